@@ -1,0 +1,266 @@
+/*
+ * qarp_hip.h — C ABI of the MI355X statevector backend (libqarp_hip.so).
+ *
+ * Plain C: fixed-width integers, doubles and pointers only.  Host arrays are
+ * passed by pointer + count; device state lives behind an opaque handle.
+ * Every entry point returns a qhip_status; the message of the last failure
+ * on the calling thread is read back with qhip_last_error().
+ *
+ * The entry points replace the native methods that OpenQARP's nanobind
+ * module binds for `qx.QarpSimulator` (src/qarpx/python/bindings.cpp:1337-1517)
+ * and the C++ kernels behind them; each declaration names the reference
+ * function it stands in for.
+ *
+ * Conventions (src/qarpx/../docs/contracts/qarp_conventions.md §1-§3):
+ *   - amplitude index i = sum_q b_q 2^q (qubit 0 is the least significant bit);
+ *   - complex128 amplitudes are interleaved (re, im) doubles, complex64 (re, im)
+ *     floats;
+ *   - dense matrices are row-major, U[y*D + x], local bit b <-> qubits[b].
+ */
+#ifndef QARP_HIP_H
+#define QARP_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ── status codes ─────────────────────────────────────────────────────────
+ * Mapped by the Python host onto the exception types the reference raises
+ * (bindings.cpp:272-290): INVALID -> ValueError, RUNTIME -> RuntimeError,
+ * CAPABILITY -> CapabilityError(ValueError), DEVICE/NOMEM -> RuntimeError. */
+typedef enum qhip_status {
+    QHIP_OK         = 0,
+    QHIP_INVALID    = 1,  /* std::invalid_argument in the reference            */
+    QHIP_RUNTIME    = 2,  /* std::runtime_error in the reference               */
+    QHIP_CAPABILITY = 3,  /* qarpx::capability_error                           */
+    QHIP_DEVICE     = 4,  /* a HIP runtime call failed                         */
+    QHIP_NOMEM      = 5   /* device or host allocation failed                  */
+} qhip_status;
+
+/* ── gate identifiers: same order as qarpx::GateType (core/gates.h:8-57) ── */
+typedef enum qhip_gate {
+    QHIP_X = 0, QHIP_Y, QHIP_Z, QHIP_H, QHIP_S, QHIP_SDG, QHIP_T, QHIP_TDG,
+    QHIP_SX, QHIP_SXDG, QHIP_ID,
+    QHIP_RX, QHIP_RY, QHIP_RZ, QHIP_P,
+    QHIP_U,
+    QHIP_CX, QHIP_CY, QHIP_CZ, QHIP_SWAP, QHIP_ECR, QHIP_ISWAP, QHIP_ISWAPDG,
+    QHIP_CH, QHIP_CS, QHIP_CSDG, QHIP_CSX, QHIP_CSXDG,
+    QHIP_CRX, QHIP_CRY, QHIP_CRZ, QHIP_CP, QHIP_RZZ, QHIP_RXX, QHIP_RYY,
+    QHIP_CU,
+    QHIP_CCX, QHIP_CSWAP,
+    QHIP_MCZ,
+    QHIP_GPHASE,
+    QHIP_BARRIER, QHIP_MEASURE, QHIP_RESET,
+    QHIP_BRANCH_BEGIN, QHIP_BRANCH_ELSE, QHIP_BRANCH_END,
+    QHIP_CUSTOM,
+    QHIP_NUM_GATES
+} qhip_gate;
+
+enum { QHIP_MAX_PARAMS = 4, QHIP_MAX_FUSION_QUBITS = 8 };
+
+/* Precision of a device state. */
+enum { QHIP_C128 = 0, QHIP_C64 = 1 };
+
+/* Op flags. */
+enum {
+    QHIP_OP_CONDITIONAL = 1  /* carries a classical condition (trajectory only) */
+};
+
+/* One command of a flat stream: the C image of qarpx::Command
+ * (core/command.h:29-103).  Parameters are concrete values; the affine
+ * dependence of a symbolic angle on the tracked parameters (what
+ * gate_param_dependencies, qarp_simulator.cpp:1487-1507, derives from
+ * affine_coefficients, param.cpp:462-497) is carried as (index, alpha)
+ * pairs in the stream's dep pools, used by the gradient entry points only. */
+typedef struct qhip_op {
+    int32_t gate;            /* qhip_gate                                      */
+    int32_t n_qubits;        /* operand count                                  */
+    int64_t qubit_offset;    /* first operand in qhip_stream.qubits            */
+    int32_t n_params;        /* 0..QHIP_MAX_PARAMS                             */
+    int32_t flags;           /* QHIP_OP_*                                      */
+    double  params[QHIP_MAX_PARAMS];
+    int64_t unitary_offset;  /* Custom: first entry in qhip_stream.unitaries   */
+                             /* (complex entries, row-major), else -1          */
+    int64_t dep_offset;      /* gradient: first pair in dep_index / dep_alpha  */
+    int32_t n_deps;          /* gradient: number of (index, alpha) pairs       */
+    int32_t cbit;            /* Measure: recorded cbit, else -1                */
+} qhip_op;
+
+typedef struct qhip_stream {
+    const qhip_op*  ops;
+    int64_t         n_ops;
+    const uint32_t* qubits;     /* operand pool                                 */
+    const double*   unitaries;  /* complex128 pool, (re, im) interleaved        */
+    const int32_t*  dep_index;  /* may be NULL when no op has n_deps > 0        */
+    const double*   dep_alpha;
+} qhip_stream;
+
+/* One Pauli string of the observable ABI (simulator/pauli_expectation.h:15-16):
+ * letters 'X' 'Y' 'Z' 'I' on the listed qubits; an empty string is the
+ * identity term. */
+typedef struct qhip_pauli_term {
+    int64_t offset;     /* first (qubit, letter) in the observable pools */
+    int32_t n_letters;
+    int32_t reserved;
+    double  coeff_re;
+    double  coeff_im;
+} qhip_pauli_term;
+
+typedef struct qhip_observable {
+    const qhip_pauli_term* terms;
+    int64_t                n_terms;
+    const uint32_t*        qubits;
+    const char*            letters;
+} qhip_observable;
+
+/* Simulator knobs.  fusion_max_qubits / fusion_min_qubits have the meaning
+ * of QarpSimulator::fusion_max_qubits / fusion_min_qubits
+ * (simulator/qarp_simulator.h:41-61).  The rest steer the device schedule:
+ * 0 (or -1 for use_mfma) selects the tuned default. */
+typedef struct qhip_config {
+    int32_t fusion_max_qubits;   /* 0..8                                       */
+    int32_t fusion_min_qubits;   /* registers below this get 1q fusion only    */
+    int32_t tile_qubits;         /* qubits held on chip per tile pass, 0=auto  */
+    int32_t low_qubits;          /* contiguous low qubits per tile, 0=auto     */
+    int32_t max_ops_per_pass;    /* 0 = auto                                   */
+    int32_t use_mfma;            /* -1 auto, 0 never, 1 for k>=5 blocks        */
+} qhip_config;
+
+/* What one apply did (filled when the caller passes a non-NULL pointer). */
+typedef struct qhip_apply_stats {
+    int64_t fused_ops;       /* ops after the fusion pass                      */
+    int64_t passes;          /* full-state HBM passes launched                 */
+    int64_t diagonal_ops;    /* ops applied as index-phase (no tile qubits)    */
+    int64_t mfma_ops;        /* k>=5 blocks applied on the matrix cores        */
+    double  flops;           /* real FP64/FP32 flops issued for matvecs        */
+} qhip_apply_stats;
+
+typedef struct qhip_state qhip_state;   /* opaque device statevector */
+typedef struct qhip_fused qhip_fused;   /* opaque host result of the fusion pass */
+
+/* ── errors / device ─────────────────────────────────────────────────────── */
+
+/* Message of the last failed call on this thread (empty when none).  Returns
+ * the status code of that failure. */
+int qhip_last_error(char* buf, size_t len);
+
+/* Number of visible HIP devices (0 when none). */
+int qhip_device_count(int32_t* out);
+
+/* Synchronise the library's stream on the current device. */
+int qhip_synchronize(void);
+
+/* ── device states ────────────────────────────────────────────────────────
+ * Stand in for csim allocate_quantum_state / initialize_quantum_state /
+ * release_quantum_state (third_party/csim/memory_ops.cpp, init_ops_fill.cpp)
+ * and the initial_state copy of QarpSimulator::statevector
+ * (qarp_simulator.cpp:924-932). */
+int qhip_state_create(int32_t n_qubits, int32_t precision, qhip_state** out);
+int qhip_state_destroy(qhip_state* s);
+int qhip_state_info(const qhip_state* s, int32_t* n_qubits, int32_t* precision);
+/* |0...0>. */
+int qhip_state_reset(qhip_state* s);
+/* Copy 2^n interleaved amplitudes (doubles for C128, floats for C64). */
+int qhip_state_upload(qhip_state* s, const void* host, uint64_t n_amps);
+int qhip_state_download(const qhip_state* s, void* host, uint64_t n_amps);
+/* Copy from a complex128 host array, converting for C64 states. */
+int qhip_state_upload_c128(qhip_state* s, const double* host, uint64_t n_amps);
+int qhip_state_download_c128(const qhip_state* s, double* host, uint64_t n_amps);
+int qhip_state_copy(qhip_state* dst, const qhip_state* src);
+/* FP64 sum |amp|^2 (validate_initial_state, qarp_simulator.cpp:731-746). */
+int qhip_state_norm2(const qhip_state* s, double* out);
+/* Device address of the amplitudes (for zero-copy interop). */
+int qhip_state_device_ptr(const qhip_state* s, void** out);
+
+/* ── fusion pass ──────────────────────────────────────────────────────────
+ * QarpSimulator::fuse_for_dispatch (qarp_simulator.cpp:120-129) ->
+ * fuse_for_simulation (simulator/fusion.cpp:120-268) or
+ * fuse_single_qubit_gates (transpiler/fusion.cpp:196-301).  Exposed so the
+ * fused stream itself can be checked; qhip_apply runs it internally. */
+int qhip_fuse(const qhip_config* cfg, const qhip_stream* in, int32_t n_qubits,
+              qhip_fused** out);
+int qhip_fused_count(const qhip_fused* f, int64_t* n_ops);
+/* Op i: gate id, operand count (<= 64 written to qubits), params (4), and
+ * for Custom the 2^k x 2^k matrix into `matrix` (capacity in complex
+ * entries; pass NULL to skip). */
+int qhip_fused_get(const qhip_fused* f, int64_t i, int32_t* gate, int32_t* n_qubits,
+                   uint32_t* qubits, double* params, double* matrix, int64_t capacity);
+int qhip_fused_destroy(qhip_fused* f);
+
+/* ── hot path: apply a command stream ─────────────────────────────────────
+ * The gate loop of QarpSimulator::statevector (qarp_simulator.cpp:901-943):
+ * fuse_for_dispatch, then every fused command on the device state in
+ * order (apply_command, :356-596; apply_dense_block, dense_kernel.cpp:23-121).
+ * Rejects trajectory commands (mid-circuit Measure with cbit, Reset,
+ * conditions, branches) with QHIP_RUNTIME, as :906-916 does. */
+int qhip_apply(const qhip_config* cfg, const qhip_stream* stream, qhip_state* s,
+               qhip_apply_stats* stats);
+
+/* As qhip_apply but recorded Measure / Barrier commands are skipped — the
+ * terminal-measurement fast path of QarpSimulator::run (qarp_simulator.cpp:1017-1036).
+ * Non-terminal measurements are rejected with QHIP_INVALID. */
+int qhip_apply_terminal(const qhip_config* cfg, const qhip_stream* stream, qhip_state* s,
+                        qhip_apply_stats* stats);
+
+/* ── Pauli-sum expectation ────────────────────────────────────────────────
+ * pauli_transition (simulator/pauli_expectation.cpp:159-167): <bra|H|ket>
+ * as (re, im).  bra and ket may be the same handle.  n in [0, 62]. */
+int qhip_transition(const qhip_state* bra, const qhip_state* ket,
+                    const qhip_observable* obs, double out[2]);
+
+/* |out> = H |in> (apply_observable, qarp_simulator.cpp:1459-1479). */
+int qhip_apply_observable(const qhip_state* in, const qhip_observable* obs, qhip_state* out);
+
+/* ── terminal sampling ────────────────────────────────────────────────────
+ * sample_terminal (qarp_simulator.cpp:751-783): n_shots draws of outcome
+ * indices from |amp|^2, identical to std::discrete_distribution<uint64_t>
+ * over std::mt19937(seed) as libstdc++ computes it (sequential normalising
+ * sum, sequential partial sums, last entry 1.0, lower_bound per draw).
+ * Outcomes are written in draw order. */
+int qhip_sample(const qhip_state* s, int64_t n_shots, uint32_t seed, uint64_t* outcomes);
+/* Same search for caller-supplied uniforms in [0, 1). */
+int qhip_sample_uniforms(const qhip_state* s, const double* uniforms, int64_t n,
+                         uint64_t* outcomes);
+/* |amp|^2 of every index into a host array of 2^n doubles (the EXACT readout
+ * of _exact_result, src/qarp/engines/_engine.py:79-116, before pruning). */
+int qhip_probabilities(const qhip_state* s, double* out, uint64_t n);
+
+/* ── adjoint gradient ─────────────────────────────────────────────────────
+ * run_gradient_core (qarp_simulator.cpp:1519-1605) with the observable
+ * builder of run_gradient (:1607-1624): forward sweep from `psi` (which is
+ * overwritten: on entry the root state, on exit U|root>), phi = H psi, then
+ * the reverse sweep un-applying every gate on psi and phi while all
+ * generator inner products accumulate in FP64.  grad has n_params entries
+ * (unused indices stay 0); energy receives <psi|H|psi> (may be NULL).
+ * `stream` is the concrete stream with dep pools filled. */
+int qhip_run_gradient(const qhip_config* cfg, const qhip_stream* stream, int32_t n_params,
+                      qhip_state* psi, const qhip_observable* obs,
+                      double* grad, double* energy);
+
+/* run_gradient_phi (qarp_simulator.cpp:1626-1639): caller-supplied phi
+ * (copied; `phi` is not modified). */
+int qhip_run_gradient_phi(const qhip_config* cfg, const qhip_stream* stream, int32_t n_params,
+                          qhip_state* psi, const qhip_state* phi, double* grad);
+
+/* ── batched instances ────────────────────────────────────────────────────
+ * Many independent circuits, each with its own register width, stream and
+ * observable, evaluated together (replaces the sequential loops of
+ * batch_expectation, qarp_simulator.cpp:971-984, and the per-parameter-set
+ * sweep of QarpEngine._sweep, src/qarp/engines/_qarp_engine.py:450-503).
+ * energies[i] = <psi_i|H_i|psi_i>; when grads is non-NULL the adjoint
+ * gradient of instance i is written at grads + grad_offsets[i].
+ * precision selects complex128 or complex64 states (FP64 reductions). */
+int qhip_batch_energy_grad(const qhip_config* cfg, int32_t n_instances,
+                           const int32_t* n_qubits, const qhip_stream* streams,
+                           const qhip_observable* observables, const int32_t* n_params,
+                           int32_t precision, double* energies,
+                           double* grads, const int64_t* grad_offsets);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QARP_HIP_H */

@@ -1,0 +1,193 @@
+"""ctypes image of include/qarp_hip.h and the packing of Python IR into it.
+
+The stream and observable layouts are shared with every C consumer of the
+header; ``pack_stream`` / ``pack_observable`` return objects that keep the
+numpy buffers alive for as long as the ctypes structs point at them.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Mapping, Optional, Sequence
+
+import numpy as np
+
+from .ir import Command, GateType, Param, affine_coefficients
+from .errors import CapabilityError
+
+QHIP_OK, QHIP_INVALID, QHIP_RUNTIME, QHIP_CAPABILITY, QHIP_DEVICE, QHIP_NOMEM = range(6)
+QHIP_C128, QHIP_C64 = 0, 1
+QHIP_OP_CONDITIONAL = 1
+
+OP_DTYPE = np.dtype({
+    "names": ["gate", "n_qubits", "qubit_offset", "n_params", "flags", "params",
+              "unitary_offset", "dep_offset", "n_deps", "cbit"],
+    "formats": [np.int32, np.int32, np.int64, np.int32, np.int32, (np.float64, 4),
+                np.int64, np.int64, np.int32, np.int32],
+    "offsets": [0, 4, 8, 16, 20, 24, 56, 64, 72, 76],
+    "itemsize": 80,
+})
+
+TERM_DTYPE = np.dtype({
+    "names": ["offset", "n_letters", "reserved", "coeff_re", "coeff_im"],
+    "formats": [np.int64, np.int32, np.int32, np.float64, np.float64],
+    "offsets": [0, 8, 12, 16, 24],
+    "itemsize": 32,
+})
+
+
+class QhipStream(C.Structure):
+    _fields_ = [("ops", C.c_void_p), ("n_ops", C.c_int64), ("qubits", C.c_void_p),
+                ("unitaries", C.c_void_p), ("dep_index", C.c_void_p), ("dep_alpha", C.c_void_p)]
+
+
+class QhipObservable(C.Structure):
+    _fields_ = [("terms", C.c_void_p), ("n_terms", C.c_int64), ("qubits", C.c_void_p),
+                ("letters", C.c_void_p)]
+
+
+class QhipConfig(C.Structure):
+    _fields_ = [("fusion_max_qubits", C.c_int32), ("fusion_min_qubits", C.c_int32),
+                ("tile_qubits", C.c_int32), ("low_qubits", C.c_int32),
+                ("max_ops_per_pass", C.c_int32), ("use_mfma", C.c_int32)]
+
+
+class QhipApplyStats(C.Structure):
+    _fields_ = [("fused_ops", C.c_int64), ("passes", C.c_int64), ("diagonal_ops", C.c_int64),
+                ("mfma_ops", C.c_int64), ("flops", C.c_double)]
+
+
+def _ptr(a: Optional[np.ndarray]) -> Optional[int]:
+    return None if a is None or a.size == 0 else a.ctypes.data
+
+
+@dataclass
+class PackedStream:
+    """A command stream as the C structs see it (buffers kept alive here)."""
+
+    ops: np.ndarray
+    qubits: np.ndarray
+    unitaries: np.ndarray
+    dep_index: np.ndarray
+    dep_alpha: np.ndarray
+    struct: QhipStream
+
+    @property
+    def ref(self):
+        return C.byref(self.struct)
+
+
+def _param_values(cmd: Command, values: Optional[Mapping[str, float]]) -> list[float]:
+    out = []
+    for p in cmd.params:
+        if p.is_symbolic():
+            if values is None:
+                raise RuntimeError(f"gate '{cmd.gate.name}' has unresolved symbolic parameter")
+            out.append(p.evaluate(values))
+        else:
+            out.append(p.value())
+    return out
+
+
+def pack_stream(commands: Sequence[Command], values: Optional[Mapping[str, float]] = None,
+                param_index: Optional[Mapping[str, int]] = None) -> PackedStream:
+    """Flatten commands into qhip_op records.
+
+    ``values`` binds symbolic parameters; ``param_index`` (name -> gradient
+    slot) fills the dep pools with d(angle)/d(parameter) for every tracked
+    symbol, as gate_param_dependencies (qarp_simulator.cpp:1487-1507) does."""
+    n = len(commands)
+    ops = np.zeros(n, dtype=OP_DTYPE)
+    qubits: list[int] = []
+    unitaries: list[np.ndarray] = []
+    u_off = 0
+    dep_idx: list[int] = []
+    dep_alpha: list[float] = []
+    for i, cmd in enumerate(commands):
+        rec = ops[i]
+        rec["gate"] = int(cmd.gate)
+        rec["n_qubits"] = len(cmd.qubits)
+        rec["qubit_offset"] = len(qubits)
+        qubits.extend(cmd.qubits)
+        vals = _param_values(cmd, values)
+        if len(vals) > 4:
+            raise ValueError(f"gate '{cmd.gate.name}' carries {len(vals)} parameters")
+        rec["n_params"] = len(vals)
+        rec["params"][: len(vals)] = vals
+        rec["flags"] = QHIP_OP_CONDITIONAL if cmd.condition_bits else 0
+        rec["cbit"] = cmd.cbits[0] if cmd.cbits else -1
+        if cmd.gate == GateType.Custom and cmd.unitary is not None:
+            rec["unitary_offset"] = u_off
+            unitaries.append(np.ascontiguousarray(cmd.unitary, dtype=np.complex128).ravel())
+            u_off += cmd.unitary.size
+        else:
+            rec["unitary_offset"] = -1
+        rec["dep_offset"] = len(dep_idx)
+        nd = 0
+        if param_index is not None:
+            for p in cmd.params:
+                if not p.is_symbolic():
+                    continue
+                try:
+                    coeffs = affine_coefficients(p)
+                except CapabilityError as e:
+                    raise CapabilityError(f"run_gradient: {e}") from None
+                for name, alpha in coeffs.items():
+                    if name in param_index:
+                        dep_idx.append(int(param_index[name]))
+                        dep_alpha.append(float(alpha))
+                        nd += 1
+        rec["n_deps"] = nd
+    q = np.asarray(qubits, dtype=np.uint32)
+    u = np.concatenate(unitaries).view(np.float64) if unitaries else np.zeros(0, np.float64)
+    di = np.asarray(dep_idx, dtype=np.int32)
+    da = np.asarray(dep_alpha, dtype=np.float64)
+    st = QhipStream(_ptr(ops), n, _ptr(q), _ptr(u), _ptr(di), _ptr(da))
+    return PackedStream(ops, q, u, di, da, st)
+
+
+Observable = Sequence[tuple]
+
+
+@dataclass
+class PackedObservable:
+    terms: np.ndarray
+    qubits: np.ndarray
+    letters: np.ndarray
+    struct: QhipObservable
+
+    @property
+    def ref(self):
+        return C.byref(self.struct)
+
+
+def pack_observable(observable: Observable) -> PackedObservable:
+    """The observable ABI ``[([(qubit, 'X'|'Y'|'Z'|'I'), ...], coeff), ...]``
+    (simulator/pauli_expectation.h:15-16)."""
+    terms = np.zeros(len(observable), dtype=TERM_DTYPE)
+    qs: list[int] = []
+    ls: list[int] = []
+    for t, (pauli, coeff) in enumerate(observable):
+        c = complex(coeff)
+        terms[t]["offset"] = len(qs)
+        terms[t]["n_letters"] = len(pauli)
+        terms[t]["coeff_re"] = c.real
+        terms[t]["coeff_im"] = c.imag
+        for q, letter in pauli:
+            if int(q) < 0:
+                raise ValueError(f"pauli_transition: observable qubit {q} out of range")
+            qs.append(int(q))
+            s = str(letter)
+            if len(s) != 1:
+                raise ValueError(f"pauli_transition: invalid Pauli '{s}'")
+            ls.append(ord(s))
+    q = np.asarray(qs, dtype=np.uint32)
+    lt = np.asarray(ls, dtype=np.uint8)
+    st = QhipObservable(_ptr(terms), len(observable), _ptr(q), _ptr(lt))
+    return PackedObservable(terms, q, lt, st)
+
+
+def config_struct(fusion_max_qubits: int, fusion_min_qubits: int, tile_qubits: int = 0,
+                  low_qubits: int = 0, max_ops_per_pass: int = 0, use_mfma: int = -1) -> QhipConfig:
+    return QhipConfig(fusion_max_qubits, fusion_min_qubits, tile_qubits, low_qubits,
+                      max_ops_per_pass, use_mfma)

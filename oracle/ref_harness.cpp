@@ -1,0 +1,288 @@
+// ref_harness.cpp — C entry points over the OpenQARP reference simulator.
+//
+// TEST INFRASTRUCTURE ONLY.  Compiled (by oracle/Makefile) together with the
+// reference's own simulator sources where they lie in the OpenQARP tree, into
+// oracle/_ref/libqarp_ref.so.  Used to write golden vectors, to pin the C
+// restatement (qarp_oracle.c), and as the timed CPU baseline.  Never linked
+// by the product.
+//
+// Each function converts the plain-C stream types of include/qarp_hip.h into
+// qarpx::Command / PauliObservable and calls the reference method named in
+// its comment (src/qarpx/include/qarpx/simulator/qarp_simulator.h).
+#include "../include/qarp_hip.h"
+
+#include "qarpx/core/command.h"
+#include "qarpx/core/errors.h"
+#include "qarpx/simulator/fusion.h"
+#include "qarpx/simulator/pauli_expectation.h"
+#include "qarpx/simulator/qarp_simulator.h"
+#include "qarpx/transpiler/fusion.h"
+
+#include <Eigen/Dense>
+
+#include <complex>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+using namespace qarpx;
+using cd = std::complex<double>;
+
+namespace {
+
+thread_local std::string g_err;
+
+int catch_all(const std::exception& e, int code) {
+    g_err = e.what();
+    return code;
+}
+
+std::string sym_name(int k) { return "p" + std::to_string(k); }
+
+// Build a qarpx::Command for op i.  When `symbolic` and the op has deps, the
+// first parameter becomes sum_k alpha_k * p_k + offset with the offset chosen
+// so the value at `values` equals params[0].
+Command to_command(const qhip_stream* s, int64_t i, bool symbolic, const double* values) {
+    const qhip_op& op = s->ops[i];
+    Command c;
+    c.gate = static_cast<GateType>(op.gate);
+    for (int b = 0; b < op.n_qubits; ++b) c.qubits.push_back(s->qubits[op.qubit_offset + b]);
+    for (int p = 0; p < op.n_params; ++p) c.params.push_back(Param(op.params[p]));
+    if (op.cbit >= 0) c.cbits.push_back(static_cast<uint32_t>(op.cbit));
+    if (op.gate == QHIP_CUSTOM && op.unitary_offset >= 0) {
+        const int D = 1 << op.n_qubits;
+        auto m = std::make_shared<Eigen::MatrixXcd>(D, D);
+        const double* u = s->unitaries + 2 * op.unitary_offset;
+        for (int y = 0; y < D; ++y)
+            for (int x = 0; x < D; ++x) (*m)(y, x) = cd(u[2 * (y * D + x)], u[2 * (y * D + x) + 1]);
+        c.unitary = m;
+    }
+    if (symbolic && op.n_deps > 0 && op.n_params > 0) {
+        double offset = op.params[0];
+        for (int d = 0; d < op.n_deps; ++d)
+            offset -= s->dep_alpha[op.dep_offset + d] * values[s->dep_index[op.dep_offset + d]];
+        Param p = Param::linear(s->dep_alpha[op.dep_offset], sym_name(s->dep_index[op.dep_offset]), offset);
+        for (int d = 1; d < op.n_deps; ++d)
+            p = p + Param::linear(s->dep_alpha[op.dep_offset + d], sym_name(s->dep_index[op.dep_offset + d]), 0.0);
+        c.params[0] = p;
+    }
+    return c;
+}
+
+std::vector<Command> to_commands(const qhip_stream* s, bool symbolic = false, const double* values = nullptr) {
+    std::vector<Command> out;
+    out.reserve(static_cast<size_t>(s->n_ops));
+    for (int64_t i = 0; i < s->n_ops; ++i) out.push_back(to_command(s, i, symbolic, values));
+    return out;
+}
+
+PauliObservable to_observable(const qhip_observable* o) {
+    PauliObservable obs;
+    for (int64_t t = 0; t < o->n_terms; ++t) {
+        const auto& term = o->terms[t];
+        std::vector<std::pair<uint32_t, char>> letters;
+        for (int l = 0; l < term.n_letters; ++l)
+            letters.emplace_back(o->qubits[term.offset + l], o->letters[term.offset + l]);
+        obs.emplace_back(std::move(letters), cd(term.coeff_re, term.coeff_im));
+    }
+    return obs;
+}
+
+std::optional<std::vector<cd>> to_state(const double* init, int n) {
+    if (!init) return std::nullopt;
+    const uint64_t dim = uint64_t{1} << n;
+    std::vector<cd> v(dim);
+    std::memcpy(v.data(), init, sizeof(cd) * dim);
+    return v;
+}
+
+QarpSimulator make_sim(int fusion_max, int fusion_min) {
+    QarpSimulator sim;
+    sim.set_fusion_max_qubits(static_cast<std::size_t>(fusion_max));
+    sim.set_fusion_min_qubits(static_cast<std::size_t>(fusion_min));
+    return sim;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ref_last_error(void) { return g_err.c_str(); }
+
+// QarpSimulator::statevector
+int ref_statevector(const qhip_stream* s, int n, int fusion_max, int fusion_min,
+                    const double* init, double* out) {
+    try {
+        auto sim = make_sim(fusion_max, fusion_min);
+        auto sv = sim.statevector(to_commands(s), n, to_state(init, n));
+        std::memcpy(out, sv.data(), sizeof(cd) * sv.size());
+        return 0;
+    } catch (const std::invalid_argument& e) { return catch_all(e, QHIP_INVALID);
+    } catch (const capability_error& e) { return catch_all(e, QHIP_CAPABILITY);
+    } catch (const std::exception& e) { return catch_all(e, QHIP_RUNTIME); }
+}
+
+// QarpSimulator::expectation
+int ref_expectation(const qhip_stream* s, int n, int fusion_max, int fusion_min,
+                    const qhip_observable* obs, const double* init, double* out) {
+    try {
+        auto sim = make_sim(fusion_max, fusion_min);
+        *out = sim.expectation(to_commands(s), n, to_observable(obs), to_state(init, n));
+        return 0;
+    } catch (const std::invalid_argument& e) { return catch_all(e, QHIP_INVALID);
+    } catch (const std::exception& e) { return catch_all(e, QHIP_RUNTIME); }
+}
+
+// pauli_transition
+int ref_transition(const double* bra, const double* ket, int n, const qhip_observable* obs,
+                   double out[2]) {
+    try {
+        const cd v = pauli_transition(reinterpret_cast<const cd*>(bra), reinterpret_cast<const cd*>(ket),
+                                      n, to_observable(obs));
+        out[0] = v.real();
+        out[1] = v.imag();
+        return 0;
+    } catch (const std::invalid_argument& e) { return catch_all(e, QHIP_INVALID);
+    } catch (const std::exception& e) { return catch_all(e, QHIP_RUNTIME); }
+}
+
+// The fused stream of fuse_for_dispatch: fuse_for_simulation for k >= 2 and
+// n >= fusion_min, fuse_single_qubit_gates otherwise.  Writes per fused op:
+// gate id, operand count, operands (8 max), params (4), and for Custom the
+// matrix into mats at op_index * 256 * 2 doubles (capacity checked).
+int ref_fuse(const qhip_stream* s, int n, int fusion_max, int fusion_min, int64_t cap_ops,
+             int64_t* n_out, int32_t* gates, int32_t* nqs, uint32_t* qubits, double* params,
+             double* mats) {
+    try {
+        auto sim = make_sim(fusion_max, fusion_min);
+        const auto cmds = to_commands(s);
+        std::vector<Command> fused;
+        if (fusion_max == 0) fused = cmds;
+        else if (fusion_max == 1 || n < fusion_min) fused = fuse_single_qubit_gates(cmds);
+        else fused = fuse_for_simulation(cmds, static_cast<std::size_t>(fusion_max),
+                                         [&sim](const Command& c) { return sim.local_unitary(c); });
+        *n_out = static_cast<int64_t>(fused.size());
+        if (static_cast<int64_t>(fused.size()) > cap_ops) return 0;
+        for (size_t i = 0; i < fused.size(); ++i) {
+            const auto& c = fused[i];
+            gates[i] = static_cast<int32_t>(c.gate);
+            nqs[i] = static_cast<int32_t>(c.qubits.size());
+            for (size_t b = 0; b < c.qubits.size() && b < 8; ++b) qubits[i * 8 + b] = c.qubits[b];
+            for (size_t p = 0; p < 4; ++p)
+                params[i * 4 + p] = p < c.params.size() && c.params[p].is_concrete() ? c.params[p].value() : 0.0;
+            if (c.gate == GateType::Custom && c.unitary) {
+                const auto& U = *c.unitary;
+                const int D = static_cast<int>(U.rows());
+                if (D <= 16)
+                    for (int y = 0; y < D; ++y)
+                        for (int x = 0; x < D; ++x) {
+                            mats[(i * 256 + y * D + x) * 2] = U(y, x).real();
+                            mats[(i * 256 + y * D + x) * 2 + 1] = U(y, x).imag();
+                        }
+            }
+        }
+        return 0;
+    } catch (const std::exception& e) { return catch_all(e, QHIP_RUNTIME); }
+}
+
+// QarpSimulator::run with a recorded terminal Measure on every qubit, so the
+// per-shot cbit_history (draw order) gives each shot's outcome index.
+int ref_sample_state(const double* sv, int n, int n_shots, uint32_t seed, uint64_t* outcomes) {
+    try {
+        QarpSimulator sim;
+        std::vector<Command> cmds;
+        for (int q = 0; q < n; ++q) {
+            Command m(GateType::Measure, static_cast<uint32_t>(q));
+            m.cbits.push_back(static_cast<uint32_t>(q));
+            cmds.push_back(m);
+        }
+        auto res = sim.run(cmds, n, n_shots, seed, to_state(sv, n));
+        if (n == 0) {
+            for (int i = 0; i < n_shots; ++i) outcomes[i] = 0;
+            return 0;
+        }
+        for (int i = 0; i < n_shots; ++i) {
+            uint64_t v = 0;
+            for (int q = 0; q < n; ++q) if (res.cbit_history[i][q]) v |= uint64_t{1} << q;
+            outcomes[i] = v;
+        }
+        return 0;
+    } catch (const std::invalid_argument& e) { return catch_all(e, QHIP_INVALID);
+    } catch (const std::exception& e) { return catch_all(e, QHIP_RUNTIME); }
+}
+
+// QarpSimulator::run counts (no recorded measures): the fast path itself.
+int ref_run_counts(const qhip_stream* s, int n, int n_shots, uint32_t seed, int fusion_max,
+                   int fusion_min, int64_t cap, int64_t* n_keys, uint64_t* keys, int32_t* counts) {
+    try {
+        auto sim = make_sim(fusion_max, fusion_min);
+        auto res = sim.run(to_commands(s), n, n_shots, seed);
+        *n_keys = static_cast<int64_t>(res.counts.size());
+        int64_t i = 0;
+        for (const auto& [k, v] : res.counts) {
+            if (i >= cap) break;
+            keys[i] = k;
+            counts[i] = v;
+            ++i;
+        }
+        return 0;
+    } catch (const std::invalid_argument& e) { return catch_all(e, QHIP_INVALID);
+    } catch (const std::exception& e) { return catch_all(e, QHIP_RUNTIME); }
+}
+
+// The toolchain's std::mt19937 words and generate_canonical<double,53> values.
+void ref_mt19937(uint32_t seed, int64_t n, uint32_t* words) {
+    std::mt19937 g(seed);
+    for (int64_t i = 0; i < n; ++i) words[i] = static_cast<uint32_t>(g());
+}
+void ref_uniforms(uint32_t seed, int64_t n, double* u) {
+    std::mt19937 g(seed);
+    for (int64_t i = 0; i < n; ++i) u[i] = std::generate_canonical<double, 53>(g);
+}
+// std::discrete_distribution over given weights, draws in order.
+void ref_discrete(const double* w, int64_t len, uint32_t seed, int64_t n, uint64_t* out) {
+    std::mt19937 g(seed);
+    std::discrete_distribution<uint64_t> d(w, w + len);
+    for (int64_t i = 0; i < n; ++i) out[i] = d(g);
+}
+
+// QarpSimulator::run_gradient with symbols p<k> built from the dep pools.
+int ref_run_gradient(const qhip_stream* s, int n, const qhip_observable* obs, int n_params,
+                     const double* values, const double* init, double* grad) {
+    try {
+        QarpSimulator sim;
+        std::unordered_map<std::string, double> params;
+        std::vector<std::string> order;
+        for (int k = 0; k < n_params; ++k) { params[sym_name(k)] = values[k]; order.push_back(sym_name(k)); }
+        auto g = sim.run_gradient(to_commands(s, true, values), n, to_observable(obs), params, order,
+                                  to_state(init, n));
+        std::memcpy(grad, g.data(), sizeof(double) * g.size());
+        return 0;
+    } catch (const std::invalid_argument& e) { return catch_all(e, QHIP_INVALID);
+    } catch (const capability_error& e) { return catch_all(e, QHIP_CAPABILITY);
+    } catch (const std::exception& e) { return catch_all(e, QHIP_RUNTIME); }
+}
+
+// QarpSimulator::run_gradient_phi
+int ref_run_gradient_phi(const qhip_stream* s, int n, const double* phi, int n_params,
+                         const double* values, const double* init, double* grad) {
+    try {
+        QarpSimulator sim;
+        std::unordered_map<std::string, double> params;
+        std::vector<std::string> order;
+        for (int k = 0; k < n_params; ++k) { params[sym_name(k)] = values[k]; order.push_back(sym_name(k)); }
+        const uint64_t dim = uint64_t{1} << n;
+        std::vector<cd> ph(dim);
+        std::memcpy(ph.data(), phi, sizeof(cd) * dim);
+        auto g = sim.run_gradient_phi(to_commands(s, true, values), n, ph, params, order, to_state(init, n));
+        std::memcpy(grad, g.data(), sizeof(double) * g.size());
+        return 0;
+    } catch (const std::invalid_argument& e) { return catch_all(e, QHIP_INVALID);
+    } catch (const capability_error& e) { return catch_all(e, QHIP_CAPABILITY);
+    } catch (const std::exception& e) { return catch_all(e, QHIP_RUNTIME); }
+}
+
+}  // extern "C"
