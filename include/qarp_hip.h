@@ -230,12 +230,13 @@ int qhip_probabilities(const qhip_state* s, double* out, uint64_t n);
 
 /* ── adjoint gradient ─────────────────────────────────────────────────────
  * run_gradient_core (qarp_simulator.cpp:1519-1605) with the observable
- * builder of run_gradient (:1607-1624): forward sweep from `psi` (which is
- * overwritten: on entry the root state, on exit U|root>), phi = H psi, then
+ * builder of run_gradient (:1607-1624): forward sweep from `psi`, phi = H psi, then
  * the reverse sweep un-applying every gate on psi and phi while all
- * generator inner products accumulate in FP64.  grad has n_params entries
- * (unused indices stay 0); energy receives <psi|H|psi> (may be NULL).
- * `stream` is the concrete stream with dep pools filled. */
+ * generator inner products accumulate in FP64.  `psi` holds the root state on
+ * entry and, the reverse sweep having un-applied every gate, again on exit.
+ * grad has n_params entries (unused indices stay 0); energy receives
+ * <psi|H|psi> of the evolved state (may be NULL).  `stream` is the concrete
+ * stream with dep pools filled. */
 int qhip_run_gradient(const qhip_config* cfg, const qhip_stream* stream, int32_t n_params,
                       qhip_state* psi, const qhip_observable* obs,
                       double* grad, double* energy);

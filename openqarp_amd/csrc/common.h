@@ -1,0 +1,207 @@
+// common.h — internal declarations of libqarp_hip.so.
+//
+// Host-side runtime (fusion, scheduling, sampling resolve) is C++17; device
+// code is HIP for gfx950.  Nothing here is part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/qarp_hip.h"
+
+namespace qhip {
+
+using cplx = std::complex<double>;
+
+// ── errors ──────────────────────────────────────────────────────────────────
+// Library functions throw qhip::Error; the C entry points catch it, latch the
+// message in a thread-local string and return the code.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+[[noreturn]] void fail(int code, const std::string& msg);
+void hip_check(hipError_t e, const char* what);
+#define QHIP_CHECK(x) ::qhip::hip_check((x), #x)
+
+// ── device context ──────────────────────────────────────────────────────────
+// One context per process for the current device: its own non-blocking
+// stream (default priority) and grow-only scratch buffers.
+struct DeviceBuffer {
+    void*  ptr = nullptr;
+    size_t bytes = 0;
+    void*  ensure(size_t n);
+    ~DeviceBuffer();
+};
+
+struct Context {
+    int          device = -1;
+    hipStream_t  stream = nullptr;
+    int          n_cu = 0;
+    DeviceBuffer ops;       // device op tables
+    DeviceBuffer coefs;     // gate coefficients
+    DeviceBuffer red;       // reduction partials
+    DeviceBuffer misc;      // sampling / adjoint scratch
+    DeviceBuffer misc2;
+    DeviceBuffer pinned_dummy;
+};
+Context& ctx();   // initialises on first use (throws QHIP_DEVICE without a GPU)
+
+// ── device states ───────────────────────────────────────────────────────────
+}  // namespace qhip
+
+struct qhip_state {
+    int32_t n = 0;
+    int32_t precision = QHIP_C128;
+    void*   d = nullptr;      // 2^n amplitudes
+    bool    owned = true;
+    uint64_t dim() const { return uint64_t{1} << n; }
+    size_t amp_bytes() const { return precision == QHIP_C128 ? 16 : 8; }
+};
+
+namespace qhip {
+
+// ── host op representation after fusion ─────────────────────────────────────
+enum OpKind : int32_t {
+    OP_DENSE = 0,     // 2^k x 2^k matrix on k operands
+    OP_DIAG = 1,      // 2^k diagonal on k operands (no locality requirement)
+    OP_MASKPHASE = 2, // amplitude *= phase where (i & mask) == mask
+    OP_SCALAR = 3,    // amplitude *= phase everywhere
+    OP_IP_DENSE = 4,  // adjoint: accumulate <phi| G |psi> for a dense G
+    OP_IP_DIAG = 5,   // adjoint: accumulate <phi| G |psi> for a diagonal G
+};
+
+struct HostOp {
+    int32_t kind = OP_DENSE;
+    int32_t k = 0;
+    std::array<uint32_t, 8> q{};     // local bit b <-> q[b]
+    std::vector<cplx> m;             // dense: D*D row-major; diag: D entries
+    uint64_t mask = 0;               // maskphase operands
+    cplx phase{1.0, 0.0};            // maskphase / scalar
+    int32_t slot = -1;               // IP ops: accumulator slot
+    uint64_t qmask() const {
+        if (kind == OP_MASKPHASE) return mask;
+        uint64_t r = 0;
+        for (int b = 0; b < k; ++b) r |= uint64_t{1} << q[b];
+        return r;
+    }
+    bool needs_local() const { return kind == OP_DENSE || kind == OP_IP_DENSE; }
+};
+
+// Matrix of one concrete named command (gate id, operands, params) over its
+// own operands, local bit b <-> operand b.  Returns false for gates that are
+// not a fixed-width unitary (markers, MCZ wider than 8, GPhase).
+bool named_matrix(int gate, int k, const double* p, std::vector<cplx>& m);
+
+// Stream view.
+struct Stream {
+    const qhip_stream* s;
+    const qhip_op& op(int64_t i) const { return s->ops[i]; }
+    const uint32_t* qubits(int64_t i) const { return s->qubits + s->ops[i].qubit_offset; }
+    int64_t size() const { return s->n_ops; }
+};
+
+// Fusion pass of fuse_for_dispatch: emits HostOps (markers dropped when
+// `drop_markers`).  Also used by qhip_fuse with the raw op/Custom split.
+struct FusedItem {
+    int64_t src = -1;                // verbatim input op, or -1 for a fused block
+    int32_t k = 0;
+    std::array<uint32_t, 8> q{};
+    std::vector<cplx> m;             // fused block matrix
+};
+std::vector<FusedItem> fuse_for_dispatch(const Stream& st, int n, int fusion_max, int fusion_min);
+
+// Lower one input op (verbatim) or fused block into HostOps.
+void lower_item(const Stream& st, const FusedItem& it, int n, std::vector<HostOp>& out,
+                bool skip_terminal_markers);
+void lower_op(const Stream& st, int64_t i, std::vector<HostOp>& out);
+
+// Adjoint helpers: the dagger of op i as HostOps, and its generator.
+void lower_dagger(const Stream& st, int64_t i, std::vector<HostOp>& out);
+// Generator of a parametric op: fills G (dense or diagonal HostOp without
+// slot) and prefactor c; returns false when the gate has no rule.
+bool generator(const Stream& st, int64_t i, HostOp& G, cplx& c);
+
+// Classify a dense HostOp as diagonal when every off-diagonal entry is 0.
+void classify_diagonal(HostOp& op);
+
+// ── device schedule ─────────────────────────────────────────────────────────
+struct DevOp {          // 64 bytes, one per op per pass
+    int32_t kind;
+    int32_t k;
+    int8_t  pos[8];     // dense/IP: local positions; diag: local pos or 32+global qubit
+    int8_t  spos[8];    // dense/IP: local positions sorted ascending
+    int64_t coef;       // first coefficient in the pass coefficient table
+    uint64_t lmask;     // maskphase: local-position mask; IP ops: accumulator slot
+    uint64_t gmask;     // maskphase: global-bit mask outside the tile
+    int64_t reserved[2];
+};
+static_assert(sizeof(DevOp) == 64, "DevOp layout");
+
+struct PassDesc {
+    int32_t n;           // register width
+    int32_t T;           // tile qubits; 0 marks a wide pass (one dense op, k > 5)
+    int32_t op_begin;
+    int32_t op_end;
+    int32_t kmax;        // widest dense / IP op in the pass
+    int32_t reserved;
+    uint64_t tmask;      // global bits that are tile-local
+    uint64_t hmask;      // global bits of local positions >= 8
+    int32_t qpos[40];    // qpos[l] = global qubit of local position l
+};
+
+struct Plan {
+    std::vector<PassDesc> passes;
+    std::vector<DevOp> ops;
+    std::vector<cplx> coefs;
+    int64_t dense_ops = 0, diag_ops = 0, big_ops = 0;
+    double flops = 0.0;
+};
+
+struct ScheduleOptions {
+    int tile_qubits = 0;     // 0 = default for the precision / two-state mode
+    int low_qubits = 0;      // 0 = default
+    int max_ops = 0;         // 0 = no cap
+    double max_flops_per_amp = 0.0;  // 0 = no cap
+    bool two_states = false; // adjoint passes hold psi and phi
+};
+Plan schedule(const std::vector<HostOp>& ops, int n, int precision, const ScheduleOptions& opt);
+int default_tile_qubits(int precision, bool two_states);
+
+// ── device entry points (kernels_*.hip) ────────────────────────────────────
+void launch_plan(const Plan& plan, qhip_state* s, qhip_state* phi, void* d_ip_partials,
+                 int n_slots);
+void state_reset(qhip_state* s);
+double state_norm2(const qhip_state* s);
+void state_convert_upload(qhip_state* s, const double* host, uint64_t n);
+void state_convert_download(const qhip_state* s, double* host, uint64_t n);
+
+struct PauliTables;   // grouped observable on the device
+cplx transition(const qhip_state* bra, const qhip_state* ket, const qhip_observable* obs);
+void apply_observable(const qhip_state* in, const qhip_observable* obs, qhip_state* out);
+cplx inner(const qhip_state* a, const qhip_state* b);   // <a|b> in FP64
+
+void sample(const qhip_state* s, const double* uniforms, int64_t n, uint64_t* out);
+void probabilities(const qhip_state* s, double* out);
+
+// IP partial reduction: sum tile partials per slot (deterministic order).
+void reduce_ip(const void* d_partials, int64_t n_tiles, int n_slots, cplx* host_out);
+
+void apply_stream(const qhip_config* cfg, const qhip_stream* stream, qhip_state* s,
+                  qhip_apply_stats* stats, bool terminal);
+void run_gradient(const qhip_config* cfg, const qhip_stream* stream, int n_params,
+                  qhip_state* psi, const qhip_observable* obs, const qhip_state* phi_in,
+                  double* grad, double* energy);
+void batch_energy_grad(const qhip_config* cfg, int n_instances, const int32_t* n_qubits,
+                       const qhip_stream* streams, const qhip_observable* observables,
+                       const int32_t* n_params, int precision, double* energies, double* grads,
+                       const int64_t* grad_offsets);
+
+}  // namespace qhip

@@ -1,0 +1,327 @@
+// kernels_sample.hip — terminal sampling, identical to libstdc++'s
+// std::discrete_distribution over std::mt19937 as sample_terminal calls it
+// (src/qarpx/src/simulator/qarp_simulator.cpp:751-783).
+//
+// libstdc++ builds the CDF sequentially: S = (((0 + p_0) + p_1) + ...),
+// q_i = p_i / S, cp_i = cp_{i-1} + q_i, last entry forced to 1.0, and a draw
+// u returns lower_bound(cp, u).  A parallel scan rounds differently, so the
+// device reproduces the sequential recurrence exactly:
+//
+//   While a running sum s stays inside one binade [2^e, 2^(e+1)), every
+//   double there is an integer multiple of ulp = 2^(e-52), so
+//   s_i = RN(s_{i-1} + v_i) = s_{i-1} + ulp * r_i with r_i = round(v_i / ulp)
+//   — exact integer arithmetic, independent of s_{i-1} except on an exact
+//   rounding tie (fraction 1/2), where ties-to-even looks at s's parity.
+//
+// Pass A (device): approximate FP64 chunk sums -> a guess of each chunk's
+//   binade.  Pass B (device): per chunk, the exact integer sum of r_i on the
+//   guessed grid plus tie / overflow flags.  Resolve (host, one step per
+//   chunk): start from s = 0 and advance s by ulp * R_c when s really lies in
+//   the guessed binade, no flag is set and the end stays below 2^(e+1);
+//   otherwise download that chunk and run the float recurrence serially.
+//   The resolve is exact either way; the guesses only decide how often the
+//   serial path runs (chunks that cross a power of two, about one per
+//   binade).  It runs once over p (giving S) and once over q = p / S (giving
+//   the exact CDF at every chunk boundary).
+// Search (device, one thread per draw): binary search over the exact chunk
+//   ends, then the float recurrence from the chunk's exact start.
+//
+// Every sum and product here is separately rounded: contraction to FMA is
+// off for this file.
+#pragma clang fp contract(off)
+
+#include "common.h"
+
+#include <cmath>
+#include <random>
+
+namespace qhip {
+
+constexpr int kST = 256;
+constexpr int kPerS = 8;
+constexpr uint64_t kSChunk = kST * kPerS;   // 2048 amplitudes
+constexpr int kSerial = -100000;
+
+__device__ __forceinline__ double prob(const double2* s, uint64_t i) {
+    const double2 a = s[i];
+    const double xx = a.x * a.x;
+    const double yy = a.y * a.y;
+    return xx + yy;                       // std::norm: x*x + y*y
+}
+__device__ __forceinline__ double prob(const float2* s, uint64_t i) {
+    const float2 a = s[i];
+    const double x = a.x, y = a.y;
+    const double xx = x * x;
+    const double yy = y * y;
+    return xx + yy;
+}
+
+template <typename A>
+__global__ __launch_bounds__(kST) void chunk_sums(const A* __restrict__ s, uint64_t dim, double* __restrict__ out) {
+    __shared__ double red[kST];
+    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kSChunk;
+    double acc = 0.0;
+    for (int r = 0; r < kPerS; ++r) {
+        const uint64_t i = base + static_cast<uint64_t>(r) * kST + threadIdx.x;
+        if (i < dim) acc += prob(s, i);
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kST / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// Exact integer increments on the guessed grid.  flags: bit0 tie, bit1 overflow.
+template <typename A>
+__global__ __launch_bounds__(kST) void chunk_increments(const A* __restrict__ s, uint64_t dim,
+                                                        const int32_t* __restrict__ grid_e, double divisor,
+                                                        int use_div, unsigned long long* __restrict__ R,
+                                                        int32_t* __restrict__ flags) {
+    __shared__ unsigned long long rr[kST];
+    __shared__ int rf[kST];
+    const int e = grid_e[blockIdx.x];
+    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kSChunk;
+    unsigned long long acc = 0;
+    int flag = 0;
+    if (e != kSerial) {
+        for (int r = 0; r < kPerS; ++r) {
+            const uint64_t i = base + static_cast<uint64_t>(r) * kST + threadIdx.x;
+            if (i >= dim) continue;
+            double v = prob(s, i);
+            if (use_div) v = v / divisor;
+            const double x = ldexp(v, 52 - e);          // exact power-of-two scaling
+            // A chunk that stays in the binade has every term below 2^e, i.e.
+            // x < 2^52; anything larger (or NaN) sends the chunk to the serial
+            // path and keeps the 2048-term sum below 2^63.
+            if (!(x < 4503599627370496.0)) { flag |= 2; continue; }
+            const double fl = floor(x);
+            const double fr = x - fl;                   // exact (Sterbenz)
+            acc += static_cast<unsigned long long>(fl) + (fr > 0.5 ? 1u : 0u);
+            if (fr == 0.5) flag |= 1;
+        }
+    }
+    rr[threadIdx.x] = acc;
+    rf[threadIdx.x] = flag;
+    __syncthreads();
+    for (int w = kST / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) { rr[threadIdx.x] += rr[threadIdx.x + w]; rf[threadIdx.x] |= rf[threadIdx.x + w]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        R[blockIdx.x] = rr[0];
+        flags[blockIdx.x] = e == kSerial ? 4 : rf[0];
+    }
+}
+
+template <typename A>
+__global__ __launch_bounds__(kST) void search_draws(const A* __restrict__ s, uint64_t dim, double S,
+                                                    const double* __restrict__ starts,
+                                                    const double* __restrict__ ends, int64_t n_chunks,
+                                                    const double* __restrict__ u, int64_t n_draws,
+                                                    unsigned long long* __restrict__ out) {
+    const int64_t d = static_cast<int64_t>(blockIdx.x) * kST + threadIdx.x;
+    if (d >= n_draws) return;
+    const double x = u[d];
+    // lower_bound over chunk ends (the last one is 1.0 > x).
+    int64_t lo = 0, count = n_chunks;
+    while (count > 0) {
+        const int64_t step = count / 2, mid = lo + step;
+        if (ends[mid] < x) { lo = mid + 1; count -= step + 1; }
+        else count = step;
+    }
+    if (lo >= n_chunks) lo = n_chunks - 1;
+    const uint64_t b = static_cast<uint64_t>(lo) * kSChunk;
+    const uint64_t e = b + kSChunk < dim ? b + kSChunk : dim;
+    double run = starts[lo];
+    uint64_t found = e - 1;
+    for (uint64_t i = b; i < e; ++i) {
+        const double q = prob(s, i) / S;
+        run = (i == 0) ? q : run + q;
+        if (i == dim - 1) run = 1.0;
+        if (!(run < x)) { found = i; break; }
+    }
+    out[d] = found;
+}
+
+template <typename A>
+__global__ void probs_kernel(const A* __restrict__ s, uint64_t dim, double* __restrict__ out) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < dim; i += stride)
+        out[i] = prob(s, i);
+}
+
+// ── host side ───────────────────────────────────────────────────────────────
+
+static int binade(double v) {      // e with v in [2^e, 2^(e+1)), v > 0 finite
+    int E;
+    std::frexp(v, &E);
+    return E - 1;
+}
+
+static double host_prob(const qhip_state* st, const std::vector<char>& raw, uint64_t j) {
+    if (st->precision == QHIP_C128) {
+        const double* a = reinterpret_cast<const double*>(raw.data()) + 2 * j;
+        const double xx = a[0] * a[0];
+        const double yy = a[1] * a[1];
+        return xx + yy;
+    }
+    const float* a = reinterpret_cast<const float*>(raw.data()) + 2 * j;
+    const double x = a[0], y = a[1];
+    const double xx = x * x;
+    const double yy = y * y;
+    return xx + yy;
+}
+
+// Serial recurrence over one chunk from `start` (exactly as libstdc++).
+static double serial_chunk(const qhip_state* st, int64_t c, double start, double divisor, bool use_div,
+                           bool first_is_init) {
+    const uint64_t b = static_cast<uint64_t>(c) * kSChunk;
+    const uint64_t e = std::min(b + kSChunk, st->dim());
+    std::vector<char> raw((e - b) * st->amp_bytes());
+    QHIP_CHECK(hipMemcpy(raw.data(), static_cast<const char*>(st->d) + b * st->amp_bytes(), raw.size(),
+                         hipMemcpyDeviceToHost));
+    double s = start;
+    for (uint64_t i = b; i < e; ++i) {
+        double v = host_prob(st, raw, i - b);
+        if (use_div) v = v / divisor;
+        s = (first_is_init && i == 0) ? v : s + v;
+    }
+    return s;
+}
+
+struct ChunkScan {
+    std::vector<double> starts, ends;
+    int64_t serial_chunks = 0;
+};
+
+template <typename A>
+static ChunkScan resolve(const qhip_state* st, const std::vector<double>& approx, double divisor, bool use_div,
+                         bool first_is_init) {
+    auto& C = ctx();
+    const int64_t nc = static_cast<int64_t>(approx.size());
+    // Guess each chunk's binade from approximate running sums.
+    std::vector<int32_t> grid(nc, kSerial);
+    double run = 0.0;
+    for (int64_t c = 0; c < nc; ++c) {
+        const double a = use_div ? approx[c] / divisor : approx[c];
+        const double lo = run, hi = run + a;
+        run = hi;
+        if (!(lo > 0.0) || !std::isfinite(hi)) continue;
+        const int e0 = binade(lo), e1 = binade(hi);
+        if (e0 != e1) continue;
+        const double edge_lo = std::ldexp(1.0, e0), edge_hi = std::ldexp(1.0, e0 + 1);
+        if ((lo - edge_lo) < 1e-6 * edge_lo || (edge_hi - hi) < 1e-6 * edge_hi) continue;
+        grid[c] = e0;
+    }
+    int32_t* d_grid = static_cast<int32_t*>(C.misc.ensure(nc * (sizeof(int32_t) * 2 + sizeof(uint64_t)) + 64));
+    unsigned long long* d_R = reinterpret_cast<unsigned long long*>(d_grid + 2 * nc + 2);
+    int32_t* d_flags = d_grid + nc;
+    QHIP_CHECK(hipMemcpyAsync(d_grid, grid.data(), nc * sizeof(int32_t), hipMemcpyHostToDevice, C.stream));
+    hipLaunchKernelGGL(chunk_increments<A>, dim3(static_cast<uint32_t>(nc)), dim3(kST), 0, C.stream,
+                       static_cast<const A*>(st->d), st->dim(), d_grid, divisor, use_div ? 1 : 0, d_R, d_flags);
+    QHIP_CHECK(hipGetLastError());
+    std::vector<unsigned long long> R(nc);
+    std::vector<int32_t> flags(nc);
+    QHIP_CHECK(hipMemcpyAsync(R.data(), d_R, nc * sizeof(uint64_t), hipMemcpyDeviceToHost, C.stream));
+    QHIP_CHECK(hipMemcpyAsync(flags.data(), d_flags, nc * sizeof(int32_t), hipMemcpyDeviceToHost, C.stream));
+    QHIP_CHECK(hipStreamSynchronize(C.stream));
+
+    ChunkScan out;
+    out.starts.resize(nc);
+    out.ends.resize(nc);
+    double s = 0.0;
+    for (int64_t c = 0; c < nc; ++c) {
+        out.starts[c] = s;
+        bool fast = flags[c] == 0 && grid[c] != kSerial && s > 0.0 && !(first_is_init && c == 0);
+        if (fast) fast = binade(s) == grid[c];
+        if (fast) {
+            const double ulp = std::ldexp(1.0, grid[c] - 52);
+            const unsigned long long K = static_cast<unsigned long long>(s / ulp);   // exact: s is on the grid
+            const unsigned long long Kend = K + R[c];
+            fast = Kend < (1ULL << 53);
+            if (fast) s = static_cast<double>(Kend) * ulp;
+        }
+        if (!fast) {
+            s = serial_chunk(st, c, s, divisor, use_div, first_is_init);
+            ++out.serial_chunks;
+        }
+        out.ends[c] = s;
+    }
+    return out;
+}
+
+template <typename A>
+static void sample_impl(const qhip_state* st, const double* uniforms, int64_t n, uint64_t* outcomes) {
+    auto& C = ctx();
+    const uint64_t dim = st->dim();
+    if (dim < 2) {     // discrete_distribution with < 2 weights always yields 0
+        for (int64_t i = 0; i < n; ++i) outcomes[i] = 0;
+        return;
+    }
+    const int64_t nc = static_cast<int64_t>((dim + kSChunk - 1) / kSChunk);
+    double* d_approx = static_cast<double*>(C.red.ensure(sizeof(double) * nc));
+    hipLaunchKernelGGL(chunk_sums<A>, dim3(static_cast<uint32_t>(nc)), dim3(kST), 0, C.stream,
+                       static_cast<const A*>(st->d), dim, d_approx);
+    QHIP_CHECK(hipGetLastError());
+    std::vector<double> approx(nc);
+    QHIP_CHECK(hipMemcpyAsync(approx.data(), d_approx, sizeof(double) * nc, hipMemcpyDeviceToHost, C.stream));
+    QHIP_CHECK(hipStreamSynchronize(C.stream));
+
+    // S = accumulate(p, 0.0): the same recurrence started at 0.0.
+    const ChunkScan total = resolve<A>(st, approx, 1.0, false, false);
+    const double S = total.ends.back();
+    if (!(S > 0.0) || !std::isfinite(S))
+        fail(QHIP_INVALID, "sample: the state has zero (or non-finite) total probability");
+    // cp = partial_sum(p / S), last entry 1.0.
+    ChunkScan cdf = resolve<A>(st, approx, S, true, true);
+    cdf.ends.back() = 1.0;
+    if (n == 0) return;
+
+    char* base = static_cast<char*>(C.misc2.ensure(sizeof(double) * (2 * nc + n) + sizeof(uint64_t) * n + 64));
+    double* d_starts = reinterpret_cast<double*>(base);
+    double* d_ends = d_starts + nc;
+    double* d_u = d_ends + nc;
+    unsigned long long* d_out = reinterpret_cast<unsigned long long*>(d_u + n);
+    QHIP_CHECK(hipMemcpyAsync(d_starts, cdf.starts.data(), sizeof(double) * nc, hipMemcpyHostToDevice, C.stream));
+    QHIP_CHECK(hipMemcpyAsync(d_ends, cdf.ends.data(), sizeof(double) * nc, hipMemcpyHostToDevice, C.stream));
+    QHIP_CHECK(hipMemcpyAsync(d_u, uniforms, sizeof(double) * n, hipMemcpyHostToDevice, C.stream));
+    const uint32_t blocks = static_cast<uint32_t>((n + kST - 1) / kST);
+    hipLaunchKernelGGL(search_draws<A>, dim3(blocks), dim3(kST), 0, C.stream, static_cast<const A*>(st->d), dim, S,
+                       d_starts, d_ends, nc, d_u, n, d_out);
+    QHIP_CHECK(hipGetLastError());
+    QHIP_CHECK(hipMemcpyAsync(outcomes, d_out, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, C.stream));
+    QHIP_CHECK(hipStreamSynchronize(C.stream));
+}
+
+void sample(const qhip_state* s, const double* uniforms, int64_t n, uint64_t* out) {
+    if (s->precision == QHIP_C128) sample_impl<double2>(s, uniforms, n, out);
+    else sample_impl<float2>(s, uniforms, n, out);
+}
+
+void probabilities(const qhip_state* s, double* out) {
+    auto& C = ctx();
+    const uint64_t dim = s->dim();
+    double* d = static_cast<double*>(C.misc.ensure(sizeof(double) * dim));
+    const int blocks = static_cast<int>(std::min<uint64_t>((dim + kST - 1) / kST, 65535));
+    if (s->precision == QHIP_C128)
+        hipLaunchKernelGGL(probs_kernel<double2>, dim3(blocks), dim3(kST), 0, C.stream,
+                           static_cast<const double2*>(s->d), dim, d);
+    else
+        hipLaunchKernelGGL(probs_kernel<float2>, dim3(blocks), dim3(kST), 0, C.stream,
+                           static_cast<const float2*>(s->d), dim, d);
+    QHIP_CHECK(hipGetLastError());
+    QHIP_CHECK(hipMemcpyAsync(out, d, sizeof(double) * dim, hipMemcpyDeviceToHost, C.stream));
+    QHIP_CHECK(hipStreamSynchronize(C.stream));
+}
+
+// generate_canonical<double, 53> over std::mt19937: the toolchain's own
+// libstdc++ implementation, the one the reference calls.
+void mt_uniforms(uint32_t seed, int64_t n, double* u) {
+    std::mt19937 g(seed);
+    for (int64_t i = 0; i < n; ++i) u[i] = std::generate_canonical<double, 53>(g);
+}
+
+}  // namespace qhip

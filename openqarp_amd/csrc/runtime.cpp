@@ -1,0 +1,579 @@
+// runtime.cpp — device context, orchestration and the C entry points of
+// include/qarp_hip.h.
+#include "common.h"
+
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <mutex>
+
+namespace qhip {
+
+void mt_uniforms(uint32_t seed, int64_t n, double* u);   // kernels_sample.hip
+
+namespace {
+thread_local std::string t_err;
+thread_local int t_code = QHIP_OK;
+std::mutex g_lock;   // one call on the device at a time (shared stream and scratch)
+}  // namespace
+
+[[noreturn]] void fail(int code, const std::string& msg) { throw Error(code, msg); }
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) fail(QHIP_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void* DeviceBuffer::ensure(size_t n) {
+    if (n <= bytes && ptr) return ptr;
+    if (ptr) {
+        QHIP_CHECK(hipStreamSynchronize(ctx().stream));
+        QHIP_CHECK(hipFree(ptr));
+        ptr = nullptr;
+        bytes = 0;
+    }
+    const size_t want = std::max<size_t>(n, 1 << 20);
+    if (hipMalloc(&ptr, want) != hipSuccess) {
+        (void)hipGetLastError();
+        ptr = nullptr;
+        fail(QHIP_NOMEM, "device scratch allocation of " + std::to_string(want) + " bytes failed");
+    }
+    bytes = want;
+    return ptr;
+}
+
+DeviceBuffer::~DeviceBuffer() {
+    // Process teardown: the runtime may already be gone; leak rather than fault.
+}
+
+Context& ctx() {
+    static Context* c = nullptr;
+    if (!c) {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+            (void)hipGetLastError();
+            fail(QHIP_DEVICE, "no HIP device visible");
+        }
+        auto* nc = new Context();
+        QHIP_CHECK(hipGetDevice(&nc->device));
+        QHIP_CHECK(hipStreamCreateWithFlags(&nc->stream, hipStreamNonBlocking));
+        hipDeviceProp_t prop;
+        QHIP_CHECK(hipGetDeviceProperties(&prop, nc->device));
+        nc->n_cu = prop.multiProcessorCount;
+        c = nc;
+    }
+    return *c;
+}
+
+// ── validation ──────────────────────────────────────────────────────────────
+
+static bool needs_trajectory(const qhip_op& op) {
+    return op.gate == QHIP_RESET || (op.gate == QHIP_MEASURE && op.cbit >= 0) ||
+           (op.flags & QHIP_OP_CONDITIONAL) || op.gate == QHIP_BRANCH_BEGIN;
+}
+
+static void validate(const Stream& st, int n) {
+    for (int64_t i = 0; i < st.size(); ++i) {
+        const qhip_op& op = st.op(i);
+        if (op.gate < 0 || op.gate >= QHIP_NUM_GATES)
+            fail(QHIP_INVALID, "unknown gate id " + std::to_string(op.gate));
+        if (op.n_params < 0 || op.n_params > QHIP_MAX_PARAMS)
+            fail(QHIP_INVALID, "bad parameter count " + std::to_string(op.n_params));
+        uint64_t seen = 0;
+        for (int b = 0; b < op.n_qubits; ++b) {
+            const uint32_t q = st.qubits(i)[b];
+            if (q >= static_cast<uint32_t>(n))
+                fail(QHIP_INVALID, "QarpSimulator: qubit " + std::to_string(q) + " out of range for a " +
+                                       std::to_string(n) + "-qubit register");
+            if ((seen >> q) & 1) fail(QHIP_INVALID, "QarpSimulator: repeated qubit " + std::to_string(q));
+            seen |= uint64_t{1} << q;
+        }
+    }
+}
+
+// measurements_are_terminal (qarp_simulator.cpp:226-252)
+static bool terminal_only(const Stream& st) {
+    uint64_t measured = 0;
+    for (int64_t i = 0; i < st.size(); ++i) {
+        const qhip_op& op = st.op(i);
+        if (op.gate == QHIP_RESET || (op.flags & QHIP_OP_CONDITIONAL)) return false;
+        if (op.gate == QHIP_BRANCH_BEGIN || op.gate == QHIP_BRANCH_ELSE || op.gate == QHIP_BRANCH_END) return false;
+        if (op.gate == QHIP_BARRIER) continue;
+        if (op.gate == QHIP_MEASURE) {
+            if (op.cbit >= 0) {
+                if (op.n_qubits == 0 || st.qubits(i)[0] >= 64) return false;
+                measured |= uint64_t{1} << st.qubits(i)[0];
+            }
+            continue;
+        }
+        for (int b = 0; b < op.n_qubits; ++b) {
+            const uint32_t q = st.qubits(i)[b];
+            if (q >= 64 || ((measured >> q) & 1)) return false;
+        }
+    }
+    return true;
+}
+
+static ScheduleOptions options(const qhip_config* cfg, bool two) {
+    ScheduleOptions o;
+    o.two_states = two;
+    if (cfg) {
+        o.tile_qubits = cfg->tile_qubits;
+        o.low_qubits = cfg->low_qubits;
+        o.max_ops = cfg->max_ops_per_pass;
+    }
+    return o;
+}
+
+void apply_stream(const qhip_config* cfg, const qhip_stream* stream, qhip_state* s, qhip_apply_stats* stats,
+                  bool terminal) {
+    const Stream st{stream};
+    const int n = s->n;
+    validate(st, n);
+    if (!terminal) {
+        for (int64_t i = 0; i < st.size(); ++i)
+            if (needs_trajectory(st.op(i)))
+                fail(QHIP_RUNTIME,
+                     "QarpSimulator::statevector: circuit contains mid-circuit measurement, Reset, or classical "
+                     "condition — statevector is not deterministic. Use run() for trajectory simulation.");
+    } else if (!terminal_only(st)) {
+        fail(QHIP_CAPABILITY,
+             "QarpSimulator::run: mid-circuit measurement, Reset, classical conditions and branches need the "
+             "per-shot trajectory path, which this backend does not provide");
+    }
+    const int kmax = cfg ? cfg->fusion_max_qubits : 3;
+    const int kmin = cfg ? cfg->fusion_min_qubits : 12;
+    const auto fused = fuse_for_dispatch(st, n, kmax, kmin);
+    std::vector<HostOp> ops;
+    ops.reserve(fused.size());
+    for (const auto& it : fused) lower_item(st, it, n, ops, terminal);
+    const Plan plan = schedule(ops, n, s->precision, options(cfg, false));
+    launch_plan(plan, s, nullptr, nullptr, 0);
+    if (stats) {
+        stats->fused_ops = static_cast<int64_t>(fused.size());
+        stats->passes = static_cast<int64_t>(plan.passes.size());
+        stats->diagonal_ops = plan.diag_ops;
+        stats->mfma_ops = 0;
+        stats->flops = plan.flops;
+    }
+}
+
+struct StateHolder {
+    qhip_state st;
+    explicit StateHolder(const qhip_state* like) {
+        st.n = like->n;
+        st.precision = like->precision;
+        st.owned = true;
+        if (hipMalloc(&st.d, st.dim() * st.amp_bytes()) != hipSuccess) {
+            (void)hipGetLastError();
+            st.d = nullptr;
+            fail(QHIP_NOMEM, "adjoint: allocation of the phi state failed");
+        }
+    }
+    ~StateHolder() {
+        if (st.d) {
+            (void)hipStreamSynchronize(ctx().stream);
+            (void)hipFree(st.d);
+        }
+    }
+};
+
+void run_gradient(const qhip_config* cfg, const qhip_stream* stream, int n_params, qhip_state* psi,
+                  const qhip_observable* obs, const qhip_state* phi_in, double* grad, double* energy) {
+    const Stream st{stream};
+    const int n = psi->n;
+    validate(st, n);
+    for (int64_t i = 0; i < st.size(); ++i)
+        if (needs_trajectory(st.op(i)))
+            fail(QHIP_RUNTIME, "QarpSimulator::run_gradient: trajectory commands (Measure / Reset / classical "
+                               "condition) are not differentiable.");
+    // Backward op list (reverse order): generator inner product, then dagger.
+    struct SlotInfo { cplx c; int64_t op; };
+    std::vector<SlotInfo> slots;
+    std::vector<HostOp> back;
+    for (int64_t i = st.size() - 1; i >= 0; --i) {
+        const qhip_op& op = st.op(i);
+        if (op.n_deps > 0) {
+            HostOp G;
+            cplx c;
+            if (!generator(st, i, G, c))
+                fail(QHIP_CAPABILITY, "run_gradient: gate id " + std::to_string(op.gate) +
+                                          " carries a symbolic parameter but has no adjoint generator rule "
+                                          "(Rx/Ry/Rz/P/GPhase/CRx/CRy/CRz/CP/RXX/RYY/RZZ); rebase it to those "
+                                          "gates before differentiating.");
+            G.slot = static_cast<int32_t>(slots.size());
+            slots.push_back({c, i});
+            back.push_back(std::move(G));
+        }
+        lower_dagger(st, i, back);
+    }
+    // Forward sweep: psi <- U psi (fused; the gradient sweep itself is unfused).
+    apply_stream(cfg, stream, psi, nullptr, false);
+    StateHolder phi(psi);
+    if (obs) {
+        apply_observable(psi, obs, &phi.st);
+        if (energy) *energy = inner(psi, &phi.st).real();
+    } else {
+        QHIP_CHECK(hipMemcpyAsync(phi.st.d, phi_in->d, psi->dim() * psi->amp_bytes(), hipMemcpyDeviceToDevice,
+                                  ctx().stream));
+        if (energy) *energy = 0.0;
+    }
+    const Plan plan = schedule(back, n, psi->precision, options(cfg, true));
+    auto& C = ctx();
+    std::vector<cplx> ip(slots.size());
+    if (!slots.empty()) {
+        double2* d_tot = static_cast<double2*>(C.misc2.ensure(sizeof(double2) * slots.size()));
+        launch_plan(plan, psi, &phi.st, d_tot, static_cast<int>(slots.size()));
+        QHIP_CHECK(hipMemcpyAsync(ip.data(), d_tot, sizeof(double2) * slots.size(), hipMemcpyDeviceToHost,
+                                  C.stream));
+        QHIP_CHECK(hipStreamSynchronize(C.stream));
+    } else {
+        launch_plan(plan, psi, &phi.st, nullptr, 0);
+    }
+    for (int p = 0; p < n_params; ++p) grad[p] = 0.0;
+    for (size_t s = 0; s < slots.size(); ++s) {
+        const qhip_op& op = st.op(slots[s].op);
+        const double base = 2.0 * (slots[s].c * ip[s]).real();
+        for (int d = 0; d < op.n_deps; ++d) {
+            const int p = stream->dep_index[op.dep_offset + d];
+            if (p >= 0 && p < n_params) grad[p] += stream->dep_alpha[op.dep_offset + d] * base;
+        }
+    }
+    // psi is back at the root state after the reverse sweep; leave U|root>
+    // as documented: re-run the forward sweep only if the caller needs it —
+    // the energy is already computed, so the root state is returned.
+}
+
+}  // namespace qhip
+
+// ── C ABI ───────────────────────────────────────────────────────────────────
+
+using qhip::Error;
+
+template <typename F>
+static int guarded(F&& f) {
+    try {
+        std::lock_guard<std::mutex> lk(qhip::g_lock);
+        f();
+        qhip::t_code = QHIP_OK;
+        return QHIP_OK;
+    } catch (const Error& e) {
+        qhip::t_err = e.what();
+        qhip::t_code = e.code;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        qhip::t_err = "host allocation failed";
+        qhip::t_code = QHIP_NOMEM;
+        return QHIP_NOMEM;
+    } catch (const std::exception& e) {
+        qhip::t_err = e.what();
+        qhip::t_code = QHIP_RUNTIME;
+        return QHIP_RUNTIME;
+    }
+}
+
+static void need(bool cond, const char* msg) {
+    if (!cond) qhip::fail(QHIP_INVALID, msg);
+}
+
+struct qhip_fused {
+    std::vector<qhip::FusedItem> items;
+    std::vector<qhip_op> src_ops;
+    std::vector<std::vector<uint32_t>> src_qubits;
+    std::vector<std::vector<qhip::cplx>> src_mats;   // Custom ops re-emitted verbatim
+};
+
+extern "C" {
+
+int qhip_last_error(char* buf, size_t len) {
+    if (buf && len) {
+        const size_t n = std::min(len - 1, qhip::t_err.size());
+        std::memcpy(buf, qhip::t_err.data(), n);
+        buf[n] = '\0';
+    }
+    return qhip::t_code;
+}
+
+int qhip_device_count(int32_t* out) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        n = 0;
+    }
+    *out = n;
+    return QHIP_OK;
+}
+
+int qhip_synchronize(void) {
+    return guarded([&] { QHIP_CHECK(hipStreamSynchronize(qhip::ctx().stream)); });
+}
+
+int qhip_state_create(int32_t n_qubits, int32_t precision, qhip_state** out) {
+    return guarded([&] {
+        need(out != nullptr, "qhip_state_create: out is NULL");
+        need(n_qubits >= 0 && n_qubits <= 40, "qhip_state_create: n_qubits must be in [0, 40]");
+        need(precision == QHIP_C128 || precision == QHIP_C64, "qhip_state_create: bad precision");
+        qhip::ctx();
+        auto s = std::make_unique<qhip_state>();
+        s->n = n_qubits;
+        s->precision = precision;
+        const size_t bytes = s->dim() * s->amp_bytes();
+        if (hipMalloc(&s->d, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            qhip::fail(QHIP_NOMEM, "qhip_state_create: device allocation of " + std::to_string(bytes) +
+                                       " bytes failed");
+        }
+        qhip::state_reset(s.get());
+        QHIP_CHECK(hipStreamSynchronize(qhip::ctx().stream));
+        *out = s.release();
+    });
+}
+
+int qhip_state_destroy(qhip_state* s) {
+    return guarded([&] {
+        if (!s) return;
+        QHIP_CHECK(hipStreamSynchronize(qhip::ctx().stream));
+        if (s->owned && s->d) QHIP_CHECK(hipFree(s->d));
+        delete s;
+    });
+}
+
+int qhip_state_info(const qhip_state* s, int32_t* n_qubits, int32_t* precision) {
+    return guarded([&] {
+        need(s != nullptr, "qhip_state_info: NULL state");
+        if (n_qubits) *n_qubits = s->n;
+        if (precision) *precision = s->precision;
+    });
+}
+
+int qhip_state_reset(qhip_state* s) {
+    return guarded([&] {
+        need(s != nullptr, "NULL state");
+        qhip::state_reset(s);
+        QHIP_CHECK(hipStreamSynchronize(qhip::ctx().stream));
+    });
+}
+
+int qhip_state_upload(qhip_state* s, const void* host, uint64_t n_amps) {
+    return guarded([&] {
+        need(s && host, "qhip_state_upload: NULL argument");
+        need(n_amps == s->dim(), "qhip_state_upload: length does not match 2^n_qubits");
+        QHIP_CHECK(hipMemcpyAsync(s->d, host, n_amps * s->amp_bytes(), hipMemcpyHostToDevice, qhip::ctx().stream));
+        QHIP_CHECK(hipStreamSynchronize(qhip::ctx().stream));
+    });
+}
+
+int qhip_state_download(const qhip_state* s, void* host, uint64_t n_amps) {
+    return guarded([&] {
+        need(s && host, "qhip_state_download: NULL argument");
+        need(n_amps == s->dim(), "qhip_state_download: length does not match 2^n_qubits");
+        QHIP_CHECK(hipMemcpyAsync(host, s->d, n_amps * s->amp_bytes(), hipMemcpyDeviceToHost, qhip::ctx().stream));
+        QHIP_CHECK(hipStreamSynchronize(qhip::ctx().stream));
+    });
+}
+
+int qhip_state_upload_c128(qhip_state* s, const double* host, uint64_t n_amps) {
+    return guarded([&] {
+        need(s && host, "qhip_state_upload_c128: NULL argument");
+        need(n_amps == s->dim(), "qhip_state_upload_c128: length does not match 2^n_qubits");
+        qhip::state_convert_upload(s, host, n_amps);
+    });
+}
+
+int qhip_state_download_c128(const qhip_state* s, double* host, uint64_t n_amps) {
+    return guarded([&] {
+        need(s && host, "qhip_state_download_c128: NULL argument");
+        need(n_amps == s->dim(), "qhip_state_download_c128: length does not match 2^n_qubits");
+        qhip::state_convert_download(s, host, n_amps);
+    });
+}
+
+int qhip_state_copy(qhip_state* dst, const qhip_state* src) {
+    return guarded([&] {
+        need(dst && src, "qhip_state_copy: NULL argument");
+        need(dst->n == src->n && dst->precision == src->precision, "qhip_state_copy: shape mismatch");
+        QHIP_CHECK(hipMemcpyAsync(dst->d, src->d, src->dim() * src->amp_bytes(), hipMemcpyDeviceToDevice,
+                                  qhip::ctx().stream));
+        QHIP_CHECK(hipStreamSynchronize(qhip::ctx().stream));
+    });
+}
+
+int qhip_state_norm2(const qhip_state* s, double* out) {
+    return guarded([&] {
+        need(s && out, "qhip_state_norm2: NULL argument");
+        *out = qhip::state_norm2(s);
+    });
+}
+
+int qhip_state_device_ptr(const qhip_state* s, void** out) {
+    return guarded([&] {
+        need(s && out, "qhip_state_device_ptr: NULL argument");
+        *out = s->d;
+    });
+}
+
+int qhip_fuse(const qhip_config* cfg, const qhip_stream* in, int32_t n_qubits, qhip_fused** out) {
+    return guarded([&] {
+        need(cfg && in && out, "qhip_fuse: NULL argument");
+        const qhip::Stream st{in};
+        qhip::validate(st, n_qubits);
+        auto f = std::make_unique<qhip_fused>();
+        f->items = qhip::fuse_for_dispatch(st, n_qubits, cfg->fusion_max_qubits, cfg->fusion_min_qubits);
+        for (const auto& it : f->items) {
+            if (it.src >= 0) {
+                const qhip_op& op = st.op(it.src);
+                f->src_ops.push_back(op);
+                f->src_qubits.emplace_back(st.qubits(it.src), st.qubits(it.src) + op.n_qubits);
+                std::vector<qhip::cplx> m;
+                if (op.gate == QHIP_CUSTOM && op.unitary_offset >= 0 && op.n_qubits <= QHIP_MAX_FUSION_QUBITS) {
+                    const size_t D = size_t{1} << op.n_qubits;
+                    m.resize(D * D);
+                    std::memcpy(m.data(), in->unitaries + 2 * op.unitary_offset, sizeof(qhip::cplx) * D * D);
+                }
+                f->src_mats.push_back(std::move(m));
+            } else {
+                f->src_ops.push_back(qhip_op{});
+                f->src_qubits.emplace_back();
+                f->src_mats.emplace_back();
+            }
+        }
+        *out = f.release();
+    });
+}
+
+int qhip_fused_count(const qhip_fused* f, int64_t* n_ops) {
+    return guarded([&] {
+        need(f && n_ops, "qhip_fused_count: NULL argument");
+        *n_ops = static_cast<int64_t>(f->items.size());
+    });
+}
+
+int qhip_fused_get(const qhip_fused* f, int64_t i, int32_t* gate, int32_t* n_qubits, uint32_t* qubits,
+                   double* params, double* matrix, int64_t capacity) {
+    return guarded([&] {
+        need(f && gate && n_qubits && qubits && params, "qhip_fused_get: NULL argument");
+        need(i >= 0 && i < static_cast<int64_t>(f->items.size()), "qhip_fused_get: index out of range");
+        const auto& it = f->items[i];
+        if (it.src >= 0) {
+            const qhip_op& op = f->src_ops[i];
+            *gate = op.gate;
+            *n_qubits = op.n_qubits;
+            for (int b = 0; b < op.n_qubits && b < 64; ++b) qubits[b] = f->src_qubits[i][b];
+            for (int p = 0; p < 4; ++p) params[p] = p < op.n_params ? op.params[p] : 0.0;
+            const auto& m = f->src_mats[i];
+            if (matrix && !m.empty()) {
+                need(capacity >= static_cast<int64_t>(m.size()), "qhip_fused_get: matrix capacity too small");
+                std::memcpy(matrix, m.data(), sizeof(qhip::cplx) * m.size());
+            }
+        } else {
+            *gate = QHIP_CUSTOM;
+            *n_qubits = it.k;
+            for (int b = 0; b < it.k; ++b) qubits[b] = it.q[b];
+            for (int p = 0; p < 4; ++p) params[p] = 0.0;
+            const int64_t D = int64_t{1} << it.k;
+            if (matrix) {
+                need(capacity >= D * D, "qhip_fused_get: matrix capacity too small");
+                std::memcpy(matrix, it.m.data(), sizeof(qhip::cplx) * D * D);
+            }
+        }
+    });
+}
+
+int qhip_fused_destroy(qhip_fused* f) {
+    return guarded([&] { delete f; });
+}
+
+int qhip_apply(const qhip_config* cfg, const qhip_stream* stream, qhip_state* s, qhip_apply_stats* stats) {
+    return guarded([&] {
+        need(stream && s, "qhip_apply: NULL argument");
+        qhip::apply_stream(cfg, stream, s, stats, false);
+    });
+}
+
+int qhip_apply_terminal(const qhip_config* cfg, const qhip_stream* stream, qhip_state* s,
+                        qhip_apply_stats* stats) {
+    return guarded([&] {
+        need(stream && s, "qhip_apply_terminal: NULL argument");
+        qhip::apply_stream(cfg, stream, s, stats, true);
+    });
+}
+
+int qhip_transition(const qhip_state* bra, const qhip_state* ket, const qhip_observable* obs, double out[2]) {
+    return guarded([&] {
+        need(bra && ket && obs && out, "qhip_transition: NULL argument");
+        if (ket->n < 0 || ket->n > 62)
+            qhip::fail(QHIP_INVALID, "pauli_transition: n_qubits must be in [0, 62], got " + std::to_string(ket->n));
+        need(bra->n == ket->n && bra->precision == ket->precision,
+             "QarpSimulator::transition: bra/ket lengths do not match");
+        const qhip::cplx v = qhip::transition(bra, ket, obs);
+        out[0] = v.real();
+        out[1] = v.imag();
+    });
+}
+
+int qhip_apply_observable(const qhip_state* in, const qhip_observable* obs, qhip_state* out) {
+    return guarded([&] {
+        need(in && obs && out, "qhip_apply_observable: NULL argument");
+        need(in->n == out->n && in->precision == out->precision && in != out,
+             "qhip_apply_observable: in/out must be distinct states of one shape");
+        qhip::apply_observable(in, obs, out);
+    });
+}
+
+int qhip_sample(const qhip_state* s, int64_t n_shots, uint32_t seed, uint64_t* outcomes) {
+    return guarded([&] {
+        need(s && (outcomes || n_shots == 0), "qhip_sample: NULL argument");
+        need(n_shots >= 0, "qhip_sample: n_shots must be >= 0");
+        std::vector<double> u(static_cast<size_t>(n_shots));
+        qhip::mt_uniforms(seed, n_shots, u.data());
+        qhip::sample(s, u.data(), n_shots, outcomes);
+    });
+}
+
+int qhip_sample_uniforms(const qhip_state* s, const double* uniforms, int64_t n, uint64_t* outcomes) {
+    return guarded([&] {
+        need(s && (n == 0 || (uniforms && outcomes)), "qhip_sample_uniforms: NULL argument");
+        need(n >= 0, "qhip_sample_uniforms: n must be >= 0");
+        for (int64_t i = 0; i < n; ++i)
+            need(uniforms[i] >= 0.0 && uniforms[i] < 1.0, "qhip_sample_uniforms: uniforms must lie in [0, 1)");
+        qhip::sample(s, uniforms, n, outcomes);
+    });
+}
+
+int qhip_probabilities(const qhip_state* s, double* out, uint64_t n) {
+    return guarded([&] {
+        need(s && out, "qhip_probabilities: NULL argument");
+        need(n == s->dim(), "qhip_probabilities: length does not match 2^n_qubits");
+        qhip::probabilities(s, out);
+    });
+}
+
+int qhip_run_gradient(const qhip_config* cfg, const qhip_stream* stream, int32_t n_params, qhip_state* psi,
+                      const qhip_observable* obs, double* grad, double* energy) {
+    return guarded([&] {
+        need(stream && psi && obs && (grad || n_params == 0), "qhip_run_gradient: NULL argument");
+        need(n_params >= 0, "qhip_run_gradient: n_params must be >= 0");
+        qhip::run_gradient(cfg, stream, n_params, psi, obs, nullptr, grad, energy);
+    });
+}
+
+int qhip_run_gradient_phi(const qhip_config* cfg, const qhip_stream* stream, int32_t n_params, qhip_state* psi,
+                          const qhip_state* phi, double* grad) {
+    return guarded([&] {
+        need(stream && psi && phi && (grad || n_params == 0), "qhip_run_gradient_phi: NULL argument");
+        need(phi->n == psi->n && phi->precision == psi->precision, "run_gradient: |phi> size mismatch");
+        qhip::run_gradient(cfg, stream, n_params, psi, nullptr, phi, grad, nullptr);
+    });
+}
+
+int qhip_batch_energy_grad(const qhip_config* cfg, int32_t n_instances, const int32_t* n_qubits,
+                           const qhip_stream* streams, const qhip_observable* observables, const int32_t* n_params,
+                           int32_t precision, double* energies, double* grads, const int64_t* grad_offsets) {
+    return guarded([&] {
+        need(n_instances >= 0 && n_qubits && streams && observables && energies, "qhip_batch_energy_grad: NULL argument");
+        need(!grads || (grad_offsets && n_params), "qhip_batch_energy_grad: grads need offsets and counts");
+        qhip::batch_energy_grad(cfg, n_instances, n_qubits, streams, observables, n_params, precision, energies,
+                                grads, grad_offsets);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,178 @@
+// schedule.cpp — packs an op sequence into tile passes.
+//
+// A pass streams the whole register through the chip once: every workgroup
+// loads one tile of 2^T amplitudes (the tile qubits are the `low` contiguous
+// qubits, for coalesced 16-B loads, plus up to T-low other qubits), applies a
+// run of ops in LDS, and writes the tile back.  A dense op can join a pass
+// only if all its operands are tile qubits; diagonal / mask-phase / scalar
+// ops know each amplitude's full index and join any pass.  Ops are taken in
+// program order; an op that cannot join blocks its qubits for the rest of the
+// pass, so every later op touching them waits for the next pass (ops on
+// disjoint qubits commute, which is what lets them overtake).
+#include "common.h"
+
+#include <algorithm>
+#include <bit>
+
+namespace qhip {
+
+int default_tile_qubits(int precision, bool two_states) {
+    // 64 KiB of LDS per workgroup: two workgroups share a CU's 160 KiB.
+    const int amp_log = precision == QHIP_C128 ? 4 : 3;   // log2 bytes per amplitude
+    return 16 - amp_log - (two_states ? 1 : 0);
+}
+
+static double op_flops(const HostOp& op) {
+    switch (op.kind) {
+        case OP_DENSE: return 8.0 * static_cast<double>(1 << op.k);
+        case OP_IP_DENSE: return 8.0 * static_cast<double>(1 << op.k) + 8.0;
+        case OP_DIAG: case OP_MASKPHASE: case OP_SCALAR: return 6.0;
+        case OP_IP_DIAG: return 14.0;
+        default: return 0.0;
+    }
+}
+
+static void emit_op(Plan& plan, const HostOp& op, const int* local_of, int n) {
+    DevOp d{};
+    d.kind = op.kind;
+    d.k = op.k;
+    d.coef = static_cast<int64_t>(plan.coefs.size());
+    d.lmask = d.gmask = 0;
+    switch (op.kind) {
+        case OP_DENSE: case OP_IP_DENSE: {
+            int sorted[8];
+            for (int b = 0; b < op.k; ++b) sorted[b] = d.pos[b] = static_cast<int8_t>(local_of[op.q[b]]);
+            std::sort(sorted, sorted + op.k);
+            for (int b = 0; b < op.k; ++b) d.spos[b] = static_cast<int8_t>(sorted[b]);
+            plan.coefs.insert(plan.coefs.end(), op.m.begin(), op.m.end());
+            ++plan.dense_ops;
+            break;
+        }
+        case OP_DIAG: case OP_IP_DIAG:
+            for (int b = 0; b < op.k; ++b)
+                d.pos[b] = static_cast<int8_t>(local_of[op.q[b]] >= 0 ? local_of[op.q[b]]
+                                                                       : 32 + static_cast<int>(op.q[b]));
+            plan.coefs.insert(plan.coefs.end(), op.m.begin(), op.m.end());
+            ++plan.diag_ops;
+            break;
+        case OP_MASKPHASE:
+            for (int q = 0; q < n; ++q)
+                if ((op.mask >> q) & 1) {
+                    if (local_of[q] >= 0) d.lmask |= uint64_t{1} << local_of[q];
+                    else d.gmask |= uint64_t{1} << q;
+                }
+            plan.coefs.push_back(op.phase);
+            ++plan.diag_ops;
+            break;
+        case OP_SCALAR:
+            plan.coefs.push_back(op.phase);
+            ++plan.diag_ops;
+            break;
+    }
+    if (op.kind == OP_IP_DENSE || op.kind == OP_IP_DIAG) d.lmask = static_cast<uint64_t>(op.slot);
+    plan.ops.push_back(d);
+    plan.flops += op_flops(op) * static_cast<double>(uint64_t{1} << n);
+}
+
+// Dense ops wider than the tile kernel handles (k > 5) run as passes of
+// their own, straight from HBM.
+constexpr int kMaxTileK = 5;
+
+static void schedule_segment(Plan& plan, const std::vector<HostOp>& ops, const std::vector<size_t>& seg, int n,
+                             int T, int low, const ScheduleOptions& opt) {
+    const uint64_t low_mask = low >= 64 ? ~uint64_t{0} : ((uint64_t{1} << low) - 1);
+    const uint64_t all_mask = n >= 64 ? ~uint64_t{0} : ((uint64_t{1} << n) - 1);
+    const int max_ip = 64;
+    std::vector<size_t> remaining = seg;
+    while (!remaining.empty()) {
+        uint64_t Q = low_mask;
+        uint64_t blocked = 0;
+        std::vector<size_t> chosen, deferred;
+        double flops = 0.0;
+        int n_ip = 0, n_dense = 0;
+        for (size_t idx : remaining) {
+            const HostOp& op = ops[idx];
+            const uint64_t qm = op.qmask();
+            if (qm & blocked) { blocked |= qm; deferred.push_back(idx); continue; }
+            const bool is_ip = op.kind == OP_IP_DENSE || op.kind == OP_IP_DIAG;
+            bool fits = true;
+            if (is_ip && n_ip >= max_ip) fits = false;
+            if (opt.max_ops > 0 && n_dense >= opt.max_ops && op.needs_local()) fits = false;
+            if (opt.max_flops_per_amp > 0 && !chosen.empty() &&
+                flops + op_flops(op) > opt.max_flops_per_amp)
+                fits = false;
+            uint64_t need = 0;
+            if (fits && op.needs_local()) {
+                need = qm & ~Q;
+                if (std::popcount(Q | need) > T) fits = false;
+            }
+            if (!fits) { blocked |= qm; deferred.push_back(idx); continue; }
+            Q |= need;
+            chosen.push_back(idx);
+            flops += op_flops(op);
+            n_ip += is_ip;
+            n_dense += op.needs_local();
+        }
+        // Fill the tile up to T qubits with the lowest unused ones.
+        for (int q = 0; q < n && std::popcount(Q) < T; ++q) Q |= uint64_t{1} << q;
+        Q &= all_mask;
+
+        PassDesc P{};
+        P.n = n;
+        P.T = T;
+        P.tmask = Q;
+        int local_of[64];
+        for (int q = 0; q < 64; ++q) local_of[q] = -1;
+        int l = 0;
+        for (int q = 0; q < n; ++q)
+            if ((Q >> q) & 1) {
+                P.qpos[l] = q;
+                local_of[q] = l;
+                if (l >= 8) P.hmask |= uint64_t{1} << q;
+                ++l;
+            }
+        P.op_begin = static_cast<int32_t>(plan.ops.size());
+        P.kmax = 0;
+        for (size_t idx : chosen) {
+            emit_op(plan, ops[idx], local_of, n);
+            if (ops[idx].needs_local()) P.kmax = std::max(P.kmax, ops[idx].k);
+        }
+        P.op_end = static_cast<int32_t>(plan.ops.size());
+        plan.passes.push_back(P);
+        remaining.swap(deferred);
+    }
+}
+
+Plan schedule(const std::vector<HostOp>& ops, int n, int precision, const ScheduleOptions& opt) {
+    Plan plan;
+    int T = opt.tile_qubits > 0 ? opt.tile_qubits : default_tile_qubits(precision, opt.two_states);
+    T = std::max(T, std::min(kMaxTileK, n));
+    T = std::min(T, n);
+    int low = opt.low_qubits > 0 ? opt.low_qubits : 4;
+    low = std::min(low, T);
+    std::vector<size_t> seg;
+    for (size_t i = 0; i < ops.size(); ++i) {
+        const HostOp& op = ops[i];
+        if (op.needs_local() && op.k > kMaxTileK) {
+            schedule_segment(plan, ops, seg, n, T, low, opt);
+            seg.clear();
+            if (op.kind != OP_DENSE) fail(QHIP_RUNTIME, "inner-product op wider than 5 qubits");
+            PassDesc P{};
+            P.n = n;
+            P.T = 0;   // wide pass
+            P.kmax = op.k;
+            int local_of[64];
+            for (int q = 0; q < 64; ++q) local_of[q] = q;   // positions are global qubits
+            P.op_begin = static_cast<int32_t>(plan.ops.size());
+            emit_op(plan, op, local_of, n);
+            P.op_end = static_cast<int32_t>(plan.ops.size());
+            plan.passes.push_back(P);
+            continue;
+        }
+        seg.push_back(i);
+    }
+    schedule_segment(plan, ops, seg, n, T, low, opt);
+    return plan;
+}
+
+}  // namespace qhip

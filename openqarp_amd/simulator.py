@@ -1,0 +1,351 @@
+"""``QarpSimulator`` for the MI355X backend.
+
+Same methods, arguments and error behaviour as the reference's
+``qx.QarpSimulator`` binding (src/qarpx/python/bindings.cpp:1337-1517;
+stub src/qarpx/python/qarpx.pyi:1594-1680), so ``QarpEngine``-style callers
+can hold one in place of the CPU simulator.  Every method runs on the GPU
+through libqarp_hip.so; states live in device memory (``DeviceState``) and
+reach the host only when a method returns amplitudes.
+
+Beyond the reference surface: ``precision="complex64"`` keeps complex64
+states with FP64 reductions, and the ``*_state`` methods work on
+device-resident states without host copies.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import secrets
+from typing import Mapping, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._abi import (QHIP_C64, QHIP_C128, PackedStream, QhipApplyStats, config_struct,
+                   pack_observable, pack_stream)
+from .ir import Command, GateType, SamplingResult, cbit_register_width
+
+Observable = Sequence[tuple]
+
+
+def _env_fusion_default() -> int:
+    """QARP_FUSION_MAX_QUBITS, ignored unless it parses to 0..8
+    (qarp_simulator.cpp:48-63)."""
+    env = os.environ.get("QARP_FUSION_MAX_QUBITS")
+    if env is not None:
+        try:
+            k = int(env, 10)
+            if 0 <= k <= QarpSimulator.MAX_FUSION_QUBITS:
+                return k
+        except ValueError:
+            pass
+    return QarpSimulator.DEFAULT_FUSION_QUBITS
+
+
+class DeviceState:
+    """A statevector in device memory (complex128 or complex64)."""
+
+    __slots__ = ("_h", "n_qubits", "precision", "__weakref__")
+
+    def __init__(self, n_qubits: int, precision: str = "complex128"):
+        lib = _lib.load()
+        if precision not in ("complex128", "complex64"):
+            raise ValueError(f"precision must be complex128 or complex64, got {precision!r}")
+        self._h = None
+        h = C.c_void_p()
+        _lib.check(lib.qhip_state_create(int(n_qubits), QHIP_C128 if precision == "complex128" else QHIP_C64,
+                                         C.byref(h)))
+        self._h = h
+        self.n_qubits = int(n_qubits)
+        self.precision = precision
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    @property
+    def dim(self) -> int:
+        return 1 << self.n_qubits
+
+    def reset(self) -> "DeviceState":
+        _lib.check(_lib.load().qhip_state_reset(self._h))
+        return self
+
+    def upload(self, amps: np.ndarray) -> "DeviceState":
+        a = np.ascontiguousarray(amps, dtype=np.complex128).ravel()
+        if a.size != self.dim:
+            raise ValueError(f"initial_state length {a.size} != 2^n_qubits = {self.dim}")
+        _lib.check(_lib.load().qhip_state_upload_c128(self._h, a.ctypes.data, a.size))
+        return self
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.dim, dtype=np.complex128)
+        _lib.check(_lib.load().qhip_state_download_c128(self._h, out.ctypes.data, out.size))
+        return out
+
+    def norm2(self) -> float:
+        v = C.c_double()
+        _lib.check(_lib.load().qhip_state_norm2(self._h, C.byref(v)))
+        return v.value
+
+    def copy_from(self, other: "DeviceState") -> "DeviceState":
+        _lib.check(_lib.load().qhip_state_copy(self._h, other._h))
+        return self
+
+    def device_ptr(self) -> int:
+        p = C.c_void_p()
+        _lib.check(_lib.load().qhip_state_device_ptr(self._h, C.byref(p)))
+        return p.value or 0
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib._lib is not None:
+            _lib._lib.qhip_state_destroy(h)
+            self._h = None
+
+
+def _check_initial_state(psi, dim: int, where: str) -> np.ndarray:
+    """validate_initial_state (qarp_simulator.cpp:731-746)."""
+    a = np.ascontiguousarray(psi, dtype=np.complex128).ravel()
+    if a.size != dim:
+        raise ValueError(f"QarpSimulator::{where}: initial_state length {a.size} != 2^n_qubits = {dim}")
+    norm = math.sqrt(float(np.vdot(a, a).real))
+    if abs(norm - 1.0) > 1e-10:
+        raise ValueError(
+            f"QarpSimulator::{where}: initial_state is not normalised: |psi| = {norm:.6f} differs from 1 "
+            "by more than 1e-10.  Normalise the input yourself — no silent renormalisation.")
+    return a
+
+
+def _random_seed() -> int:
+    return secrets.randbits(32)
+
+
+class QarpSimulator:
+    """GPU statevector simulator with the reference ``QarpSimulator`` API."""
+
+    MAX_FUSION_QUBITS = 8
+    DEFAULT_FUSION_QUBITS = 3
+    DEFAULT_FUSION_MIN_QUBITS = 12
+
+    def __init__(self, *, precision: str = "complex128", tile_qubits: int = 0, low_qubits: int = 0,
+                 max_ops_per_pass: int = 0):
+        if precision not in ("complex128", "complex64"):
+            raise ValueError(f"precision must be complex128 or complex64, got {precision!r}")
+        _lib.load()
+        self.precision = precision
+        self._fusion_max = _env_fusion_default()
+        self._fusion_min = self.DEFAULT_FUSION_MIN_QUBITS
+        self.tile_qubits = int(tile_qubits)
+        self.low_qubits = int(low_qubits)
+        self.max_ops_per_pass = int(max_ops_per_pass)
+        self.last_stats: Optional[dict] = None
+        self._pool: dict = {}
+
+    # ── knobs ──
+    @property
+    def fusion_max_qubits(self) -> int:
+        return self._fusion_max
+
+    @fusion_max_qubits.setter
+    def fusion_max_qubits(self, k: int) -> None:
+        k = int(k)
+        if k < 0 or k > self.MAX_FUSION_QUBITS:
+            raise ValueError(f"QarpSimulator::set_fusion_max_qubits: {k} exceeds the maximum block width "
+                             f"{self.MAX_FUSION_QUBITS}")
+        self._fusion_max = k
+
+    @property
+    def fusion_min_qubits(self) -> int:
+        return self._fusion_min
+
+    @fusion_min_qubits.setter
+    def fusion_min_qubits(self, n: int) -> None:
+        self._fusion_min = int(n)
+
+    def _config(self):
+        return config_struct(self._fusion_max, self._fusion_min, self.tile_qubits, self.low_qubits,
+                             self.max_ops_per_pass, -1)
+
+    # ── device-resident primitives ──
+    def new_state(self, n_qubits: int, initial_state=None, where: str = "statevector") -> DeviceState:
+        st = DeviceState(n_qubits, self.precision)
+        if initial_state is not None:
+            st.upload(_check_initial_state(initial_state, st.dim, where))
+        return st
+
+    def apply(self, commands: Sequence[Command], state: DeviceState, *, terminal: bool = False,
+              packed: Optional[PackedStream] = None) -> dict:
+        """Run the command stream on a device state in place (the gate loop of
+        QarpSimulator::statevector); returns the schedule statistics."""
+        ps = packed if packed is not None else pack_stream(commands)
+        cfg = self._config()
+        stats = QhipApplyStats()
+        fn = _lib.load().qhip_apply_terminal if terminal else _lib.load().qhip_apply
+        _lib.check(fn(C.byref(cfg), ps.ref, state.handle, C.byref(stats)))
+        self.last_stats = {"fused_ops": stats.fused_ops, "passes": stats.passes,
+                           "diagonal_ops": stats.diagonal_ops, "mfma_ops": stats.mfma_ops,
+                           "flops": stats.flops}
+        return self.last_stats
+
+    def transition_state(self, bra: DeviceState, ket: DeviceState, observable: Observable) -> complex:
+        po = pack_observable(observable)
+        out = (C.c_double * 2)()
+        _lib.check(_lib.load().qhip_transition(bra.handle, ket.handle, po.ref, out))
+        return complex(out[0], out[1])
+
+    def expectation_state(self, state: DeviceState, observable: Observable) -> float:
+        return self.transition_state(state, state, observable).real
+
+    def sample_state(self, state: DeviceState, n_shots: int, seed: Optional[int] = None) -> np.ndarray:
+        out = np.zeros(max(int(n_shots), 0), dtype=np.uint64)
+        s = _random_seed() if seed is None else int(seed) & 0xFFFFFFFF
+        _lib.check(_lib.load().qhip_sample(state.handle, int(n_shots), s, out.ctypes.data if out.size else None))
+        return out
+
+    def sample_uniforms(self, state: DeviceState, uniforms: np.ndarray) -> np.ndarray:
+        u = np.ascontiguousarray(uniforms, dtype=np.float64)
+        out = np.zeros(u.size, dtype=np.uint64)
+        _lib.check(_lib.load().qhip_sample_uniforms(state.handle, u.ctypes.data if u.size else None, u.size,
+                                                    out.ctypes.data if out.size else None))
+        return out
+
+    def probabilities_state(self, state: DeviceState) -> np.ndarray:
+        out = np.zeros(state.dim, dtype=np.float64)
+        _lib.check(_lib.load().qhip_probabilities(state.handle, out.ctypes.data, out.size))
+        return out
+
+    # ── reference surface ──
+    def statevector(self, commands: Sequence[Command], n_qubits: int, *, initial_state=None) -> np.ndarray:
+        st = self.new_state(n_qubits, initial_state, "statevector")
+        self.apply(commands, st)
+        return st.download()
+
+    def expectation(self, commands: Sequence[Command], n_qubits: int, observable: Observable, *,
+                    initial_state=None) -> float:
+        st = self.new_state(n_qubits, initial_state, "expectation")
+        self.apply(commands, st)
+        return self.expectation_state(st, observable)
+
+    def transition(self, bra, ket, n_qubits: int, observable: Observable) -> complex:
+        n = int(n_qubits)
+        if n < 0 or n > 62:
+            raise ValueError(f"pauli_transition: n_qubits must be in [0, 62], got {n}")
+        dim = 1 << n
+        b = np.ascontiguousarray(bra, dtype=np.complex128).ravel()
+        k = np.ascontiguousarray(ket, dtype=np.complex128).ravel()
+        if b.size != dim or k.size != dim:
+            raise ValueError(f"QarpSimulator::transition: bra/ket lengths {b.size}/{k.size} do not match "
+                             f"2^{n} = {dim}")
+        sb = DeviceState(n, self.precision).upload(b)
+        sk = DeviceState(n, self.precision).upload(k)
+        return self.transition_state(sb, sk, observable)
+
+    def batch_expectation(self, commands: Sequence[Command], n_qubits: int, observable: Observable,
+                          param_sets: Sequence[Mapping[str, float]]) -> list:
+        st = DeviceState(n_qubits, self.precision)
+        po = pack_observable(observable)
+        out = []
+        for params in param_sets:
+            st.reset()
+            self.apply(commands, st, packed=pack_stream(commands, values=params))
+            v = (C.c_double * 2)()
+            _lib.check(_lib.load().qhip_transition(st.handle, st.handle, po.ref, v))
+            out.append(v[0])
+        return out
+
+    def run(self, commands: Sequence[Command], n_qubits: int, n_shots: int, seed: Optional[int] = None, *,
+            initial_state=None) -> SamplingResult:
+        """Terminal-measurement fast path of QarpSimulator::run
+        (qarp_simulator.cpp:1017-1036) and sample_terminal (:751-783)."""
+        st = self.new_state(n_qubits, initial_state, "run")
+        self.apply(commands, st, terminal=True)
+        outcomes = self.sample_state(st, n_shots, seed)
+        res = SamplingResult(n_qubits=int(n_qubits), n_shots=int(n_shots))
+        vals, cnt = np.unique(outcomes, return_counts=True)
+        res.counts = {int(v): int(c) for v, c in zip(vals, cnt)}
+        recorded = [(c.qubits[0], c.cbits[0]) for c in commands
+                    if c.gate == GateType.Measure and c.cbits]
+        res.n_cbits = cbit_register_width(commands)
+        if recorded:
+            hist = []
+            for o in outcomes.tolist():
+                reg = [False] * res.n_cbits
+                for q, cb in recorded:
+                    reg[cb] = bool((o >> q) & 1)
+                hist.append(reg)
+            res.cbit_history = hist
+        return res
+
+    def batch_run(self, commands: Sequence[Command], n_qubits: int, n_shots: int,
+                  param_sets: Sequence[Mapping[str, float]], seed: Optional[int] = None) -> list:
+        out = []
+        for i, params in enumerate(param_sets):
+            s = None if seed is None else (int(seed) + i) & 0xFFFFFFFF
+            concrete = [c.substitute(params) for c in commands]
+            out.append(self.run(concrete, n_qubits, n_shots, s))
+        return out
+
+    def run_gradient(self, commands: Sequence[Command], n_qubits: int, observable: Observable,
+                     params: Mapping[str, float], param_order: Sequence[str], *, initial_state=None) -> list:
+        """Adjoint gradient of <H> (run_gradient, qarp_simulator.cpp:1607-1624)."""
+        grad, _ = self.run_gradient_energy(commands, n_qubits, observable, params, param_order,
+                                           initial_state=initial_state)
+        return grad.tolist()
+
+    def run_gradient_energy(self, commands, n_qubits, observable, params, param_order, *,
+                            initial_state=None, state: Optional[DeviceState] = None):
+        """(gradient array, energy) in one adjoint sweep."""
+        index = {name: i for i, name in enumerate(param_order)}
+        ps = pack_stream(commands, values=params, param_index=index)
+        po = pack_observable(observable)
+        st = state if state is not None else self.new_state(n_qubits, initial_state, "run_gradient")
+        grad = np.zeros(len(param_order))
+        energy = C.c_double()
+        cfg = self._config()
+        _lib.check(_lib.load().qhip_run_gradient(C.byref(cfg), ps.ref, len(param_order), st.handle, po.ref,
+                                                 grad.ctypes.data if grad.size else None, C.byref(energy)))
+        return grad, energy.value
+
+    def run_gradient_phi(self, commands: Sequence[Command], n_qubits: int, phi, params: Mapping[str, float],
+                         param_order: Sequence[str], *, initial_state=None) -> list:
+        index = {name: i for i, name in enumerate(param_order)}
+        ps = pack_stream(commands, values=params, param_index=index)
+        st = self.new_state(n_qubits, initial_state, "run_gradient")
+        ph = np.ascontiguousarray(np.asarray(phi, dtype=np.complex128).ravel())
+        if ph.size != st.dim:
+            raise RuntimeError("run_gradient: |φ⟩ size mismatch (expected 2^n_qubits)")
+        sp = DeviceState(n_qubits, self.precision).upload(ph)
+        grad = np.zeros(len(param_order))
+        cfg = self._config()
+        _lib.check(_lib.load().qhip_run_gradient_phi(C.byref(cfg), ps.ref, len(param_order), st.handle,
+                                                     sp.handle, grad.ctypes.data if grad.size else None))
+        return grad.tolist()
+
+    def fused_stream(self, commands: Sequence[Command], n_qubits: int) -> list:
+        """The fusion pass output: list of (gate id, qubits, params, matrix-or-None)."""
+        ps = pack_stream(commands)
+        cfg = self._config()
+        lib = _lib.load()
+        h = C.c_void_p()
+        _lib.check(lib.qhip_fuse(C.byref(cfg), ps.ref, int(n_qubits), C.byref(h)))
+        try:
+            n = C.c_int64()
+            _lib.check(lib.qhip_fused_count(h, C.byref(n)))
+            out = []
+            qs = np.zeros(64, np.uint32)
+            pr = np.zeros(4)
+            m = np.zeros(256 * 256, np.complex128)
+            for i in range(n.value):
+                g, k = C.c_int32(), C.c_int32()
+                _lib.check(lib.qhip_fused_get(h, i, C.byref(g), C.byref(k), qs.ctypes.data, pr.ctypes.data,
+                                              m.ctypes.data, m.size))
+                mat = None
+                if g.value == int(GateType.Custom) and k.value > 0:
+                    d = 1 << k.value
+                    mat = m[: d * d].reshape(d, d).copy()
+                out.append((g.value, tuple(int(q) for q in qs[: k.value]), tuple(pr), mat))
+            return out
+        finally:
+            lib.qhip_fused_destroy(h)

@@ -152,7 +152,7 @@ int ref_transition(const double* bra, const double* ket, int n, const qhip_obser
 // The fused stream of fuse_for_dispatch: fuse_for_simulation for k >= 2 and
 // n >= fusion_min, fuse_single_qubit_gates otherwise.  Writes per fused op:
 // gate id, operand count, operands (8 max), params (4), and for Custom the
-// matrix into mats at op_index * 256 * 2 doubles (capacity checked).
+// matrix (k <= 5) into mats at op_index * 1024 complex entries.
 int ref_fuse(const qhip_stream* s, int n, int fusion_max, int fusion_min, int64_t cap_ops,
              int64_t* n_out, int32_t* gates, int32_t* nqs, uint32_t* qubits, double* params,
              double* mats) {
@@ -176,11 +176,11 @@ int ref_fuse(const qhip_stream* s, int n, int fusion_max, int fusion_min, int64_
             if (c.gate == GateType::Custom && c.unitary) {
                 const auto& U = *c.unitary;
                 const int D = static_cast<int>(U.rows());
-                if (D <= 16)
+                if (D <= 32)
                     for (int y = 0; y < D; ++y)
                         for (int x = 0; x < D; ++x) {
-                            mats[(i * 256 + y * D + x) * 2] = U(y, x).real();
-                            mats[(i * 256 + y * D + x) * 2 + 1] = U(y, x).imag();
+                            mats[(i * 1024 + y * D + x) * 2] = U(y, x).real();
+                            mats[(i * 1024 + y * D + x) * 2 + 1] = U(y, x).imag();
                         }
             }
         }

@@ -222,7 +222,7 @@ def ref_fuse(packed, n, fusion_max, fusion_min, cap=100000):
     nqs = np.zeros(cap, np.int32)
     qubits = np.zeros((cap, 8), np.uint32)
     params = np.zeros((cap, 4))
-    mats = np.zeros((cap, 256), np.complex128)
+    mats = np.zeros((cap, 1024), np.complex128)
     _check(L, L.ref_fuse(packed.ref, n, fusion_max, fusion_min, cap, C.byref(n_out),
                          gates.ctypes.data, nqs.ctypes.data, qubits.ctypes.data,
                          params.ctypes.data, mats.ctypes.data), "ref")

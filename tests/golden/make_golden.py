@@ -1,0 +1,141 @@
+"""Writes the golden vectors under tests/golden/ from the reference simulator
+(oracle/_ref/libqarp_ref.so, compiled from the OpenQARP sources by
+oracle/Makefile).  Run in a container that has the reference tree:
+
+    make -C oracle && python tests/golden/make_golden.py
+
+Every fixture is inputs (seeds and sizes of the workload builders in
+openqarp_amd.workloads, or explicit arrays) plus the reference's outputs.
+"""
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parents[1]
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "tests"))
+
+import _oracle as O  # noqa: E402
+from openqarp_amd import workloads as W  # noqa: E402
+from openqarp_amd._abi import pack_observable, pack_stream  # noqa: E402
+
+
+def obs_arrays(obs):
+    qs, ls, offs, lens, cs = [], [], [], [], []
+    for pauli, c in obs:
+        offs.append(len(qs))
+        lens.append(len(pauli))
+        for q, l in pauli:
+            qs.append(q)
+            ls.append(ord(l))
+        cs.append(complex(c))
+    return dict(obs_qubits=np.array(qs, np.int64), obs_letters=np.array(ls, np.uint8),
+                obs_offsets=np.array(offs, np.int64), obs_lens=np.array(lens, np.int64),
+                obs_coeffs=np.array(cs, np.complex128))
+
+
+def hea_fixture(name, n, layers, seed, shots, shot_seed, fusion=(3, 12)):
+    w = W.c1_hea(n, layers, seed)
+    ps = pack_stream(w.commands)
+    k, kmin = fusion
+    sv = O.statevector(ps, n, k, kmin, lib="ref")
+    energy = O.transition(sv, sv, n, pack_observable(w.observable), lib="ref").real
+    samples = O.sample(sv, n, shots, shot_seed, lib="ref")
+    fused = O.ref_fuse(ps, n, k, kmin)
+    np.savez_compressed(HERE / name, n=n, layers=layers, seed=seed, fusion=np.array(fusion), state=sv,
+                        energy=energy, shots=shots, shot_seed=shot_seed, samples=samples,
+                        n_fused=len(fused), **obs_arrays(w.observable))
+
+
+def brickwork_fixture(name, n, layers, seed):
+    cmds = W.brickwork_haar(n, layers, seed)
+    ps = pack_stream(cmds)
+    out = {}
+    for k in (2, 3, 4, 5):
+        out[f"state_k{k}"] = O.statevector(ps, n, k, 0, lib="ref")
+    fused = O.ref_fuse(ps, n, 5, 0)
+    qubits = np.full((len(fused), 8), -1, np.int64)
+    mats = np.zeros((len(fused), 32, 32), np.complex128)
+    for i, (g, qs, _, m) in enumerate(fused):
+        qubits[i, : len(qs)] = qs
+        if m is not None:
+            mats[i, : m.shape[0], : m.shape[1]] = m
+    np.savez_compressed(HERE / name, n=n, layers=layers, seed=seed, fused_qubits=qubits, fused_mats=mats,
+                        fused_gates=np.array([f[0] for f in fused]), **out)
+
+
+def transition_fixture(name, n, n_terms, seed):
+    rng = np.random.default_rng(seed)
+    bra = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
+    ket = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
+    bra /= np.linalg.norm(bra)
+    ket /= np.linalg.norm(ket)
+    obs = W.random_pauli_sum(n, n_terms, seed + 1, probs=(0.55, 0.15, 0.15, 0.15))
+    obs = [(p, c * complex(1.0, 0.3)) for p, c in obs]      # complex coefficients: H need not be Hermitian
+    val = O.transition(bra, ket, n, pack_observable(obs), lib="ref")
+    np.savez_compressed(HERE / name, n=n, bra=bra, ket=ket, value=val, **obs_arrays(obs))
+
+
+def gradient_fixture(name, n, layers, seed, n_terms):
+    w = W.c3_gradient(n, layers, seed, n_terms)
+    idx = {s: i for i, s in enumerate(w.symbols)}
+    ps = pack_stream(w.commands, w.values, idx)
+    vals = np.array([w.values[s] for s in w.symbols])
+    po = pack_observable(w.observable)
+    grad = O.run_gradient(ps, n, po, len(vals), values=vals, lib="ref")
+    sv = O.statevector(ps, n, 0, 0, lib="ref")
+    energy = O.transition(sv, sv, n, po, lib="ref").real
+    np.savez_compressed(HERE / name, n=n, layers=layers, seed=seed, n_terms=n_terms, values=vals, grad=grad,
+                        energy=energy, **obs_arrays(w.observable))
+
+
+def qaoa_fixture(name, n, p, seed):
+    w = W.qaoa_maxcut(n, p, seed, symbolic=True)
+    idx = {s: i for i, s in enumerate(w.symbols)}
+    ps = pack_stream(w.commands, w.values, idx)
+    vals = np.array([w.values[s] for s in w.symbols])
+    po = pack_observable(w.observable)
+    grad = O.run_gradient(ps, n, po, len(vals), values=vals, lib="ref")
+    sv = O.statevector(ps, n, 3, 12, lib="ref")
+    energy = O.transition(sv, sv, n, po, lib="ref").real
+    np.savez_compressed(HERE / name, n=n, p=p, seed=seed, values=vals, grad=grad, energy=energy, state=sv,
+                        **obs_arrays(w.observable))
+
+
+def rng_fixture(name):
+    np.savez_compressed(HERE / name, words_5489=O.mt19937(5489, 2000, lib="ref"),
+                        words_1234=O.mt19937(1234, 2000, lib="ref"),
+                        uniforms_1234=O.uniforms(1234, 2000, lib="ref"))
+
+
+def sample_fixture(name, n, seed, shots, shot_seed):
+    rng = np.random.default_rng(seed)
+    # Heavy-tailed amplitudes with exact zeros, so the CDF has flat runs.
+    amp = rng.standard_normal(1 << n) * rng.exponential(1.0, 1 << n) + 1j * rng.standard_normal(1 << n)
+    amp[rng.random(1 << n) < 0.2] = 0.0
+    amp /= np.linalg.norm(amp)
+    out = O.sample(amp, n, shots, shot_seed, lib="ref")
+    np.savez_compressed(HERE / name, n=n, state=amp, shots=shots, shot_seed=shot_seed, samples=out)
+
+
+def main():
+    if O.reference() is None:
+        raise SystemExit("oracle/_ref/libqarp_ref.so is not built (make -C oracle)")
+    hea_fixture("hea12.npz", 12, 4, 7, 4000, 1234)
+    hea_fixture("hea16.npz", 16, 10, 1, 4000, 99)
+    hea_fixture("hea8_unfused.npz", 8, 3, 5, 500, 7, fusion=(3, 12))
+    brickwork_fixture("brickwork10.npz", 10, 6, 3)
+    transition_fixture("transition8.npz", 8, 60, 11)
+    gradient_fixture("gradient8.npz", 8, 2, 3, 40)
+    qaoa_fixture("qaoa10.npz", 10, 2, 5)
+    rng_fixture("mt19937.npz")
+    sample_fixture("sample10.npz", 10, 21, 5000, 77)
+    print("golden fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()
