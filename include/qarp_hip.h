@@ -199,6 +199,21 @@ int qhip_fused_destroy(qhip_fused* f);
 int qhip_apply(const qhip_config* cfg, const qhip_stream* stream, qhip_state* s,
                qhip_apply_stats* stats);
 
+/* Compiled stream: the fusion pass and the device schedule of qhip_apply done
+ * once, op tables resident on the device, applied to any state of the given
+ * width and precision.  The counterpart of compiling a circuit once and
+ * dispatching it per call (QarpEngine._compile_one, src/qarp/engines/
+ * _qarp_engine.py:283-300, with the program cache of qx.Program,
+ * src/qarpx/include/qarpx/simulator/program.h:51-83). */
+typedef struct qhip_plan qhip_plan;
+int qhip_plan_create(const qhip_config* cfg, const qhip_stream* stream, int32_t n_qubits,
+                     int32_t precision, qhip_plan** out);
+int qhip_plan_apply(const qhip_plan* plan, qhip_state* s, qhip_apply_stats* stats);
+/* As qhip_plan_apply, recording HIP events around every pass launch on the
+ * library's stream: pass_ms[i] = duration of pass i (capacity entries). */
+int qhip_plan_apply_timed(const qhip_plan* plan, qhip_state* s, double* pass_ms, int64_t capacity);
+int qhip_plan_destroy(qhip_plan* plan);
+
 /* As qhip_apply but recorded Measure / Barrier commands are skipped — the
  * terminal-measurement fast path of QarpSimulator::run (qarp_simulator.cpp:1017-1036).
  * Non-terminal measurements are rejected with QHIP_INVALID. */

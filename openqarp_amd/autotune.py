@@ -1,0 +1,59 @@
+"""Picks fusion width and tile shape for a circuit by timing candidate
+schedules on the device.
+
+Each candidate (fusion width k, tile qubits T, contiguous low qubits) is
+compiled once and applied once on the caller's state with HIP-event timing
+of every pass; the fastest total wins.  Results are cached per (register
+width, precision, circuit fingerprint).  Applying a candidate changes the
+state's amplitudes, not the work done, so the caller resets the state
+afterwards when its contents matter.
+"""
+from __future__ import annotations
+
+import hashlib
+from typing import Iterable, Optional, Sequence
+
+import numpy as np
+
+from .ir import Command
+from .simulator import CompiledCircuit, DeviceState, QarpSimulator
+
+_cache: dict = {}
+
+
+def fingerprint(commands: Sequence[Command]) -> str:
+    h = hashlib.sha1()
+    for c in commands:
+        h.update(bytes([int(c.gate)]))
+        h.update(np.asarray(c.qubits, np.int64).tobytes())
+    return h.hexdigest()
+
+
+def candidates(n: int, precision: str, widths: Iterable[int] = (2, 3, 4, 5)) -> list:
+    base_t = 12 if precision == "complex128" else 13
+    out = []
+    for k in widths:
+        for T in (base_t - 1, base_t):
+            for low in (4, 5):
+                if T <= n:
+                    out.append((k, T, low))
+    return out or [(3, min(n, base_t), 4)]
+
+
+def tune(sim: QarpSimulator, commands: Sequence[Command], state: DeviceState,
+         widths: Iterable[int] = (2, 3, 4, 5), repeats: int = 1) -> dict:
+    key = (state.n_qubits, sim.precision, fingerprint(commands), tuple(widths))
+    if key in _cache:
+        return _cache[key]
+    results = []
+    for k, T, low in candidates(state.n_qubits, sim.precision, widths):
+        cc = CompiledCircuit(sim, commands, state.n_qubits, fusion_max_qubits=k, tile_qubits=T,
+                             low_qubits=low)
+        cc.apply(state)                         # warm (first launch of each kernel)
+        ms = min(float(cc.apply_timed(state).sum()) for _ in range(repeats))
+        results.append({"fusion_max_qubits": k, "tile_qubits": T, "low_qubits": low, "ms": ms,
+                        "passes": cc.stats["passes"], "fused_ops": cc.stats["fused_ops"]})
+    best = min(results, key=lambda r: r["ms"])
+    out = {"best": best, "trials": results}
+    _cache[key] = out
+    return out

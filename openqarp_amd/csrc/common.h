@@ -175,6 +175,22 @@ struct ScheduleOptions {
 Plan schedule(const std::vector<HostOp>& ops, int n, int precision, const ScheduleOptions& opt);
 int default_tile_qubits(int precision, bool two_states);
 
+// A schedule with its op tables resident on the device.
+struct CompiledPlan {
+    Plan plan;
+    int n = 0;
+    int precision = QHIP_C128;
+    void* d_tables = nullptr;
+    void* d_ops = nullptr;
+    void* d_coefs = nullptr;
+    void* d_slots = nullptr;
+    int max_ip = 0;
+    int64_t fused_ops = 0;
+};
+CompiledPlan* compile_plan(Plan&& plan, int n, int precision);
+void run_compiled(const CompiledPlan& cp, qhip_state* s, std::vector<double>* pass_ms);
+void free_compiled(CompiledPlan* cp);
+
 // ── device entry points (kernels_*.hip) ────────────────────────────────────
 void launch_plan(const Plan& plan, qhip_state* s, qhip_state* phi, void* d_ip_partials,
                  int n_slots);
@@ -196,6 +212,8 @@ void reduce_ip(const void* d_partials, int64_t n_tiles, int n_slots, cplx* host_
 
 void apply_stream(const qhip_config* cfg, const qhip_stream* stream, qhip_state* s,
                   qhip_apply_stats* stats, bool terminal);
+Plan plan_stream(const qhip_config* cfg, const qhip_stream* stream, int n, int precision,
+                 bool terminal, int64_t* fused_ops);
 void run_gradient(const qhip_config* cfg, const qhip_stream* stream, int n_params,
                   qhip_state* psi, const qhip_observable* obs, const qhip_state* phi_in,
                   double* grad, double* energy);

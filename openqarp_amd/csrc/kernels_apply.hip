@@ -486,6 +486,17 @@ static void launch_tile(const PassDesc& P, A* psi, A* phi, double2* ip_part, con
                         int64_t n_tiles, size_t lds) {
     auto& C = ctx();
     const dim3 grid(static_cast<uint32_t>(n_tiles)), block(kThreads);
+    // Above 64 KiB of dynamic LDS a kernel must opt in (160 KiB per CU).
+    static bool opted[4] = {false, false, false, false};
+    const int kk = (P.kmax <= 2 ? 2 : std::min(P.kmax, 5)) - 2;
+    if (lds > 65536 && !opted[kk]) {
+        const void* fn = kk == 0 ? reinterpret_cast<const void*>(&tile_pass_kernel<A, TWO, 2>)
+                       : kk == 1 ? reinterpret_cast<const void*>(&tile_pass_kernel<A, TWO, 3>)
+                       : kk == 2 ? reinterpret_cast<const void*>(&tile_pass_kernel<A, TWO, 4>)
+                                 : reinterpret_cast<const void*>(&tile_pass_kernel<A, TWO, 5>);
+        QHIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        opted[kk] = true;
+    }
     switch (P.kmax <= 2 ? 2 : P.kmax) {
         case 2: hipLaunchKernelGGL((tile_pass_kernel<A, TWO, 2>), grid, block, lds, C.stream, psi, phi, P, d_ops,
                                    d_coefs, ip_part, n_tiles); break;
@@ -501,10 +512,13 @@ static void launch_tile(const PassDesc& P, A* psi, A* phi, double2* ip_part, con
 
 template <typename A>
 static void launch_passes(const Plan& plan, A* psi, A* phi, double2* ip_part, double2* ip_tot,
-                          const DevOp* d_ops, const A* d_coefs, const int32_t* d_slots) {
+                          const DevOp* d_ops, const A* d_coefs, const int32_t* d_slots,
+                          std::vector<hipEvent_t>* events) {
     auto& C = ctx();
     int slot_base = 0;
+    size_t ev = 0;
     for (const PassDesc& P : plan.passes) {
+        if (events) QHIP_CHECK(hipEventRecord((*events)[ev++], C.stream));
         if (P.T == 0) {   // wide dense op, straight from HBM
             const uint64_t groups = uint64_t{1} << (P.n - P.kmax);
             const int blocks = static_cast<int>(std::min<uint64_t>(groups, static_cast<uint64_t>(C.n_cu) * 8));
@@ -514,34 +528,42 @@ static void launch_passes(const Plan& plan, A* psi, A* phi, double2* ip_part, do
                                    d_ops + P.op_begin, d_coefs + plan.ops[P.op_begin].coef);
                 QHIP_CHECK(hipGetLastError());
             }
-            continue;
-        }
-        const int64_t n_tiles = int64_t{1} << (P.n - P.T);
-        const size_t lds = (size_t{1} << P.T) * sizeof(A) * (phi ? 2 : 1) +
-                           (phi ? 64 * (kThreads / 64) * sizeof(double2) : 0);
-        if (phi) launch_tile<A, true>(P, psi, phi, ip_part, d_ops, d_coefs, n_tiles, lds);
-        else launch_tile<A, false>(P, psi, phi, ip_part, d_ops, d_coefs, n_tiles, lds);
-        if (phi) {
-            int n_ip = 0;
-            for (int o = P.op_begin; o < P.op_end; ++o)
-                n_ip += plan.ops[o].kind == OP_IP_DENSE || plan.ops[o].kind == OP_IP_DIAG;
-            if (n_ip > 0) {
-                hipLaunchKernelGGL(reduce_pass_ip, dim3(n_ip), dim3(kThreads), 0, C.stream, ip_part, n_tiles,
-                                   d_slots + slot_base, ip_tot);
-                QHIP_CHECK(hipGetLastError());
-                slot_base += n_ip;
+        } else {
+            const int64_t n_tiles = int64_t{1} << (P.n - P.T);
+            const size_t lds = (size_t{1} << P.T) * sizeof(A) * (phi ? 2 : 1) +
+                               (phi ? 64 * (kThreads / 64) * sizeof(double2) : 0);
+            if (phi) launch_tile<A, true>(P, psi, phi, ip_part, d_ops, d_coefs, n_tiles, lds);
+            else launch_tile<A, false>(P, psi, phi, ip_part, d_ops, d_coefs, n_tiles, lds);
+            if (phi) {
+                int n_ip = 0;
+                for (int o = P.op_begin; o < P.op_end; ++o)
+                    n_ip += plan.ops[o].kind == OP_IP_DENSE || plan.ops[o].kind == OP_IP_DIAG;
+                if (n_ip > 0) {
+                    hipLaunchKernelGGL(reduce_pass_ip, dim3(n_ip), dim3(kThreads), 0, C.stream, ip_part, n_tiles,
+                                       d_slots + slot_base, ip_tot);
+                    QHIP_CHECK(hipGetLastError());
+                    slot_base += n_ip;
+                }
             }
         }
+        if (events) QHIP_CHECK(hipEventRecord((*events)[ev++], C.stream));
     }
 }
 
-void launch_plan(const Plan& plan, qhip_state* s, qhip_state* phi, void* d_ip_totals, int n_slots) {
-    if (plan.passes.empty()) return;
-    auto& C = ctx();
-    // Upload op tables and coefficients (converted for complex64 states).
-    const size_t ops_bytes = plan.ops.size() * sizeof(DevOp);
-    std::vector<int32_t> slots;
+// Device copies of a plan's op tables: either in the shared scratch buffers
+// (one-shot applies) or in a plan-owned allocation (compiled plans).
+struct DeviceTables {
+    DevOp* ops = nullptr;
+    void* coefs = nullptr;
+    int32_t* slots = nullptr;
+    void* owned = nullptr;
     int max_ip = 0;
+};
+
+static DeviceTables upload_tables(const Plan& plan, int precision, bool own) {
+    auto& C = ctx();
+    DeviceTables t;
+    std::vector<int32_t> slots;
     for (const PassDesc& P : plan.passes) {
         int n_ip = 0;
         for (int o = P.op_begin; o < P.op_end; ++o)
@@ -549,41 +571,125 @@ void launch_plan(const Plan& plan, qhip_state* s, qhip_state* phi, void* d_ip_to
                 slots.push_back(static_cast<int32_t>(plan.ops[o].lmask));
                 ++n_ip;
             }
-        max_ip = std::max(max_ip, n_ip);
+        t.max_ip = std::max(t.max_ip, n_ip);
     }
-    const size_t slot_bytes = slots.size() * sizeof(int32_t);
-    char* tab = static_cast<char*>(C.ops.ensure(ops_bytes + slot_bytes + 256));
-    DevOp* d_ops = reinterpret_cast<DevOp*>(tab);
-    int32_t* d_slots = reinterpret_cast<int32_t*>(tab + ((ops_bytes + 255) / 256) * 256);
-    QHIP_CHECK(hipMemcpyAsync(d_ops, plan.ops.data(), ops_bytes, hipMemcpyHostToDevice, C.stream));
-    if (slot_bytes)
-        QHIP_CHECK(hipMemcpyAsync(d_slots, slots.data(), slot_bytes, hipMemcpyHostToDevice, C.stream));
-    double2* ip_part = nullptr;
-    if (phi && max_ip > 0) {
-        int T0 = 0;
-        for (const PassDesc& P : plan.passes) T0 = P.T > 0 ? P.T : T0;
-        const int64_t n_tiles = int64_t{1} << (plan.passes[0].n - T0);
-        ip_part = static_cast<double2*>(C.red.ensure(sizeof(double2) * static_cast<size_t>(max_ip) * n_tiles));
+    const size_t align = 256;
+    auto up = [&](size_t b) { return (b + align - 1) / align * align; };
+    const size_t ops_b = plan.ops.size() * sizeof(DevOp);
+    const size_t slot_b = slots.size() * sizeof(int32_t);
+    const size_t coef_b = plan.coefs.size() * (precision == QHIP_C128 ? sizeof(double2) : sizeof(float2));
+    const size_t total = up(ops_b) + up(slot_b) + up(coef_b) + align;
+    char* base;
+    if (own) {
+        if (hipMalloc(reinterpret_cast<void**>(&base), total) != hipSuccess) {
+            (void)hipGetLastError();
+            fail(QHIP_NOMEM, "plan: device table allocation failed");
+        }
+        t.owned = base;
+    } else {
+        base = static_cast<char*>(C.ops.ensure(total));
     }
-    (void)n_slots;
-    if (s->precision == QHIP_C128) {
-        const size_t cb = plan.coefs.size() * sizeof(double2);
-        double2* d_c = static_cast<double2*>(C.coefs.ensure(std::max<size_t>(cb, 16)));
-        QHIP_CHECK(hipMemcpyAsync(d_c, plan.coefs.data(), cb, hipMemcpyHostToDevice, C.stream));
-        launch_passes<double2>(plan, static_cast<double2*>(s->d), phi ? static_cast<double2*>(phi->d) : nullptr,
-                               ip_part, static_cast<double2*>(d_ip_totals), d_ops, d_c, d_slots);
+    t.ops = reinterpret_cast<DevOp*>(base);
+    t.slots = reinterpret_cast<int32_t*>(base + up(ops_b));
+    t.coefs = base + up(ops_b) + up(slot_b);
+    QHIP_CHECK(hipMemcpyAsync(t.ops, plan.ops.data(), ops_b, hipMemcpyHostToDevice, C.stream));
+    if (slot_b) QHIP_CHECK(hipMemcpyAsync(t.slots, slots.data(), slot_b, hipMemcpyHostToDevice, C.stream));
+    if (precision == QHIP_C128) {
+        QHIP_CHECK(hipMemcpyAsync(t.coefs, plan.coefs.data(), coef_b, hipMemcpyHostToDevice, C.stream));
+        QHIP_CHECK(hipStreamSynchronize(C.stream));
     } else {
         std::vector<float2> cf(plan.coefs.size());
         for (size_t i = 0; i < cf.size(); ++i)
             cf[i] = make_float2(static_cast<float>(plan.coefs[i].real()), static_cast<float>(plan.coefs[i].imag()));
-        const size_t cb = cf.size() * sizeof(float2);
-        float2* d_c = static_cast<float2*>(C.coefs.ensure(std::max<size_t>(cb, 16)));
-        QHIP_CHECK(hipMemcpyAsync(d_c, cf.data(), cb, hipMemcpyHostToDevice, C.stream));
-        launch_passes<float2>(plan, static_cast<float2*>(s->d), phi ? static_cast<float2*>(phi->d) : nullptr,
-                              ip_part, static_cast<double2*>(d_ip_totals), d_ops, d_c, d_slots);
+        QHIP_CHECK(hipMemcpyAsync(t.coefs, cf.data(), coef_b, hipMemcpyHostToDevice, C.stream));
+        QHIP_CHECK(hipStreamSynchronize(C.stream));   // cf must outlive the copy
     }
-    // The host staging vectors above must outlive the copies.
+    return t;
+}
+
+static void run_tables(const Plan& plan, const DeviceTables& t, qhip_state* s, qhip_state* phi, void* d_ip_totals,
+                       std::vector<double>* pass_ms) {
+    auto& C = ctx();
+    double2* ip_part = nullptr;
+    if (phi && t.max_ip > 0) {
+        int T0 = 0;
+        for (const PassDesc& P : plan.passes) T0 = P.T > 0 ? P.T : T0;
+        const int64_t n_tiles = int64_t{1} << (plan.passes[0].n - T0);
+        ip_part = static_cast<double2*>(C.red.ensure(sizeof(double2) * static_cast<size_t>(t.max_ip) * n_tiles));
+    }
+    std::vector<hipEvent_t> events;
+    if (pass_ms) {
+        events.resize(2 * plan.passes.size());
+        for (auto& e : events) QHIP_CHECK(hipEventCreate(&e));
+    }
+    if (s->precision == QHIP_C128)
+        launch_passes<double2>(plan, static_cast<double2*>(s->d), phi ? static_cast<double2*>(phi->d) : nullptr,
+                               ip_part, static_cast<double2*>(d_ip_totals), t.ops,
+                               static_cast<const double2*>(t.coefs), t.slots, pass_ms ? &events : nullptr);
+    else
+        launch_passes<float2>(plan, static_cast<float2*>(s->d), phi ? static_cast<float2*>(phi->d) : nullptr,
+                              ip_part, static_cast<double2*>(d_ip_totals), t.ops,
+                              static_cast<const float2*>(t.coefs), t.slots, pass_ms ? &events : nullptr);
     QHIP_CHECK(hipStreamSynchronize(C.stream));
+    if (pass_ms) {
+        pass_ms->assign(plan.passes.size(), 0.0);
+        for (size_t i = 0; i < plan.passes.size(); ++i) {
+            float ms = 0.0f;
+            QHIP_CHECK(hipEventElapsedTime(&ms, events[2 * i], events[2 * i + 1]));
+            (*pass_ms)[i] = ms;
+        }
+        for (auto& e : events) (void)hipEventDestroy(e);
+    }
+}
+
+void launch_plan(const Plan& plan, qhip_state* s, qhip_state* phi, void* d_ip_totals, int n_slots) {
+    (void)n_slots;
+    if (plan.passes.empty()) return;
+    const DeviceTables t = upload_tables(plan, s->precision, false);
+    run_tables(plan, t, s, phi, d_ip_totals, nullptr);
+}
+
+CompiledPlan* compile_plan(Plan&& plan, int n, int precision) {
+    auto* cp = new CompiledPlan();
+    cp->plan = std::move(plan);
+    cp->n = n;
+    cp->precision = precision;
+    if (!cp->plan.passes.empty()) {
+        try {
+            const DeviceTables t = upload_tables(cp->plan, precision, true);
+            cp->d_tables = t.owned;
+            cp->d_ops = t.ops;
+            cp->d_coefs = t.coefs;
+            cp->d_slots = t.slots;
+            cp->max_ip = t.max_ip;
+        } catch (...) {
+            delete cp;
+            throw;
+        }
+    }
+    return cp;
+}
+
+void run_compiled(const CompiledPlan& cp, qhip_state* s, std::vector<double>* pass_ms) {
+    if (cp.plan.passes.empty()) {
+        if (pass_ms) pass_ms->clear();
+        return;
+    }
+    DeviceTables t;
+    t.ops = static_cast<DevOp*>(cp.d_ops);
+    t.coefs = cp.d_coefs;
+    t.slots = static_cast<int32_t*>(cp.d_slots);
+    t.max_ip = cp.max_ip;
+    run_tables(cp.plan, t, s, nullptr, nullptr, pass_ms);
+}
+
+void free_compiled(CompiledPlan* cp) {
+    if (!cp) return;
+    if (cp->d_tables) {
+        (void)hipStreamSynchronize(ctx().stream);
+        (void)hipFree(cp->d_tables);
+    }
+    delete cp;
 }
 
 }  // namespace qhip

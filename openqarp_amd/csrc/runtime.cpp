@@ -124,10 +124,9 @@ static ScheduleOptions options(const qhip_config* cfg, bool two) {
     return o;
 }
 
-void apply_stream(const qhip_config* cfg, const qhip_stream* stream, qhip_state* s, qhip_apply_stats* stats,
-                  bool terminal) {
+Plan plan_stream(const qhip_config* cfg, const qhip_stream* stream, int n, int precision, bool terminal,
+                 int64_t* fused_ops) {
     const Stream st{stream};
-    const int n = s->n;
     validate(st, n);
     if (!terminal) {
         for (int64_t i = 0; i < st.size(); ++i)
@@ -146,15 +145,25 @@ void apply_stream(const qhip_config* cfg, const qhip_stream* stream, qhip_state*
     std::vector<HostOp> ops;
     ops.reserve(fused.size());
     for (const auto& it : fused) lower_item(st, it, n, ops, terminal);
-    const Plan plan = schedule(ops, n, s->precision, options(cfg, false));
+    if (fused_ops) *fused_ops = static_cast<int64_t>(fused.size());
+    return schedule(ops, n, precision, options(cfg, false));
+}
+
+static void fill_stats(qhip_apply_stats* stats, const Plan& plan, int64_t fused) {
+    if (!stats) return;
+    stats->fused_ops = fused;
+    stats->passes = static_cast<int64_t>(plan.passes.size());
+    stats->diagonal_ops = plan.diag_ops;
+    stats->mfma_ops = 0;
+    stats->flops = plan.flops;
+}
+
+void apply_stream(const qhip_config* cfg, const qhip_stream* stream, qhip_state* s, qhip_apply_stats* stats,
+                  bool terminal) {
+    int64_t fused = 0;
+    const Plan plan = plan_stream(cfg, stream, s->n, s->precision, terminal, &fused);
     launch_plan(plan, s, nullptr, nullptr, 0);
-    if (stats) {
-        stats->fused_ops = static_cast<int64_t>(fused.size());
-        stats->passes = static_cast<int64_t>(plan.passes.size());
-        stats->diagonal_ops = plan.diag_ops;
-        stats->mfma_ops = 0;
-        stats->flops = plan.flops;
-    }
+    fill_stats(stats, plan, fused);
 }
 
 struct StateHolder {
@@ -487,6 +496,46 @@ int qhip_apply(const qhip_config* cfg, const qhip_stream* stream, qhip_state* s,
         need(stream && s, "qhip_apply: NULL argument");
         qhip::apply_stream(cfg, stream, s, stats, false);
     });
+}
+
+int qhip_plan_create(const qhip_config* cfg, const qhip_stream* stream, int32_t n_qubits, int32_t precision,
+                     qhip_plan** out) {
+    return guarded([&] {
+        need(stream && out, "qhip_plan_create: NULL argument");
+        need(n_qubits >= 0 && n_qubits <= 40, "qhip_plan_create: n_qubits must be in [0, 40]");
+        need(precision == QHIP_C128 || precision == QHIP_C64, "qhip_plan_create: bad precision");
+        qhip::ctx();
+        int64_t fused = 0;
+        qhip::Plan plan = qhip::plan_stream(cfg, stream, n_qubits, precision, false, &fused);
+        qhip::CompiledPlan* cp = qhip::compile_plan(std::move(plan), n_qubits, precision);
+        cp->fused_ops = fused;
+        *out = reinterpret_cast<qhip_plan*>(cp);
+    });
+}
+
+int qhip_plan_apply(const qhip_plan* plan, qhip_state* s, qhip_apply_stats* stats) {
+    return guarded([&] {
+        need(plan && s, "qhip_plan_apply: NULL argument");
+        const auto* cp = reinterpret_cast<const qhip::CompiledPlan*>(plan);
+        need(cp->n == s->n && cp->precision == s->precision, "qhip_plan_apply: state shape differs from the plan");
+        qhip::run_compiled(*cp, s, nullptr);
+        qhip::fill_stats(stats, cp->plan, cp->fused_ops);
+    });
+}
+
+int qhip_plan_apply_timed(const qhip_plan* plan, qhip_state* s, double* pass_ms, int64_t capacity) {
+    return guarded([&] {
+        need(plan && s && (pass_ms || capacity == 0), "qhip_plan_apply_timed: NULL argument");
+        const auto* cp = reinterpret_cast<const qhip::CompiledPlan*>(plan);
+        need(cp->n == s->n && cp->precision == s->precision, "qhip_plan_apply_timed: state shape differs");
+        std::vector<double> ms;
+        qhip::run_compiled(*cp, s, &ms);
+        for (int64_t i = 0; i < capacity && i < static_cast<int64_t>(ms.size()); ++i) pass_ms[i] = ms[i];
+    });
+}
+
+int qhip_plan_destroy(qhip_plan* plan) {
+    return guarded([&] { qhip::free_compiled(reinterpret_cast<qhip::CompiledPlan*>(plan)); });
 }
 
 int qhip_apply_terminal(const qhip_config* cfg, const qhip_stream* stream, qhip_state* s,

@@ -349,3 +349,52 @@ class QarpSimulator:
             return out
         finally:
             lib.qhip_fused_destroy(h)
+
+
+class CompiledCircuit:
+    """A command stream fused and scheduled once, op tables resident on the
+    device (qhip_plan_*); ``apply`` launches only the passes."""
+
+    def __init__(self, sim: "QarpSimulator", commands: Sequence[Command], n_qubits: int, *,
+                 fusion_max_qubits: Optional[int] = None, tile_qubits: Optional[int] = None,
+                 low_qubits: Optional[int] = None):
+        self.n_qubits = int(n_qubits)
+        self.precision = sim.precision
+        cfg = config_struct(sim.fusion_max_qubits if fusion_max_qubits is None else int(fusion_max_qubits),
+                            sim.fusion_min_qubits,
+                            sim.tile_qubits if tile_qubits is None else int(tile_qubits),
+                            sim.low_qubits if low_qubits is None else int(low_qubits),
+                            sim.max_ops_per_pass, -1)
+        self.config = {"fusion_max_qubits": cfg.fusion_max_qubits, "tile_qubits": cfg.tile_qubits,
+                       "low_qubits": cfg.low_qubits}
+        ps = pack_stream(commands)
+        self._h = None
+        h = C.c_void_p()
+        _lib.check(_lib.load().qhip_plan_create(C.byref(cfg), ps.ref, self.n_qubits,
+                                                QHIP_C128 if self.precision == "complex128" else QHIP_C64,
+                                                C.byref(h)))
+        self._h = h
+        self.stats: dict = {}     # filled by the first apply
+
+    def apply(self, state: DeviceState) -> dict:
+        stats = QhipApplyStats()
+        _lib.check(_lib.load().qhip_plan_apply(self._h, state.handle, C.byref(stats)))
+        self.stats = {"fused_ops": stats.fused_ops, "passes": stats.passes,
+                      "diagonal_ops": stats.diagonal_ops, "flops": stats.flops}
+        return self.stats
+
+    def apply_timed(self, state: DeviceState) -> np.ndarray:
+        """Apply once; per-pass kernel durations (ms) from HIP events recorded
+        on the library's stream around every pass launch."""
+        if not self.stats:
+            self.apply(state)
+        n = int(self.stats["passes"])
+        out = np.zeros(max(n, 1))
+        _lib.check(_lib.load().qhip_plan_apply_timed(self._h, state.handle, out.ctypes.data, n))
+        return out[:n]
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib._lib is not None:
+            _lib._lib.qhip_plan_destroy(h)
+            self._h = None
