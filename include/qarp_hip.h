@@ -190,6 +190,15 @@ int qhip_fused_get(const qhip_fused* f, int64_t i, int32_t* gate, int32_t* n_qub
                    uint32_t* qubits, double* params, double* matrix, int64_t capacity);
 int qhip_fused_destroy(qhip_fused* f);
 
+/* Host only (no device call): the device schedule of qhip_apply for a
+ * register of n_qubits, as a sequence of global operations in execution
+ * order — dense and diagonal blocks as Custom with their full matrix, mask
+ * phases as MCZ, scalar phases as GPhase — with the pass each belongs to
+ * (qhip_fused_pass).  For checking the scheduler; no reference counterpart. */
+int qhip_schedule_inspect(const qhip_config* cfg, const qhip_stream* in, int32_t n_qubits,
+                          int32_t precision, qhip_fused** out);
+int qhip_fused_pass(const qhip_fused* f, int64_t i, int32_t* pass);
+
 /* ── hot path: apply a command stream ─────────────────────────────────────
  * The gate loop of QarpSimulator::statevector (qarp_simulator.cpp:901-943):
  * fuse_for_dispatch, then every fused command on the device state in

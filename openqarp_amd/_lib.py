@@ -21,6 +21,7 @@ EXPORTS = [
     "qhip_state_upload", "qhip_state_download", "qhip_state_upload_c128",
     "qhip_state_download_c128", "qhip_state_copy", "qhip_state_norm2", "qhip_state_device_ptr",
     "qhip_fuse", "qhip_fused_count", "qhip_fused_get", "qhip_fused_destroy",
+    "qhip_schedule_inspect", "qhip_fused_pass",
     "qhip_plan_create", "qhip_plan_apply", "qhip_plan_apply_timed", "qhip_plan_destroy",
     "qhip_apply", "qhip_apply_terminal", "qhip_transition", "qhip_apply_observable",
     "qhip_sample", "qhip_sample_uniforms", "qhip_probabilities",
@@ -49,6 +50,8 @@ def _declare(lib):
         "qhip_fused_count": [vp, C.POINTER(i64)],
         "qhip_fused_get": [vp, i64, C.POINTER(i32), C.POINTER(i32), vp, vp, vp, i64],
         "qhip_fused_destroy": [vp],
+        "qhip_schedule_inspect": [vp, vp, i32, i32, C.POINTER(vp)],
+        "qhip_fused_pass": [vp, i64, C.POINTER(i32)],
         "qhip_plan_create": [vp, vp, i32, i32, C.POINTER(vp)],
         "qhip_plan_apply": [vp, vp, vp],
         "qhip_plan_apply_timed": [vp, vp, vp, i64],

@@ -115,6 +115,12 @@ struct FusedItem {
     int32_t k = 0;
     std::array<uint32_t, 8> q{};
     std::vector<cplx> m;             // fused block matrix
+    // Schedule inspection only: gate id of the emitted operation, its pass,
+    // mask-phase operands beyond 8 and the phase of mask / scalar ops.
+    int32_t gate = QHIP_CUSTOM;
+    int32_t pass = -1;
+    std::vector<uint32_t> wide_q;
+    cplx phase{1.0, 0.0};
 };
 std::vector<FusedItem> fuse_for_dispatch(const Stream& st, int n, int fusion_max, int fusion_min);
 

@@ -449,6 +449,75 @@ int qhip_fuse(const qhip_config* cfg, const qhip_stream* in, int32_t n_qubits, q
     });
 }
 
+int qhip_schedule_inspect(const qhip_config* cfg, const qhip_stream* in, int32_t n_qubits, int32_t precision,
+                          qhip_fused** out) {
+    return guarded([&] {
+        need(in && out, "qhip_schedule_inspect: NULL argument");
+        need(n_qubits >= 0 && n_qubits <= 40, "qhip_schedule_inspect: n_qubits must be in [0, 40]");
+        need(precision == QHIP_C128 || precision == QHIP_C64, "qhip_schedule_inspect: bad precision");
+        const qhip::Plan plan = qhip::plan_stream(cfg, in, n_qubits, precision, false, nullptr);
+        auto f = std::make_unique<qhip_fused>();
+        for (size_t pi = 0; pi < plan.passes.size(); ++pi) {
+            const qhip::PassDesc& P = plan.passes[pi];
+            auto global = [&](int pos) -> uint32_t {
+                if (P.T == 0) return static_cast<uint32_t>(pos);
+                return pos < 32 ? static_cast<uint32_t>(P.qpos[pos]) : static_cast<uint32_t>(pos - 32);
+            };
+            for (int o = P.op_begin; o < P.op_end; ++o) {
+                const qhip::DevOp& d = plan.ops[o];
+                qhip::FusedItem it;
+                it.pass = static_cast<int32_t>(pi);
+                switch (d.kind) {
+                    case qhip::OP_DENSE: {
+                        it.k = d.k;
+                        for (int b = 0; b < d.k; ++b) it.q[b] = global(d.pos[b]);
+                        const size_t D = size_t{1} << d.k;
+                        it.m.assign(plan.coefs.begin() + d.coef, plan.coefs.begin() + d.coef + D * D);
+                        break;
+                    }
+                    case qhip::OP_DIAG: {
+                        it.k = d.k;
+                        for (int b = 0; b < d.k; ++b) it.q[b] = global(d.pos[b]);
+                        const size_t D = size_t{1} << d.k;
+                        it.m.assign(D * D, qhip::cplx(0.0));
+                        for (size_t y = 0; y < D; ++y) it.m[y * D + y] = plan.coefs[d.coef + y];
+                        break;
+                    }
+                    case qhip::OP_MASKPHASE: {
+                        it.gate = QHIP_MCZ;
+                        for (int l = 0; l < 64; ++l)
+                            if ((d.lmask >> l) & 1) it.wide_q.push_back(global(l));
+                        for (int q = 0; q < 64; ++q)
+                            if ((d.gmask >> q) & 1) it.wide_q.push_back(static_cast<uint32_t>(q));
+                        it.k = static_cast<int32_t>(it.wide_q.size());
+                        it.phase = plan.coefs[d.coef];
+                        break;
+                    }
+                    case qhip::OP_SCALAR:
+                        it.gate = QHIP_GPHASE;
+                        it.phase = plan.coefs[d.coef];
+                        break;
+                    default:
+                        qhip::fail(QHIP_RUNTIME, "qhip_schedule_inspect: unexpected op kind");
+                }
+                f->items.push_back(std::move(it));
+                f->src_ops.push_back(qhip_op{});
+                f->src_qubits.emplace_back();
+                f->src_mats.emplace_back();
+            }
+        }
+        *out = f.release();
+    });
+}
+
+int qhip_fused_pass(const qhip_fused* f, int64_t i, int32_t* pass) {
+    return guarded([&] {
+        need(f && pass, "qhip_fused_pass: NULL argument");
+        need(i >= 0 && i < static_cast<int64_t>(f->items.size()), "qhip_fused_pass: index out of range");
+        *pass = f->items[i].pass;
+    });
+}
+
 int qhip_fused_count(const qhip_fused* f, int64_t* n_ops) {
     return guarded([&] {
         need(f && n_ops, "qhip_fused_count: NULL argument");
@@ -473,6 +542,13 @@ int qhip_fused_get(const qhip_fused* f, int64_t i, int32_t* gate, int32_t* n_qub
                 need(capacity >= static_cast<int64_t>(m.size()), "qhip_fused_get: matrix capacity too small");
                 std::memcpy(matrix, m.data(), sizeof(qhip::cplx) * m.size());
             }
+        } else if (it.gate == QHIP_MCZ || it.gate == QHIP_GPHASE) {
+            *gate = it.gate;
+            *n_qubits = it.k;
+            for (int b = 0; b < it.k && b < 64; ++b) qubits[b] = it.wide_q[b];
+            params[0] = it.phase.real();
+            params[1] = it.phase.imag();
+            params[2] = params[3] = 0.0;
         } else {
             *gate = QHIP_CUSTOM;
             *n_qubits = it.k;

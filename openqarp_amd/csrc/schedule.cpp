@@ -85,7 +85,20 @@ static void schedule_segment(Plan& plan, const std::vector<HostOp>& ops, const s
     const int max_ip = 64;
     std::vector<size_t> remaining = seg;
     while (!remaining.empty()) {
+        // The first op of a pass always fits: it may displace forced low
+        // qubits (the tile is then less coalesced, never wrong), so every
+        // pass makes progress.
         uint64_t Q = low_mask;
+        {
+            const HostOp& first = ops[remaining.front()];
+            if (first.needs_local()) {
+                const uint64_t qm = first.qmask();
+                if (std::popcount(Q | qm) > T) {
+                    Q = qm;
+                    for (int q = 0; q < n && std::popcount(Q) < T && q < low; ++q) Q |= uint64_t{1} << q;
+                }
+            }
+        }
         uint64_t blocked = 0;
         std::vector<size_t> chosen, deferred;
         double flops = 0.0;
@@ -138,6 +151,7 @@ static void schedule_segment(Plan& plan, const std::vector<HostOp>& ops, const s
             if (ops[idx].needs_local()) P.kmax = std::max(P.kmax, ops[idx].k);
         }
         P.op_end = static_cast<int32_t>(plan.ops.size());
+        if (chosen.empty()) fail(QHIP_RUNTIME, "schedule: no op could join a pass");
         plan.passes.push_back(P);
         remaining.swap(deferred);
     }
@@ -148,8 +162,11 @@ Plan schedule(const std::vector<HostOp>& ops, int n, int precision, const Schedu
     int T = opt.tile_qubits > 0 ? opt.tile_qubits : default_tile_qubits(precision, opt.two_states);
     T = std::max(T, std::min(kMaxTileK, n));
     T = std::min(T, n);
+    int kdense = 0;
+    for (const auto& op : ops)
+        if (op.needs_local() && op.k <= kMaxTileK) kdense = std::max(kdense, op.k);
     int low = opt.low_qubits > 0 ? opt.low_qubits : 4;
-    low = std::min(low, T);
+    low = std::max(0, std::min(low, T - kdense));
     std::vector<size_t> seg;
     for (size_t i = 0; i < ops.size(); ++i) {
         const HostOp& op = ops[i];

@@ -398,3 +398,43 @@ class CompiledCircuit:
         if h is not None and _lib._lib is not None:
             _lib._lib.qhip_plan_destroy(h)
             self._h = None
+
+
+def inspect_schedule(commands: Sequence[Command], n_qubits: int, *, fusion_max_qubits: int = 3,
+                     fusion_min_qubits: int = 12, tile_qubits: int = 0, low_qubits: int = 0,
+                     precision: str = "complex128") -> list:
+    """The device schedule of ``commands`` as [(pass, Command)] in execution
+    order, computed on the host only (qhip_schedule_inspect): dense and
+    diagonal blocks as Custom, mask phases as MCZ, scalar phases as GPhase."""
+    lib = _lib.load()
+    cfg = config_struct(fusion_max_qubits, fusion_min_qubits, tile_qubits, low_qubits, 0, -1)
+    ps = pack_stream(commands)
+    h = C.c_void_p()
+    _lib.check(lib.qhip_schedule_inspect(C.byref(cfg), ps.ref, int(n_qubits),
+                                         QHIP_C128 if precision == "complex128" else QHIP_C64, C.byref(h)))
+    try:
+        n = C.c_int64()
+        _lib.check(lib.qhip_fused_count(h, C.byref(n)))
+        out = []
+        qs = np.zeros(64, np.uint32)
+        pr = np.zeros(4)
+        m = np.zeros(256 * 256, np.complex128)
+        for i in range(n.value):
+            g, k, p = C.c_int32(), C.c_int32(), C.c_int32()
+            _lib.check(lib.qhip_fused_get(h, i, C.byref(g), C.byref(k), qs.ctypes.data, pr.ctypes.data,
+                                          m.ctypes.data, m.size))
+            _lib.check(lib.qhip_fused_pass(h, i, C.byref(p)))
+            q = [int(x) for x in qs[: k.value]]
+            if g.value == int(GateType.GPhase):
+                cmd = Command(GateType.GPhase, params=[math.atan2(pr[1], pr[0])])
+            elif g.value == int(GateType.MCZ):
+                if abs(complex(pr[0], pr[1]) + 1) > 1e-15:
+                    raise AssertionError("mask phase other than -1")
+                cmd = Command(GateType.MCZ, qubits=q)
+            else:
+                d = 1 << k.value
+                cmd = Command(GateType.Custom, qubits=q, unitary=m[: d * d].reshape(d, d).copy())
+            out.append((p.value, cmd))
+        return out
+    finally:
+        lib.qhip_fused_destroy(h)

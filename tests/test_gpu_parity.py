@@ -84,6 +84,8 @@ def random_circuit(n, n_gates, seed, with_custom=True):
         if g in (GateType.CCX, GateType.CSWAP):
             nq = 3
         elif g == GateType.MCZ:
+            if n < 2:
+                continue
             nq = int(rng.integers(2, min(n, 6) + 1))
         elif g == GateType.GPhase:
             nq = 0
