@@ -341,10 +341,12 @@ class Command:
         qs, ps = list(qubits), [Param._coerce(p) for p in params]
         # Positional form of the reference constructors: Command(g, q),
         # Command(g, q0, q1), Command(g, q, param), Command(g, q0, q1, param).
+        positional_param = False
         for a in args:
             if isinstance(a, (Param, float, np.floating)):
                 ps.append(Param._coerce(a))
-            elif isinstance(a, (int, np.integer)) and not ps:
+                positional_param = True
+            elif isinstance(a, (int, np.integer)) and not positional_param:
                 qs.append(int(a))
             else:
                 raise TypeError(f"Command: unexpected argument {a!r}")
