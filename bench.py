@@ -64,19 +64,23 @@ def fused_apply(args, sim):
     widths = (args.fusion,) if args.fusion else (2, 3, 4, 5)
     tuned = tune(sim, cmds, state, widths=widths)
     best = tuned["best"]
+    for t in tuned["trials"]:
+        log(f"  trial k={t['fusion_max_qubits']} T={t['tile_qubits']} low={t['low_qubits']}: "
+            f"{t['ms']:.1f} ms, {t['passes']} passes, {t['fused_ops']} fused ops")
     log(f"autotune best {best}")
     cc = CompiledCircuit(sim, cmds, n, fusion_max_qubits=best["fusion_max_qubits"],
                          tile_qubits=best["tile_qubits"], low_qubits=best["low_qubits"])
     state.reset()
     for _ in range(args.warmup):
         cc.apply(state)
-    import torch
-    torch.cuda.synchronize()
+    # The library runs on its own HIP stream; qhip_synchronize drains it.
+    from openqarp_amd import _lib
+    _lib.check(_lib.load().qhip_synchronize())
     pass_ms = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         pass_ms.append(cc.apply_timed(state))
-    torch.cuda.synchronize()
+    _lib.check(_lib.load().qhip_synchronize())
     wall = (time.perf_counter() - t0) / args.steps
     passes = int(cc.stats["passes"])
     bytes_per_pass = 2 * 16 * (1 << n)

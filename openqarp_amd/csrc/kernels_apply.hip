@@ -41,6 +41,30 @@ __device__ __forceinline__ uint64_t tile_base(uint64_t t, uint64_t outer_mask) {
     return g;
 }
 
+// Tile loads / stores touch every amplitude exactly once per pass: marked
+// non-temporal so the streamed state does not evict reused data from L2.
+constexpr int kBatch = 8;
+__device__ __forceinline__ double2 stream_load(const double2* p) {
+    double2 v;
+    v.x = __builtin_nontemporal_load(&p->x);
+    v.y = __builtin_nontemporal_load(&p->y);
+    return v;
+}
+__device__ __forceinline__ float2 stream_load(const float2* p) {
+    float2 v;
+    v.x = __builtin_nontemporal_load(&p->x);
+    v.y = __builtin_nontemporal_load(&p->y);
+    return v;
+}
+__device__ __forceinline__ void stream_store(double2* p, double2 v) {
+    __builtin_nontemporal_store(v.x, &p->x);
+    __builtin_nontemporal_store(v.y, &p->y);
+}
+__device__ __forceinline__ void stream_store(float2* p, float2 v) {
+    __builtin_nontemporal_store(v.x, &p->x);
+    __builtin_nontemporal_store(v.y, &p->y);
+}
+
 template <typename A>
 __device__ __forceinline__ A cmad(A acc, A u, A x) {
     acc.x += u.x * x.x - u.y * x.y;
@@ -194,7 +218,29 @@ __global__ __launch_bounds__(kThreads) void tile_pass_kernel(
 #pragma unroll
     for (int b = 0; b < 8; ++b)
         if (b < P.T) dj |= static_cast<uint64_t>((threadIdx.x >> b) & 1u) << P.qpos[b];
-    {
+    // Loads are issued kBatch at a time before any LDS write, so each thread
+    // keeps kBatch HBM requests in flight (tiles of >= 2^11 amplitudes; the
+    // R = 2^(T-8) rows per thread are a multiple of kBatch there).
+    const uint32_t R = L / kThreads;
+    if (R >= kBatch) {
+        uint64_t dr = 0;
+        for (uint32_t r0 = 0; r0 < R; r0 += kBatch) {
+            A v0[kBatch], v1[kBatch];
+#pragma unroll
+            for (int i = 0; i < kBatch; ++i) {
+                const uint64_t g = gbase | dj | dr;
+                v0[i] = stream_load(psi + g);
+                if (TWO) v1[i] = stream_load(phi + g);
+                dr = (dr - P.hmask) & P.hmask;
+            }
+#pragma unroll
+            for (int i = 0; i < kBatch; ++i) {
+                const uint32_t l = threadIdx.x + (r0 + i) * kThreads;
+                s0[l] = v0[i];
+                if (TWO) s1[l] = v1[i];
+            }
+        }
+    } else {
         uint64_t dr = 0;
         for (uint32_t l = threadIdx.x; l < L; l += kThreads) {
             const uint64_t g = gbase | dj | dr;
@@ -277,7 +323,25 @@ __global__ __launch_bounds__(kThreads) void tile_pass_kernel(
         ip_out[static_cast<int64_t>(k) * n_tiles + static_cast<int64_t>(t)] = v;
     }
 
-    {
+    if (R >= kBatch) {
+        uint64_t dr = 0;
+        for (uint32_t r0 = 0; r0 < R; r0 += kBatch) {
+            A v0[kBatch], v1[kBatch];
+#pragma unroll
+            for (int i = 0; i < kBatch; ++i) {
+                const uint32_t l = threadIdx.x + (r0 + i) * kThreads;
+                v0[i] = s0[l];
+                if (TWO) v1[i] = s1[l];
+            }
+#pragma unroll
+            for (int i = 0; i < kBatch; ++i) {
+                const uint64_t g = gbase | dj | dr;
+                stream_store(psi + g, v0[i]);
+                if (TWO) stream_store(phi + g, v1[i]);
+                dr = (dr - P.hmask) & P.hmask;
+            }
+        }
+    } else {
         uint64_t dr = 0;
         for (uint32_t l = threadIdx.x; l < L; l += kThreads) {
             const uint64_t g = gbase | dj | dr;
