@@ -62,7 +62,8 @@ def fused_apply(args, sim):
     state = sim.new_state(n)
     log(f"brickwork n={n} layers={args.layers}: {len(cmds)} gates; autotuning")
     widths = (args.fusion,) if args.fusion else (2, 3, 4, 5)
-    tuned = tune(sim, cmds, state, widths=widths)
+    tuned = tune(sim, cmds, state, widths=widths, tiles=(args.tile,) if args.tile else None,
+                 lows=(args.low,) if args.low else (4, 5))
     best = tuned["best"]
     for t in tuned["trials"]:
         log(f"  trial k={t['fusion_max_qubits']} T={t['tile_qubits']} low={t['low_qubits']}: "
@@ -205,6 +206,8 @@ def main(argv=None) -> int:
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--layers", type=int, default=30)
     ap.add_argument("--fusion", type=int, default=0, help="fix the fusion width (0: autotune 2..5)")
+    ap.add_argument("--tile", type=int, default=0, help="fix the tile qubits (0: autotune)")
+    ap.add_argument("--low", type=int, default=0, help="fix the contiguous low qubits (0: autotune)")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-expectation", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")

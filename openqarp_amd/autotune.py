@@ -29,24 +29,29 @@ def fingerprint(commands: Sequence[Command]) -> str:
     return h.hexdigest()
 
 
-def candidates(n: int, precision: str, widths: Iterable[int] = (2, 3, 4, 5)) -> list:
+def candidates(n: int, precision: str, widths: Iterable[int] = (2, 3, 4, 5),
+               tiles: Optional[Iterable[int]] = None, lows: Iterable[int] = (4, 5)) -> list:
     base_t = 12 if precision == "complex128" else 13
+    tiles = tuple(tiles) if tiles else (base_t - 1, base_t)
     out = []
     for k in widths:
-        for T in (base_t - 1, base_t):
-            for low in (4, 5):
+        for T in tiles:
+            for low in lows:
                 if T <= n:
                     out.append((k, T, low))
     return out or [(3, min(n, base_t), 4)]
 
 
 def tune(sim: QarpSimulator, commands: Sequence[Command], state: DeviceState,
-         widths: Iterable[int] = (2, 3, 4, 5), repeats: int = 1) -> dict:
-    key = (state.n_qubits, sim.precision, fingerprint(commands), tuple(widths))
+         widths: Iterable[int] = (2, 3, 4, 5), repeats: int = 1, tiles: Optional[Iterable[int]] = None,
+         lows: Iterable[int] = (4, 5)) -> dict:
+    widths, lows = tuple(widths), tuple(lows)
+    tiles = tuple(tiles) if tiles else None
+    key = (state.n_qubits, sim.precision, fingerprint(commands), widths, tiles, lows)
     if key in _cache:
         return _cache[key]
     results = []
-    for k, T, low in candidates(state.n_qubits, sim.precision, widths):
+    for k, T, low in candidates(state.n_qubits, sim.precision, widths, tiles, lows):
         cc = CompiledCircuit(sim, commands, state.n_qubits, fusion_max_qubits=k, tile_qubits=T,
                              low_qubits=low)
         cc.apply(state)                         # warm (first launch of each kernel)
