@@ -21,6 +21,8 @@
 // workgroup and every unitary op acts on both.
 #include "common.h"
 
+#include <cstdlib>
+
 namespace qhip {
 
 template <typename A> struct Real;
@@ -196,61 +198,14 @@ __device__ __forceinline__ double2 ip_dense(const A* s0, const A* s1, int T, con
     return acc;
 }
 
-// Tile of pass P: load, run the pass's ops in LDS, store.  KMAX bounds the
-// widest dense op of the pass (register budget of the group vectors).
+// The ops of pass P over the tile in LDS (s0; s1 too in two-state mode),
+// a barrier after each.  Returns the number of inner-product ops, whose
+// per-wave partials are left in wave_ip.
 template <typename A, bool TWO, int KMAX>
-__global__ __launch_bounds__(kThreads) void tile_pass_kernel(
-    A* __restrict__ psi, A* __restrict__ phi, const PassDesc P, const DevOp* __restrict__ ops,
-    const A* __restrict__ coefs, double2* __restrict__ ip_out, int64_t n_tiles) {
-    extern __shared__ unsigned char smem_raw[];
-    A* s0 = reinterpret_cast<A*>(smem_raw);
-    A* s1 = s0 + (1u << P.T);
-    double2* wave_ip = reinterpret_cast<double2*>(s0 + (TWO ? 2u : 1u) * (1u << P.T));
+__device__ __forceinline__ int run_ops(A* s0, A* s1, const PassDesc& P, uint64_t gbase,
+                                       const DevOp* __restrict__ ops, const A* __restrict__ coefs,
+                                       double2* wave_ip) {
     const uint32_t L = 1u << P.T;
-    const uint64_t t = blockIdx.x;
-    const uint64_t full = P.n >= 64 ? ~0ull : ((1ull << P.n) - 1ull);
-    const uint64_t gbase = tile_base(t, full & ~P.tmask);
-
-    // Global index of local l = threadIdx.x + 256 r: the thread's own low
-    // eight local bits are deposited once; the subsets of hmask enumerate
-    // the deposits of r in increasing order.
-    uint64_t dj = 0;
-#pragma unroll
-    for (int b = 0; b < 8; ++b)
-        if (b < P.T) dj |= static_cast<uint64_t>((threadIdx.x >> b) & 1u) << P.qpos[b];
-    // Loads are issued kBatch at a time before any LDS write, so each thread
-    // keeps kBatch HBM requests in flight (tiles of >= 2^11 amplitudes; the
-    // R = 2^(T-8) rows per thread are a multiple of kBatch there).
-    const uint32_t R = L / kThreads;
-    if (R >= kBatch) {
-        uint64_t dr = 0;
-        for (uint32_t r0 = 0; r0 < R; r0 += kBatch) {
-            A v0[kBatch], v1[kBatch];
-#pragma unroll
-            for (int i = 0; i < kBatch; ++i) {
-                const uint64_t g = gbase | dj | dr;
-                v0[i] = stream_load(psi + g);
-                if (TWO) v1[i] = stream_load(phi + g);
-                dr = (dr - P.hmask) & P.hmask;
-            }
-#pragma unroll
-            for (int i = 0; i < kBatch; ++i) {
-                const uint32_t l = threadIdx.x + (r0 + i) * kThreads;
-                s0[l] = v0[i];
-                if (TWO) s1[l] = v1[i];
-            }
-        }
-    } else {
-        uint64_t dr = 0;
-        for (uint32_t l = threadIdx.x; l < L; l += kThreads) {
-            const uint64_t g = gbase | dj | dr;
-            s0[l] = psi[g];
-            if (TWO) s1[l] = phi[g];
-            dr = (dr - P.hmask) & P.hmask;
-        }
-    }
-    __syncthreads();
-
     int n_ip = 0;
     for (int o = P.op_begin; o < P.op_end; ++o) {
         const DevOp* op = ops + o;
@@ -312,6 +267,66 @@ __global__ __launch_bounds__(kThreads) void tile_pass_kernel(
         __syncthreads();
     }
 
+    return n_ip;
+}
+
+// Tile of pass P: load, run the pass's ops in LDS, store.  KMAX bounds the
+// widest dense op of the pass (register budget of the group vectors).
+template <typename A, bool TWO, int KMAX>
+__global__ __launch_bounds__(kThreads) void tile_pass_kernel(
+    A* __restrict__ psi, A* __restrict__ phi, const PassDesc P, const DevOp* __restrict__ ops,
+    const A* __restrict__ coefs, double2* __restrict__ ip_out, int64_t n_tiles) {
+    extern __shared__ unsigned char smem_raw[];
+    A* s0 = reinterpret_cast<A*>(smem_raw);
+    A* s1 = s0 + (1u << P.T);
+    double2* wave_ip = reinterpret_cast<double2*>(s0 + (TWO ? 2u : 1u) * (1u << P.T));
+    const uint32_t L = 1u << P.T;
+    const uint64_t t = blockIdx.x;
+    const uint64_t full = P.n >= 64 ? ~0ull : ((1ull << P.n) - 1ull);
+    const uint64_t gbase = tile_base(t, full & ~P.tmask);
+
+    // Global index of local l = threadIdx.x + 256 r: the thread's own low
+    // eight local bits are deposited once; the subsets of hmask enumerate
+    // the deposits of r in increasing order.
+    uint64_t dj = 0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+        if (b < P.T) dj |= static_cast<uint64_t>((threadIdx.x >> b) & 1u) << P.qpos[b];
+    // Loads are issued kBatch at a time before any LDS write, so each thread
+    // keeps kBatch HBM requests in flight (tiles of >= 2^11 amplitudes; the
+    // R = 2^(T-8) rows per thread are a multiple of kBatch there).
+    const uint32_t R = L / kThreads;
+    if (R >= kBatch) {
+        uint64_t dr = 0;
+        for (uint32_t r0 = 0; r0 < R; r0 += kBatch) {
+            A v0[kBatch], v1[kBatch];
+#pragma unroll
+            for (int i = 0; i < kBatch; ++i) {
+                const uint64_t g = gbase | dj | dr;
+                v0[i] = stream_load(psi + g);
+                if (TWO) v1[i] = stream_load(phi + g);
+                dr = (dr - P.hmask) & P.hmask;
+            }
+#pragma unroll
+            for (int i = 0; i < kBatch; ++i) {
+                const uint32_t l = threadIdx.x + (r0 + i) * kThreads;
+                s0[l] = v0[i];
+                if (TWO) s1[l] = v1[i];
+            }
+        }
+    } else {
+        uint64_t dr = 0;
+        for (uint32_t l = threadIdx.x; l < L; l += kThreads) {
+            const uint64_t g = gbase | dj | dr;
+            s0[l] = psi[g];
+            if (TWO) s1[l] = phi[g];
+            dr = (dr - P.hmask) & P.hmask;
+        }
+    }
+    __syncthreads();
+
+    const int n_ip = run_ops<A, TWO, KMAX>(s0, s1, P, gbase, ops, coefs, wave_ip);
+
     if (TWO && static_cast<int>(threadIdx.x) < n_ip) {
         // Sum the per-wave values in fixed order: partial k of this tile.
         const int k = threadIdx.x;
@@ -349,6 +364,57 @@ __global__ __launch_bounds__(kThreads) void tile_pass_kernel(
             if (TWO) phi[g] = s1[l];
             dr = (dr - P.hmask) & P.hmask;
         }
+    }
+}
+
+// Persistent, software-pipelined variant for single-state passes with
+// R = 2^(T-8) rows per thread: each workgroup walks tiles blockIdx.x,
+// blockIdx.x + gridDim.x, ...; the next tile's HBM loads are issued into
+// registers before the current tile's ops run in LDS, so load latency hides
+// behind the compute and the store of the current tile.
+template <typename A, int KMAX, int R>
+__global__ __launch_bounds__(kThreads) void stream_pass_kernel(
+    A* __restrict__ psi, const PassDesc P, const DevOp* __restrict__ ops, const A* __restrict__ coefs,
+    int64_t n_tiles) {
+    extern __shared__ unsigned char smem_raw[];
+    A* s0 = reinterpret_cast<A*>(smem_raw);
+    const uint64_t full = P.n >= 64 ? ~0ull : ((1ull << P.n) - 1ull);
+    const uint64_t outer = full & ~P.tmask;
+    uint64_t dj = 0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) dj |= static_cast<uint64_t>((threadIdx.x >> b) & 1u) << P.qpos[b];
+    uint64_t dr_tab[R];
+    {
+        uint64_t dr = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            dr_tab[r] = dj | dr;
+            dr = (dr - P.hmask) & P.hmask;
+        }
+    }
+    int64_t t = blockIdx.x;
+    if (t >= n_tiles) return;
+    A pre[R];
+    uint64_t gbase = tile_base(static_cast<uint64_t>(t), outer);
+#pragma unroll
+    for (int r = 0; r < R; ++r) pre[r] = stream_load(psi + (gbase | dr_tab[r]));
+    while (true) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) s0[threadIdx.x + r * kThreads] = pre[r];
+        __syncthreads();
+        const int64_t tn = t + gridDim.x;
+        const uint64_t gnext = tn < n_tiles ? tile_base(static_cast<uint64_t>(tn), outer) : 0;
+        if (tn < n_tiles) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) pre[r] = stream_load(psi + (gnext | dr_tab[r]));
+        }
+        (void)run_ops<A, false, KMAX>(s0, nullptr, P, gbase, ops, coefs, nullptr);
+#pragma unroll
+        for (int r = 0; r < R; ++r) stream_store(psi + (gbase | dr_tab[r]), s0[threadIdx.x + r * kThreads]);
+        if (tn >= n_tiles) break;
+        __syncthreads();   // LDS reads of this tile done before the next tile lands
+        t = tn;
+        gbase = gnext;
     }
 }
 
@@ -472,6 +538,15 @@ __global__ void convert_to_c128(const A* __restrict__ in, double2* __restrict__ 
 
 // ── host launchers ──────────────────────────────────────────────────────────
 
+// QHIP_PIPELINE=0 selects the one-tile-per-workgroup kernel for every pass.
+static bool tile_pipelining() {
+    static const bool on = [] {
+        const char* e = std::getenv("QHIP_PIPELINE");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 static int grid_for(uint64_t n, int per_block = kThreads) {
     const uint64_t blocks = (n + per_block - 1) / per_block;
     const uint64_t cap = static_cast<uint64_t>(std::max(1, ctx().n_cu)) * 32;
@@ -545,23 +620,67 @@ void state_convert_download(const qhip_state* s, double* host, uint64_t n) {
     QHIP_CHECK(hipStreamSynchronize(C.stream));
 }
 
+template <typename A, bool TWO, int KMAX>
+static const void* tile_fn() {
+    return reinterpret_cast<const void*>(&tile_pass_kernel<A, TWO, KMAX>);
+}
+
+template <typename A, int KMAX, int R>
+static void launch_stream(const PassDesc& P, A* psi, const DevOp* d_ops, const A* d_coefs, int64_t n_tiles,
+                          size_t lds) {
+    auto& C = ctx();
+    static int per_cu = 0;   // resident workgroups per CU for this instantiation
+    if (per_cu == 0) {
+        QHIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&stream_pass_kernel<A, KMAX, R>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        int blocks = 0;
+        QHIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &blocks, reinterpret_cast<const void*>(&stream_pass_kernel<A, KMAX, R>), kThreads, lds));
+        per_cu = std::max(1, blocks);
+    }
+    const int64_t grid = std::min<int64_t>(n_tiles, static_cast<int64_t>(C.n_cu) * per_cu);
+    hipLaunchKernelGGL((stream_pass_kernel<A, KMAX, R>), dim3(static_cast<uint32_t>(grid)), dim3(kThreads), lds,
+                       C.stream, psi, P, d_ops, d_coefs, n_tiles);
+    QHIP_CHECK(hipGetLastError());
+}
+
+template <typename A, int KMAX>
+static bool try_stream(const PassDesc& P, A* psi, const DevOp* d_ops, const A* d_coefs, int64_t n_tiles,
+                       size_t lds) {
+    switch (P.T) {
+        case 11: launch_stream<A, KMAX, 8>(P, psi, d_ops, d_coefs, n_tiles, lds); return true;
+        case 12: launch_stream<A, KMAX, 16>(P, psi, d_ops, d_coefs, n_tiles, lds); return true;
+        case 13: launch_stream<A, KMAX, 32>(P, psi, d_ops, d_coefs, n_tiles, lds); return true;
+        default: return false;
+    }
+}
+
 template <typename A, bool TWO>
 static void launch_tile(const PassDesc& P, A* psi, A* phi, double2* ip_part, const DevOp* d_ops, const A* d_coefs,
                         int64_t n_tiles, size_t lds) {
     auto& C = ctx();
+    const int km = P.kmax <= 2 ? 2 : std::min(P.kmax, 5);
+    if (!TWO && tile_pipelining()) {
+        bool done = false;
+        switch (km) {
+            case 2: done = try_stream<A, 2>(P, psi, d_ops, d_coefs, n_tiles, lds); break;
+            case 3: done = try_stream<A, 3>(P, psi, d_ops, d_coefs, n_tiles, lds); break;
+            case 4: done = try_stream<A, 4>(P, psi, d_ops, d_coefs, n_tiles, lds); break;
+            default: done = try_stream<A, 5>(P, psi, d_ops, d_coefs, n_tiles, lds); break;
+        }
+        if (done) return;
+    }
     const dim3 grid(static_cast<uint32_t>(n_tiles)), block(kThreads);
     // Above 64 KiB of dynamic LDS a kernel must opt in (160 KiB per CU).
     static bool opted[4] = {false, false, false, false};
-    const int kk = (P.kmax <= 2 ? 2 : std::min(P.kmax, 5)) - 2;
+    const int kk = km - 2;
     if (lds > 65536 && !opted[kk]) {
-        const void* fn = kk == 0 ? reinterpret_cast<const void*>(&tile_pass_kernel<A, TWO, 2>)
-                       : kk == 1 ? reinterpret_cast<const void*>(&tile_pass_kernel<A, TWO, 3>)
-                       : kk == 2 ? reinterpret_cast<const void*>(&tile_pass_kernel<A, TWO, 4>)
-                                 : reinterpret_cast<const void*>(&tile_pass_kernel<A, TWO, 5>);
+        const void* fn = kk == 0 ? tile_fn<A, TWO, 2>() : kk == 1 ? tile_fn<A, TWO, 3>()
+                       : kk == 2 ? tile_fn<A, TWO, 4>() : tile_fn<A, TWO, 5>();
         QHIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         opted[kk] = true;
     }
-    switch (P.kmax <= 2 ? 2 : P.kmax) {
+    switch (km) {
         case 2: hipLaunchKernelGGL((tile_pass_kernel<A, TWO, 2>), grid, block, lds, C.stream, psi, phi, P, d_ops,
                                    d_coefs, ip_part, n_tiles); break;
         case 3: hipLaunchKernelGGL((tile_pass_kernel<A, TWO, 3>), grid, block, lds, C.stream, psi, phi, P, d_ops,
