@@ -538,11 +538,13 @@ __global__ void convert_to_c128(const A* __restrict__ in, double2* __restrict__ 
 
 // ── host launchers ──────────────────────────────────────────────────────────
 
-// QHIP_PIPELINE=0 selects the one-tile-per-workgroup kernel for every pass.
+// QHIP_PIPELINE=1 selects the persistent pipelined kernel for single-state
+// passes of 2^11..2^13 tiles; the default one-tile-per-workgroup kernel keeps
+// more workgroups resident per CU, which measured faster.
 static bool tile_pipelining() {
     static const bool on = [] {
         const char* e = std::getenv("QHIP_PIPELINE");
-        return !(e && e[0] == '0');
+        return e && e[0] == '1';
     }();
     return on;
 }
