@@ -146,10 +146,18 @@ struct DevOp {          // 64 bytes, one per op per pass
     int8_t  spos[8];    // dense/IP: local positions sorted ascending
     int64_t coef;       // first coefficient in the pass coefficient table
     uint64_t lmask;     // maskphase: local-position mask; IP ops: accumulator slot
-    uint64_t gmask;     // maskphase: global-bit mask outside the tile
-    int64_t reserved[2];
+    uint64_t gmask;     // maskphase: global-bit mask outside the tile;
+                        // dense/IP dense: local positions of group bits >= 8
+    int8_t  gpos[16];   // dense/IP dense: local position of group-number bit b
 };
 static_assert(sizeof(DevOp) == 64, "DevOp layout");
+
+// Inner-product ops per pass (per-wave LDS slots of the adjoint tile kernel).
+constexpr int kMaxIpPerPass = 64;
+
+// Widest tile: 2^T amplitudes of each state must fit the LDS of one workgroup
+// (160 KiB per CU; two states share it in adjoint passes).
+inline int max_tile_qubits(int precision) { return precision == QHIP_C128 ? 12 : 13; }
 
 struct PassDesc {
     int32_t n;           // register width
@@ -198,6 +206,12 @@ void run_compiled(const CompiledPlan& cp, qhip_state* s, std::vector<double>* pa
 void free_compiled(CompiledPlan* cp);
 
 // ── device entry points (kernels_*.hip) ────────────────────────────────────
+// One tile pass (T >= 1) of psi, or of psi and phi together when phi is set,
+// on `stream`: tile_c128.hip / tile_c64.hip.
+void launch_tile_pass(const PassDesc& P, double2* psi, double2* phi, double2* ip_part, const DevOp* d_ops,
+                      const double2* d_coefs, int64_t n_tiles, hipStream_t stream);
+void launch_tile_pass(const PassDesc& P, float2* psi, float2* phi, double2* ip_part, const DevOp* d_ops,
+                      const float2* d_coefs, int64_t n_tiles, hipStream_t stream);
 void launch_plan(const Plan& plan, qhip_state* s, qhip_state* phi, void* d_ip_partials,
                  int n_slots);
 void state_reset(qhip_state* s);

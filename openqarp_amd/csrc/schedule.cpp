@@ -17,9 +17,11 @@
 namespace qhip {
 
 int default_tile_qubits(int precision, bool two_states) {
-    // 64 KiB of LDS per workgroup: two workgroups share a CU's 160 KiB.
-    const int amp_log = precision == QHIP_C128 ? 4 : 3;   // log2 bytes per amplitude
-    return 16 - amp_log - (two_states ? 1 : 0);
+    // 32 KiB of LDS per state (2^11 complex128 or 2^12 complex64
+    // amplitudes): several workgroups share a CU's 160 KiB, so one tile's
+    // ops overlap other tiles' loads and stores.
+    (void)two_states;
+    return precision == QHIP_C128 ? 11 : 12;
 }
 
 static double op_flops(const HostOp& op) {
@@ -32,7 +34,41 @@ static double op_flops(const HostOp& op) {
     }
 }
 
-static void emit_op(Plan& plan, const HostOp& op, const int* local_of, int n) {
+// Local positions of a dense op's group-number bits (the tile positions that
+// are not operands).  The tile kernel keeps LDS slot l at
+// l ^ (((l >> 4) ^ (l >> 8)) & 15), so position p < 12 lands on slot bit
+// p % 4.  Group bits 0-3 index lanes 0-15 of a wavefront: they take one free
+// position per residue 0..3 where the operands leave one, so those 16 lanes
+// touch 16 distinct 16-B slots modulo 16 (no bank conflict); the other
+// positions follow in ascending order.
+static void group_positions(DevOp& d, int T) {
+    bool operand[64] = {};
+    for (int b = 0; b < d.k; ++b) operand[d.pos[b]] = true;
+    std::vector<int> free;
+    for (int p = 0; p < T; ++p)
+        if (!operand[p]) free.push_back(p);
+    std::vector<int> order;
+    std::vector<bool> used(free.size(), false);
+    for (int r = 0; r < 4; ++r)
+        for (size_t i = 0; i < free.size(); ++i)
+            if (!used[i] && free[i] < 12 && free[i] % 4 == r) {
+                order.push_back(free[i]);
+                used[i] = true;
+                break;
+            }
+    for (size_t i = 0; i < free.size(); ++i)
+        if (!used[i]) order.push_back(free[i]);
+    if (order.size() > sizeof(d.gpos)) fail(QHIP_RUNTIME, "schedule: tile too wide for the group map");
+    d.gmask = 0;
+    for (size_t b = 0; b < order.size(); ++b) {
+        d.gpos[b] = static_cast<int8_t>(order[b]);
+        if (b >= 8) d.gmask |= uint64_t{1} << order[b];
+    }
+    // Positions of bits >= 8 ascend, so the kernel can step through their
+    // deposits with (x - mask) & mask.
+}
+
+static void emit_op(Plan& plan, const HostOp& op, const int* local_of, int n, int T) {
     DevOp d{};
     d.kind = op.kind;
     d.k = op.k;
@@ -44,6 +80,7 @@ static void emit_op(Plan& plan, const HostOp& op, const int* local_of, int n) {
             for (int b = 0; b < op.k; ++b) sorted[b] = d.pos[b] = static_cast<int8_t>(local_of[op.q[b]]);
             std::sort(sorted, sorted + op.k);
             for (int b = 0; b < op.k; ++b) d.spos[b] = static_cast<int8_t>(sorted[b]);
+            group_positions(d, T);
             plan.coefs.insert(plan.coefs.end(), op.m.begin(), op.m.end());
             ++plan.dense_ops;
             break;
@@ -82,7 +119,7 @@ static void schedule_segment(Plan& plan, const std::vector<HostOp>& ops, const s
                              int T, int low, const ScheduleOptions& opt) {
     const uint64_t low_mask = low >= 64 ? ~uint64_t{0} : ((uint64_t{1} << low) - 1);
     const uint64_t all_mask = n >= 64 ? ~uint64_t{0} : ((uint64_t{1} << n) - 1);
-    const int max_ip = 64;
+    const int max_ip = kMaxIpPerPass;
     std::vector<size_t> remaining = seg;
     while (!remaining.empty()) {
         // The first op of a pass always fits: it may displace forced low
@@ -147,7 +184,7 @@ static void schedule_segment(Plan& plan, const std::vector<HostOp>& ops, const s
         P.op_begin = static_cast<int32_t>(plan.ops.size());
         P.kmax = 0;
         for (size_t idx : chosen) {
-            emit_op(plan, ops[idx], local_of, n);
+            emit_op(plan, ops[idx], local_of, n, T);
             if (ops[idx].needs_local()) P.kmax = std::max(P.kmax, ops[idx].k);
         }
         P.op_end = static_cast<int32_t>(plan.ops.size());
@@ -160,6 +197,9 @@ static void schedule_segment(Plan& plan, const std::vector<HostOp>& ops, const s
 Plan schedule(const std::vector<HostOp>& ops, int n, int precision, const ScheduleOptions& opt) {
     Plan plan;
     int T = opt.tile_qubits > 0 ? opt.tile_qubits : default_tile_qubits(precision, opt.two_states);
+    if (T > max_tile_qubits(precision))
+        fail(QHIP_INVALID, "tile_qubits " + std::to_string(T) + " exceeds the LDS limit of " +
+                               std::to_string(max_tile_qubits(precision)) + " for this precision");
     T = std::max(T, std::min(kMaxTileK, n));
     T = std::min(T, n);
     int kdense = 0;
@@ -181,7 +221,7 @@ Plan schedule(const std::vector<HostOp>& ops, int n, int precision, const Schedu
             int local_of[64];
             for (int q = 0; q < 64; ++q) local_of[q] = q;   // positions are global qubits
             P.op_begin = static_cast<int32_t>(plan.ops.size());
-            emit_op(plan, op, local_of, n);
+            emit_op(plan, op, local_of, n, 0);   // no tile: the wide kernel reads spos
             P.op_end = static_cast<int32_t>(plan.ops.size());
             plan.passes.push_back(P);
             continue;

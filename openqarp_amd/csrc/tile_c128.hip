@@ -1,0 +1,12 @@
+// tile_c128.hip — tile-pass kernels for complex128 states (tile arrays of
+// 2^10 .. 2^12 amplitudes: 16 / 32 / 64 KiB per state).
+#include "tile_kernel.h"
+
+namespace qhip {
+
+void launch_tile_pass(const PassDesc& P, double2* psi, double2* phi, double2* ip_part, const DevOp* d_ops,
+                      const double2* d_coefs, int64_t n_tiles, hipStream_t stream) {
+    tile::launch<double2, 10>(P, psi, phi, ip_part, d_ops, d_coefs, n_tiles, stream);
+}
+
+}  // namespace qhip

@@ -1,0 +1,554 @@
+// tile_kernel.h — the tile-pass gate kernel (gfx950), instantiated once per
+// amplitude type by tile_c128.hip and tile_c64.hip.
+//
+// Tile pass: one workgroup of 256 threads per tile.  The tile holds 2^T
+// amplitudes in a static LDS array sized for 2^TB >= 2^T (TB is a template
+// parameter, so the array's address is a link-time constant and every LDS
+// access is one XOR of a per-thread and a wave-uniform byte offset); local
+// index l maps to the global index by depositing l's bits onto the pass's
+// tile qubits and the tile number's bits onto the remaining qubits.  Load and
+// store move 16-B (c128) or 8-B (c64) elements with consecutive threads on
+// consecutive low qubits, so every wavefront reads whole 2^low-amplitude
+// runs.  In between, each op of the pass runs out of LDS with a barrier
+// after it:
+//   dense   — every thread takes groups of 2^k amplitudes (the group number's
+//             bits deposited on the non-operand positions, in an order chosen
+//             on the host so that 16 neighbouring lanes hit 16 distinct LDS
+//             slots modulo 16), multiplies the 2^k x 2^k matrix (read
+//             through the scalar cache: the indices are wave-uniform) and
+//             writes the group back; for k <= 2 two groups are in flight;
+//   diag    — per amplitude, phase d[bits of the full index at the operands];
+//   mask    — per amplitude, phase where every mask bit is set;
+//   scalar  — global phase;
+//   ip_*    — (adjoint only) <phi| G |psi> over the tile, wave-reduced with
+//             lane shuffles into a per-wave LDS slot.
+// In two-state mode (adjoint) the tile of psi and the tile of phi share the
+// workgroup and every unitary op acts on both.
+//
+// LDS layout: amplitude l of state z sits at element z * 2^TB + swz(l) with
+// swz(l) = l ^ (((l >> 4) ^ (l >> 8)) & 15).  swz is linear over XOR and
+// only changes bits 0-3, so the slot of (thread bits) | (uniform bits) is
+// swz(thread bits) ^ swz(uniform bits).
+#pragma once
+
+#include <cstddef>
+
+#include "common.h"
+
+namespace qhip {
+namespace tile {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+constexpr int kBatch = 8;
+
+template <typename A> struct Amp;
+template <> struct Amp<double2> { using R = double; static constexpr int kShift = 4; };
+template <> struct Amp<float2> { using R = float; static constexpr int kShift = 3; };
+
+__host__ __device__ constexpr uint32_t swz(uint32_t l) { return l ^ (((l >> 4) ^ (l >> 8)) & 15u); }
+
+template <typename A>
+__device__ __forceinline__ A& at(A* s, uint32_t byte_offset) {
+    return *reinterpret_cast<A*>(reinterpret_cast<char*>(s) + byte_offset);
+}
+
+// Deposit of the tile number onto the non-tile qubits.
+__device__ __forceinline__ uint64_t tile_base(uint64_t t, uint64_t outer_mask) {
+    uint64_t g = 0;
+    while (outer_mask) {
+        const uint64_t low = outer_mask & (~outer_mask + 1);
+        if (t & 1ull) g |= low;
+        t >>= 1;
+        outer_mask ^= low;
+    }
+    return g;
+}
+
+// Tile loads / stores touch every amplitude exactly once per pass: marked
+// non-temporal so the streamed state does not evict reused data from L2.
+__device__ __forceinline__ double2 stream_load(const double2* p) {
+    double2 v;
+    v.x = __builtin_nontemporal_load(&p->x);
+    v.y = __builtin_nontemporal_load(&p->y);
+    return v;
+}
+__device__ __forceinline__ float2 stream_load(const float2* p) {
+    float2 v;
+    v.x = __builtin_nontemporal_load(&p->x);
+    v.y = __builtin_nontemporal_load(&p->y);
+    return v;
+}
+__device__ __forceinline__ void stream_store(double2* p, double2 v) {
+    __builtin_nontemporal_store(v.x, &p->x);
+    __builtin_nontemporal_store(v.y, &p->y);
+}
+__device__ __forceinline__ void stream_store(float2* p, float2 v) {
+    __builtin_nontemporal_store(v.x, &p->x);
+    __builtin_nontemporal_store(v.y, &p->y);
+}
+
+// Complex multiply-accumulate as four explicit FMAs (the compiler otherwise
+// splits it into separate multiplies and adds).
+__device__ __forceinline__ double2 cmad(double2 acc, double2 u, double2 x) {
+    acc.x = __fma_rn(u.x, x.x, acc.x);
+    acc.x = __fma_rn(-u.y, x.y, acc.x);
+    acc.y = __fma_rn(u.x, x.y, acc.y);
+    acc.y = __fma_rn(u.y, x.x, acc.y);
+    return acc;
+}
+__device__ __forceinline__ float2 cmad(float2 acc, float2 u, float2 x) {
+    acc.x = __fmaf_rn(u.x, x.x, acc.x);
+    acc.x = __fmaf_rn(-u.y, x.y, acc.x);
+    acc.y = __fmaf_rn(u.x, x.y, acc.y);
+    acc.y = __fmaf_rn(u.y, x.x, acc.y);
+    return acc;
+}
+template <typename A>
+__device__ __forceinline__ A cmul(A a, A b) {
+    A z;
+    z.x = 0;
+    z.y = 0;
+    return cmad(z, a, b);
+}
+
+__device__ __forceinline__ double2 wave_sum(double2 v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        v.x += __shfl_xor(v.x, off);
+        v.y += __shfl_xor(v.y, off);
+    }
+    return v;
+}
+
+// One DevOp as read through the scalar cache: eight 8-B loads from a
+// wave-uniform address, byte fields unpacked with shifts (sub-dword loads
+// would go through the vector memory path, one round trip each).
+struct OpRegs {
+    int kind, k;
+    uint64_t pos, spos, gpos0, gpos1;   // packed int8 fields of DevOp
+    int64_t coef;
+    uint64_t lmask, gmask;
+    __device__ int p(int b) const { return static_cast<int>((pos >> (8 * b)) & 0xffu); }
+    __device__ int g(int b) const {
+        return static_cast<int>(((b < 8 ? gpos0 : gpos1) >> (8 * (b & 7))) & 0xffu);
+    }
+};
+static_assert(sizeof(DevOp) == 64 && offsetof(DevOp, pos) == 8 && offsetof(DevOp, spos) == 16 &&
+                  offsetof(DevOp, coef) == 24 && offsetof(DevOp, lmask) == 32 && offsetof(DevOp, gmask) == 40 &&
+                  offsetof(DevOp, gpos) == 48,
+              "DevOp layout read by load_op");
+
+__device__ __forceinline__ OpRegs load_op(const DevOp* __restrict__ op) {
+    const uint64_t* q = reinterpret_cast<const uint64_t*>(op);
+    OpRegs r;
+    const uint64_t w0 = q[0];
+    r.kind = static_cast<int>(static_cast<uint32_t>(w0));
+    r.k = static_cast<int>(w0 >> 32);
+    r.pos = q[1];
+    r.spos = q[2];
+    r.coef = static_cast<int64_t>(q[3]);
+    r.lmask = q[4];
+    r.gmask = q[5];
+    r.gpos0 = q[6];
+    r.gpos1 = q[7];
+    return r;
+}
+
+template <int K>
+__device__ __forceinline__ uint32_t local_offset(uint32_t x, const OpRegs& op) {
+    uint32_t o = 0;
+#pragma unroll
+    for (int b = 0; b < K; ++b) o |= ((x >> b) & 1u) << op.p(b);
+    return o;
+}
+
+// Byte offset (swizzled) of this thread's part of a dense op's group number:
+// thread bit b of the group number sits at local position gpos[b].
+template <typename A>
+__device__ __forceinline__ uint32_t thread_part(const OpRegs& op, int G) {
+    uint32_t tp = 0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+        if (b < G) tp |= ((threadIdx.x >> b) & 1u) << op.g(b);
+    return swz(tp) << Amp<A>::kShift;
+}
+
+// y = U x for one group held in registers, written back to LDS: row y goes
+// to byte offset a[y] (K <= 3), or to tp ^ h ^ slot(y) computed per row
+// (K >= 4, whose rows are not unrolled).
+template <typename A, int K>
+__device__ __forceinline__ void matvec_store(A* s, const uint32_t* a, uint32_t tp, uint32_t h, const OpRegs& op,
+                                             const A* in, const A* __restrict__ U) {
+    constexpr int D = 1 << K;
+    if constexpr (K <= 3) {
+#pragma unroll
+        for (int y = 0; y < D; ++y) {
+            A acc;
+            acc.x = 0;
+            acc.y = 0;
+#pragma unroll
+            for (int x = 0; x < D; ++x) acc = cmad(acc, U[y * D + x], in[x]);
+            at(s, a[y]) = acc;
+        }
+    } else {
+#pragma unroll 1
+        for (int y = 0; y < D; ++y) {
+            A acc;
+            acc.x = 0;
+            acc.y = 0;
+#pragma unroll
+            for (int x = 0; x < D; ++x) acc = cmad(acc, U[y * D + x], in[x]);
+            const uint32_t o = swz(local_offset<K>(static_cast<uint32_t>(y), op)) << Amp<A>::kShift;
+            at(s, tp ^ (h ^ o)) = acc;
+        }
+    }
+}
+
+// One dense k-qubit op over one state's LDS tile.  tp: this thread's byte
+// offset (thread_part | state offset); G = T - K group-number bits, whose
+// bits >= 8 sit on the local positions of op.gmask.
+template <typename A, int K>
+__device__ __forceinline__ void dense_groups(A* s, uint32_t tp, int G, const OpRegs& op, const A* __restrict__ U) {
+    constexpr int D = 1 << K;
+    constexpr int sh = Amp<A>::kShift;
+    if (G < 8 && threadIdx.x >= (1u << G)) return;
+    uint32_t so[D];
+#pragma unroll
+    for (int x = 0; x < D; ++x) so[x] = swz(local_offset<K>(x, op)) << sh;
+    const uint32_t hm = static_cast<uint32_t>(op.gmask);
+    const uint32_t iters = G > 8 ? 1u << (G - 8) : 1u;
+    uint32_t hi = 0;   // deposit of the current group number's bits >= 8
+    uint32_t i = 0;
+    if constexpr (K <= 2) {
+        for (; i + 1 < iters; i += 2) {
+            const uint32_t h0 = swz(hi) << sh;
+            hi = (hi - hm) & hm;
+            const uint32_t h1 = swz(hi) << sh;
+            hi = (hi - hm) & hm;
+            uint32_t a0[D], a1[D];
+            A in0[D], in1[D];
+#pragma unroll
+            for (int x = 0; x < D; ++x) {
+                a0[x] = tp ^ (h0 ^ so[x]);
+                a1[x] = tp ^ (h1 ^ so[x]);
+                in0[x] = at(s, a0[x]);
+                in1[x] = at(s, a1[x]);
+            }
+            matvec_store<A, K>(s, a0, tp, h0, op, in0, U);
+            matvec_store<A, K>(s, a1, tp, h1, op, in1, U);
+        }
+    }
+    for (; i < iters; ++i) {
+        const uint32_t h0 = swz(hi) << sh;
+        hi = (hi - hm) & hm;
+        uint32_t a0[D];
+        A in0[D];
+#pragma unroll
+        for (int x = 0; x < D; ++x) {
+            a0[x] = tp ^ (h0 ^ so[x]);
+            in0[x] = at(s, a0[x]);
+        }
+        matvec_store<A, K>(s, a0, tp, h0, op, in0, U);
+    }
+}
+
+template <typename A, int KMAX>
+__device__ __forceinline__ void dense_op(A* s, uint32_t tp, int G, const OpRegs& op, const A* __restrict__ U) {
+    switch (op.k) {
+        case 1: dense_groups<A, 1>(s, tp, G, op, U); break;
+        case 2: if constexpr (KMAX >= 2) dense_groups<A, 2>(s, tp, G, op, U); break;
+        case 3: if constexpr (KMAX >= 3) dense_groups<A, 3>(s, tp, G, op, U); break;
+        case 4: if constexpr (KMAX >= 4) dense_groups<A, 4>(s, tp, G, op, U); break;
+        case 5: if constexpr (KMAX >= 5) dense_groups<A, 5>(s, tp, G, op, U); break;
+        default: break;
+    }
+}
+
+__device__ __forceinline__ uint32_t diag_index(uint32_t l, uint64_t gbase, const OpRegs& op, int k) {
+    uint32_t idx = 0;
+    for (int b = 0; b < k; ++b) {
+        const int p = op.p(b);
+        const uint32_t bit = p < 32 ? (l >> p) & 1u : static_cast<uint32_t>((gbase >> (p - 32)) & 1ull);
+        idx |= bit << b;
+    }
+    return idx;
+}
+
+// <phi| G |psi> over the tile for a dense generator on K <= 2 qubits; tp0 /
+// tp1 are the thread's byte offsets into psi's and phi's tiles.
+template <typename A, int K>
+__device__ __forceinline__ double2 ip_dense(A* s, uint32_t tp0, uint32_t tp1, int G, const OpRegs& op,
+                                            const A* __restrict__ Gm) {
+    constexpr int D = 1 << K;
+    constexpr int sh = Amp<A>::kShift;
+    double2 acc = make_double2(0.0, 0.0);
+    if (G < 8 && threadIdx.x >= (1u << G)) return acc;
+    uint32_t so[D];
+#pragma unroll
+    for (int x = 0; x < D; ++x) so[x] = swz(local_offset<K>(x, op)) << sh;
+    const uint32_t hm = static_cast<uint32_t>(op.gmask);
+    const uint32_t iters = G > 8 ? 1u << (G - 8) : 1u;
+    uint32_t hi = 0;
+    for (uint32_t i = 0; i < iters; ++i) {
+        const uint32_t h = swz(hi) << sh;
+        hi = (hi - hm) & hm;
+        A in[D];
+#pragma unroll
+        for (int x = 0; x < D; ++x) in[x] = at(s, tp0 ^ (h ^ so[x]));
+#pragma unroll
+        for (int y = 0; y < D; ++y) {
+            A gy;
+            gy.x = 0;
+            gy.y = 0;
+#pragma unroll
+            for (int x = 0; x < D; ++x) gy = cmad(gy, Gm[y * D + x], in[x]);
+            const A f = at(s, tp1 ^ (h ^ so[y]));
+            const double fr = f.x, fi = f.y, gr = gy.x, gi = gy.y;   // conj(f) * gy
+            acc.x += fr * gr + fi * gi;
+            acc.y += fr * gi - fi * gr;
+        }
+    }
+    return acc;
+}
+
+// The ops of pass P over the tile in LDS (psi at element 0; phi at element
+// 2^TB in two-state mode), a barrier after each.  te: this thread's byte
+// offset for amplitudes l = threadIdx.x + 256 r.  Returns the number of
+// inner-product ops, whose per-wave partials are left in wave_ip.
+template <typename A, bool TWO, int KMAX, int TB>
+__device__ __forceinline__ int run_ops(A* s, const PassDesc& P, uint64_t gbase, uint32_t te,
+                                       const DevOp* __restrict__ ops, const A* __restrict__ coefs,
+                                       double2* wave_ip) {
+    constexpr int sh = Amp<A>::kShift;
+    constexpr uint32_t kPhi = (1u << TB) << sh;   // byte offset of phi's tile
+    const uint32_t L = 1u << P.T;
+    const uint32_t R = L >= kThreads ? L / kThreads : 1u;
+    const bool active = threadIdx.x < L;
+    int n_ip = 0;
+    for (int o = P.op_begin; o < P.op_end; ++o) {
+        const OpRegs op = load_op(ops + o);
+        const int kind = op.kind, k = op.k;
+        const A* U = coefs + op.coef;
+        switch (kind) {
+            case OP_DENSE: {
+                const int G = P.T - k;
+                const uint32_t tp = thread_part<A>(op, G);
+                dense_op<A, KMAX>(s, tp, G, op, U);
+                if (TWO) dense_op<A, KMAX>(s, tp | kPhi, G, op, U);
+                break;
+            }
+            case OP_DIAG:
+                if (active)
+                    for (uint32_t r = 0; r < R; ++r) {
+                        const uint32_t l = threadIdx.x + r * kThreads;
+                        const uint32_t a = te ^ (swz(r << 8) << sh);
+                        const A d = U[diag_index(l, gbase, op, k)];
+                        at(s, a) = cmul(at(s, a), d);
+                        if (TWO) at(s, a | kPhi) = cmul(at(s, a | kPhi), d);
+                    }
+                break;
+            case OP_MASKPHASE: {
+                const A ph = U[0];
+                const uint64_t lm = op.lmask, gm = op.gmask;
+                if ((gbase & gm) == gm && active)
+                    for (uint32_t r = 0; r < R; ++r) {
+                        const uint32_t l = threadIdx.x + r * kThreads;
+                        if ((l & lm) == lm) {
+                            const uint32_t a = te ^ (swz(r << 8) << sh);
+                            at(s, a) = cmul(at(s, a), ph);
+                            if (TWO) at(s, a | kPhi) = cmul(at(s, a | kPhi), ph);
+                        }
+                    }
+                break;
+            }
+            case OP_SCALAR: {
+                const A ph = U[0];
+                if (active)
+                    for (uint32_t r = 0; r < R; ++r) {
+                        const uint32_t a = te ^ (swz(r << 8) << sh);
+                        at(s, a) = cmul(at(s, a), ph);
+                        if (TWO) at(s, a | kPhi) = cmul(at(s, a | kPhi), ph);
+                    }
+                break;
+            }
+            case OP_IP_DENSE:
+            case OP_IP_DIAG:
+                if constexpr (TWO) {
+                    double2 acc = make_double2(0.0, 0.0);
+                    if (kind == OP_IP_DENSE) {
+                        const int G = P.T - k;
+                        const uint32_t tp = thread_part<A>(op, G);
+                        acc = k == 1 ? ip_dense<A, 1>(s, tp, tp | kPhi, G, op, U)
+                                     : ip_dense<A, 2>(s, tp, tp | kPhi, G, op, U);
+                    } else if (active) {
+                        for (uint32_t r = 0; r < R; ++r) {
+                            const uint32_t l = threadIdx.x + r * kThreads;
+                            const uint32_t a = te ^ (swz(r << 8) << sh);
+                            const A d = U[k == 0 ? 0 : diag_index(l, gbase, op, k)];
+                            const A x = at(s, a), f = at(s, a | kPhi);
+                            const double gr = static_cast<double>(d.x) * x.x - static_cast<double>(d.y) * x.y;
+                            const double gi = static_cast<double>(d.x) * x.y + static_cast<double>(d.y) * x.x;
+                            acc.x += static_cast<double>(f.x) * gr + static_cast<double>(f.y) * gi;
+                            acc.y += static_cast<double>(f.x) * gi - static_cast<double>(f.y) * gr;
+                        }
+                    }
+                    acc = wave_sum(acc);
+                    if ((threadIdx.x & 63) == 0) wave_ip[n_ip * kWaves + (threadIdx.x >> 6)] = acc;
+                    ++n_ip;
+                }
+                break;
+        }
+        __syncthreads();
+    }
+    return n_ip;
+}
+
+// Tile of pass P: load, run the pass's ops in LDS, store.  KMAX bounds the
+// widest dense op of the pass (register budget of the group vectors); the
+// tile array holds 2^TB >= 2^T amplitudes per state.
+template <typename A, bool TWO, int KMAX, int TB>
+__global__ __launch_bounds__(kThreads) void tile_pass_kernel(
+    A* __restrict__ psi, A* __restrict__ phi, const PassDesc P, const DevOp* __restrict__ ops,
+    const A* __restrict__ coefs, double2* __restrict__ ip_out, int64_t n_tiles) {
+    constexpr int sh = Amp<A>::kShift;
+    constexpr uint32_t kTile = 1u << TB;
+    constexpr uint32_t kPhi = kTile << sh;
+    constexpr uint32_t kIpElems = TWO ? kMaxIpPerPass * kWaves * sizeof(double2) / sizeof(A) : 0;
+    __shared__ __align__(16) A s[(TWO ? 2 : 1) * kTile + kIpElems];
+    double2* wave_ip = reinterpret_cast<double2*>(s + (TWO ? 2 : 1) * kTile);
+
+    const uint32_t L = 1u << P.T;
+    const uint32_t R = L >= kThreads ? L / kThreads : 1u;
+    const bool active = threadIdx.x < L;
+    const uint64_t t = blockIdx.x;
+    const uint64_t full = P.n >= 64 ? ~0ull : ((1ull << P.n) - 1ull);
+    const uint64_t gbase = tile_base(t, full & ~P.tmask);
+    const uint32_t te = swz(threadIdx.x) << sh;
+
+    // Global index of local l = threadIdx.x + 256 r: the thread's own low
+    // eight local bits are deposited once; the subsets of hmask enumerate
+    // the deposits of r in increasing order.
+    uint64_t dj = 0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+        if (b < P.T) dj |= static_cast<uint64_t>((threadIdx.x >> b) & 1u) << P.qpos[b];
+    // Loads are issued kBatch at a time before any LDS write, so each thread
+    // keeps kBatch HBM requests in flight (tiles of >= 2^11 amplitudes; the
+    // R = 2^(T-8) rows per thread are a multiple of kBatch there).
+    if (R >= kBatch) {
+        uint64_t dr = 0;
+        for (uint32_t r0 = 0; r0 < R; r0 += kBatch) {
+            A v0[kBatch], v1[kBatch];
+#pragma unroll
+            for (int i = 0; i < kBatch; ++i) {
+                const uint64_t g = gbase | dj | dr;
+                v0[i] = stream_load(psi + g);
+                if (TWO) v1[i] = stream_load(phi + g);
+                dr = (dr - P.hmask) & P.hmask;
+            }
+#pragma unroll
+            for (int i = 0; i < kBatch; ++i) {
+                const uint32_t a = te ^ (swz((r0 + i) << 8) << sh);
+                at(s, a) = v0[i];
+                if (TWO) at(s, a | kPhi) = v1[i];
+            }
+        }
+    } else if (active) {
+        uint64_t dr = 0;
+        for (uint32_t r = 0; r < R; ++r) {
+            const uint64_t g = gbase | dj | dr;
+            const uint32_t a = te ^ (swz(r << 8) << sh);
+            at(s, a) = psi[g];
+            if (TWO) at(s, a | kPhi) = phi[g];
+            dr = (dr - P.hmask) & P.hmask;
+        }
+    }
+    __syncthreads();
+
+    const int n_ip = run_ops<A, TWO, KMAX, TB>(s, P, gbase, te, ops, coefs, wave_ip);
+
+    if (TWO && static_cast<int>(threadIdx.x) < n_ip) {
+        // Sum the per-wave values in fixed order: partial k of this tile.
+        const int k = threadIdx.x;
+        double2 v = wave_ip[k * kWaves];
+        for (int w = 1; w < kWaves; ++w) {
+            v.x += wave_ip[k * kWaves + w].x;
+            v.y += wave_ip[k * kWaves + w].y;
+        }
+        ip_out[static_cast<int64_t>(k) * n_tiles + static_cast<int64_t>(t)] = v;
+    }
+
+    if (R >= kBatch) {
+        uint64_t dr = 0;
+        for (uint32_t r0 = 0; r0 < R; r0 += kBatch) {
+            A v0[kBatch], v1[kBatch];
+#pragma unroll
+            for (int i = 0; i < kBatch; ++i) {
+                const uint32_t a = te ^ (swz((r0 + i) << 8) << sh);
+                v0[i] = at(s, a);
+                if (TWO) v1[i] = at(s, a | kPhi);
+            }
+#pragma unroll
+            for (int i = 0; i < kBatch; ++i) {
+                const uint64_t g = gbase | dj | dr;
+                stream_store(psi + g, v0[i]);
+                if (TWO) stream_store(phi + g, v1[i]);
+                dr = (dr - P.hmask) & P.hmask;
+            }
+        }
+    } else if (active) {
+        uint64_t dr = 0;
+        for (uint32_t r = 0; r < R; ++r) {
+            const uint64_t g = gbase | dj | dr;
+            const uint32_t a = te ^ (swz(r << 8) << sh);
+            psi[g] = at(s, a);
+            if (TWO) phi[g] = at(s, a | kPhi);
+            dr = (dr - P.hmask) & P.hmask;
+        }
+    }
+}
+
+// ── launch ──────────────────────────────────────────────────────────────────
+// TB0 .. TB0 + 2 are the instantiated tile-array sizes of one amplitude type:
+// a pass runs with the smallest array that holds its 2^T amplitudes.
+
+template <typename A, bool TWO, int TB>
+static void launch_k(int km, const PassDesc& P, A* psi, A* phi, double2* ip, const DevOp* ops, const A* coefs,
+                     int64_t n_tiles, hipStream_t st) {
+    const dim3 grid(static_cast<uint32_t>(n_tiles)), block(kThreads);
+    switch (km) {
+        case 2: hipLaunchKernelGGL((tile_pass_kernel<A, TWO, 2, TB>), grid, block, 0, st, psi, phi, P, ops, coefs, ip,
+                                   n_tiles); break;
+        case 3: hipLaunchKernelGGL((tile_pass_kernel<A, TWO, 3, TB>), grid, block, 0, st, psi, phi, P, ops, coefs, ip,
+                                   n_tiles); break;
+        case 4: hipLaunchKernelGGL((tile_pass_kernel<A, TWO, 4, TB>), grid, block, 0, st, psi, phi, P, ops, coefs, ip,
+                                   n_tiles); break;
+        default: hipLaunchKernelGGL((tile_pass_kernel<A, TWO, 5, TB>), grid, block, 0, st, psi, phi, P, ops, coefs,
+                                    ip, n_tiles); break;
+    }
+}
+
+template <typename A, bool TWO, int TB0>
+static void launch_tb(const PassDesc& P, A* psi, A* phi, double2* ip, const DevOp* ops, const A* coefs,
+                      int64_t n_tiles, hipStream_t st) {
+    const int km = P.kmax <= 2 ? 2 : std::min(P.kmax, 5);
+    const int tb = std::max(P.T, TB0);
+    switch (tb - TB0) {
+        case 0: launch_k<A, TWO, TB0>(km, P, psi, phi, ip, ops, coefs, n_tiles, st); break;
+        case 1: launch_k<A, TWO, TB0 + 1>(km, P, psi, phi, ip, ops, coefs, n_tiles, st); break;
+        case 2: launch_k<A, TWO, TB0 + 2>(km, P, psi, phi, ip, ops, coefs, n_tiles, st); break;
+        default: fail(QHIP_INVALID, "tile pass: " + std::to_string(P.T) + " tile qubits exceed the LDS tile");
+    }
+}
+
+template <typename A, int TB0>
+static void launch(const PassDesc& P, A* psi, A* phi, double2* ip, const DevOp* ops, const A* coefs, int64_t n_tiles,
+                   hipStream_t st) {
+    if (P.T < 1 || P.T > P.n) fail(QHIP_RUNTIME, "tile pass: bad tile width");
+    if (phi) launch_tb<A, true, TB0>(P, psi, phi, ip, ops, coefs, n_tiles, st);
+    else launch_tb<A, false, TB0>(P, psi, phi, ip, ops, coefs, n_tiles, st);
+    QHIP_CHECK(hipGetLastError());
+}
+
+}  // namespace tile
+}  // namespace qhip
