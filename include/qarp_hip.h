@@ -126,7 +126,7 @@ typedef struct qhip_config {
     int32_t tile_qubits;         /* qubits held on chip per tile pass, 0=auto  */
     int32_t low_qubits;          /* contiguous low qubits per tile, 0=auto     */
     int32_t max_ops_per_pass;    /* 0 = auto                                   */
-    int32_t use_mfma;            /* -1 auto, 0 never, 1 for k>=5 blocks        */
+    int32_t use_mfma;            /* -1 auto, 0 never: dense k=3..5 (complex128) on the matrix cores */
 } qhip_config;
 
 /* What one apply did (filled when the caller passes a non-NULL pointer). */
@@ -134,7 +134,7 @@ typedef struct qhip_apply_stats {
     int64_t fused_ops;       /* ops after the fusion pass                      */
     int64_t passes;          /* full-state HBM passes launched                 */
     int64_t diagonal_ops;    /* ops applied as index-phase (no tile qubits)    */
-    int64_t mfma_ops;        /* k>=5 blocks applied on the matrix cores        */
+    int64_t mfma_ops;        /* dense ops applied on the matrix cores          */
     double  flops;           /* real FP64/FP32 flops issued for matvecs        */
 } qhip_apply_stats;
 

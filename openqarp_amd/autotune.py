@@ -57,7 +57,8 @@ def tune(sim: QarpSimulator, commands: Sequence[Command], state: DeviceState,
         cc.apply(state)                         # warm (first launch of each kernel)
         ms = min(float(cc.apply_timed(state).sum()) for _ in range(repeats))
         results.append({"fusion_max_qubits": k, "tile_qubits": T, "low_qubits": low, "ms": ms,
-                        "passes": cc.stats["passes"], "fused_ops": cc.stats["fused_ops"]})
+                        "passes": cc.stats["passes"], "fused_ops": cc.stats["fused_ops"],
+                        "mfma_ops": cc.stats["mfma_ops"]})
     best = min(results, key=lambda r: r["ms"])
     out = {"best": best, "trials": results}
     _cache[key] = out

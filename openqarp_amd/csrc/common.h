@@ -76,6 +76,16 @@ enum OpKind : int32_t {
     OP_SCALAR = 3,    // amplitude *= phase everywhere
     OP_IP_DENSE = 4,  // adjoint: accumulate <phi| G |psi> for a dense G
     OP_IP_DIAG = 5,   // adjoint: accumulate <phi| G |psi> for a diagonal G
+    // Device-only: a register block of three tile positions (pos[0..2],
+    // ascending) whose 8 amplitudes a thread holds while the lmask following
+    // OP_SUB entries act on them; an OP_SUB's k is its pattern (0-2: one
+    // qubit on block bit k; 3, 4, 5: two qubits on block bits 01, 02, 12, the
+    // lower bit being matrix operand 0) and coef its matrix.
+    OP_BLOCK = 6,
+    OP_SUB = 7,
+    // Device-only: dense op (complex128, k = 3..5) on the matrix cores; coef
+    // points at the per-lane MFMA fragments of its real 2^(k+1)-square form.
+    OP_MFMA = 8,
 };
 
 struct HostOp {
@@ -164,7 +174,7 @@ struct PassDesc {
     int32_t T;           // tile qubits; 0 marks a wide pass (one dense op, k > 5)
     int32_t op_begin;
     int32_t op_end;
-    int32_t kmax;        // widest dense / IP op in the pass
+    int32_t kmax;        // widest VALU dense / IP op in the pass (matrix-core ops excluded)
     int32_t reserved;
     uint64_t tmask;      // global bits that are tile-local
     uint64_t hmask;      // global bits of local positions >= 8
@@ -175,7 +185,7 @@ struct Plan {
     std::vector<PassDesc> passes;
     std::vector<DevOp> ops;
     std::vector<cplx> coefs;
-    int64_t dense_ops = 0, diag_ops = 0, big_ops = 0;
+    int64_t dense_ops = 0, diag_ops = 0, big_ops = 0, mfma_ops = 0;
     double flops = 0.0;
 };
 
@@ -185,8 +195,14 @@ struct ScheduleOptions {
     int max_ops = 0;         // 0 = no cap
     double max_flops_per_amp = 0.0;  // 0 = no cap
     bool two_states = false; // adjoint passes hold psi and phi
+    bool register_blocks = true;  // group k <= 2 dense ops into OP_BLOCKs
+    int mfma = -1;           // dense k >= 3 on the matrix cores: -1 auto, 0 never, 1 where possible
+    int precision = QHIP_C128;
 };
 Plan schedule(const std::vector<HostOp>& ops, int n, int precision, const ScheduleOptions& opt);
+// Per-lane fragments of a dense op for the matrix-core path, and back.
+void mfma_fragments(const std::vector<cplx>& U, int k, std::vector<cplx>& out);
+std::vector<cplx> mfma_matrix(const cplx* fragments, int k);
 int default_tile_qubits(int precision, bool two_states);
 
 // A schedule with its op tables resident on the device.

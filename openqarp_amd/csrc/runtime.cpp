@@ -120,6 +120,7 @@ static ScheduleOptions options(const qhip_config* cfg, bool two) {
         o.tile_qubits = cfg->tile_qubits;
         o.low_qubits = cfg->low_qubits;
         o.max_ops = cfg->max_ops_per_pass;
+        o.mfma = cfg->use_mfma;
     }
     return o;
 }
@@ -154,7 +155,7 @@ static void fill_stats(qhip_apply_stats* stats, const Plan& plan, int64_t fused)
     stats->fused_ops = fused;
     stats->passes = static_cast<int64_t>(plan.passes.size());
     stats->diagonal_ops = plan.diag_ops;
-    stats->mfma_ops = 0;
+    stats->mfma_ops = plan.mfma_ops;
     stats->flops = plan.flops;
 }
 
@@ -468,10 +469,24 @@ int qhip_schedule_inspect(const qhip_config* cfg, const qhip_stream* in, int32_t
                 qhip::FusedItem it;
                 it.pass = static_cast<int32_t>(pi);
                 switch (d.kind) {
-                    case qhip::OP_DENSE: {
+                    case qhip::OP_BLOCK:
+                        continue;   // register-block header: its OP_SUB entries follow
+                    case qhip::OP_MFMA:
                         it.k = d.k;
                         for (int b = 0; b < d.k; ++b) it.q[b] = global(d.pos[b]);
-                        const size_t D = size_t{1} << d.k;
+                        it.m = qhip::mfma_matrix(plan.coefs.data() + d.coef, d.k);
+                        break;
+                    case qhip::OP_SUB:   // pos[] holds the member op's own operand positions
+                    case qhip::OP_DENSE: {
+                        it.k = d.kind == qhip::OP_SUB ? (d.k <= 2 ? 1 : 2) : d.k;
+                        const size_t D = size_t{1} << it.k;
+                        if (d.kind == qhip::OP_SUB && it.k == 2 && d.pos[0] > d.pos[1]) {
+                            // stored in block-bit order: report it on ascending positions
+                            it.q[0] = global(d.pos[1]);
+                            it.q[1] = global(d.pos[0]);
+                        } else {
+                            for (int b = 0; b < it.k; ++b) it.q[b] = global(d.pos[b]);
+                        }
                         it.m.assign(plan.coefs.begin() + d.coef, plan.coefs.begin() + d.coef + D * D);
                         break;
                     }

@@ -68,7 +68,53 @@ static void group_positions(DevOp& d, int T) {
     // deposits with (x - mask) & mask.
 }
 
-static void emit_op(Plan& plan, const HostOp& op, const int* local_of, int n, int T) {
+// MFMA fragments of a dense k-qubit op (k = 3..5) for v_mfma_f64_16x16x4.
+// Real form: M[(b,q)][(a,p)] over output amplitude b / part q and input
+// amplitude a / part p (part 0 = re, 1 = im): re U_ba, -im U_ba, im U_ba,
+// re U_ba for (q,p) = (0,0), (0,1), (1,0), (1,1).  The instruction holds
+// A[i][c] in lane (i = lane % 16, c = lane / 16) and returns D[4r + lane/16]
+// [lane % 16] in register r.  Rows: M-tile mt, row i = 4r + g is output
+// amplitude 8mt + 2g + (r >> 1), part r & 1, so lane group g receives two
+// whole amplitudes.  K-steps: step s, column c is input amplitude
+// 4(s >> 1) + c, part s & 1, so lane group c feeds whole amplitudes c,
+// c + 4, ....  Stored as complex pairs (steps 2t, 2t+1) at
+// [(mt * D/4 + t) * 64 + lane].
+void mfma_fragments(const std::vector<cplx>& U, int k, std::vector<cplx>& out) {
+    const int D = 1 << k, NT = D / 8, S = D / 2;
+    auto M = [&](int b, int q, int a, int p) -> double {
+        const cplx u = U[static_cast<size_t>(b) * D + a];
+        if (q == 0) return p == 0 ? u.real() : -u.imag();
+        return p == 0 ? u.imag() : u.real();
+    };
+    auto frag = [&](int mt, int s, int lane) {
+        const int i = lane % 16, c = lane / 16;
+        const int g = i % 4, r = i / 4;
+        return M(8 * mt + 2 * g + (r >> 1), r & 1, 4 * (s >> 1) + c, s & 1);
+    };
+    for (int mt = 0; mt < NT; ++mt)
+        for (int t = 0; t < S / 2; ++t)
+            for (int lane = 0; lane < 64; ++lane)
+                out.emplace_back(frag(mt, 2 * t, lane), frag(mt, 2 * t + 1, lane));
+}
+
+// Inverse of mfma_fragments (schedule inspection).
+std::vector<cplx> mfma_matrix(const cplx* f, int k) {
+    const int D = 1 << k, NT = D / 8, S = D / 2;
+    std::vector<cplx> U(static_cast<size_t>(D) * D);
+    for (int mt = 0; mt < NT; ++mt)
+        for (int t = 0; t < S / 2; ++t)
+            for (int lane = 0; lane < 64; ++lane) {
+                const cplx v = f[(mt * (S / 2) + t) * 64 + lane];
+                const int i = lane % 16, c = lane / 16, g = i % 4, r = i / 4;
+                const int b = 8 * mt + 2 * g + (r >> 1), q = r & 1, a = 4 * t + c;   // step 2t: part 0
+                auto& u = U[static_cast<size_t>(b) * D + a];
+                if (q == 0) u.real(v.real());
+                else u.imag(v.real());
+            }
+    return U;
+}
+
+static void emit_op(Plan& plan, const HostOp& op, const int* local_of, int n, int T, const ScheduleOptions& opt) {
     DevOp d{};
     d.kind = op.kind;
     d.k = op.k;
@@ -81,7 +127,14 @@ static void emit_op(Plan& plan, const HostOp& op, const int* local_of, int n, in
             std::sort(sorted, sorted + op.k);
             for (int b = 0; b < op.k; ++b) d.spos[b] = static_cast<int8_t>(sorted[b]);
             group_positions(d, T);
-            plan.coefs.insert(plan.coefs.end(), op.m.begin(), op.m.end());
+            if (op.kind == OP_DENSE && op.k >= 3 && op.k <= 5 && opt.mfma != 0 && opt.precision == QHIP_C128 &&
+                T - op.k >= 4) {
+                d.kind = OP_MFMA;
+                mfma_fragments(op.m, op.k, plan.coefs);
+                ++plan.mfma_ops;
+            } else {
+                plan.coefs.insert(plan.coefs.end(), op.m.begin(), op.m.end());
+            }
             ++plan.dense_ops;
             break;
         }
@@ -109,6 +162,105 @@ static void emit_op(Plan& plan, const HostOp& op, const int* local_of, int n, in
     if (op.kind == OP_IP_DENSE || op.kind == OP_IP_DIAG) d.lmask = static_cast<uint64_t>(op.slot);
     plan.ops.push_back(d);
     plan.flops += op_flops(op) * static_cast<double>(uint64_t{1} << n);
+}
+
+// Register blocks.  Dense ops on at most two qubits are gathered into blocks
+// whose qubit union has at most three qubits; the kernel loads a block's
+// 2^3-amplitude groups into registers once, applies every member op there
+// and stores once, so LDS traffic and addressing are paid per block, not per
+// op.  An op joins the latest block that touches its qubits (it must follow
+// that block, and no later unit touches them, so moving it up commutes it
+// past disjoint ops only); an op touching no unit may join any block.
+struct Unit {
+    std::vector<size_t> ops;
+    uint64_t qset = 0;
+    bool block_ok = false;
+};
+
+static std::vector<Unit> form_units(const std::vector<HostOp>& ops, const std::vector<size_t>& chosen, bool enable) {
+    std::vector<Unit> units;
+    for (size_t idx : chosen) {
+        const HostOp& op = ops[idx];
+        const uint64_t Q = op.qmask();
+        const bool mergeable = enable && op.kind == OP_DENSE && op.k >= 1 && op.k <= 2;
+        int last = -1;
+        for (int u = static_cast<int>(units.size()) - 1; u >= 0; --u)
+            if (op.kind == OP_SCALAR || (units[u].qset & Q)) { last = u; break; }
+        int target = -1;
+        if (mergeable) {
+            if (last >= 0) {
+                if (units[last].block_ok && std::popcount(units[last].qset | Q) <= 3) target = last;
+            } else {
+                for (int u = static_cast<int>(units.size()) - 1; u >= 0; --u)
+                    if (units[u].block_ok && std::popcount(units[u].qset | Q) <= 3) { target = u; break; }
+            }
+        }
+        if (target >= 0) {
+            units[target].ops.push_back(idx);
+            units[target].qset |= Q;
+        } else {
+            Unit u;
+            u.ops.push_back(idx);
+            u.qset = Q;
+            u.block_ok = mergeable;
+            units.push_back(std::move(u));
+        }
+    }
+    return units;
+}
+
+// Emits a block of >= 2 member ops: the OP_BLOCK header, then one OP_SUB per
+// member with its matrix in block-bit order.
+static void emit_block(Plan& plan, const std::vector<HostOp>& ops, const Unit& u, const int* local_of, int n, int T) {
+    int bp[3];
+    int m = 0;
+    for (int q = 0; q < n; ++q)
+        if ((u.qset >> q) & 1) bp[m++] = local_of[q];
+    // Pad to three block bits with the lowest free tile positions.
+    for (int p = 0; m < 3 && p < T; ++p) {
+        bool used = false;
+        for (int i = 0; i < m; ++i) used |= bp[i] == p;
+        if (!used) bp[m++] = p;
+    }
+    std::sort(bp, bp + 3);
+    DevOp h{};
+    h.kind = OP_BLOCK;
+    h.k = 3;
+    for (int b = 0; b < 3; ++b) h.pos[b] = h.spos[b] = static_cast<int8_t>(bp[b]);
+    group_positions(h, T);
+    h.lmask = u.ops.size();
+    h.coef = static_cast<int64_t>(plan.coefs.size());
+    plan.ops.push_back(h);
+    auto bit_of = [&](uint32_t q) {
+        for (int b = 0; b < 3; ++b)
+            if (bp[b] == local_of[q]) return b;
+        fail(QHIP_RUNTIME, "schedule: block operand outside the block");
+    };
+    for (size_t idx : u.ops) {
+        const HostOp& op = ops[idx];
+        DevOp d{};
+        d.kind = OP_SUB;
+        d.coef = static_cast<int64_t>(plan.coefs.size());
+        if (op.k == 1) {
+            d.k = bit_of(op.q[0]);
+            plan.coefs.insert(plan.coefs.end(), op.m.begin(), op.m.end());
+        } else {
+            const int b0 = bit_of(op.q[0]), b1 = bit_of(op.q[1]);
+            const int lo = std::min(b0, b1), hi = std::max(b0, b1);
+            d.k = lo == 0 ? (hi == 1 ? 3 : 4) : 5;
+            if (b0 < b1) {
+                plan.coefs.insert(plan.coefs.end(), op.m.begin(), op.m.end());
+            } else {   // swap the two operand bits of the 4x4 matrix
+                auto sw = [](int x) { return ((x & 1) << 1) | ((x >> 1) & 1); };
+                for (int y = 0; y < 4; ++y)
+                    for (int x = 0; x < 4; ++x) plan.coefs.push_back(op.m[sw(y) * 4 + sw(x)]);
+            }
+        }
+        for (int b = 0; b < op.k; ++b) d.pos[b] = static_cast<int8_t>(local_of[op.q[b]]);
+        plan.ops.push_back(d);
+        ++plan.dense_ops;
+        plan.flops += op_flops(op) * static_cast<double>(uint64_t{1} << n);
+    }
 }
 
 // Dense ops wider than the tile kernel handles (k > 5) run as passes of
@@ -183,9 +335,16 @@ static void schedule_segment(Plan& plan, const std::vector<HostOp>& ops, const s
             }
         P.op_begin = static_cast<int32_t>(plan.ops.size());
         P.kmax = 0;
-        for (size_t idx : chosen) {
-            emit_op(plan, ops[idx], local_of, n, T);
-            if (ops[idx].needs_local()) P.kmax = std::max(P.kmax, ops[idx].k);
+        const bool blocks = opt.register_blocks && T >= 3;
+        for (const Unit& u : form_units(ops, chosen, blocks)) {
+            if (u.ops.size() >= 2) {
+                emit_block(plan, ops, u, local_of, n, T);   // runs in every kernel variant
+            } else {
+                emit_op(plan, ops[u.ops[0]], local_of, n, T, opt);
+                // kmax sizes the VALU dense path; matrix-core ops run in every variant.
+                if (ops[u.ops[0]].needs_local() && plan.ops.back().kind != OP_MFMA)
+                    P.kmax = std::max(P.kmax, ops[u.ops[0]].k);
+            }
         }
         P.op_end = static_cast<int32_t>(plan.ops.size());
         if (chosen.empty()) fail(QHIP_RUNTIME, "schedule: no op could join a pass");
@@ -194,8 +353,10 @@ static void schedule_segment(Plan& plan, const std::vector<HostOp>& ops, const s
     }
 }
 
-Plan schedule(const std::vector<HostOp>& ops, int n, int precision, const ScheduleOptions& opt) {
+Plan schedule(const std::vector<HostOp>& ops, int n, int precision, const ScheduleOptions& opt_in) {
     Plan plan;
+    ScheduleOptions opt = opt_in;
+    opt.precision = precision;
     int T = opt.tile_qubits > 0 ? opt.tile_qubits : default_tile_qubits(precision, opt.two_states);
     if (T > max_tile_qubits(precision))
         fail(QHIP_INVALID, "tile_qubits " + std::to_string(T) + " exceeds the LDS limit of " +
@@ -221,7 +382,7 @@ Plan schedule(const std::vector<HostOp>& ops, int n, int precision, const Schedu
             int local_of[64];
             for (int q = 0; q < 64; ++q) local_of[q] = q;   // positions are global qubits
             P.op_begin = static_cast<int32_t>(plan.ops.size());
-            emit_op(plan, op, local_of, n, 0);   // no tile: the wide kernel reads spos
+            emit_op(plan, op, local_of, n, 0, opt);   // no tile: the wide kernel reads spos
             P.op_end = static_cast<int32_t>(plan.ops.size());
             plan.passes.push_back(P);
             continue;

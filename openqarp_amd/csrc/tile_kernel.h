@@ -32,6 +32,7 @@
 #pragma once
 
 #include <cstddef>
+#include <type_traits>
 
 #include "common.h"
 
@@ -264,6 +265,138 @@ __device__ __forceinline__ void dense_op(A* s, uint32_t tp, int G, const OpRegs&
     }
 }
 
+// Dense k-qubit op (k = 3..5, complex128) on the matrix cores.  Groups go
+// 16 to a chunk (the MFMA's 16 columns, lane % 16); wave w takes chunks w,
+// w + 4, ....  Lane group c = lane / 16 loads input amplitudes c, c + 4, ...
+// of its group (re then im feed consecutive K-steps), the D/8 M-tiles of 16
+// real rows are accumulated over D/2 K-steps from per-lane fragments F,
+// and lane group g writes output amplitudes 8mt + 2g and 8mt + 2g + 1 of
+// its group.  A wave reads its whole chunk before writing any of it.
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+
+template <int K>
+__device__ __forceinline__ void dense_mfma(double2* s, uint32_t zoff, int G, const OpRegs& op,
+                                           const double2* __restrict__ F) {
+    constexpr int D = 1 << K, NT = D / 8, H = D / 4;   // M-tiles; input amplitudes (= fragment pairs) per lane
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6, j = lane & 15u, c = lane >> 4;
+    uint32_t lp = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) lp |= ((j >> b) & 1u) << op.g(b);
+    lp = (swz(lp) << 4) | zoff;
+    uint32_t so_in[H];
+#pragma unroll
+    for (int t = 0; t < H; ++t) so_in[t] = swz(local_offset<K>(c + 4u * t, op)) << 4;
+    const uint32_t chunks = 1u << (G - 4);
+    for (uint32_t ch = w; ch < chunks; ch += kWaves) {
+        uint32_t up = 0;   // deposit of the group-number bits >= 4 (wave-uniform)
+        for (int b = 4; b < G; ++b) up |= ((ch >> (b - 4)) & 1u) << op.g(b);
+        const uint32_t base = lp ^ (swz(up) << 4);
+        double2 x[H];
+#pragma unroll
+        for (int t = 0; t < H; ++t) x[t] = at(s, base ^ so_in[t]);
+        // M-tiles one at a time (k = 5 keeps only one tile's fragments live).
+#pragma unroll(K < 5 ? NT : 1)
+        for (int mt = 0; mt < NT; ++mt) {
+            mfma_d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int t = 0; t < H; ++t) {
+                const double2 f = F[(mt * H + t) * 64 + lane];
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(f.x, x[t].x, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(f.y, x[t].y, acc, 0, 0, 0);
+            }
+            const uint32_t b0 = 8u * static_cast<uint32_t>(mt) + 2u * c;
+            at(s, base ^ (swz(local_offset<K>(b0, op)) << 4)) = make_double2(acc[0], acc[1]);
+            at(s, base ^ (swz(local_offset<K>(b0 + 1u, op)) << 4)) = make_double2(acc[2], acc[3]);
+        }
+    }
+}
+
+template <typename A>
+__device__ __forceinline__ void mfma_op(A* s, uint32_t zoff, int G, const OpRegs& op, const A* __restrict__ F) {
+    if constexpr (std::is_same_v<A, double2>) {
+        switch (op.k) {
+            case 3: dense_mfma<3>(s, zoff, G, op, F); break;
+            case 4: dense_mfma<4>(s, zoff, G, op, F); break;
+            case 5: dense_mfma<5>(s, zoff, G, op, F); break;
+            default: break;
+        }
+    }
+}
+
+// Register blocks: a thread holds the 8 amplitudes of one group of the
+// block's three tile positions (v[x], x = block bits) while the member ops
+// act on them.
+template <typename A, int B>   // 2x2 matrix on block bit B
+__device__ __forceinline__ void sub_single(A* v, const A* __restrict__ U) {
+    const A u00 = U[0], u01 = U[1], u10 = U[2], u11 = U[3];
+#pragma unroll
+    for (int x = 0; x < 8; ++x)
+        if (!((x >> B) & 1)) {
+            const A a = v[x], b = v[x | (1 << B)];
+            v[x] = cmad(cmul(u00, a), u01, b);
+            v[x | (1 << B)] = cmad(cmul(u10, a), u11, b);
+        }
+}
+
+template <typename A, int B0, int B1>   // 4x4 matrix, operand 0 on block bit B0 < B1
+__device__ __forceinline__ void sub_pair(A* v, const A* __restrict__ U) {
+    constexpr int C = 3 - B0 - B1;   // the block bit the op does not touch
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const int i[4] = {r << C, (r << C) | (1 << B0), (r << C) | (1 << B1), (r << C) | (1 << B0) | (1 << B1)};
+        const A in[4] = {v[i[0]], v[i[1]], v[i[2]], v[i[3]]};
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+            A acc;
+            acc.x = 0;
+            acc.y = 0;
+#pragma unroll
+            for (int x = 0; x < 4; ++x) acc = cmad(acc, U[y * 4 + x], in[x]);
+            v[i[y]] = acc;
+        }
+    }
+}
+
+// One OP_BLOCK over one state's tile: load each group's 8 amplitudes, run
+// the nsub OP_SUB entries that follow the header, store.
+template <typename A>
+__device__ __forceinline__ void block_apply(A* s, uint32_t tp, int G, const OpRegs& hdr,
+                                            const DevOp* __restrict__ subs, int nsub, const A* __restrict__ coefs) {
+    constexpr int sh = Amp<A>::kShift;
+    if (G < 8 && threadIdx.x >= (1u << G)) return;
+    uint32_t so[8];
+#pragma unroll
+    for (int x = 0; x < 8; ++x) so[x] = swz(local_offset<3>(x, hdr)) << sh;
+    const uint32_t hm = static_cast<uint32_t>(hdr.gmask);
+    const uint32_t iters = G > 8 ? 1u << (G - 8) : 1u;
+    uint32_t hi = 0;
+    for (uint32_t i = 0; i < iters; ++i) {
+        const uint32_t h = swz(hi) << sh;
+        hi = (hi - hm) & hm;
+        uint32_t a[8];
+        A v[8];
+#pragma unroll
+        for (int x = 0; x < 8; ++x) {
+            a[x] = tp ^ (h ^ so[x]);
+            v[x] = at(s, a[x]);
+        }
+        for (int j = 0; j < nsub; ++j) {
+            const OpRegs sub = load_op(subs + j);
+            const A* U = coefs + sub.coef;
+            switch (sub.k) {
+                case 0: sub_single<A, 0>(v, U); break;
+                case 1: sub_single<A, 1>(v, U); break;
+                case 2: sub_single<A, 2>(v, U); break;
+                case 3: sub_pair<A, 0, 1>(v, U); break;
+                case 4: sub_pair<A, 0, 2>(v, U); break;
+                default: sub_pair<A, 1, 2>(v, U); break;
+            }
+        }
+#pragma unroll
+        for (int x = 0; x < 8; ++x) at(s, a[x]) = v[x];
+    }
+}
+
 __device__ __forceinline__ uint32_t diag_index(uint32_t l, uint64_t gbase, const OpRegs& op, int k) {
     uint32_t idx = 0;
     for (int b = 0; b < k; ++b) {
@@ -335,6 +468,21 @@ __device__ __forceinline__ int run_ops(A* s, const PassDesc& P, uint64_t gbase, 
                 const uint32_t tp = thread_part<A>(op, G);
                 dense_op<A, KMAX>(s, tp, G, op, U);
                 if (TWO) dense_op<A, KMAX>(s, tp | kPhi, G, op, U);
+                break;
+            }
+            case OP_MFMA: {
+                const int G = P.T - k;
+                mfma_op<A>(s, 0u, G, op, U);
+                if (TWO) mfma_op<A>(s, kPhi, G, op, U);
+                break;
+            }
+            case OP_BLOCK: {
+                const int nsub = static_cast<int>(op.lmask);
+                const int G = P.T - 3;
+                const uint32_t tp = thread_part<A>(op, G);
+                block_apply<A>(s, tp, G, op, ops + o + 1, nsub, coefs);
+                if (TWO) block_apply<A>(s, tp | kPhi, G, op, ops + o + 1, nsub, coefs);
+                o += nsub;   // the OP_SUB entries are consumed here
                 break;
             }
             case OP_DIAG:

@@ -130,7 +130,7 @@ class QarpSimulator:
     DEFAULT_FUSION_MIN_QUBITS = 12
 
     def __init__(self, *, precision: str = "complex128", tile_qubits: int = 0, low_qubits: int = 0,
-                 max_ops_per_pass: int = 0):
+                 max_ops_per_pass: int = 0, use_mfma: int = -1):
         if precision not in ("complex128", "complex64"):
             raise ValueError(f"precision must be complex128 or complex64, got {precision!r}")
         _lib.load()
@@ -140,6 +140,7 @@ class QarpSimulator:
         self.tile_qubits = int(tile_qubits)
         self.low_qubits = int(low_qubits)
         self.max_ops_per_pass = int(max_ops_per_pass)
+        self.use_mfma = int(use_mfma)   # dense k = 3..5 on the matrix cores: -1 auto, 0 never
         self.last_stats: Optional[dict] = None
         self._pool: dict = {}
 
@@ -166,7 +167,7 @@ class QarpSimulator:
 
     def _config(self):
         return config_struct(self._fusion_max, self._fusion_min, self.tile_qubits, self.low_qubits,
-                             self.max_ops_per_pass, -1)
+                             self.max_ops_per_pass, self.use_mfma)
 
     # ── device-resident primitives ──
     def new_state(self, n_qubits: int, initial_state=None, where: str = "statevector") -> DeviceState:
@@ -357,16 +358,16 @@ class CompiledCircuit:
 
     def __init__(self, sim: "QarpSimulator", commands: Sequence[Command], n_qubits: int, *,
                  fusion_max_qubits: Optional[int] = None, tile_qubits: Optional[int] = None,
-                 low_qubits: Optional[int] = None):
+                 low_qubits: Optional[int] = None, use_mfma: Optional[int] = None):
         self.n_qubits = int(n_qubits)
         self.precision = sim.precision
         cfg = config_struct(sim.fusion_max_qubits if fusion_max_qubits is None else int(fusion_max_qubits),
                             sim.fusion_min_qubits,
                             sim.tile_qubits if tile_qubits is None else int(tile_qubits),
                             sim.low_qubits if low_qubits is None else int(low_qubits),
-                            sim.max_ops_per_pass, -1)
+                            sim.max_ops_per_pass, sim.use_mfma if use_mfma is None else int(use_mfma))
         self.config = {"fusion_max_qubits": cfg.fusion_max_qubits, "tile_qubits": cfg.tile_qubits,
-                       "low_qubits": cfg.low_qubits}
+                       "low_qubits": cfg.low_qubits, "use_mfma": cfg.use_mfma}
         ps = pack_stream(commands)
         self._h = None
         h = C.c_void_p()
@@ -380,7 +381,7 @@ class CompiledCircuit:
         stats = QhipApplyStats()
         _lib.check(_lib.load().qhip_plan_apply(self._h, state.handle, C.byref(stats)))
         self.stats = {"fused_ops": stats.fused_ops, "passes": stats.passes,
-                      "diagonal_ops": stats.diagonal_ops, "flops": stats.flops}
+                      "diagonal_ops": stats.diagonal_ops, "mfma_ops": stats.mfma_ops, "flops": stats.flops}
         return self.stats
 
     def apply_timed(self, state: DeviceState) -> np.ndarray:

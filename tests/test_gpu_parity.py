@@ -129,6 +129,25 @@ def test_multi_pass_schedules(tile, low):
     assert np.abs(out - ref).max() < ATOL
 
 
+@pytest.mark.parametrize("k", [3, 4, 5])
+def test_matrix_core_path_matches_oracle(k):
+    """Dense k = 3..5 ops run on the matrix cores in complex128; the VALU
+    path (use_mfma = 0) and the oracle must agree with them."""
+    n = 15
+    cmds = W.brickwork_haar(n, 10, 40 + k)
+    ref = O.statevector(pack_stream(cmds), n, k, 0)
+    on = gpu_sim()
+    on.fusion_max_qubits, on.fusion_min_qubits = k, 0
+    got = on.statevector(cmds, n)
+    assert on.last_stats["mfma_ops"] > 0
+    assert np.abs(got - ref).max() < ATOL
+    off = gpu_sim(use_mfma=0)
+    off.fusion_max_qubits, off.fusion_min_qubits = k, 0
+    got_off = off.statevector(cmds, n)
+    assert off.last_stats["mfma_ops"] == 0
+    assert np.abs(got_off - ref).max() < ATOL
+
+
 def test_wide_fusion_blocks_6_to_8():
     n = 11
     cmds = W.brickwork_haar(n, 8, 4)
