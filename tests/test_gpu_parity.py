@@ -148,6 +148,18 @@ def test_matrix_core_path_matches_oracle(k):
     assert np.abs(got_off - ref).max() < ATOL
 
 
+@pytest.mark.parametrize("fusion,tile", [(2, 11), (2, 12), (1, 11)])
+def test_register_blocks_at_22_qubits(fusion, tile):
+    """2^22 amplitudes in 2^11 / 2^12 tiles, passes of register blocks and
+    dense k <= 2 ops."""
+    n = 22
+    cmds = W.brickwork_haar(n, 12, 9) + random_circuit(n, 40, 5)
+    ref = O.statevector(pack_stream(cmds), n, fusion, 0)
+    sim = gpu_sim(tile_qubits=tile)
+    sim.fusion_max_qubits, sim.fusion_min_qubits = fusion, 0
+    assert np.abs(sim.statevector(cmds, n) - ref).max() < ATOL
+
+
 def test_wide_fusion_blocks_6_to_8():
     n = 11
     cmds = W.brickwork_haar(n, 8, 4)
