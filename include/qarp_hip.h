@@ -251,6 +251,13 @@ int qhip_sample_uniforms(const qhip_state* s, const double* uniforms, int64_t n,
 /* |amp|^2 of every index into a host array of 2^n doubles (the EXACT readout
  * of _exact_result, src/qarp/engines/_engine.py:79-116, before pruning). */
 int qhip_probabilities(const qhip_state* s, double* out, uint64_t n);
+/* EXACT readout of _exact_result (src/qarp/engines/_engine.py:79-116,
+ * _EXACT_PRUNE_TOL at :76): every index with |amp|^2 > threshold, ascending,
+ * with its probability, compacted on the device.  *count receives the number
+ * retained; keys / probs (both NULL to query the count) must hold capacity >=
+ * *count entries. */
+int qhip_exact(const qhip_state* s, double threshold, int64_t* keys, double* probs, int64_t capacity,
+               int64_t* count);
 
 /* ── adjoint gradient ─────────────────────────────────────────────────────
  * run_gradient_core (qarp_simulator.cpp:1519-1605) with the observable

@@ -24,7 +24,7 @@ EXPORTS = [
     "qhip_schedule_inspect", "qhip_fused_pass",
     "qhip_plan_create", "qhip_plan_apply", "qhip_plan_apply_timed", "qhip_plan_destroy",
     "qhip_apply", "qhip_apply_terminal", "qhip_transition", "qhip_apply_observable",
-    "qhip_sample", "qhip_sample_uniforms", "qhip_probabilities",
+    "qhip_sample", "qhip_sample_uniforms", "qhip_probabilities", "qhip_exact",
     "qhip_run_gradient", "qhip_run_gradient_phi", "qhip_batch_energy_grad",
 ]
 
@@ -63,6 +63,7 @@ def _declare(lib):
         "qhip_sample": [vp, i64, u32, vp],
         "qhip_sample_uniforms": [vp, vp, i64, vp],
         "qhip_probabilities": [vp, vp, u64],
+        "qhip_exact": [vp, C.c_double, vp, vp, i64, C.POINTER(i64)],
         "qhip_run_gradient": [vp, vp, i32, vp, vp, vp, vp],
         "qhip_run_gradient_phi": [vp, vp, i32, vp, vp, vp],
         "qhip_batch_energy_grad": [vp, i32, vp, vp, vp, vp, i32, vp, vp, vp],

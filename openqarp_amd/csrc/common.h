@@ -242,6 +242,9 @@ cplx inner(const qhip_state* a, const qhip_state* b);   // <a|b> in FP64
 
 void sample(const qhip_state* s, const double* uniforms, int64_t n, uint64_t* out);
 void probabilities(const qhip_state* s, double* out);
+// Indices (ascending) and |amp|^2 above threshold; returns their number and
+// fills keys / probs when both are set (capacity must cover the number).
+int64_t exact(const qhip_state* s, double threshold, int64_t* keys, double* probs, int64_t capacity);
 
 // IP partial reduction: sum tile partials per slot (deterministic order).
 void reduce_ip(const void* d_partials, int64_t n_tiles, int n_slots, cplx* host_out);

@@ -153,6 +153,91 @@ __global__ void probs_kernel(const A* __restrict__ s, uint64_t dim, double* __re
         out[i] = prob(s, i);
 }
 
+// EXACT readout: indices with |amp|^2 above a threshold, compacted in
+// ascending index order (the np.nonzero of _exact_result).  Block b owns
+// amplitudes [4096 b, 4096 b + 4096): a count pass, an exclusive scan of
+// the block counts, then a scatter pass that ranks kept lanes with wave
+// ballots in index order.
+constexpr uint64_t kXChunk = kST * 16;
+
+template <typename A>
+__global__ __launch_bounds__(kST) void exact_count(const A* __restrict__ s, uint64_t dim, double thr,
+                                                   uint64_t* __restrict__ counts) {
+    __shared__ uint32_t wsum[kST / 64];
+    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kXChunk;
+    uint32_t n = 0;
+    for (int r = 0; r < 16; ++r) {
+        const uint64_t i = base + static_cast<uint64_t>(r) * kST + threadIdx.x;
+        if (i < dim && prob(s, i) > thr) ++n;
+    }
+    for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+        for (int w = 0; w < kST / 64; ++w) t += wsum[w];
+        counts[blockIdx.x] = t;
+    }
+}
+
+// out[i] = in[0] + ... + in[i-1] for i <= n (one block; thread t scans a
+// contiguous segment).
+__global__ __launch_bounds__(kST) void exclusive_scan_u64(const uint64_t* __restrict__ in, uint64_t n,
+                                                          uint64_t* __restrict__ out) {
+    __shared__ uint64_t part[kST];
+    const uint64_t seg = (n + kST - 1) / kST;
+    const uint64_t lo = threadIdx.x * seg, hi = lo + seg < n ? lo + seg : n;
+    uint64_t sum = 0;
+    for (uint64_t i = lo; i < hi; ++i) sum += in[i];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t run = 0;
+        for (int t = 0; t < kST; ++t) {
+            const uint64_t v = part[t];
+            part[t] = run;
+            run += v;
+        }
+        out[n] = run;
+    }
+    __syncthreads();
+    uint64_t run = part[threadIdx.x];
+    for (uint64_t i = lo; i < hi; ++i) {
+        out[i] = run;
+        run += in[i];
+    }
+}
+
+template <typename A>
+__global__ __launch_bounds__(kST) void exact_scatter(const A* __restrict__ s, uint64_t dim, double thr,
+                                                     const uint64_t* __restrict__ offsets,
+                                                     int64_t* __restrict__ keys, double* __restrict__ probs) {
+    __shared__ uint32_t wcount[kST / 64];
+    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kXChunk;
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint64_t pos = offsets[blockIdx.x];
+    for (int r = 0; r < 16; ++r) {
+        const uint64_t i = base + static_cast<uint64_t>(r) * kST + threadIdx.x;
+        const double p = i < dim ? prob(s, i) : 0.0;
+        const bool keep = i < dim && p > thr;
+        const uint64_t ballot = __ballot(keep);
+        const uint32_t rank = __popcll(ballot & ((1ull << lane) - 1ull));
+        if (lane == 0) wcount[w] = __popcll(ballot);
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+        for (int v = 0; v < kST / 64; ++v) {
+            before += v < static_cast<int>(w) ? wcount[v] : 0u;
+            total += wcount[v];
+        }
+        if (keep) {
+            keys[pos + before + rank] = static_cast<int64_t>(i);
+            probs[pos + before + rank] = p;
+        }
+        pos += total;
+        __syncthreads();   // wcount is rewritten next round
+    }
+}
+
 // ── host side ───────────────────────────────────────────────────────────────
 
 static int binade(double v) {      // e with v in [2^e, 2^(e+1)), v > 0 finite
@@ -315,6 +400,44 @@ void probabilities(const qhip_state* s, double* out) {
     QHIP_CHECK(hipGetLastError());
     QHIP_CHECK(hipMemcpyAsync(out, d, sizeof(double) * dim, hipMemcpyDeviceToHost, C.stream));
     QHIP_CHECK(hipStreamSynchronize(C.stream));
+}
+
+int64_t exact(const qhip_state* s, double threshold, int64_t* keys, double* probs, int64_t capacity) {
+    auto& C = ctx();
+    const uint64_t dim = s->dim();
+    const uint64_t nb = (dim + kXChunk - 1) / kXChunk;
+    uint64_t* counts = static_cast<uint64_t*>(C.red.ensure(sizeof(uint64_t) * (2 * nb + 1)));
+    uint64_t* offsets = counts + nb;
+    if (s->precision == QHIP_C128)
+        hipLaunchKernelGGL(exact_count<double2>, dim3(static_cast<uint32_t>(nb)), dim3(kST), 0, C.stream,
+                           static_cast<const double2*>(s->d), dim, threshold, counts);
+    else
+        hipLaunchKernelGGL(exact_count<float2>, dim3(static_cast<uint32_t>(nb)), dim3(kST), 0, C.stream,
+                           static_cast<const float2*>(s->d), dim, threshold, counts);
+    hipLaunchKernelGGL(exclusive_scan_u64, dim3(1), dim3(kST), 0, C.stream, counts, nb, offsets);
+    QHIP_CHECK(hipGetLastError());
+    uint64_t total = 0;
+    QHIP_CHECK(hipMemcpyAsync(&total, offsets + nb, sizeof(uint64_t), hipMemcpyDeviceToHost, C.stream));
+    QHIP_CHECK(hipStreamSynchronize(C.stream));
+    if (!keys || !probs) return static_cast<int64_t>(total);
+    if (static_cast<uint64_t>(capacity) < total)
+        fail(QHIP_INVALID, "qhip_exact: capacity " + std::to_string(capacity) + " is below the " +
+                               std::to_string(total) + " retained outcomes");
+    if (total == 0) return 0;
+    char* out = static_cast<char*>(C.misc.ensure(16 * total));
+    int64_t* d_keys = reinterpret_cast<int64_t*>(out);
+    double* d_probs = reinterpret_cast<double*>(out + 8 * total);
+    if (s->precision == QHIP_C128)
+        hipLaunchKernelGGL(exact_scatter<double2>, dim3(static_cast<uint32_t>(nb)), dim3(kST), 0, C.stream,
+                           static_cast<const double2*>(s->d), dim, threshold, offsets, d_keys, d_probs);
+    else
+        hipLaunchKernelGGL(exact_scatter<float2>, dim3(static_cast<uint32_t>(nb)), dim3(kST), 0, C.stream,
+                           static_cast<const float2*>(s->d), dim, threshold, offsets, d_keys, d_probs);
+    QHIP_CHECK(hipGetLastError());
+    QHIP_CHECK(hipMemcpyAsync(keys, d_keys, 8 * total, hipMemcpyDeviceToHost, C.stream));
+    QHIP_CHECK(hipMemcpyAsync(probs, d_probs, 8 * total, hipMemcpyDeviceToHost, C.stream));
+    QHIP_CHECK(hipStreamSynchronize(C.stream));
+    return static_cast<int64_t>(total);
 }
 
 // generate_canonical<double, 53> over std::mt19937: the toolchain's own

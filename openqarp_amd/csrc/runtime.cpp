@@ -687,6 +687,16 @@ int qhip_probabilities(const qhip_state* s, double* out, uint64_t n) {
     });
 }
 
+int qhip_exact(const qhip_state* s, double threshold, int64_t* keys, double* probs, int64_t capacity,
+               int64_t* count) {
+    return guarded([&] {
+        need(s && count, "qhip_exact: NULL argument");
+        need(capacity >= 0, "qhip_exact: capacity must be >= 0");
+        need(threshold >= 0.0, "qhip_exact: threshold must be >= 0");
+        *count = qhip::exact(s, threshold, keys, probs, capacity);
+    });
+}
+
 int qhip_run_gradient(const qhip_config* cfg, const qhip_stream* stream, int32_t n_params, qhip_state* psi,
                       const qhip_observable* obs, double* grad, double* energy) {
     return guarded([&] {

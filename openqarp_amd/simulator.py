@@ -217,6 +217,19 @@ class QarpSimulator:
         _lib.check(_lib.load().qhip_probabilities(state.handle, out.ctypes.data, out.size))
         return out
 
+    def exact_state(self, state: DeviceState, threshold: float = 1e-12):
+        """(keys, probs): every index with |amp|^2 > threshold, ascending, compacted on the device (the
+        Born distribution of _exact_result, src/qarp/engines/_engine.py:79-116)."""
+        lib = _lib.load()
+        n = C.c_int64()
+        _lib.check(lib.qhip_exact(state.handle, float(threshold), None, None, 0, C.byref(n)))
+        keys = np.zeros(n.value, dtype=np.int64)
+        probs = np.zeros(n.value, dtype=np.float64)
+        if n.value:
+            _lib.check(lib.qhip_exact(state.handle, float(threshold), keys.ctypes.data, probs.ctypes.data,
+                                      n.value, C.byref(n)))
+        return keys, probs
+
     # ── reference surface ──
     def statevector(self, commands: Sequence[Command], n_qubits: int, *, initial_state=None) -> np.ndarray:
         st = self.new_state(n_qubits, initial_state, "statevector")
