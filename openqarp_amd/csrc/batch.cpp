@@ -1,52 +1,324 @@
-// batch.cpp — many independent instances (own width, circuit, observable).
+// batch.cpp — many independent instances (own width, circuit, observable)
+// in batched launches.
 //
 // Stands in for the sequential parameter-set loops of batch_expectation
 // (src/qarpx/src/simulator/qarp_simulator.cpp:971-984) and the per-point
-// sweep of QarpEngine._sweep (src/qarp/engines/_qarp_engine.py:450-503).
-// Instances are grouped by register width so one device state per width is
-// reused; each instance runs the fused forward apply, H|psi>, and (when
-// gradients are requested) the adjoint reverse sweep.
+// sweep of QarpEngine._sweep (src/qarp/engines/_qarp_engine.py:450-503),
+// with every instance differentiated by the adjoint method of run_gradient
+// (qarp_simulator.cpp:1519-1624).
+//
+// Every instance is planned on the host (fused forward schedule, unfused
+// reverse schedule with its generator slots), all op tables, observables,
+// launch descriptors and states go to the device in one arena each, and
+// then:
+//   * forward step k: ONE tile launch runs pass k of every instance that has
+//     one (tile_batch_kernel: workgroup -> (instance, tile) by bisection of a
+//     prefix table), so the number of launches is the longest schedule, not
+//     the sum of all schedules;
+//   * per instance, phi = H psi and <psi|phi> stay on the device;
+//   * reverse step k: one two-state tile launch over all instances, then one
+//     launch reducing every inner-product op of that step (fixed tile order);
+//   * one copy brings energies and inner products back; gradients follow
+//     on the host as in run_gradient.
+// Instances whose schedules contain a pass the batched kernel does not run
+// (a dense op wider than five qubits) go through run_gradient one by one.
 #include "common.h"
 
-#include <map>
+#include <algorithm>
+#include <cstring>
+#include <memory>
 
 namespace qhip {
+
+namespace {
+
+struct Inst {
+    int n = 0;
+    bool batched = true;
+    Plan fwd, back;
+    std::vector<GradSlot> slots;
+    size_t fwd_ops = 0, fwd_coefs = 0, back_ops = 0, back_coefs = 0;   // arena offsets (elements)
+    size_t psi = 0, phi = 0;                                           // state arena offsets (bytes)
+    size_t tot = 0;                                                    // first slot in the totals arena
+};
+
+struct DevArena {
+    void* p = nullptr;
+    ~DevArena() {
+        if (p) {
+            (void)hipStreamSynchronize(ctx().stream);
+            (void)hipFree(p);
+        }
+    }
+    void* alloc(size_t bytes, const char* what) {
+        if (hipMalloc(&p, std::max<size_t>(bytes, 256)) != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            fail(QHIP_NOMEM, std::string("batch: ") + what + " allocation failed");
+        }
+        return p;
+    }
+};
+
+struct Step {   // one batched launch
+    std::vector<BatchItem> items;
+    std::vector<int64_t> first;
+    int64_t total = 0;
+    int tmax = 0, kmax = 0;
+    std::vector<IpReduce> jobs;        // two-state steps: reductions after the launch
+    std::vector<size_t> item_ip;       // byte offset of each item's partials (npos: none)
+    std::vector<size_t> job_in;        // byte offset of each job's partial row
+    size_t items_off = 0, first_off = 0, jobs_off = 0;   // byte offsets in the descriptor arena
+};
+
+constexpr size_t npos = ~size_t{0};
+
+size_t round_up(size_t b) { return (b + 255) / 256 * 256; }
+
+bool batchable(const Plan& p) {
+    for (const PassDesc& P : p.passes)
+        if (P.T <= 0) return false;
+    return true;
+}
+
+}  // namespace
 
 void batch_energy_grad(const qhip_config* cfg, int n_instances, const int32_t* n_qubits,
                        const qhip_stream* streams, const qhip_observable* observables,
                        const int32_t* n_params, int precision, double* energies, double* grads,
                        const int64_t* grad_offsets) {
     if (precision != QHIP_C128 && precision != QHIP_C64) fail(QHIP_INVALID, "batch: bad precision");
-    std::map<int, std::vector<int>> by_width;
+    if (n_instances <= 0) return;
+    auto& C = ctx();
+    const size_t amp = precision == QHIP_C128 ? 16 : 8;
+    const bool want_grad = grads != nullptr;
+
+    // ── host planning ──
+    std::vector<Inst> inst(n_instances);
     for (int i = 0; i < n_instances; ++i) {
-        if (n_qubits[i] < 0 || n_qubits[i] > 34) fail(QHIP_INVALID, "batch: n_qubits out of range");
-        by_width[n_qubits[i]].push_back(i);
+        Inst& I = inst[i];
+        I.n = n_qubits[i];
+        if (I.n < 0 || I.n > 34) fail(QHIP_INVALID, "batch: n_qubits out of range");
+        int64_t fused = 0;
+        I.fwd = plan_stream(cfg, &streams[i], I.n, precision, false, &fused);
+        if (want_grad) {
+            std::vector<HostOp> back;
+            build_backward(Stream{&streams[i]}, I.n, I.slots, back);
+            if (!I.slots.empty()) I.back = schedule(back, I.n, precision, schedule_options(cfg, true));
+        }
+        I.batched = batchable(I.fwd) && batchable(I.back);
     }
-    for (const auto& [n, members] : by_width) {
+
+    // ── arenas: op tables, coefficients, states, totals ──
+    std::vector<DevOp> ops;
+    std::vector<cplx> coefs;
+    size_t state_bytes = 0, n_tot = 0;
+    for (Inst& I : inst) {
+        if (!I.batched) continue;
+        I.fwd_ops = ops.size();
+        I.fwd_coefs = coefs.size();
+        ops.insert(ops.end(), I.fwd.ops.begin(), I.fwd.ops.end());
+        coefs.insert(coefs.end(), I.fwd.coefs.begin(), I.fwd.coefs.end());
+        I.back_ops = ops.size();
+        I.back_coefs = coefs.size();
+        ops.insert(ops.end(), I.back.ops.begin(), I.back.ops.end());
+        coefs.insert(coefs.end(), I.back.coefs.begin(), I.back.coefs.end());
+        const size_t sb = round_up((size_t{1} << I.n) * amp);
+        I.psi = state_bytes;
+        I.phi = state_bytes + sb;
+        state_bytes += 2 * sb;
+        I.tot = n_tot;
+        n_tot += I.slots.size();
+    }
+    DevArena tables, states, results, desc;
+    const size_t ops_b = round_up(ops.size() * sizeof(DevOp));
+    const size_t coef_b = round_up(coefs.size() * amp);
+    char* d_tab = static_cast<char*>(tables.alloc(ops_b + coef_b, "op table"));
+    QHIP_CHECK(hipMemcpy(d_tab, ops.data(), ops.size() * sizeof(DevOp), hipMemcpyHostToDevice));
+    if (precision == QHIP_C128) {
+        QHIP_CHECK(hipMemcpy(d_tab + ops_b, coefs.data(), coefs.size() * 16, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float2> cf(coefs.size());
+        for (size_t k = 0; k < cf.size(); ++k)
+            cf[k] = make_float2(static_cast<float>(coefs[k].real()), static_cast<float>(coefs[k].imag()));
+        QHIP_CHECK(hipMemcpy(d_tab + ops_b, cf.data(), cf.size() * 8, hipMemcpyHostToDevice));
+    }
+    const DevOp* d_ops = reinterpret_cast<const DevOp*>(d_tab);
+    const char* d_coefs = d_tab + ops_b;
+    char* d_states = static_cast<char*>(states.alloc(state_bytes, "state"));
+    // results: energies (double2 per instance), slot totals, inner scratch
+    int scratch_n = 1;
+    for (const Inst& I : inst) {
+        if (!I.batched) continue;
+        qhip_state tmp;
+        tmp.n = I.n;
+        tmp.precision = precision;
+        scratch_n = std::max(scratch_n, inner_blocks(&tmp) + 1);
+    }
+    const size_t en_b = round_up(sizeof(double2) * n_instances), tot_b = round_up(sizeof(double2) * (n_tot + 1));
+    char* d_res = static_cast<char*>(results.alloc(en_b + tot_b + sizeof(double2) * scratch_n, "result"));
+    double2* d_energy = reinterpret_cast<double2*>(d_res);
+    double2* d_tot = reinterpret_cast<double2*>(d_res + en_b);
+    double2* d_scratch = reinterpret_cast<double2*>(d_res + en_b + tot_b);
+
+    auto state_of = [&](const Inst& I, size_t off) {
+        qhip_state s;
+        s.n = I.n;
+        s.precision = precision;
+        s.d = d_states + off;
+        s.owned = false;
+        return s;
+    };
+
+    // ── launch descriptors, per step ──
+    size_t max_fwd = 0, max_back = 0;
+    for (const Inst& I : inst)
+        if (I.batched) {
+            max_fwd = std::max(max_fwd, I.fwd.passes.size());
+            max_back = std::max(max_back, I.back.passes.size());
+        }
+    std::vector<Step> fsteps(max_fwd), bsteps(max_back);
+    size_t ip_bytes = 0;   // largest per-step inner-product partial area
+    auto add_item = [&](Step& S, const Inst& I, const Plan& plan, size_t p, size_t ops_off, size_t coef_off,
+                        bool two, size_t& ip_off) {
+        const PassDesc& P = plan.passes[p];
+        BatchItem b{};
+        b.P = P;
+        b.psi = d_states + I.psi;
+        b.phi = two ? d_states + I.phi : nullptr;
+        b.ops = d_ops + ops_off;
+        b.coefs = d_coefs + coef_off * amp;
+        b.n_tiles = int64_t{1} << (P.n - P.T);
+        b.ip_out = nullptr;
+        size_t my_ip = npos;
+        if (two) {
+            int n_ip = 0;
+            for (int o = P.op_begin; o < P.op_end; ++o) {
+                const DevOp& d = plan.ops[o];
+                if (d.kind != OP_IP_DENSE && d.kind != OP_IP_DIAG) continue;
+                S.jobs.push_back({nullptr, b.n_tiles, d_tot + I.tot + static_cast<size_t>(d.lmask)});
+                S.job_in.push_back(ip_off + sizeof(double2) * static_cast<size_t>(n_ip) * b.n_tiles);
+                ++n_ip;
+            }
+            if (n_ip > 0) {
+                my_ip = ip_off;
+                ip_off += sizeof(double2) * static_cast<size_t>(n_ip) * b.n_tiles;
+            }
+        }
+        S.item_ip.push_back(my_ip);
+        S.first.push_back(S.total);
+        S.total += b.n_tiles;
+        S.tmax = std::max(S.tmax, P.T);
+        S.kmax = std::max(S.kmax, P.kmax);
+        S.items.push_back(b);
+    };
+    for (size_t k = 0; k < max_fwd; ++k) {
+        size_t dummy = 0;
+        for (const Inst& I : inst)
+            if (I.batched && k < I.fwd.passes.size()) add_item(fsteps[k], I, I.fwd, k, I.fwd_ops, I.fwd_coefs, false, dummy);
+    }
+    for (size_t k = 0; k < max_back; ++k) {
+        size_t ip_off = 0;
+        for (const Inst& I : inst)
+            if (I.batched && k < I.back.passes.size())
+                add_item(bsteps[k], I, I.back, k, I.back_ops, I.back_coefs, true, ip_off);
+        ip_bytes = std::max(ip_bytes, ip_off);
+    }
+    DevArena ip_area;
+    char* d_ip = static_cast<char*>(ip_area.alloc(ip_bytes, "inner-product partial"));
+    // Partial rows live in the partial area, at the offsets planned above.
+    for (Step& S : bsteps) {
+        for (size_t k = 0; k < S.items.size(); ++k)
+            if (S.item_ip[k] != npos) S.items[k].ip_out = reinterpret_cast<double2*>(d_ip + S.item_ip[k]);
+        for (size_t k = 0; k < S.jobs.size(); ++k) S.jobs[k].in = reinterpret_cast<const double2*>(d_ip + S.job_in[k]);
+    }
+    size_t desc_bytes = 0;
+    for (auto* steps : {&fsteps, &bsteps})
+        for (Step& S : *steps) {
+            S.items_off = desc_bytes;
+            desc_bytes += round_up(S.items.size() * sizeof(BatchItem));
+            S.first_off = desc_bytes;
+            desc_bytes += round_up(S.first.size() * sizeof(int64_t));
+            S.jobs_off = desc_bytes;
+            desc_bytes += round_up(S.jobs.size() * sizeof(IpReduce));
+        }
+    std::vector<char> hdesc(std::max<size_t>(desc_bytes, 1));
+    for (auto* steps : {&fsteps, &bsteps})
+        for (const Step& S : *steps) {
+            std::memcpy(hdesc.data() + S.items_off, S.items.data(), S.items.size() * sizeof(BatchItem));
+            std::memcpy(hdesc.data() + S.first_off, S.first.data(), S.first.size() * sizeof(int64_t));
+            std::memcpy(hdesc.data() + S.jobs_off, S.jobs.data(), S.jobs.size() * sizeof(IpReduce));
+        }
+    char* d_desc = static_cast<char*>(desc.alloc(hdesc.size(), "descriptor"));
+    QHIP_CHECK(hipMemcpy(d_desc, hdesc.data(), hdesc.size(), hipMemcpyHostToDevice));
+    std::unique_ptr<ObsArena, void (*)(ObsArena*)> obs(obs_arena_create(n_instances, n_qubits, observables),
+                                                        obs_arena_destroy);
+
+    // ── device work ──
+    for (Inst& I : inst) {
+        if (!I.batched) continue;
+        qhip_state s = state_of(I, I.psi);
+        state_reset(&s);
+    }
+    for (const Step& S : fsteps)
+        launch_tile_batch(precision, false, reinterpret_cast<const BatchItem*>(d_desc + S.items_off),
+                          reinterpret_cast<const int64_t*>(d_desc + S.first_off), static_cast<int>(S.items.size()),
+                          S.total, S.tmax, S.kmax, C.stream);
+    for (int i = 0; i < n_instances; ++i) {
+        const Inst& I = inst[i];
+        if (!I.batched) continue;
+        qhip_state psi = state_of(I, I.psi), phi = state_of(I, I.phi);
+        obs_apply_async(obs.get(), i, &psi, &phi);
+        inner_async(&psi, &phi, d_scratch, d_energy + i);
+    }
+    for (const Step& S : bsteps) {
+        launch_tile_batch(precision, true, reinterpret_cast<const BatchItem*>(d_desc + S.items_off),
+                          reinterpret_cast<const int64_t*>(d_desc + S.first_off), static_cast<int>(S.items.size()),
+                          S.total, S.tmax, S.kmax, C.stream);
+        launch_ip_reduce(reinterpret_cast<const IpReduce*>(d_desc + S.jobs_off), static_cast<int>(S.jobs.size()),
+                         C.stream);
+    }
+    std::vector<double2> h_energy(n_instances), h_tot(n_tot + 1);
+    QHIP_CHECK(hipMemcpyAsync(h_energy.data(), d_energy, sizeof(double2) * n_instances, hipMemcpyDeviceToHost,
+                              C.stream));
+    QHIP_CHECK(hipMemcpyAsync(h_tot.data(), d_tot, sizeof(double2) * (n_tot + 1), hipMemcpyDeviceToHost, C.stream));
+    QHIP_CHECK(hipStreamSynchronize(C.stream));
+    for (int i = 0; i < n_instances; ++i) {
+        const Inst& I = inst[i];
+        if (!I.batched) continue;
+        energies[i] = h_energy[i].x;
+        if (want_grad) {
+            std::vector<cplx> ip(I.slots.size());
+            for (size_t s = 0; s < ip.size(); ++s) ip[s] = cplx(h_tot[I.tot + s].x, h_tot[I.tot + s].y);
+            accumulate_gradient(&streams[i], I.slots, ip.data(), n_params[i], grads + grad_offsets[i]);
+        }
+    }
+
+    // ── instances outside the batched kernels, one by one ──
+    for (int i = 0; i < n_instances; ++i) {
+        if (inst[i].batched) continue;
         qhip_state st;
-        st.n = n;
+        st.n = inst[i].n;
         st.precision = precision;
         if (hipMalloc(&st.d, st.dim() * st.amp_bytes()) != hipSuccess) {
             (void)hipGetLastError();
             fail(QHIP_NOMEM, "batch: state allocation failed");
         }
         try {
-            for (int i : members) {
-                state_reset(&st);
-                if (grads) {
-                    run_gradient(cfg, &streams[i], n_params[i], &st, &observables[i], nullptr,
-                                 grads + grad_offsets[i], &energies[i]);
-                } else {
-                    apply_stream(cfg, &streams[i], &st, nullptr, false);
-                    energies[i] = transition(&st, &st, &observables[i]).real();
-                }
+            state_reset(&st);
+            if (want_grad) {
+                run_gradient(cfg, &streams[i], n_params[i], &st, &observables[i], nullptr,
+                             grads + grad_offsets[i], &energies[i]);
+            } else {
+                apply_stream(cfg, &streams[i], &st, nullptr, false);
+                energies[i] = transition(&st, &st, &observables[i]).real();
             }
         } catch (...) {
-            (void)hipStreamSynchronize(ctx().stream);
+            (void)hipStreamSynchronize(C.stream);
             (void)hipFree(st.d);
             throw;
         }
-        QHIP_CHECK(hipStreamSynchronize(ctx().stream));
+        QHIP_CHECK(hipStreamSynchronize(C.stream));
         QHIP_CHECK(hipFree(st.d));
     }
 }

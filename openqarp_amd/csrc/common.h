@@ -181,6 +181,17 @@ struct PassDesc {
     int32_t qpos[40];    // qpos[l] = global qubit of local position l
 };
 
+// One instance's pass in a batched launch (tile_batch_kernel).
+struct BatchItem {
+    PassDesc P;
+    void* psi;
+    void* phi;             // two-state passes only
+    const void* ops;       // DevOp table of the pass (indices in P are relative to it)
+    const void* coefs;
+    double2* ip_out;       // per-tile inner-product partials (two-state passes)
+    int64_t n_tiles;
+};
+
 struct Plan {
     std::vector<PassDesc> passes;
     std::vector<DevOp> ops;
@@ -228,6 +239,17 @@ void launch_tile_pass(const PassDesc& P, double2* psi, double2* phi, double2* ip
                       const double2* d_coefs, int64_t n_tiles, hipStream_t stream);
 void launch_tile_pass(const PassDesc& P, float2* psi, float2* phi, double2* ip_part, const DevOp* d_ops,
                       const float2* d_coefs, int64_t n_tiles, hipStream_t stream);
+// One reduction job: *out = sum of n_tiles partials at in (tile order).
+struct IpReduce {
+    const double2* in;
+    int64_t n_tiles;
+    double2* out;
+};
+void launch_ip_reduce(const IpReduce* d_jobs, int n_jobs, hipStream_t stream);
+// All items' tiles in one launch (items / first on the device; tmax and kmax
+// bound the items' tile widths and VALU dense widths).
+void launch_tile_batch(int precision, bool two, const BatchItem* d_items, const int64_t* d_first, int n_items,
+                       int64_t total_tiles, int tmax, int kmax, hipStream_t stream);
 void launch_plan(const Plan& plan, qhip_state* s, qhip_state* phi, void* d_ip_partials,
                  int n_slots);
 void state_reset(qhip_state* s);
@@ -239,6 +261,14 @@ struct PauliTables;   // grouped observable on the device
 cplx transition(const qhip_state* bra, const qhip_state* ket, const qhip_observable* obs);
 void apply_observable(const qhip_state* in, const qhip_observable* obs, qhip_state* out);
 cplx inner(const qhip_state* a, const qhip_state* b);   // <a|b> in FP64
+// Grouped observables of many instances in one device allocation, and
+// stream-ordered H|in> and <a|b> (results stay on the device).
+struct ObsArena;
+ObsArena* obs_arena_create(int n_obs, const int32_t* n_qubits, const qhip_observable* obs);
+void obs_arena_destroy(ObsArena* a);
+void obs_apply_async(const ObsArena* a, int i, const qhip_state* in, qhip_state* out);
+int inner_blocks(const qhip_state* a);
+void inner_async(const qhip_state* a, const qhip_state* b, double2* scratch, double2* d_out);
 
 void sample(const qhip_state* s, const double* uniforms, int64_t n, uint64_t* out);
 void probabilities(const qhip_state* s, double* out);
@@ -253,6 +283,17 @@ void apply_stream(const qhip_config* cfg, const qhip_stream* stream, qhip_state*
                   qhip_apply_stats* stats, bool terminal);
 Plan plan_stream(const qhip_config* cfg, const qhip_stream* stream, int n, int precision,
                  bool terminal, int64_t* fused_ops);
+// Adjoint pieces shared by run_gradient and the batched driver: the reverse
+// op list (generator inner products and daggers) with its slots, and the
+// gradient from the slots' inner products.
+struct GradSlot {
+    cplx c;        // generator prefactor
+    int64_t op;    // stream op index
+};
+void build_backward(const Stream& st, int n, std::vector<GradSlot>& slots, std::vector<HostOp>& back);
+void accumulate_gradient(const qhip_stream* stream, const std::vector<GradSlot>& slots, const cplx* ip,
+                         int n_params, double* grad);
+ScheduleOptions schedule_options(const qhip_config* cfg, bool two_states);
 void run_gradient(const qhip_config* cfg, const qhip_stream* stream, int n_params,
                   qhip_state* psi, const qhip_observable* obs, const qhip_state* phi_in,
                   double* grad, double* energy);

@@ -113,6 +113,24 @@ __global__ __launch_bounds__(kThreads) void reduce_pass_ip(const double2* __rest
     if (threadIdx.x == 0) totals[slots[blockIdx.x]] = make_double2(rx[0], ry[0]);
 }
 
+// Sums of many inner-product partial rows at once: block b adds
+// jobs[b].n_tiles partials in tile order (the tree of reduce_pass_ip) into
+// *jobs[b].out.
+__global__ __launch_bounds__(kThreads) void reduce_ip_batch(const IpReduce* __restrict__ jobs) {
+    __shared__ double rx[kThreads], ry[kThreads];
+    const IpReduce J = jobs[blockIdx.x];
+    double ax = 0.0, ay = 0.0;
+    for (int64_t i = threadIdx.x; i < J.n_tiles; i += kThreads) { ax += J.in[i].x; ay += J.in[i].y; }
+    rx[threadIdx.x] = ax;
+    ry[threadIdx.x] = ay;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) { rx[threadIdx.x] += rx[threadIdx.x + w]; ry[threadIdx.x] += ry[threadIdx.x + w]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *J.out = make_double2(rx[0], ry[0]);
+}
+
 template <typename A>
 __global__ void convert_c128_to(const double2* __restrict__ in, A* __restrict__ out, uint64_t n) {
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
@@ -333,6 +351,25 @@ static void run_tables(const Plan& plan, const DeviceTables& t, qhip_state* s, q
         }
         for (auto& e : events) (void)hipEventDestroy(e);
     }
+}
+
+void launch_ip_reduce(const IpReduce* d_jobs, int n_jobs, hipStream_t stream) {
+    if (n_jobs <= 0) return;
+    hipLaunchKernelGGL(reduce_ip_batch, dim3(static_cast<uint32_t>(n_jobs)), dim3(kThreads), 0, stream, d_jobs);
+    QHIP_CHECK(hipGetLastError());
+}
+
+void launch_tile_batch_c128(bool two, const BatchItem* items, const int64_t* first, int n_items, int64_t total,
+                            int tmax, int kmax, hipStream_t stream);
+void launch_tile_batch_c64(bool two, const BatchItem* items, const int64_t* first, int n_items, int64_t total,
+                           int tmax, int kmax, hipStream_t stream);
+
+void launch_tile_batch(int precision, bool two, const BatchItem* d_items, const int64_t* d_first, int n_items,
+                       int64_t total_tiles, int tmax, int kmax, hipStream_t stream) {
+    if (precision == QHIP_C128)
+        launch_tile_batch_c128(two, d_items, d_first, n_items, total_tiles, tmax, kmax, stream);
+    else
+        launch_tile_batch_c64(two, d_items, d_first, n_items, total_tiles, tmax, kmax, stream);
 }
 
 void launch_plan(const Plan& plan, qhip_state* s, qhip_state* phi, void* d_ip_totals, int n_slots) {

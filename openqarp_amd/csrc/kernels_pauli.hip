@@ -15,6 +15,9 @@
 // per-chunk partials in index order — the result does not depend on timing.
 #include "common.h"
 
+#include <cstring>
+#include <memory>
+
 #include <unordered_map>
 
 namespace qhip {
@@ -281,6 +284,93 @@ void apply_observable(const qhip_state* in, const qhip_observable* obs, qhip_sta
                            dg.n_groups, dg.z, dg.c);
     QHIP_CHECK(hipGetLastError());
     QHIP_CHECK(hipStreamSynchronize(C.stream));
+}
+
+// ── many observables at once (batched instances) ───────────────────────────
+
+struct ObsArena {
+    void* d = nullptr;
+    std::vector<DeviceGroups> per;
+    ~ObsArena() {
+        if (d) {
+            (void)hipStreamSynchronize(ctx().stream);
+            (void)hipFree(d);
+        }
+    }
+};
+
+ObsArena* obs_arena_create(int n_obs, const int32_t* n_qubits, const qhip_observable* obs) {
+    std::vector<Grouped> gs;
+    gs.reserve(n_obs);
+    size_t gb = 0, zb = 0, cb = 0;
+    for (int i = 0; i < n_obs; ++i) {
+        gs.push_back(group_terms(&obs[i], n_qubits[i], "run_gradient"));
+        gb += gs.back().groups.size() * sizeof(GroupDev);
+        zb += gs.back().z.size() * sizeof(uint64_t);
+        cb += gs.back().c.size() * sizeof(double2);
+    }
+    auto a = std::make_unique<ObsArena>();
+    std::vector<char> host(gb + zb + cb + 64);
+    if (hipMalloc(&a->d, host.size()) != hipSuccess) {
+        (void)hipGetLastError();
+        a->d = nullptr;
+        fail(QHIP_NOMEM, "batch: observable table allocation failed");
+    }
+    char* dev = static_cast<char*>(a->d);
+    size_t og = 0, oz = gb, oc = gb + zb;
+    for (const Grouped& g : gs) {
+        DeviceGroups dg{reinterpret_cast<const GroupDev*>(dev + og), reinterpret_cast<const uint64_t*>(dev + oz),
+                        reinterpret_cast<const double2*>(dev + oc), static_cast<int>(g.groups.size())};
+        std::memcpy(host.data() + og, g.groups.data(), g.groups.size() * sizeof(GroupDev));
+        std::memcpy(host.data() + oz, g.z.data(), g.z.size() * sizeof(uint64_t));
+        std::memcpy(host.data() + oc, g.c.data(), g.c.size() * sizeof(double2));
+        og += g.groups.size() * sizeof(GroupDev);
+        oz += g.z.size() * sizeof(uint64_t);
+        oc += g.c.size() * sizeof(double2);
+        a->per.push_back(dg);
+    }
+    QHIP_CHECK(hipMemcpy(a->d, host.data(), host.size(), hipMemcpyHostToDevice));
+    return a.release();
+}
+
+void obs_arena_destroy(ObsArena* a) { delete a; }
+
+// out = H_i in, stream-ordered (no host synchronisation).
+void obs_apply_async(const ObsArena* a, int i, const qhip_state* in, qhip_state* out) {
+    auto& C = ctx();
+    const DeviceGroups& dg = a->per[i];
+    const uint64_t dim = in->dim();
+    const int blocks = static_cast<int>(std::min<uint64_t>((dim + kT - 1) / kT,
+                                                           static_cast<uint64_t>(std::max(1, C.n_cu)) * 64));
+    if (in->precision == QHIP_C128)
+        hipLaunchKernelGGL(observable_kernel<double2>, dim3(blocks), dim3(kT), 0, C.stream,
+                           static_cast<const double2*>(in->d), static_cast<double2*>(out->d), dim, dg.g,
+                           dg.n_groups, dg.z, dg.c);
+    else
+        hipLaunchKernelGGL(observable_kernel<float2>, dim3(blocks), dim3(kT), 0, C.stream,
+                           static_cast<const float2*>(in->d), static_cast<float2*>(out->d), dim, dg.g,
+                           dg.n_groups, dg.z, dg.c);
+    QHIP_CHECK(hipGetLastError());
+}
+
+// *d_out = <a|b> in FP64, stream-ordered; scratch holds inner_blocks(a) + 1 entries.
+int inner_blocks(const qhip_state* a) {
+    return static_cast<int>(std::min<uint64_t>((a->dim() + kT - 1) / kT,
+                                               static_cast<uint64_t>(std::max(1, ctx().n_cu)) * 32));
+}
+
+void inner_async(const qhip_state* a, const qhip_state* b, double2* scratch, double2* d_out) {
+    auto& C = ctx();
+    const uint64_t dim = a->dim();
+    const int blocks = inner_blocks(a);
+    if (a->precision == QHIP_C128)
+        hipLaunchKernelGGL(inner_kernel<double2>, dim3(blocks), dim3(kT), 0, C.stream,
+                           static_cast<const double2*>(a->d), static_cast<const double2*>(b->d), dim, scratch);
+    else
+        hipLaunchKernelGGL(inner_kernel<float2>, dim3(blocks), dim3(kT), 0, C.stream,
+                           static_cast<const float2*>(a->d), static_cast<const float2*>(b->d), dim, scratch);
+    hipLaunchKernelGGL(sum_partials, dim3(1), dim3(kT), 0, C.stream, scratch, static_cast<int64_t>(blocks), d_out);
+    QHIP_CHECK(hipGetLastError());
 }
 
 cplx inner(const qhip_state* a, const qhip_state* b) {

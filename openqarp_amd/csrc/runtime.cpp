@@ -187,19 +187,13 @@ struct StateHolder {
     }
 };
 
-void run_gradient(const qhip_config* cfg, const qhip_stream* stream, int n_params, qhip_state* psi,
-                  const qhip_observable* obs, const qhip_state* phi_in, double* grad, double* energy) {
-    const Stream st{stream};
-    const int n = psi->n;
+void build_backward(const Stream& st, int n, std::vector<GradSlot>& slots, std::vector<HostOp>& back) {
     validate(st, n);
     for (int64_t i = 0; i < st.size(); ++i)
         if (needs_trajectory(st.op(i)))
             fail(QHIP_RUNTIME, "QarpSimulator::run_gradient: trajectory commands (Measure / Reset / classical "
                                "condition) are not differentiable.");
     // Backward op list (reverse order): generator inner product, then dagger.
-    struct SlotInfo { cplx c; int64_t op; };
-    std::vector<SlotInfo> slots;
-    std::vector<HostOp> back;
     for (int64_t i = st.size() - 1; i >= 0; --i) {
         const qhip_op& op = st.op(i);
         if (op.n_deps > 0) {
@@ -216,6 +210,31 @@ void run_gradient(const qhip_config* cfg, const qhip_stream* stream, int n_param
         }
         lower_dagger(st, i, back);
     }
+}
+
+void accumulate_gradient(const qhip_stream* stream, const std::vector<GradSlot>& slots, const cplx* ip,
+                         int n_params, double* grad) {
+    const Stream st{stream};
+    for (int p = 0; p < n_params; ++p) grad[p] = 0.0;
+    for (size_t s = 0; s < slots.size(); ++s) {
+        const qhip_op& op = st.op(slots[s].op);
+        const double base = 2.0 * (slots[s].c * ip[s]).real();
+        for (int d = 0; d < op.n_deps; ++d) {
+            const int p = stream->dep_index[op.dep_offset + d];
+            if (p >= 0 && p < n_params) grad[p] += stream->dep_alpha[op.dep_offset + d] * base;
+        }
+    }
+}
+
+ScheduleOptions schedule_options(const qhip_config* cfg, bool two) { return options(cfg, two); }
+
+void run_gradient(const qhip_config* cfg, const qhip_stream* stream, int n_params, qhip_state* psi,
+                  const qhip_observable* obs, const qhip_state* phi_in, double* grad, double* energy) {
+    const Stream st{stream};
+    const int n = psi->n;
+    std::vector<GradSlot> slots;
+    std::vector<HostOp> back;
+    build_backward(st, n, slots, back);
     // Forward sweep: psi <- U psi (fused; the gradient sweep itself is unfused).
     apply_stream(cfg, stream, psi, nullptr, false);
     StateHolder phi(psi);
@@ -239,15 +258,7 @@ void run_gradient(const qhip_config* cfg, const qhip_stream* stream, int n_param
     } else {
         launch_plan(plan, psi, &phi.st, nullptr, 0);
     }
-    for (int p = 0; p < n_params; ++p) grad[p] = 0.0;
-    for (size_t s = 0; s < slots.size(); ++s) {
-        const qhip_op& op = st.op(slots[s].op);
-        const double base = 2.0 * (slots[s].c * ip[s]).real();
-        for (int d = 0; d < op.n_deps; ++d) {
-            const int p = stream->dep_index[op.dep_offset + d];
-            if (p >= 0 && p < n_params) grad[p] += stream->dep_alpha[op.dep_offset + d] * base;
-        }
-    }
+    accumulate_gradient(stream, slots, ip.data(), n_params, grad);
     // psi is back at the root state after the reverse sweep; leave U|root>
     // as documented: re-run the forward sweep only if the caller needs it —
     // the energy is already computed, so the root state is returned.

@@ -554,21 +554,27 @@ __device__ __forceinline__ int run_ops(A* s, const PassDesc& P, uint64_t gbase, 
 // Tile of pass P: load, run the pass's ops in LDS, store.  KMAX bounds the
 // widest dense op of the pass (register budget of the group vectors); the
 // tile array holds 2^TB >= 2^T amplitudes per state.
+template <typename A, bool TWO, int TB>
+struct TileLds {
+    static constexpr uint32_t kTile = 1u << TB;
+    static constexpr uint32_t kIpElems = TWO ? kMaxIpPerPass * kWaves * sizeof(double2) / sizeof(A) : 0;
+    static constexpr uint32_t kElems = (TWO ? 2 : 1) * kTile + kIpElems;
+};
+
+// One tile t of pass P in the LDS array s (TileLds<A, TWO, TB>::kElems
+// elements).
 template <typename A, bool TWO, int KMAX, int TB>
-__global__ __launch_bounds__(kThreads) void tile_pass_kernel(
-    A* __restrict__ psi, A* __restrict__ phi, const PassDesc P, const DevOp* __restrict__ ops,
-    const A* __restrict__ coefs, double2* __restrict__ ip_out, int64_t n_tiles) {
+__device__ __forceinline__ void tile_body(A* s, A* __restrict__ psi, A* __restrict__ phi, const PassDesc& P,
+                                          const DevOp* __restrict__ ops, const A* __restrict__ coefs,
+                                          double2* __restrict__ ip_out, int64_t n_tiles, uint64_t t) {
     constexpr int sh = Amp<A>::kShift;
     constexpr uint32_t kTile = 1u << TB;
     constexpr uint32_t kPhi = kTile << sh;
-    constexpr uint32_t kIpElems = TWO ? kMaxIpPerPass * kWaves * sizeof(double2) / sizeof(A) : 0;
-    __shared__ __align__(16) A s[(TWO ? 2 : 1) * kTile + kIpElems];
     double2* wave_ip = reinterpret_cast<double2*>(s + (TWO ? 2 : 1) * kTile);
 
     const uint32_t L = 1u << P.T;
     const uint32_t R = L >= kThreads ? L / kThreads : 1u;
     const bool active = threadIdx.x < L;
-    const uint64_t t = blockIdx.x;
     const uint64_t full = P.n >= 64 ? ~0ull : ((1ull << P.n) - 1ull);
     const uint64_t gbase = tile_base(t, full & ~P.tmask);
     const uint32_t te = swz(threadIdx.x) << sh;
@@ -656,6 +662,34 @@ __global__ __launch_bounds__(kThreads) void tile_pass_kernel(
     }
 }
 
+template <typename A, bool TWO, int KMAX, int TB>
+__global__ __launch_bounds__(kThreads) void tile_pass_kernel(
+    A* __restrict__ psi, A* __restrict__ phi, const PassDesc P, const DevOp* __restrict__ ops,
+    const A* __restrict__ coefs, double2* __restrict__ ip_out, int64_t n_tiles) {
+    __shared__ __align__(16) A s[TileLds<A, TWO, TB>::kElems];
+    tile_body<A, TWO, KMAX, TB>(s, psi, phi, P, ops, coefs, ip_out, n_tiles, blockIdx.x);
+}
+
+// Many instances' passes in one launch: workgroup b runs tile b - first[j]
+// of item j, where first[j] <= b < first[j + 1] (first[] is the exclusive
+// prefix sum of the items' tile counts, found by bisection; wave-uniform).
+template <typename A, bool TWO, int KMAX, int TB>
+__global__ __launch_bounds__(kThreads) void tile_batch_kernel(const BatchItem* __restrict__ items,
+                                                              const int64_t* __restrict__ first, int n_items) {
+    __shared__ __align__(16) A s[TileLds<A, TWO, TB>::kElems];
+    const int64_t b = blockIdx.x;
+    int lo = 0, hi = n_items - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const BatchItem& it = items[lo];
+    tile_body<A, TWO, KMAX, TB>(s, static_cast<A*>(it.psi), static_cast<A*>(it.phi), it.P,
+                                static_cast<const DevOp*>(it.ops), static_cast<const A*>(it.coefs), it.ip_out,
+                                it.n_tiles, static_cast<uint64_t>(b - first[lo]));
+}
+
 // ── launch ──────────────────────────────────────────────────────────────────
 // TB0 .. TB0 + 2 are the instantiated tile-array sizes of one amplitude type:
 // a pass runs with the smallest array that holds its 2^T amplitudes.
@@ -695,6 +729,38 @@ static void launch(const PassDesc& P, A* psi, A* phi, double2* ip, const DevOp* 
     if (P.T < 1 || P.T > P.n) fail(QHIP_RUNTIME, "tile pass: bad tile width");
     if (phi) launch_tb<A, true, TB0>(P, psi, phi, ip, ops, coefs, n_tiles, st);
     else launch_tb<A, false, TB0>(P, psi, phi, ip, ops, coefs, n_tiles, st);
+    QHIP_CHECK(hipGetLastError());
+}
+
+template <typename A, bool TWO, int TB>
+static void launch_batch_k(int km, const BatchItem* items, const int64_t* first, int n_items, int64_t total,
+                           hipStream_t st) {
+    const dim3 grid(static_cast<uint32_t>(total)), block(kThreads);
+    switch (km) {
+        case 2: hipLaunchKernelGGL((tile_batch_kernel<A, TWO, 2, TB>), grid, block, 0, st, items, first, n_items); break;
+        case 3: hipLaunchKernelGGL((tile_batch_kernel<A, TWO, 3, TB>), grid, block, 0, st, items, first, n_items); break;
+        case 4: hipLaunchKernelGGL((tile_batch_kernel<A, TWO, 4, TB>), grid, block, 0, st, items, first, n_items); break;
+        default: hipLaunchKernelGGL((tile_batch_kernel<A, TWO, 5, TB>), grid, block, 0, st, items, first, n_items); break;
+    }
+}
+
+template <typename A, int TB0>
+static void launch_batch(bool two, const BatchItem* items, const int64_t* first, int n_items, int64_t total, int tmax,
+                         int kmax, hipStream_t st) {
+    if (n_items <= 0 || total <= 0) return;
+    const int km = kmax <= 2 ? 2 : std::min(kmax, 5);
+    const int tb = std::max(tmax, TB0) - TB0;
+    if (tb > 2) fail(QHIP_INVALID, "batched pass: " + std::to_string(tmax) + " tile qubits exceed the LDS tile");
+    auto go = [&](auto two_tag) {
+        constexpr bool TWO = decltype(two_tag)::value;
+        switch (tb) {
+            case 0: launch_batch_k<A, TWO, TB0>(km, items, first, n_items, total, st); break;
+            case 1: launch_batch_k<A, TWO, TB0 + 1>(km, items, first, n_items, total, st); break;
+            default: launch_batch_k<A, TWO, TB0 + 2>(km, items, first, n_items, total, st); break;
+        }
+    };
+    if (two) go(std::true_type{});
+    else go(std::false_type{});
     QHIP_CHECK(hipGetLastError());
 }
 

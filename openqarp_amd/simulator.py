@@ -337,6 +337,36 @@ class QarpSimulator:
                                                      sp.handle, grad.ctypes.data if grad.size else None))
         return grad.tolist()
 
+    def batch_energy_grad(self, instances: Sequence, gradients: bool = True):
+        """Energies (and adjoint gradients) of many independent instances in
+        batched launches (qhip_batch_energy_grad).  ``instances`` holds tuples
+        (commands, n_qubits, observable, params, param_order).  Returns
+        (energies array, list of gradient arrays or None)."""
+        from ._abi import QhipObservable, QhipStream
+        m = len(instances)
+        packed_s, packed_o = [], []
+        nq = np.zeros(m, dtype=np.int32)
+        npar = np.zeros(m, dtype=np.int32)
+        offs = np.zeros(m, dtype=np.int64)
+        total = 0
+        for i, (cmds, n, obs, params, order) in enumerate(instances):
+            index = {name: k for k, name in enumerate(order)}
+            packed_s.append(pack_stream(cmds, values=params, param_index=index))
+            packed_o.append(pack_observable(obs))
+            nq[i], npar[i], offs[i] = int(n), len(order), total
+            total += len(order)
+        streams = (QhipStream * max(m, 1))(*[p.struct for p in packed_s])
+        observables = (QhipObservable * max(m, 1))(*[p.struct for p in packed_o])
+        energies = np.zeros(m)
+        grads = np.zeros(max(total, 1)) if gradients else None
+        cfg = self._config()
+        _lib.check(_lib.load().qhip_batch_energy_grad(
+            C.byref(cfg), m, nq.ctypes.data, streams, observables, npar.ctypes.data,
+            QHIP_C128 if self.precision == "complex128" else QHIP_C64, energies.ctypes.data,
+            grads.ctypes.data if gradients else None, offs.ctypes.data))
+        out = [grads[offs[i]:offs[i] + npar[i]].copy() for i in range(m)] if gradients else None
+        return energies, out
+
     def fused_stream(self, commands: Sequence[Command], n_qubits: int) -> list:
         """The fusion pass output: list of (gate id, qubits, params, matrix-or-None)."""
         ps = pack_stream(commands)
