@@ -17,6 +17,10 @@ Also on the JSON line:
   expectation   <H> of a 2000-string random Pauli sum on the 30-qubit state;
   adjoint       24-qubit, 384-parameter VQE energy gradient: adjoint sweep vs
                 a parameter-shift sweep (768 shifted energies) on the GPU;
+  batched_vqe   512 independent instances, energies + adjoint gradients in
+                batched launches, complex128 and complex64 (error per instance);
+  qaoa          34-qubit complex64 QAOA MaxCut: <C>, 1e5 shots, the adjoint
+                gradient at 33 qubits and complex64 vs complex128 at 32;
   cpu_baseline  the reference C++ simulator (compiled from its sources into
                 oracle/_ref) on a bounded sample of the same workload family,
                 on this host's cores.
@@ -114,6 +118,8 @@ def expectation(sim, state, n):
 
 def adjoint(sim, n=24, layers=8, points=None):
     from openqarp_amd import workloads as W
+    from openqarp_amd.engine import HipEngine
+    from openqarp_amd.primitives import StateVector
     w = W.c3_gradient(n, layers, seed=3, n_terms=1000)
     names = w.symbols
     sim.run_gradient_energy(w.commands, n, w.observable, w.values, names)   # warm
@@ -123,33 +129,107 @@ def adjoint(sim, n=24, layers=8, points=None):
         grad, energy = sim.run_gradient_energy(w.commands, n, w.observable, w.values, names)
     t_adj = (time.perf_counter() - t0) / reps
     log(f"adjoint {n}q {len(names)} params: {t_adj * 1e3:.1f} ms, E={energy:.12f}")
-    # Parameter shift: every Ry/Rz occurrence at +-pi/2, coefficient 1/2
-    # (two-term rule of src/qarp/engines/_gradients.py:258-291).
-    st = sim.new_state(n)
-    total = 2 * len(names)
-    todo = total if points is None else min(points, total)
-    ps_grad = np.zeros(len(names))
+    # Parameter shift through the engine: every occurrence shifted on its own
+    # (two-term rule for Ry/Rz, src/qarp/engines/_gradients.py:250-301), all
+    # 768 shifted energies through the batched forward + <H> launches.
+    eng = HipEngine(simulator=sim)
+    prim = StateVector(w.commands, n, w.observable)
+    sub = names if points is None else names[:max(1, points // 2)]
     t0 = time.perf_counter()
-    done = 0
-    for i, s in enumerate(names):
-        for sign in (+1, -1):
-            if done >= todo:
-                break
-            vals = dict(w.values)
-            vals[s] += sign * math.pi / 2
-            st.reset()
-            sim.apply([c.substitute(vals) for c in w.commands], st)
-            ps_grad[i] += 0.5 * sign * sim.expectation_state(st, w.observable)
-            done += 1
-    t_ps = (time.perf_counter() - t0) / done * total
-    full = done == total
-    err = float(np.abs(ps_grad - grad).max()) if full else None
-    log(f"parameter shift: {done}/{total} points, {t_ps:.2f} s per gradient, max |adj - ps| = {err}")
+    ps = eng.run_gradient(prim, w.values, sub, method="parameter-shift")
+    t_sub = time.perf_counter() - t0
+    t_ps = t_sub * len(names) / len(sub)
+    err = float(np.abs(ps - grad[:len(sub)]).max())
+    log(f"parameter shift: {2 * len(sub)}/{2 * len(names)} points in {t_sub:.2f} s -> {t_ps:.2f} s per gradient, "
+        f"max |adj - ps| = {err:.3e}")
     return {"n_qubits": n, "params": len(names), "terms": len(w.observable), "adjoint_ms": t_adj * 1e3,
-            "adjoint_grads_per_s": 1.0 / t_adj, "param_shift_s": t_ps, "param_shift_points_timed": done,
-            "param_shift_points": total, "param_shift_grads_per_s": 1.0 / t_ps,
-            "speedup_vs_param_shift": t_ps / t_adj, "max_abs_diff_vs_param_shift": err,
-            "energy": energy}
+            "adjoint_grads_per_s": 1.0 / t_adj, "param_shift_s": t_ps, "param_shift_points_timed": 2 * len(sub),
+            "param_shift_points": 2 * len(names), "param_shift_grads_per_s": 1.0 / t_ps,
+            "speedup_vs_param_shift": t_ps / t_adj, "max_abs_diff_vs_param_shift": err, "energy": energy}
+
+
+def batched(n_instances=512):
+    """512 independent HEA/TFIM instances (12..22 qubits, depth 4..16): energies
+    and adjoint gradients in batched launches, complex128 then complex64 with
+    FP64 reductions, and the complex64 error per instance."""
+    from openqarp_amd import workloads as W
+    from openqarp_amd.simulator import QarpSimulator
+    ws = W.batched_vqe(n_instances, seed=4)
+    inst = [(w.commands, w.n_qubits, w.observable, w.values, w.symbols) for w in ws]
+    out = {"instances": n_instances, "params": int(sum(len(w.symbols) for w in ws)),
+           "state_gib_c128": round(sum(16 << w.n_qubits for w in ws) / 2**30, 2)}
+    res = {}
+    for prec in ("complex128", "complex64"):
+        sim = QarpSimulator(precision=prec)
+        sim.batch_energy_grad(inst[:8])   # warm
+        t0 = time.perf_counter()
+        e, g = sim.batch_energy_grad(inst)
+        dt = time.perf_counter() - t0
+        res[prec] = (e, g)
+        out[f"{prec}_s"] = dt
+        log(f"batched {n_instances} instances {prec}: {dt:.3f} s")
+    e128, g128 = res["complex128"]
+    e64, g64 = res["complex64"]
+    de = np.abs(e64 - e128)
+    dg = np.array([np.abs(a - b).max() if a.size else 0.0 for a, b in zip(g64, g128)])
+    out.update({"c64_energy_abs_err_max": float(de.max()), "c64_energy_abs_err_median": float(np.median(de)),
+                "c64_grad_abs_err_max": float(dg.max()), "c64_grad_abs_err_median": float(np.median(dg))})
+    log(f"complex64 vs complex128: energy err max {de.max():.2e} median {np.median(de):.2e}; "
+        f"grad err max {dg.max():.2e}")
+    return out
+
+
+def qaoa(n=34, p=4, seed=5, shots=100_000, n_adj=33, n_cmp=32, fusion=4):
+    """QAOA MaxCut on a seeded random 3-regular graph: <C> and 1e5 shots on a
+    34-qubit complex64 state (128 GiB), the adjoint gradient at 33 qubits in
+    complex64, and complex64 vs complex128 at 32 qubits.  The apply fuses up to
+    4 qubits (the fastest width for this circuit at 30 qubits in both
+    precisions)."""
+    import gc
+    from openqarp_amd import workloads as W
+    from openqarp_amd.simulator import DeviceState, QarpSimulator
+    from openqarp_amd import _lib
+    sim = QarpSimulator(precision="complex64")
+    sim.fusion_max_qubits = fusion
+    w = W.qaoa_maxcut(n, p, seed)
+    st = DeviceState(n, "complex64")
+    t0 = time.perf_counter()
+    stats = sim.apply(w.commands, st)
+    _lib.check(_lib.load().qhip_synchronize())
+    t_apply = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    c = sim.expectation_state(st, w.observable)
+    t_c = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    outcomes = sim.sample_state(st, shots, seed)
+    t_s = time.perf_counter() - t0
+    edges = [(t[0][0], t[1][0]) for t, _ in w.observable]
+    zz = np.zeros(outcomes.size)
+    for a, b in edges:
+        zz += 1.0 - 2.0 * (((outcomes >> a) ^ (outcomes >> b)) & 1).astype(np.float64)
+    c_shots = float(zz.mean())
+    del st
+    gc.collect()
+    log(f"qaoa {n}q c64: apply {t_apply:.2f} s ({stats['passes']} passes), <C> {c:.6f} in {t_c:.2f} s, "
+        f"{shots} shots in {t_s:.2f} s (shot mean {c_shots:.4f})")
+    wa = W.qaoa_maxcut(n_adj, p, seed, symbolic=True)
+    t0 = time.perf_counter()
+    ga, ea = sim.run_gradient_energy(wa.commands, n_adj, wa.observable, wa.values, wa.symbols)
+    t_adj = time.perf_counter() - t0
+    gc.collect()
+    log(f"qaoa {n_adj}q c64 adjoint: {t_adj:.2f} s, E {ea:.6f}")
+    wc = W.qaoa_maxcut(n_cmp, p, seed, symbolic=True)
+    g64, e64 = sim.run_gradient_energy(wc.commands, n_cmp, wc.observable, wc.values, wc.symbols)
+    gc.collect()
+    g128, e128 = QarpSimulator().run_gradient_energy(wc.commands, n_cmp, wc.observable, wc.values, wc.symbols)
+    gc.collect()
+    log(f"qaoa {n_cmp}q: c64 vs c128 energy err {abs(e64 - e128):.2e}, grad err {np.abs(g64 - g128).max():.2e}")
+    return {"n_qubits": n, "p": p, "edges": len(edges), "state_gib": (8 << n) / 2**30,
+            "apply_s": t_apply, "passes": stats["passes"], "expectation": c, "expectation_s": t_c,
+            "shots": shots, "sample_s": t_s, "shot_mean_C": c_shots,
+            "adjoint_n_qubits": n_adj, "adjoint_s": t_adj, "adjoint_energy": ea,
+            "compare_n_qubits": n_cmp, "c64_energy_abs_err": abs(e64 - e128),
+            "c64_grad_abs_err": float(np.abs(g64 - g128).max())}
 
 
 def cpu_baseline(n=24, layers=30, fusion=3):
@@ -212,6 +292,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-expectation", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ps-points", type=int, default=0, help="time only this many parameter-shift points")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-qaoa", action="store_true")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the backend is single-GPU; "
@@ -224,6 +306,8 @@ def main(argv=None) -> int:
     ex = None if args.no_expectation else expectation(sim, fa["state"], fa["n"])
     fa["state"] = None
     adj = None if args.no_adjoint else adjoint(sim, points=args.ps_points or None)
+    bat = None if args.no_batch else batched()
+    qa = None if args.no_qaoa else qaoa()
     cpu = None if args.no_cpu else cpu_baseline()
     line = {
         "metric": METRIC, "value": round(fa["value"], 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
@@ -240,7 +324,7 @@ def main(argv=None) -> int:
                      "kernel": "tile_pass_kernel", "mean_launch_ms": round(fa["kernel_ms"], 4),
                      "bytes_per_launch": 2 * 16 * (1 << fa["n"])},
         "autotune": fa["trials"],
-        "expectation": ex, "adjoint": adj, "cpu_baseline": cpu,
+        "expectation": ex, "adjoint": adj, "batched_vqe": bat, "qaoa": qa, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -170,11 +170,29 @@ def _with_ket(prim, commands):
     raise CapabilityError(f"{type(prim).__name__} has no stencil gradient")
 
 
+# Device memory one batched evaluation may hold in states (psi and H psi of
+# every point of a chunk).
+BATCH_STATE_BYTES = 32 << 30
+
+
 def _point_evaluator(sim, prim):
-    """Batched f(params) for a primitive: device forward + ⟨H⟩ per point."""
+    """Batched f(params) for a primitive.  An expectation value from |0...0>
+    goes through qhip_batch_energy_grad (energies only), one batched launch
+    per schedule step for a chunk of points; anything else runs point by
+    point."""
     from .primitives import StateVector, Target
     if isinstance(prim, StateVector) and prim.target is Target.EXPECTATION_VALUE and prim.initial_state is None:
-        return lambda pts: sim.batch_expectation(prim.ket, prim.n_qubits, prim.observable, pts)
+        amp = 16 if sim.precision == "complex128" else 8
+        per = max(1, BATCH_STATE_BYTES // (2 * amp << prim.n_qubits))
+
+        def run(pts):
+            out = []
+            for k in range(0, len(pts), per):
+                chunk = [(prim.ket, prim.n_qubits, prim.observable, p, []) for p in pts[k:k + per]]
+                e, _ = sim.batch_energy_grad(chunk, gradients=False)
+                out.extend(e.tolist())
+            return out
+        return run
     return lambda pts: [float(prim.run(sim, p)) for p in pts]
 
 

@@ -171,10 +171,16 @@ def random_regular_graph(d: int, n: int, seed: int) -> list:
 
 def qaoa_maxcut(n: int, p: int, seed: int, symbolic: bool = False) -> Workload:
     """H on all qubits, then p layers of RZZ(w*gamma_l) per edge and
-    RX(beta_l) per qubit; <C> = sum over edges of Z_a Z_b (the Ising cost
+    RX(beta_l) per qubit (cost_operator_block.py:47-62, mixed_operator_block.py:
+    33-36); an odd n uses the (n+1)-node graph minus its last node; <C> = sum over edges of Z_a Z_b (the Ising cost
     form of graph_to_cost_hamiltonian, src/qarp/graphs/_utils.py:13-50)."""
     rng = np.random.default_rng(seed)
-    edges = random_regular_graph(3, n, seed)
+    if (3 * n) % 2:
+        # No 3-regular graph has an odd number of nodes: take the graph on
+        # n + 1 nodes from the same seed and drop node n with its edges.
+        edges = [e for e in random_regular_graph(3, n + 1, seed) if n not in e]
+    else:
+        edges = random_regular_graph(3, n, seed)
     gammas = rng.uniform(0.0, math.pi, p)
     betas = rng.uniform(0.0, math.pi / 2, p)
     cmds = [Command(GateType.H, q) for q in range(n)]

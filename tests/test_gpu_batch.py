@@ -61,3 +61,26 @@ def test_complex64_batch_error_is_float32_sized():
         scale = sum(abs(complex(c)) for _, c in inst[i][2])
         assert abs(e64[i] - e128[i]) < 2e-5 * scale
         assert np.abs(g64[i] - g128[i]).max() < 2e-5 * scale
+
+
+def test_qaoa_complex64_against_oracle():
+    """The QAOA MaxCut workload in complex64: <C> and the adjoint gradient
+    against the complex128 oracle (float32 accuracy), and shots drawn from
+    the complex64 state bit-exact with the oracle sampler on the same
+    amplitudes."""
+    n = 16
+    w = W.qaoa_maxcut(n, 3, 5, symbolic=True)
+    sim = QarpSimulator(precision="complex64")
+    index = {s: k for k, s in enumerate(w.symbols)}
+    ps = pack_stream(w.commands, values=w.values, param_index=index)
+    po = pack_observable(w.observable)
+    sv = O.statevector(ps, n, 3, 12)
+    e_ref = O.transition(sv, sv, n, po).real
+    g_ref = O.run_gradient(ps, n, po, len(w.symbols))
+    g, e = sim.run_gradient_energy(w.commands, n, w.observable, w.values, w.symbols)
+    assert abs(e - e_ref) < 1e-4 and np.abs(g - g_ref).max() < 1e-4
+    st = sim.new_state(n)
+    sim.apply([c.substitute(w.values) for c in w.commands], st)
+    amps = st.download()
+    assert abs(sim.expectation_state(st, w.observable) - O.transition(amps, amps, n, po).real) < 1e-9
+    assert np.array_equal(sim.sample_state(st, 5000, 123), O.sample(amps, n, 5000, 123))
