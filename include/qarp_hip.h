@@ -221,6 +221,19 @@ int qhip_plan_apply(const qhip_plan* plan, qhip_state* s, qhip_apply_stats* stat
 /* As qhip_plan_apply, recording HIP events around every pass launch on the
  * library's stream: pass_ms[i] = duration of pass i (capacity entries). */
 int qhip_plan_apply_timed(const qhip_plan* plan, qhip_state* s, double* pass_ms, int64_t capacity);
+/* As qhip_plan_apply, replaying the pass launches as one hipGraph captured
+ * on the first call for this state (a different state re-captures).  Same
+ * kernels, same results as qhip_plan_apply; the launch cost of a plan with
+ * many short passes drops to one graph launch.  No counterpart in the
+ * reference (a CPU simulator has no launches to batch). */
+int qhip_plan_apply_graph(qhip_plan* plan, qhip_state* s, qhip_apply_stats* stats);
+/* Rebinds a plan to new parameter values of the same circuit (the stream is
+ * re-fused and re-scheduled on the host): when the schedule keeps its
+ * structure only the device coefficient table is rewritten and a captured
+ * graph stays valid (*rebuilt = 0); otherwise the tables are rebuilt
+ * (*rebuilt = 1).  The sweep counterpart of batch_run / batch_expectation
+ * (qarp_simulator.cpp:1248-1267, 971-984). */
+int qhip_plan_update(qhip_plan* plan, const qhip_config* cfg, const qhip_stream* stream, int32_t* rebuilt);
 int qhip_plan_destroy(qhip_plan* plan);
 
 /* As qhip_apply but recorded Measure / Barrier commands are skipped — the

@@ -22,7 +22,8 @@ EXPORTS = [
     "qhip_state_download_c128", "qhip_state_copy", "qhip_state_norm2", "qhip_state_device_ptr",
     "qhip_fuse", "qhip_fused_count", "qhip_fused_get", "qhip_fused_destroy",
     "qhip_schedule_inspect", "qhip_fused_pass",
-    "qhip_plan_create", "qhip_plan_apply", "qhip_plan_apply_timed", "qhip_plan_destroy",
+    "qhip_plan_create", "qhip_plan_apply", "qhip_plan_apply_timed", "qhip_plan_apply_graph",
+    "qhip_plan_update", "qhip_plan_destroy",
     "qhip_apply", "qhip_apply_terminal", "qhip_transition", "qhip_apply_observable",
     "qhip_sample", "qhip_sample_uniforms", "qhip_probabilities", "qhip_exact",
     "qhip_run_gradient", "qhip_run_gradient_phi", "qhip_batch_energy_grad",
@@ -55,6 +56,8 @@ def _declare(lib):
         "qhip_plan_create": [vp, vp, i32, i32, C.POINTER(vp)],
         "qhip_plan_apply": [vp, vp, vp],
         "qhip_plan_apply_timed": [vp, vp, vp, i64],
+        "qhip_plan_apply_graph": [vp, vp, vp],
+        "qhip_plan_update": [vp, vp, vp, C.POINTER(i32)],
         "qhip_plan_destroy": [vp],
         "qhip_apply": [vp, vp, vp, vp],
         "qhip_apply_terminal": [vp, vp, vp, vp],

@@ -227,8 +227,15 @@ struct CompiledPlan {
     void* d_slots = nullptr;
     int max_ip = 0;
     int64_t fused_ops = 0;
+    hipGraphExec_t graph = nullptr;   // captured pass launches (qhip_plan_apply_graph)
+    void* graph_state = nullptr;      // the state the graph was captured for
 };
 CompiledPlan* compile_plan(Plan&& plan, int n, int precision);
+void run_compiled_graph(CompiledPlan& cp, qhip_state* s);
+// Rebinds a plan to a re-planned stream: when the schedule has the same
+// structure only the coefficient table is rewritten (the captured graph stays
+// valid); otherwise the device tables are rebuilt.  Returns true on rebuild.
+bool update_compiled(CompiledPlan& cp, Plan&& plan, int64_t fused_ops);
 void run_compiled(const CompiledPlan& cp, qhip_state* s, std::vector<double>* pass_ms);
 void free_compiled(CompiledPlan* cp);
 

@@ -3,6 +3,7 @@
 #include "tile_kernel.h"
 
 #include <cstdlib>
+#include <cstring>
 
 namespace qhip {
 
@@ -413,8 +414,102 @@ void run_compiled(const CompiledPlan& cp, qhip_state* s, std::vector<double>* pa
     run_tables(cp.plan, t, s, nullptr, nullptr, pass_ms);
 }
 
+static void drop_graph(CompiledPlan& cp) {
+    if (cp.graph) {
+        (void)hipStreamSynchronize(ctx().stream);
+        (void)hipGraphExecDestroy(cp.graph);
+        cp.graph = nullptr;
+        cp.graph_state = nullptr;
+    }
+}
+
+// Replays the plan's pass launches as one hipGraph: captured on the first
+// call for a state (stream capture of exactly the launches run_compiled
+// makes: tile / wide kernels only, no copies, fills or events), relaunched
+// afterwards.  A different state re-captures.
+void run_compiled_graph(CompiledPlan& cp, qhip_state* s) {
+    if (cp.plan.passes.empty()) return;
+    auto& C = ctx();
+    if (cp.graph && cp.graph_state != s->d) drop_graph(cp);
+    if (!cp.graph) {
+        hipGraph_t g = nullptr;
+        QHIP_CHECK(hipStreamBeginCapture(C.stream, hipStreamCaptureModeThreadLocal));
+        try {
+            if (s->precision == QHIP_C128)
+                launch_passes<double2>(cp.plan, static_cast<double2*>(s->d), nullptr, nullptr, nullptr,
+                                       static_cast<const DevOp*>(cp.d_ops), static_cast<const double2*>(cp.d_coefs),
+                                       static_cast<const int32_t*>(cp.d_slots), nullptr);
+            else
+                launch_passes<float2>(cp.plan, static_cast<float2*>(s->d), nullptr, nullptr, nullptr,
+                                      static_cast<const DevOp*>(cp.d_ops), static_cast<const float2*>(cp.d_coefs),
+                                      static_cast<const int32_t*>(cp.d_slots), nullptr);
+        } catch (...) {
+            (void)hipStreamEndCapture(C.stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        QHIP_CHECK(hipStreamEndCapture(C.stream, &g));
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        QHIP_CHECK(e);
+        cp.graph = exec;
+        cp.graph_state = s->d;
+    }
+    QHIP_CHECK(hipGraphLaunch(cp.graph, C.stream));
+    QHIP_CHECK(hipStreamSynchronize(C.stream));
+}
+
+static bool same_structure(const Plan& a, const Plan& b) {
+    if (a.passes.size() != b.passes.size() || a.ops.size() != b.ops.size() || a.coefs.size() != b.coefs.size())
+        return false;
+    if (std::memcmp(a.passes.data(), b.passes.data(), a.passes.size() * sizeof(PassDesc)) != 0) return false;
+    return std::memcmp(a.ops.data(), b.ops.data(), a.ops.size() * sizeof(DevOp)) == 0;
+}
+
+bool update_compiled(CompiledPlan& cp, Plan&& plan, int64_t fused_ops) {
+    auto& C = ctx();
+    if (cp.d_tables && same_structure(cp.plan, plan)) {
+        if (cp.precision == QHIP_C128) {
+            QHIP_CHECK(hipMemcpyAsync(cp.d_coefs, plan.coefs.data(), plan.coefs.size() * sizeof(double2),
+                                      hipMemcpyHostToDevice, C.stream));
+            QHIP_CHECK(hipStreamSynchronize(C.stream));
+        } else {
+            std::vector<float2> cf(plan.coefs.size());
+            for (size_t i = 0; i < cf.size(); ++i)
+                cf[i] = make_float2(static_cast<float>(plan.coefs[i].real()),
+                                    static_cast<float>(plan.coefs[i].imag()));
+            QHIP_CHECK(hipMemcpyAsync(cp.d_coefs, cf.data(), cf.size() * sizeof(float2), hipMemcpyHostToDevice,
+                                      C.stream));
+            QHIP_CHECK(hipStreamSynchronize(C.stream));
+        }
+        cp.plan.coefs = std::move(plan.coefs);
+        cp.plan.flops = plan.flops;
+        cp.fused_ops = fused_ops;
+        return false;
+    }
+    drop_graph(cp);
+    if (cp.d_tables) {
+        QHIP_CHECK(hipStreamSynchronize(C.stream));
+        QHIP_CHECK(hipFree(cp.d_tables));
+        cp.d_tables = cp.d_ops = cp.d_coefs = cp.d_slots = nullptr;
+    }
+    cp.plan = std::move(plan);
+    cp.fused_ops = fused_ops;
+    if (!cp.plan.passes.empty()) {
+        const DeviceTables t = upload_tables(cp.plan, cp.precision, true);
+        cp.d_tables = t.owned;
+        cp.d_ops = t.ops;
+        cp.d_coefs = t.coefs;
+        cp.d_slots = t.slots;
+        cp.max_ip = t.max_ip;
+    }
+    return true;
+}
+
 void free_compiled(CompiledPlan* cp) {
     if (!cp) return;
+    drop_graph(*cp);
     if (cp->d_tables) {
         (void)hipStreamSynchronize(ctx().stream);
         (void)hipFree(cp->d_tables);

@@ -625,6 +625,27 @@ int qhip_plan_apply(const qhip_plan* plan, qhip_state* s, qhip_apply_stats* stat
     });
 }
 
+int qhip_plan_apply_graph(qhip_plan* plan, qhip_state* s, qhip_apply_stats* stats) {
+    return guarded([&] {
+        need(plan && s, "qhip_plan_apply_graph: NULL argument");
+        auto* cp = reinterpret_cast<qhip::CompiledPlan*>(plan);
+        need(cp->n == s->n && cp->precision == s->precision, "qhip_plan_apply_graph: state shape differs from the plan");
+        qhip::run_compiled_graph(*cp, s);
+        qhip::fill_stats(stats, cp->plan, cp->fused_ops);
+    });
+}
+
+int qhip_plan_update(qhip_plan* plan, const qhip_config* cfg, const qhip_stream* stream, int32_t* rebuilt) {
+    return guarded([&] {
+        need(plan && stream, "qhip_plan_update: NULL argument");
+        auto* cp = reinterpret_cast<qhip::CompiledPlan*>(plan);
+        int64_t fused = 0;
+        qhip::Plan p = qhip::plan_stream(cfg, stream, cp->n, cp->precision, false, &fused);
+        const bool r = qhip::update_compiled(*cp, std::move(p), fused);
+        if (rebuilt) *rebuilt = r ? 1 : 0;
+    });
+}
+
 int qhip_plan_apply_timed(const qhip_plan* plan, qhip_state* s, double* pass_ms, int64_t capacity) {
     return guarded([&] {
         need(plan && s && (pass_ms || capacity == 0), "qhip_plan_apply_timed: NULL argument");

@@ -397,11 +397,15 @@ class QarpSimulator:
 
 class CompiledCircuit:
     """A command stream fused and scheduled once, op tables resident on the
-    device (qhip_plan_*); ``apply`` launches only the passes."""
+    device (qhip_plan_*); ``apply`` launches only the passes, ``apply_graph``
+    replays them as one captured hipGraph, and ``update(values)`` rebinds the
+    symbolic parameters of the same circuit (rewriting only the coefficient
+    table while the schedule keeps its structure)."""
 
     def __init__(self, sim: "QarpSimulator", commands: Sequence[Command], n_qubits: int, *,
                  fusion_max_qubits: Optional[int] = None, tile_qubits: Optional[int] = None,
-                 low_qubits: Optional[int] = None, use_mfma: Optional[int] = None):
+                 low_qubits: Optional[int] = None, use_mfma: Optional[int] = None,
+                 values: Optional[Mapping[str, float]] = None):
         self.n_qubits = int(n_qubits)
         self.precision = sim.precision
         cfg = config_struct(sim.fusion_max_qubits if fusion_max_qubits is None else int(fusion_max_qubits),
@@ -411,7 +415,9 @@ class CompiledCircuit:
                             sim.max_ops_per_pass, sim.use_mfma if use_mfma is None else int(use_mfma))
         self.config = {"fusion_max_qubits": cfg.fusion_max_qubits, "tile_qubits": cfg.tile_qubits,
                        "low_qubits": cfg.low_qubits, "use_mfma": cfg.use_mfma}
-        ps = pack_stream(commands)
+        self._cfg = cfg
+        self._commands = list(commands)
+        ps = pack_stream(self._commands, values=values)
         self._h = None
         h = C.c_void_p()
         _lib.check(_lib.load().qhip_plan_create(C.byref(cfg), ps.ref, self.n_qubits,
@@ -426,6 +432,22 @@ class CompiledCircuit:
         self.stats = {"fused_ops": stats.fused_ops, "passes": stats.passes,
                       "diagonal_ops": stats.diagonal_ops, "mfma_ops": stats.mfma_ops, "flops": stats.flops}
         return self.stats
+
+    def apply_graph(self, state: DeviceState) -> dict:
+        stats = QhipApplyStats()
+        _lib.check(_lib.load().qhip_plan_apply_graph(self._h, state.handle, C.byref(stats)))
+        self.stats = {"fused_ops": stats.fused_ops, "passes": stats.passes,
+                      "diagonal_ops": stats.diagonal_ops, "mfma_ops": stats.mfma_ops, "flops": stats.flops}
+        return self.stats
+
+    def update(self, values: Mapping[str, float]) -> bool:
+        """Rebind parameter values; True when the device tables had to be
+        rebuilt (the schedule changed structure), False when only the
+        coefficients were rewritten."""
+        ps = pack_stream(self._commands, values=values)
+        rebuilt = C.c_int32(0)
+        _lib.check(_lib.load().qhip_plan_update(self._h, C.byref(self._cfg), ps.ref, C.byref(rebuilt)))
+        return bool(rebuilt.value)
 
     def apply_timed(self, state: DeviceState) -> np.ndarray:
         """Apply once; per-pass kernel durations (ms) from HIP events recorded
