@@ -303,6 +303,23 @@ int qhip_batch_energy_grad(const qhip_config* cfg, int32_t n_instances,
                            const qhip_observable* observables, const int32_t* n_params,
                            int32_t precision, double* energies,
                            double* grads, const int64_t* grad_offsets);
+/* The same instances kept as a session for parameter sweeps (an optimizer's
+ * iterations over batch_expectation / run_gradient, qarp_simulator.cpp:
+ * 971-984, 1519-1624): plans, states and launch descriptors are built once;
+ * qhip_batch_run executes a sweep (use_graph: replays the sweep's launches as
+ * one hipGraph, captured on the first run); qhip_batch_update rebinds new
+ * parameter values of the same circuits — only the coefficient table is
+ * rewritten while every schedule keeps its structure (*rebuilt = 0),
+ * otherwise the session is rebuilt (*rebuilt = 1).  streams and observables
+ * must stay valid until the next update / destroy. */
+typedef struct qhip_batch qhip_batch;
+int qhip_batch_create(const qhip_config* cfg, int32_t n_instances, const int32_t* n_qubits,
+                      const qhip_stream* streams, const qhip_observable* observables,
+                      const int32_t* n_params, int32_t precision, int32_t gradients, qhip_batch** out);
+int qhip_batch_run(qhip_batch* b, double* energies, double* grads, const int64_t* grad_offsets,
+                   int32_t use_graph);
+int qhip_batch_update(qhip_batch* b, const qhip_stream* streams, int32_t* rebuilt);
+int qhip_batch_destroy(qhip_batch* b);
 
 #ifdef __cplusplus
 }

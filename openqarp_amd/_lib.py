@@ -27,6 +27,7 @@ EXPORTS = [
     "qhip_apply", "qhip_apply_terminal", "qhip_transition", "qhip_apply_observable",
     "qhip_sample", "qhip_sample_uniforms", "qhip_probabilities", "qhip_exact",
     "qhip_run_gradient", "qhip_run_gradient_phi", "qhip_batch_energy_grad",
+    "qhip_batch_create", "qhip_batch_run", "qhip_batch_update", "qhip_batch_destroy",
 ]
 
 
@@ -70,6 +71,10 @@ def _declare(lib):
         "qhip_run_gradient": [vp, vp, i32, vp, vp, vp, vp],
         "qhip_run_gradient_phi": [vp, vp, i32, vp, vp, vp],
         "qhip_batch_energy_grad": [vp, i32, vp, vp, vp, vp, i32, vp, vp, vp],
+        "qhip_batch_create": [vp, i32, vp, vp, vp, vp, i32, i32, C.POINTER(vp)],
+        "qhip_batch_run": [vp, vp, vp, vp, i32],
+        "qhip_batch_update": [vp, vp, C.POINTER(i32)],
+        "qhip_batch_destroy": [vp],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

@@ -44,13 +44,19 @@ struct Inst {
 
 struct DevArena {
     void* p = nullptr;
-    ~DevArena() {
+    DevArena() = default;
+    DevArena(const DevArena&) = delete;
+    DevArena& operator=(const DevArena&) = delete;
+    ~DevArena() { reset(); }
+    void reset() {
         if (p) {
             (void)hipStreamSynchronize(ctx().stream);
             (void)hipFree(p);
+            p = nullptr;
         }
     }
     void* alloc(size_t bytes, const char* what) {
+        reset();
         if (hipMalloc(&p, std::max<size_t>(bytes, 256)) != hipSuccess) {
             (void)hipGetLastError();
             p = nullptr;
@@ -83,36 +89,92 @@ bool batchable(const Plan& p) {
 
 }  // namespace
 
-void batch_energy_grad(const qhip_config* cfg, int n_instances, const int32_t* n_qubits,
-                       const qhip_stream* streams, const qhip_observable* observables,
-                       const int32_t* n_params, int precision, double* energies, double* grads,
-                       const int64_t* grad_offsets) {
-    if (precision != QHIP_C128 && precision != QHIP_C64) fail(QHIP_INVALID, "batch: bad precision");
-    if (n_instances <= 0) return;
-    auto& C = ctx();
-    const size_t amp = precision == QHIP_C128 ? 16 : 8;
-    const bool want_grad = grads != nullptr;
+// One set of instances: plans, device arenas and launch descriptors built
+// once; a sweep (states reset, forward steps, H psi and energies, reverse
+// steps and reductions) is a fixed sequence of kernel launches that can be
+// captured as one hipGraph and replayed after the coefficients change.
+struct BatchSession {
+    const qhip_config* cfg = nullptr;
+    qhip_config cfg_copy{};
+    int n_inst = 0;
+    int precision = QHIP_C128;
+    bool want_grad = true;
+    size_t amp = 16;
+    std::vector<int32_t> n_qubits;
+    std::vector<int32_t> n_params;
+    const qhip_stream* streams = nullptr;          // caller-owned, valid until the next update
+    const qhip_observable* observables = nullptr;
+    std::vector<Inst> inst;
+    DevArena tables, states, results, desc, ip_area;
+    std::unique_ptr<ObsArena, void (*)(ObsArena*)> obs{nullptr, obs_arena_destroy};
+    char* d_tab = nullptr;
+    size_t ops_b = 0;
+    double2* d_energy = nullptr;
+    double2* d_tot = nullptr;
+    double2* d_scratch = nullptr;
+    char* d_states = nullptr;
+    char* d_desc = nullptr;
+    size_t n_tot = 0;
+    std::vector<Step> fsteps, bsteps;
+    hipGraphExec_t graph = nullptr;
 
-    // ── host planning ──
-    std::vector<Inst> inst(n_instances);
-    for (int i = 0; i < n_instances; ++i) {
-        Inst& I = inst[i];
-        I.n = n_qubits[i];
-        if (I.n < 0 || I.n > 34) fail(QHIP_INVALID, "batch: n_qubits out of range");
-        int64_t fused = 0;
-        I.fwd = plan_stream(cfg, &streams[i], I.n, precision, false, &fused);
-        if (want_grad) {
-            std::vector<HostOp> back;
-            build_backward(Stream{&streams[i]}, I.n, I.slots, back);
-            if (!I.slots.empty()) I.back = schedule(back, I.n, precision, schedule_options(cfg, true));
+    ~BatchSession() {
+        if (graph) {
+            (void)hipStreamSynchronize(ctx().stream);
+            (void)hipGraphExecDestroy(graph);
         }
-        I.batched = batchable(I.fwd) && batchable(I.back);
     }
 
+    void plan_all(std::vector<Inst>& out) const {
+        out.assign(n_inst, Inst{});
+        for (int i = 0; i < n_inst; ++i) {
+            Inst& I = out[i];
+            I.n = n_qubits[i];
+            if (I.n < 0 || I.n > 34) fail(QHIP_INVALID, "batch: n_qubits out of range");
+            int64_t fused = 0;
+            I.fwd = plan_stream(cfg, &streams[i], I.n, precision, false, &fused);
+            if (want_grad) {
+                std::vector<HostOp> back;
+                build_backward(Stream{&streams[i]}, I.n, I.slots, back);
+                if (!I.slots.empty()) I.back = schedule(back, I.n, precision, schedule_options(cfg, true));
+            }
+            I.batched = batchable(I.fwd) && batchable(I.back);
+        }
+    }
+
+    void upload_coefs(const std::vector<cplx>& coefs) {
+        if (coefs.empty()) return;
+        if (precision == QHIP_C128) {
+            QHIP_CHECK(hipMemcpy(d_tab + ops_b, coefs.data(), coefs.size() * 16, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float2> cf(coefs.size());
+            for (size_t k = 0; k < cf.size(); ++k)
+                cf[k] = make_float2(static_cast<float>(coefs[k].real()), static_cast<float>(coefs[k].imag()));
+            QHIP_CHECK(hipMemcpy(d_tab + ops_b, cf.data(), cf.size() * 8, hipMemcpyHostToDevice));
+        }
+    }
+
+    void build();
+    void enqueue();
+    void run(double* energies, double* grads, const int64_t* grad_offsets, bool use_graph);
+    bool update(const qhip_stream* new_streams);
+};
+
+static bool same_structure(const Plan& a, const Plan& b) {
+    if (a.passes.size() != b.passes.size() || a.ops.size() != b.ops.size() || a.coefs.size() != b.coefs.size())
+        return false;
+    if (!a.passes.empty() && std::memcmp(a.passes.data(), b.passes.data(), a.passes.size() * sizeof(PassDesc)))
+        return false;
+    return a.ops.empty() || std::memcmp(a.ops.data(), b.ops.data(), a.ops.size() * sizeof(DevOp)) == 0;
+}
+
+void BatchSession::build() {
+    plan_all(inst);
     // ── arenas: op tables, coefficients, states, totals ──
     std::vector<DevOp> ops;
     std::vector<cplx> coefs;
-    size_t state_bytes = 0, n_tot = 0;
+    size_t state_bytes = 0;
+    n_tot = 0;
     for (Inst& I : inst) {
         if (!I.batched) continue;
         I.fwd_ops = ops.size();
@@ -130,22 +192,14 @@ void batch_energy_grad(const qhip_config* cfg, int n_instances, const int32_t* n
         I.tot = n_tot;
         n_tot += I.slots.size();
     }
-    DevArena tables, states, results, desc;
-    const size_t ops_b = round_up(ops.size() * sizeof(DevOp));
+    ops_b = round_up(ops.size() * sizeof(DevOp));
     const size_t coef_b = round_up(coefs.size() * amp);
-    char* d_tab = static_cast<char*>(tables.alloc(ops_b + coef_b, "op table"));
+    d_tab = static_cast<char*>(tables.alloc(ops_b + coef_b, "op table"));
     QHIP_CHECK(hipMemcpy(d_tab, ops.data(), ops.size() * sizeof(DevOp), hipMemcpyHostToDevice));
-    if (precision == QHIP_C128) {
-        QHIP_CHECK(hipMemcpy(d_tab + ops_b, coefs.data(), coefs.size() * 16, hipMemcpyHostToDevice));
-    } else {
-        std::vector<float2> cf(coefs.size());
-        for (size_t k = 0; k < cf.size(); ++k)
-            cf[k] = make_float2(static_cast<float>(coefs[k].real()), static_cast<float>(coefs[k].imag()));
-        QHIP_CHECK(hipMemcpy(d_tab + ops_b, cf.data(), cf.size() * 8, hipMemcpyHostToDevice));
-    }
+    upload_coefs(coefs);
     const DevOp* d_ops = reinterpret_cast<const DevOp*>(d_tab);
     const char* d_coefs = d_tab + ops_b;
-    char* d_states = static_cast<char*>(states.alloc(state_bytes, "state"));
+    d_states = static_cast<char*>(states.alloc(state_bytes, "state"));
     // results: energies (double2 per instance), slot totals, inner scratch
     int scratch_n = 1;
     for (const Inst& I : inst) {
@@ -155,20 +209,11 @@ void batch_energy_grad(const qhip_config* cfg, int n_instances, const int32_t* n
         tmp.precision = precision;
         scratch_n = std::max(scratch_n, inner_blocks(&tmp) + 1);
     }
-    const size_t en_b = round_up(sizeof(double2) * n_instances), tot_b = round_up(sizeof(double2) * (n_tot + 1));
+    const size_t en_b = round_up(sizeof(double2) * n_inst), tot_b = round_up(sizeof(double2) * (n_tot + 1));
     char* d_res = static_cast<char*>(results.alloc(en_b + tot_b + sizeof(double2) * scratch_n, "result"));
-    double2* d_energy = reinterpret_cast<double2*>(d_res);
-    double2* d_tot = reinterpret_cast<double2*>(d_res + en_b);
-    double2* d_scratch = reinterpret_cast<double2*>(d_res + en_b + tot_b);
-
-    auto state_of = [&](const Inst& I, size_t off) {
-        qhip_state s;
-        s.n = I.n;
-        s.precision = precision;
-        s.d = d_states + off;
-        s.owned = false;
-        return s;
-    };
+    d_energy = reinterpret_cast<double2*>(d_res);
+    d_tot = reinterpret_cast<double2*>(d_res + en_b);
+    d_scratch = reinterpret_cast<double2*>(d_res + en_b + tot_b);
 
     // ── launch descriptors, per step ──
     size_t max_fwd = 0, max_back = 0;
@@ -177,7 +222,8 @@ void batch_energy_grad(const qhip_config* cfg, int n_instances, const int32_t* n
             max_fwd = std::max(max_fwd, I.fwd.passes.size());
             max_back = std::max(max_back, I.back.passes.size());
         }
-    std::vector<Step> fsteps(max_fwd), bsteps(max_back);
+    fsteps.assign(max_fwd, Step{});
+    bsteps.assign(max_back, Step{});
     size_t ip_bytes = 0;   // largest per-step inner-product partial area
     auto add_item = [&](Step& S, const Inst& I, const Plan& plan, size_t p, size_t ops_off, size_t coef_off,
                         bool two, size_t& ip_off) {
@@ -224,7 +270,6 @@ void batch_energy_grad(const qhip_config* cfg, int n_instances, const int32_t* n
                 add_item(bsteps[k], I, I.back, k, I.back_ops, I.back_coefs, true, ip_off);
         ip_bytes = std::max(ip_bytes, ip_off);
     }
-    DevArena ip_area;
     char* d_ip = static_cast<char*>(ip_area.alloc(ip_bytes, "inner-product partial"));
     // Partial rows live in the partial area, at the offsets planned above.
     for (Step& S : bsteps) {
@@ -249,12 +294,21 @@ void batch_energy_grad(const qhip_config* cfg, int n_instances, const int32_t* n
             std::memcpy(hdesc.data() + S.first_off, S.first.data(), S.first.size() * sizeof(int64_t));
             std::memcpy(hdesc.data() + S.jobs_off, S.jobs.data(), S.jobs.size() * sizeof(IpReduce));
         }
-    char* d_desc = static_cast<char*>(desc.alloc(hdesc.size(), "descriptor"));
+    d_desc = static_cast<char*>(desc.alloc(hdesc.size(), "descriptor"));
     QHIP_CHECK(hipMemcpy(d_desc, hdesc.data(), hdesc.size(), hipMemcpyHostToDevice));
-    std::unique_ptr<ObsArena, void (*)(ObsArena*)> obs(obs_arena_create(n_instances, n_qubits, observables),
-                                                        obs_arena_destroy);
+    obs.reset(obs_arena_create(n_inst, n_qubits.data(), observables));
+}
 
-    // ── device work ──
+void BatchSession::enqueue() {
+    auto& C = ctx();
+    auto state_of = [&](const Inst& I, size_t off) {
+        qhip_state s;
+        s.n = I.n;
+        s.precision = precision;
+        s.d = d_states + off;
+        s.owned = false;
+        return s;
+    };
     for (Inst& I : inst) {
         if (!I.batched) continue;
         qhip_state s = state_of(I, I.psi);
@@ -264,7 +318,7 @@ void batch_energy_grad(const qhip_config* cfg, int n_instances, const int32_t* n
         launch_tile_batch(precision, false, reinterpret_cast<const BatchItem*>(d_desc + S.items_off),
                           reinterpret_cast<const int64_t*>(d_desc + S.first_off), static_cast<int>(S.items.size()),
                           S.total, S.tmax, S.kmax, C.stream);
-    for (int i = 0; i < n_instances; ++i) {
+    for (int i = 0; i < n_inst; ++i) {
         const Inst& I = inst[i];
         if (!I.batched) continue;
         qhip_state psi = state_of(I, I.psi), phi = state_of(I, I.phi);
@@ -278,16 +332,42 @@ void batch_energy_grad(const qhip_config* cfg, int n_instances, const int32_t* n
         launch_ip_reduce(reinterpret_cast<const IpReduce*>(d_desc + S.jobs_off), static_cast<int>(S.jobs.size()),
                          C.stream);
     }
-    std::vector<double2> h_energy(n_instances), h_tot(n_tot + 1);
-    QHIP_CHECK(hipMemcpyAsync(h_energy.data(), d_energy, sizeof(double2) * n_instances, hipMemcpyDeviceToHost,
+}
+
+void BatchSession::run(double* energies, double* grads, const int64_t* grad_offsets, bool use_graph) {
+    auto& C = ctx();
+    if (use_graph) {
+        if (!graph) {
+            hipGraph_t g = nullptr;
+            QHIP_CHECK(hipStreamBeginCapture(C.stream, hipStreamCaptureModeThreadLocal));
+            try {
+                enqueue();
+            } catch (...) {
+                (void)hipStreamEndCapture(C.stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            QHIP_CHECK(hipStreamEndCapture(C.stream, &g));
+            hipGraphExec_t exec = nullptr;
+            const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            QHIP_CHECK(e);
+            graph = exec;
+        }
+        QHIP_CHECK(hipGraphLaunch(graph, C.stream));
+    } else {
+        enqueue();
+    }
+    std::vector<double2> h_energy(n_inst), h_tot(n_tot + 1);
+    QHIP_CHECK(hipMemcpyAsync(h_energy.data(), d_energy, sizeof(double2) * n_inst, hipMemcpyDeviceToHost,
                               C.stream));
     QHIP_CHECK(hipMemcpyAsync(h_tot.data(), d_tot, sizeof(double2) * (n_tot + 1), hipMemcpyDeviceToHost, C.stream));
     QHIP_CHECK(hipStreamSynchronize(C.stream));
-    for (int i = 0; i < n_instances; ++i) {
+    for (int i = 0; i < n_inst; ++i) {
         const Inst& I = inst[i];
         if (!I.batched) continue;
         energies[i] = h_energy[i].x;
-        if (want_grad) {
+        if (want_grad && grads) {
             std::vector<cplx> ip(I.slots.size());
             for (size_t s = 0; s < ip.size(); ++s) ip[s] = cplx(h_tot[I.tot + s].x, h_tot[I.tot + s].y);
             accumulate_gradient(&streams[i], I.slots, ip.data(), n_params[i], grads + grad_offsets[i]);
@@ -295,7 +375,7 @@ void batch_energy_grad(const qhip_config* cfg, int n_instances, const int32_t* n
     }
 
     // ── instances outside the batched kernels, one by one ──
-    for (int i = 0; i < n_instances; ++i) {
+    for (int i = 0; i < n_inst; ++i) {
         if (inst[i].batched) continue;
         qhip_state st;
         st.n = inst[i].n;
@@ -306,7 +386,7 @@ void batch_energy_grad(const qhip_config* cfg, int n_instances, const int32_t* n
         }
         try {
             state_reset(&st);
-            if (want_grad) {
+            if (want_grad && grads) {
                 run_gradient(cfg, &streams[i], n_params[i], &st, &observables[i], nullptr,
                              grads + grad_offsets[i], &energies[i]);
             } else {
@@ -321,6 +401,82 @@ void batch_energy_grad(const qhip_config* cfg, int n_instances, const int32_t* n
         QHIP_CHECK(hipStreamSynchronize(C.stream));
         QHIP_CHECK(hipFree(st.d));
     }
+}
+
+bool BatchSession::update(const qhip_stream* new_streams) {
+    streams = new_streams;
+    std::vector<Inst> fresh;
+    plan_all(fresh);
+    bool same = fresh.size() == inst.size();
+    for (size_t i = 0; same && i < fresh.size(); ++i)
+        same = fresh[i].batched == inst[i].batched && fresh[i].slots.size() == inst[i].slots.size() &&
+               same_structure(fresh[i].fwd, inst[i].fwd) && same_structure(fresh[i].back, inst[i].back);
+    if (same) {
+        std::vector<cplx> coefs;
+        for (size_t i = 0; i < fresh.size(); ++i) {
+            if (!fresh[i].batched) continue;
+            coefs.insert(coefs.end(), fresh[i].fwd.coefs.begin(), fresh[i].fwd.coefs.end());
+            coefs.insert(coefs.end(), fresh[i].back.coefs.begin(), fresh[i].back.coefs.end());
+            inst[i].fwd.coefs = std::move(fresh[i].fwd.coefs);
+            inst[i].back.coefs = std::move(fresh[i].back.coefs);
+            inst[i].slots = std::move(fresh[i].slots);
+        }
+        QHIP_CHECK(hipStreamSynchronize(ctx().stream));
+        upload_coefs(coefs);
+        return false;
+    }
+    // New structure: rebuild everything (the graph, if any, is captured again).
+    if (graph) {
+        QHIP_CHECK(hipStreamSynchronize(ctx().stream));
+        (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+    }
+    for (DevArena* a : {&tables, &states, &results, &desc, &ip_area}) a->reset();
+    obs.reset();
+    build();
+    return true;
+}
+
+BatchSession* batch_create(const qhip_config* cfg, int n_instances, const int32_t* n_qubits,
+                           const qhip_stream* streams, const qhip_observable* observables, const int32_t* n_params,
+                           int precision, bool gradients) {
+    if (precision != QHIP_C128 && precision != QHIP_C64) fail(QHIP_INVALID, "batch: bad precision");
+    if (n_instances < 0) fail(QHIP_INVALID, "batch: n_instances must be >= 0");
+    auto S = std::make_unique<BatchSession>();
+    if (cfg) {
+        S->cfg_copy = *cfg;
+        S->cfg = &S->cfg_copy;
+    }
+    S->n_inst = n_instances;
+    S->precision = precision;
+    S->want_grad = gradients;
+    S->amp = precision == QHIP_C128 ? 16 : 8;
+    S->n_qubits.assign(n_qubits, n_qubits + n_instances);
+    S->n_params.assign(n_params ? n_params : n_qubits, (n_params ? n_params : n_qubits) + n_instances);
+    if (!n_params) std::fill(S->n_params.begin(), S->n_params.end(), 0);
+    S->streams = streams;
+    S->observables = observables;
+    if (n_instances > 0) S->build();
+    return S.release();
+}
+
+void batch_run(BatchSession* S, double* energies, double* grads, const int64_t* grad_offsets, bool use_graph) {
+    if (S->n_inst == 0) return;
+    if (grads && !S->want_grad) fail(QHIP_INVALID, "batch: the session was created without gradients");
+    S->run(energies, grads, grad_offsets, use_graph);
+}
+
+bool batch_update(BatchSession* S, const qhip_stream* streams) { return S->n_inst ? S->update(streams) : false; }
+
+void batch_destroy(BatchSession* S) { delete S; }
+
+void batch_energy_grad(const qhip_config* cfg, int n_instances, const int32_t* n_qubits,
+                       const qhip_stream* streams, const qhip_observable* observables,
+                       const int32_t* n_params, int precision, double* energies, double* grads,
+                       const int64_t* grad_offsets) {
+    std::unique_ptr<BatchSession> S(
+        batch_create(cfg, n_instances, n_qubits, streams, observables, n_params, precision, grads != nullptr));
+    batch_run(S.get(), energies, grads, grad_offsets, false);
 }
 
 }  // namespace qhip

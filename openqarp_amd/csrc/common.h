@@ -304,6 +304,13 @@ ScheduleOptions schedule_options(const qhip_config* cfg, bool two_states);
 void run_gradient(const qhip_config* cfg, const qhip_stream* stream, int n_params,
                   qhip_state* psi, const qhip_observable* obs, const qhip_state* phi_in,
                   double* grad, double* energy);
+struct BatchSession;
+BatchSession* batch_create(const qhip_config* cfg, int n_instances, const int32_t* n_qubits,
+                           const qhip_stream* streams, const qhip_observable* observables, const int32_t* n_params,
+                           int precision, bool gradients);
+void batch_run(BatchSession* s, double* energies, double* grads, const int64_t* grad_offsets, bool use_graph);
+bool batch_update(BatchSession* s, const qhip_stream* streams);
+void batch_destroy(BatchSession* s);
 void batch_energy_grad(const qhip_config* cfg, int n_instances, const int32_t* n_qubits,
                        const qhip_stream* streams, const qhip_observable* observables,
                        const int32_t* n_params, int precision, double* energies, double* grads,

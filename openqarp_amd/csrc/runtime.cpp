@@ -758,4 +758,36 @@ int qhip_batch_energy_grad(const qhip_config* cfg, int32_t n_instances, const in
     });
 }
 
+int qhip_batch_create(const qhip_config* cfg, int32_t n_instances, const int32_t* n_qubits,
+                      const qhip_stream* streams, const qhip_observable* observables, const int32_t* n_params,
+                      int32_t precision, int32_t gradients, qhip_batch** out) {
+    return guarded([&] {
+        need(out && n_instances >= 0 && (n_instances == 0 || (n_qubits && streams && observables)),
+             "qhip_batch_create: NULL argument");
+        need(!gradients || n_instances == 0 || n_params, "qhip_batch_create: gradients need n_params");
+        *out = reinterpret_cast<qhip_batch*>(qhip::batch_create(cfg, n_instances, n_qubits, streams, observables,
+                                                                n_params, precision, gradients != 0));
+    });
+}
+
+int qhip_batch_run(qhip_batch* b, double* energies, double* grads, const int64_t* grad_offsets, int32_t use_graph) {
+    return guarded([&] {
+        need(b && energies, "qhip_batch_run: NULL argument");
+        need(!grads || grad_offsets, "qhip_batch_run: grads need offsets");
+        qhip::batch_run(reinterpret_cast<qhip::BatchSession*>(b), energies, grads, grad_offsets, use_graph != 0);
+    });
+}
+
+int qhip_batch_update(qhip_batch* b, const qhip_stream* streams, int32_t* rebuilt) {
+    return guarded([&] {
+        need(b && streams, "qhip_batch_update: NULL argument");
+        const bool r = qhip::batch_update(reinterpret_cast<qhip::BatchSession*>(b), streams);
+        if (rebuilt) *rebuilt = r ? 1 : 0;
+    });
+}
+
+int qhip_batch_destroy(qhip_batch* b) {
+    return guarded([&] { qhip::batch_destroy(reinterpret_cast<qhip::BatchSession*>(b)); });
+}
+
 }  // extern "C"

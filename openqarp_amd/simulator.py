@@ -342,30 +342,15 @@ class QarpSimulator:
         batched launches (qhip_batch_energy_grad).  ``instances`` holds tuples
         (commands, n_qubits, observable, params, param_order).  Returns
         (energies array, list of gradient arrays or None)."""
-        from ._abi import QhipObservable, QhipStream
-        m = len(instances)
-        packed_s, packed_o = [], []
-        nq = np.zeros(m, dtype=np.int32)
-        npar = np.zeros(m, dtype=np.int32)
-        offs = np.zeros(m, dtype=np.int64)
-        total = 0
-        for i, (cmds, n, obs, params, order) in enumerate(instances):
-            index = {name: k for k, name in enumerate(order)}
-            packed_s.append(pack_stream(cmds, values=params, param_index=index))
-            packed_o.append(pack_observable(obs))
-            nq[i], npar[i], offs[i] = int(n), len(order), total
-            total += len(order)
-        streams = (QhipStream * max(m, 1))(*[p.struct for p in packed_s])
-        observables = (QhipObservable * max(m, 1))(*[p.struct for p in packed_o])
-        energies = np.zeros(m)
-        grads = np.zeros(max(total, 1)) if gradients else None
+        b = _PackedBatch(instances)
+        energies = np.zeros(b.m)
+        grads = np.zeros(max(b.total, 1)) if gradients else None
         cfg = self._config()
         _lib.check(_lib.load().qhip_batch_energy_grad(
-            C.byref(cfg), m, nq.ctypes.data, streams, observables, npar.ctypes.data,
+            C.byref(cfg), b.m, b.nq.ctypes.data, b.streams, b.observables, b.npar.ctypes.data,
             QHIP_C128 if self.precision == "complex128" else QHIP_C64, energies.ctypes.data,
-            grads.ctypes.data if gradients else None, offs.ctypes.data))
-        out = [grads[offs[i]:offs[i] + npar[i]].copy() for i in range(m)] if gradients else None
-        return energies, out
+            grads.ctypes.data if gradients else None, b.offs.ctypes.data))
+        return energies, b.split(grads) if gradients else None
 
     def fused_stream(self, commands: Sequence[Command], n_qubits: int) -> list:
         """The fusion pass output: list of (gate id, qubits, params, matrix-or-None)."""
@@ -393,6 +378,82 @@ class QarpSimulator:
             return out
         finally:
             lib.qhip_fused_destroy(h)
+
+
+class _PackedBatch:
+    """Instances (commands, n_qubits, observable, params, param_order) as the
+    C structs of qhip_batch_* see them (buffers kept alive here)."""
+
+    def __init__(self, instances: Sequence):
+        from ._abi import QhipObservable
+        self.instances = list(instances)
+        self.m = len(self.instances)
+        self.nq = np.zeros(self.m, dtype=np.int32)
+        self.npar = np.zeros(self.m, dtype=np.int32)
+        self.offs = np.zeros(self.m, dtype=np.int64)
+        total = 0
+        self.packed_o = []
+        for i, (cmds, n, obs, params, order) in enumerate(self.instances):
+            self.packed_o.append(pack_observable(obs))
+            self.nq[i], self.npar[i], self.offs[i] = int(n), len(order), total
+            total += len(order)
+        self.total = total
+        self.observables = (QhipObservable * max(self.m, 1))(*[p.struct for p in self.packed_o])
+        self.bind([inst[3] for inst in self.instances])
+
+    def bind(self, values: Sequence[Mapping[str, float]]) -> None:
+        from ._abi import QhipStream
+        self.packed_s = []
+        for (cmds, n, obs, _, order), vals in zip(self.instances, values):
+            index = {name: k for k, name in enumerate(order)}
+            self.packed_s.append(pack_stream(cmds, values=vals, param_index=index))
+        self.streams = (QhipStream * max(self.m, 1))(*[p.struct for p in self.packed_s])
+
+    def split(self, grads: np.ndarray) -> list:
+        return [grads[self.offs[i]:self.offs[i] + self.npar[i]].copy() for i in range(self.m)]
+
+
+class BatchSession:
+    """Independent instances kept on the device for parameter sweeps
+    (qhip_batch_*): ``run`` evaluates energies (and adjoint gradients) of
+    every instance, replaying the sweep as one captured hipGraph when
+    ``use_graph``; ``update(values)`` rebinds one parameter dict per
+    instance (True when the session had to be rebuilt)."""
+
+    def __init__(self, sim: QarpSimulator, instances: Sequence, gradients: bool = True):
+        self.sim = sim
+        self.gradients = bool(gradients)
+        self._b = _PackedBatch(instances)
+        self._h = None
+        h = C.c_void_p()
+        cfg = sim._config()
+        _lib.check(_lib.load().qhip_batch_create(
+            C.byref(cfg), self._b.m, self._b.nq.ctypes.data, self._b.streams, self._b.observables,
+            self._b.npar.ctypes.data, QHIP_C128 if sim.precision == "complex128" else QHIP_C64,
+            int(self.gradients), C.byref(h)))
+        self._h = h
+
+    def run(self, use_graph: bool = True):
+        energies = np.zeros(self._b.m)
+        grads = np.zeros(max(self._b.total, 1)) if self.gradients else None
+        _lib.check(_lib.load().qhip_batch_run(self._h, energies.ctypes.data,
+                                              grads.ctypes.data if self.gradients else None,
+                                              self._b.offs.ctypes.data, int(bool(use_graph))))
+        return energies, self._b.split(grads) if self.gradients else None
+
+    def update(self, values: Sequence[Mapping[str, float]]) -> bool:
+        if len(values) != self._b.m:
+            raise ValueError(f"update: expected {self._b.m} parameter dicts, got {len(values)}")
+        self._b.bind(values)
+        rebuilt = C.c_int32(0)
+        _lib.check(_lib.load().qhip_batch_update(self._h, self._b.streams, C.byref(rebuilt)))
+        return bool(rebuilt.value)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib._lib is not None:
+            _lib._lib.qhip_batch_destroy(h)
+            self._h = None
 
 
 class CompiledCircuit:
