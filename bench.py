@@ -168,6 +168,25 @@ def batched(n_instances=512):
         res[prec] = (e, g)
         out[f"{prec}_s"] = dt
         log(f"batched {n_instances} instances {prec}: {dt:.3f} s")
+    # Optimizer-style sweeps: the same instances kept in a session, the sweep
+    # replayed as one hipGraph after each coefficient update.
+    from openqarp_amd.simulator import BatchSession
+    sess = BatchSession(QarpSimulator(), inst)
+    sess.run(use_graph=True)   # capture
+    rng = np.random.default_rng(1)
+    vals = [{s: float(rng.uniform(0, 2 * math.pi)) for s in w.symbols} for w in ws]
+    t0 = time.perf_counter()
+    sess.update(vals)
+    t_upd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    sess.run(use_graph=True)
+    t_graph = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    sess.run(use_graph=False)
+    t_direct = time.perf_counter() - t0
+    out.update({"session_update_s": t_upd, "session_sweep_graph_s": t_graph, "session_sweep_direct_s": t_direct})
+    log(f"session sweep: update {t_upd:.3f} s, graph replay {t_graph:.3f} s, direct launches {t_direct:.3f} s")
+    del sess
     e128, g128 = res["complex128"]
     e64, g64 = res["complex64"]
     de = np.abs(e64 - e128)
