@@ -69,6 +69,8 @@ static void group_positions(DevOp& d, int T) {
 }
 
 // MFMA fragments of a dense k-qubit op (k = 3..5) for v_mfma_f64_16x16x4.
+// k = 4, 5 use three real products (see the k >= 4 branch); k = 3 the
+// realified matrix below.
 // Real form: M[(b,q)][(a,p)] over output amplitude b / part q and input
 // amplitude a / part p (part 0 = re, 1 = im): re U_ba, -im U_ba, im U_ba,
 // re U_ba for (q,p) = (0,0), (0,1), (1,0), (1,1).  The instruction holds
@@ -80,7 +82,25 @@ static void group_positions(DevOp& d, int T) {
 // c + 4, ....  Stored as complex pairs (steps 2t, 2t+1) at
 // [(mt * D/4 + t) * 64 + lane].
 void mfma_fragments(const std::vector<cplx>& U, int k, std::vector<cplx>& out) {
-    const int D = 1 << k, NT = D / 8, S = D / 2;
+    const int D = 1 << k;
+    if (k >= 4) {   // three real products (Gauss), layout of the k = 4, 5 kernel
+        const int NT = D / 16, HP = D / 8;
+        auto u = [&](int b, int a) { return U[static_cast<size_t>(b) * D + a]; };
+        for (int mt = 0; mt < NT; ++mt)
+            for (int t = 0; t < HP; ++t)
+                for (int p = 0; p < 3; ++p)
+                    for (int lane = 0; lane < 64; ++lane) {
+                        const int b = 16 * mt + lane % 16, c = lane / 16;
+                        double v[2];
+                        for (int h = 0; h < 2; ++h) {
+                            const cplx e = u(b, 4 * (2 * t + h) + c);
+                            v[h] = p == 0 ? e.real() : p == 1 ? e.imag() : e.real() + e.imag();
+                        }
+                        out.emplace_back(v[0], v[1]);
+                    }
+        return;
+    }
+    const int NT = D / 8, S = D / 2;
     auto M = [&](int b, int q, int a, int p) -> double {
         const cplx u = U[static_cast<size_t>(b) * D + a];
         if (q == 0) return p == 0 ? u.real() : -u.imag();
@@ -99,8 +119,22 @@ void mfma_fragments(const std::vector<cplx>& U, int k, std::vector<cplx>& out) {
 
 // Inverse of mfma_fragments (schedule inspection).
 std::vector<cplx> mfma_matrix(const cplx* f, int k) {
-    const int D = 1 << k, NT = D / 8, S = D / 2;
+    const int D = 1 << k;
     std::vector<cplx> U(static_cast<size_t>(D) * D);
+    if (k >= 4) {
+        const int NT = D / 16, HP = D / 8;
+        for (int mt = 0; mt < NT; ++mt)
+            for (int t = 0; t < HP; ++t)
+                for (int lane = 0; lane < 64; ++lane) {
+                    const int b = 16 * mt + lane % 16, c = lane / 16;
+                    const cplx re = f[((mt * HP + t) * 3 + 0) * 64 + lane];
+                    const cplx im = f[((mt * HP + t) * 3 + 1) * 64 + lane];
+                    U[static_cast<size_t>(b) * D + 4 * (2 * t) + c] = cplx(re.real(), im.real());
+                    U[static_cast<size_t>(b) * D + 4 * (2 * t + 1) + c] = cplx(re.imag(), im.imag());
+                }
+        return U;
+    }
+    const int NT = D / 8, S = D / 2;
     for (int mt = 0; mt < NT; ++mt)
         for (int t = 0; t < S / 2; ++t)
             for (int lane = 0; lane < 64; ++lane) {

@@ -311,13 +311,66 @@ __device__ __forceinline__ void dense_mfma(double2* s, uint32_t zoff, int G, con
     }
 }
 
+// k = 4, 5: the complex product as three real ones (Gauss):
+// P1 = Ur·xr, P2 = Ui·xi, P3 = (Ur + Ui)·(xr + xi); y = (P1 − P2) + i(P3 − P1 − P2).
+// Real 16-row M-tiles over output amplitudes (row i = 4r + g of tile mt is
+// amplitude 16mt + 4r + g), K-steps over input amplitudes (step s, column c
+// is amplitude 4s + c, so lane group c feeds amplitudes c, c + 4, ...): D/16
+// tiles x D/4 steps x 3 products = 3/4 of the MFMAs of the realified form.
+// Fragments: per tile and pair of steps, (Ur, Ui, Ur + Ui) as three 16-B
+// pairs per lane at [((mt * D/8 + t) * 3 + p) * 64 + lane].
+template <int K>
+__device__ __forceinline__ void dense_mfma3(double2* s, uint32_t zoff, int G, const OpRegs& op,
+                                            const double2* __restrict__ F) {
+    constexpr int D = 1 << K, NT = D / 16, H = D / 4, HP = H / 2;
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6, j = lane & 15u, c = lane >> 4;
+    uint32_t lp = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) lp |= ((j >> b) & 1u) << op.g(b);
+    lp = (swz(lp) << 4) | zoff;
+    uint32_t so_in[H];
+#pragma unroll
+    for (int t = 0; t < H; ++t) so_in[t] = swz(local_offset<K>(c + 4u * t, op)) << 4;
+    const uint32_t chunks = 1u << (G - 4);
+    for (uint32_t ch = w; ch < chunks; ch += kWaves) {
+        uint32_t up = 0;
+        for (int b = 4; b < G; ++b) up |= ((ch >> (b - 4)) & 1u) << op.g(b);
+        const uint32_t base = lp ^ (swz(up) << 4);
+        double2 x[H];
+#pragma unroll
+        for (int t = 0; t < H; ++t) x[t] = at(s, base ^ so_in[t]);
+#pragma unroll(K < 5 ? NT : 1)
+        for (int mt = 0; mt < NT; ++mt) {
+            mfma_d4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = p1, p3 = p1;
+#pragma unroll
+            for (int t = 0; t < HP; ++t) {
+                const double2* f = F + ((mt * HP + t) * 3) * 64 + lane;
+                const double2 fr = f[0], fi = f[64], fs = f[128];
+                const double2 x0 = x[2 * t], x1 = x[2 * t + 1];
+                p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(fr.x, x0.x, p1, 0, 0, 0);
+                p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(fi.x, x0.y, p2, 0, 0, 0);
+                p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(fs.x, x0.x + x0.y, p3, 0, 0, 0);
+                p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(fr.y, x1.x, p1, 0, 0, 0);
+                p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(fi.y, x1.y, p2, 0, 0, 0);
+                p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(fs.y, x1.x + x1.y, p3, 0, 0, 0);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const uint32_t b = 16u * static_cast<uint32_t>(mt) + 4u * r + c;
+                at(s, base ^ (swz(local_offset<K>(b, op)) << 4)) =
+                    make_double2(p1[r] - p2[r], p3[r] - p1[r] - p2[r]);
+            }
+        }
+    }
+}
+
 template <typename A>
 __device__ __forceinline__ void mfma_op(A* s, uint32_t zoff, int G, const OpRegs& op, const A* __restrict__ F) {
     if constexpr (std::is_same_v<A, double2>) {
         switch (op.k) {
             case 3: dense_mfma<3>(s, zoff, G, op, F); break;
-            case 4: dense_mfma<4>(s, zoff, G, op, F); break;
-            case 5: dense_mfma<5>(s, zoff, G, op, F); break;
+            case 4: dense_mfma3<4>(s, zoff, G, op, F); break;
+            case 5: dense_mfma3<5>(s, zoff, G, op, F); break;
             default: break;
         }
     }
