@@ -67,7 +67,7 @@ def fused_apply(args, sim):
     log(f"brickwork n={n} layers={args.layers}: {len(cmds)} gates; autotuning")
     widths = (args.fusion,) if args.fusion else (2, 3, 4, 5)
     tuned = tune(sim, cmds, state, widths=widths, tiles=(args.tile,) if args.tile else None,
-                 lows=(args.low,) if args.low else (4, 5))
+                 lows=(args.low,) if args.low else None)
     best = tuned["best"]
     for t in tuned["trials"]:
         log(f"  trial k={t['fusion_max_qubits']} T={t['tile_qubits']} low={t['low_qubits']}: "

@@ -29,10 +29,17 @@ def fingerprint(commands: Sequence[Command]) -> str:
     return h.hexdigest()
 
 
+def default_lows(precision: str) -> tuple:
+    """Contiguous low qubits tried: runs of 128 B and 256 B per tile row
+    (8 or 16 complex128 amplitudes, 16 or 32 complex64)."""
+    return (3, 4) if precision == "complex128" else (4, 5)
+
+
 def candidates(n: int, precision: str, widths: Iterable[int] = (2, 3, 4, 5),
-               tiles: Optional[Iterable[int]] = None, lows: Iterable[int] = (4, 5)) -> list:
+               tiles: Optional[Iterable[int]] = None, lows: Optional[Iterable[int]] = None) -> list:
     base_t = 12 if precision == "complex128" else 13
     tiles = tuple(tiles) if tiles else (base_t - 1, base_t)
+    lows = tuple(lows) if lows else default_lows(precision)
     out = []
     for k in widths:
         for T in tiles:
@@ -44,8 +51,9 @@ def candidates(n: int, precision: str, widths: Iterable[int] = (2, 3, 4, 5),
 
 def tune(sim: QarpSimulator, commands: Sequence[Command], state: DeviceState,
          widths: Iterable[int] = (2, 3, 4, 5), repeats: int = 1, tiles: Optional[Iterable[int]] = None,
-         lows: Iterable[int] = (4, 5)) -> dict:
-    widths, lows = tuple(widths), tuple(lows)
+         lows: Optional[Iterable[int]] = None) -> dict:
+    widths = tuple(widths)
+    lows = tuple(lows) if lows else default_lows(sim.precision)
     tiles = tuple(tiles) if tiles else None
     key = (state.n_qubits, sim.precision, fingerprint(commands), widths, tiles, lows)
     if key in _cache:

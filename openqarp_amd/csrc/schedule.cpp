@@ -5,10 +5,11 @@
 // qubits, for coalesced 16-B loads, plus up to T-low other qubits), applies a
 // run of ops in LDS, and writes the tile back.  A dense op can join a pass
 // only if all its operands are tile qubits; diagonal / mask-phase / scalar
-// ops know each amplitude's full index and join any pass.  Ops are taken in
-// program order; an op that cannot join blocks its qubits for the rest of the
-// pass, so every later op touching them waits for the next pass (ops on
-// disjoint qubits commute, which is what lets them overtake).
+// ops know each amplitude's full index and join any pass.  An op may run
+// once every earlier op sharing a qubit with it has run (ops on disjoint
+// qubits commute, which is what lets them overtake); each pass takes the
+// tile qubits that let the most such ops run (pick_pass below), and runs
+// them in program order.
 #include "common.h"
 
 #include <algorithm>
@@ -301,53 +302,150 @@ static void emit_block(Plan& plan, const std::vector<HostOp>& ops, const Unit& u
 // their own, straight from HBM.
 constexpr int kMaxTileK = 5;
 
+// Pass selection.  Two ops conflict when they share a qubit; an op may run
+// in a pass once every earlier conflicting op has run in an earlier pass or
+// earlier in the same one (ops on disjoint qubits commute).  A pass is built
+// best-fit: starting from the forced low qubits (plus a seed op's qubits),
+// every ready op whose qubits are already tile qubits is taken, then the
+// ready op that adds the fewest new tile qubits, until none fits in T.  The
+// pass applying the most ops over a few seeds (none, and each of the first
+// ready dense ops) wins.  Candidates come from a window of the first
+// kWindow remaining ops, which keeps the cost linear in the circuit length.
+namespace {
+constexpr size_t kWindow = 512;
+constexpr int kSeeds = 16;
+
+struct PassPick {
+    std::vector<size_t> chosen;   // positions in `remaining`, ascending
+    uint64_t Q = 0;
+};
+
+struct SegmentDag {
+    const std::vector<HostOp>& ops;
+    std::vector<std::vector<int>> preds;   // per segment position: earlier conflicting positions
+    std::vector<char> done;
+    SegmentDag(const std::vector<HostOp>& o, const std::vector<size_t>& seg) : ops(o), preds(seg.size()),
+                                                                               done(seg.size(), 0) {
+        int last[64];
+        for (int q = 0; q < 64; ++q) last[q] = -1;
+        for (size_t i = 0; i < seg.size(); ++i) {
+            uint64_t qm = ops[seg[i]].qmask();
+            while (qm) {
+                const int q = std::countr_zero(qm);
+                qm &= qm - 1;
+                if (last[q] >= 0 && (preds[i].empty() || preds[i].back() != last[q])) preds[i].push_back(last[q]);
+                last[q] = static_cast<int>(i);
+            }
+        }
+    }
+};
+
+PassPick pick_pass(const SegmentDag& dag, const std::vector<size_t>& seg, const std::vector<int>& remaining,
+                   uint64_t Q0, int T, int seed, const ScheduleOptions& opt) {
+    const size_t W = std::min(remaining.size(), kWindow);
+    std::vector<char> in_pass(seg.size(), 0);
+    PassPick pk;
+    pk.Q = Q0;
+    double flops = 0.0;
+    int n_ip = 0, n_dense = 0;
+    auto ready = [&](int i) {
+        for (int p : dag.preds[i])
+            if (!dag.done[p] && !in_pass[p]) return false;
+        return true;
+    };
+    auto take = [&](size_t r) {
+        const HostOp& op = dag.ops[seg[remaining[r]]];
+        in_pass[remaining[r]] = 1;
+        pk.chosen.push_back(r);
+        pk.Q |= op.needs_local() ? op.qmask() : 0;
+        flops += op_flops(op);
+        n_ip += op.kind == OP_IP_DENSE || op.kind == OP_IP_DIAG;
+        n_dense += op.needs_local();
+    };
+    auto admissible = [&](const HostOp& op) {
+        if ((op.kind == OP_IP_DENSE || op.kind == OP_IP_DIAG) && n_ip >= kMaxIpPerPass) return false;
+        if (opt.max_ops > 0 && n_dense >= opt.max_ops && op.needs_local()) return false;
+        if (opt.max_flops_per_amp > 0 && !pk.chosen.empty() && flops + op_flops(op) > opt.max_flops_per_amp)
+            return false;
+        return true;
+    };
+    if (seed >= 0) {
+        pk.Q |= dag.ops[seg[remaining[seed]]].qmask();
+        take(static_cast<size_t>(seed));
+    }
+    for (;;) {
+        bool progress = false;
+        size_t best = W;
+        int best_need = 65;
+        for (size_t r = 0; r < W; ++r) {
+            const int i = remaining[r];
+            if (in_pass[i] || !ready(i)) continue;
+            const HostOp& op = dag.ops[seg[i]];
+            if (!admissible(op)) continue;
+            const int need = op.needs_local() ? std::popcount(op.qmask() & ~pk.Q) : 0;
+            if (need == 0) {
+                take(r);
+                progress = true;
+            } else if (std::popcount(pk.Q) + need <= T && need < best_need) {
+                best = r;
+                best_need = need;
+            }
+        }
+        if (progress) continue;
+        if (best == W) break;
+        take(best);
+    }
+    std::sort(pk.chosen.begin(), pk.chosen.end());
+    return pk;
+}
+}  // namespace
+
 static void schedule_segment(Plan& plan, const std::vector<HostOp>& ops, const std::vector<size_t>& seg, int n,
                              int T, int low, const ScheduleOptions& opt) {
     const uint64_t low_mask = low >= 64 ? ~uint64_t{0} : ((uint64_t{1} << low) - 1);
     const uint64_t all_mask = n >= 64 ? ~uint64_t{0} : ((uint64_t{1} << n) - 1);
-    const int max_ip = kMaxIpPerPass;
-    std::vector<size_t> remaining = seg;
+    SegmentDag dag(ops, seg);
+    std::vector<int> remaining(seg.size());
+    for (size_t i = 0; i < seg.size(); ++i) remaining[i] = static_cast<int>(i);
     while (!remaining.empty()) {
-        // The first op of a pass always fits: it may displace forced low
-        // qubits (the tile is then less coalesced, never wrong), so every
-        // pass makes progress.
-        uint64_t Q = low_mask;
-        {
-            const HostOp& first = ops[remaining.front()];
-            if (first.needs_local()) {
-                const uint64_t qm = first.qmask();
-                if (std::popcount(Q | qm) > T) {
-                    Q = qm;
-                    for (int q = 0; q < n && std::popcount(Q) < T && q < low; ++q) Q |= uint64_t{1} << q;
-                }
-            }
+        PassPick best = pick_pass(dag, seg, remaining, low_mask, T, -1, opt);
+        int seeds = 0;
+        const size_t W = std::min(remaining.size(), kWindow);
+        for (size_t r = 0; r < W && seeds < kSeeds; ++r) {
+            const int i = remaining[r];
+            const HostOp& op = ops[seg[i]];
+            if (!op.needs_local()) continue;
+            bool ready = true;
+            for (int p : dag.preds[i]) ready = ready && dag.done[p];
+            if (!ready) continue;
+            ++seeds;
+            if (std::popcount(low_mask | op.qmask()) > T) continue;
+            PassPick pk = pick_pass(dag, seg, remaining, low_mask, T, static_cast<int>(r), opt);
+            if (pk.chosen.size() > best.chosen.size()) best = std::move(pk);
         }
-        uint64_t blocked = 0;
-        std::vector<size_t> chosen, deferred;
-        double flops = 0.0;
-        int n_ip = 0, n_dense = 0;
-        for (size_t idx : remaining) {
-            const HostOp& op = ops[idx];
-            const uint64_t qm = op.qmask();
-            if (qm & blocked) { blocked |= qm; deferred.push_back(idx); continue; }
-            const bool is_ip = op.kind == OP_IP_DENSE || op.kind == OP_IP_DIAG;
-            bool fits = true;
-            if (is_ip && n_ip >= max_ip) fits = false;
-            if (opt.max_ops > 0 && n_dense >= opt.max_ops && op.needs_local()) fits = false;
-            if (opt.max_flops_per_amp > 0 && !chosen.empty() &&
-                flops + op_flops(op) > opt.max_flops_per_amp)
-                fits = false;
-            uint64_t need = 0;
-            if (fits && op.needs_local()) {
-                need = qm & ~Q;
-                if (std::popcount(Q | need) > T) fits = false;
+        if (best.chosen.empty()) {
+            // The first remaining op (always ready) needs more than the free
+            // tile qubits: it displaces forced low qubits (the tile is then
+            // less coalesced, never wrong), so every pass makes progress.
+            const uint64_t qm = ops[seg[remaining.front()]].qmask();
+            uint64_t Q0 = qm;
+            for (int q = 0; q < n && std::popcount(Q0) < T && q < low; ++q) Q0 |= uint64_t{1} << q;
+            best = pick_pass(dag, seg, remaining, Q0, T, 0, opt);
+        }
+        uint64_t Q = best.Q;
+        std::vector<size_t> chosen;
+        chosen.reserve(best.chosen.size());
+        std::vector<int> rest;
+        rest.reserve(remaining.size() - best.chosen.size());
+        size_t c = 0;
+        for (size_t r = 0; r < remaining.size(); ++r) {
+            if (c < best.chosen.size() && best.chosen[c] == r) {
+                chosen.push_back(seg[remaining[r]]);
+                dag.done[remaining[r]] = 1;
+                ++c;
+            } else {
+                rest.push_back(remaining[r]);
             }
-            if (!fits) { blocked |= qm; deferred.push_back(idx); continue; }
-            Q |= need;
-            chosen.push_back(idx);
-            flops += op_flops(op);
-            n_ip += is_ip;
-            n_dense += op.needs_local();
         }
         // Fill the tile up to T qubits with the lowest unused ones.
         for (int q = 0; q < n && std::popcount(Q) < T; ++q) Q |= uint64_t{1} << q;
@@ -383,7 +481,7 @@ static void schedule_segment(Plan& plan, const std::vector<HostOp>& ops, const s
         P.op_end = static_cast<int32_t>(plan.ops.size());
         if (chosen.empty()) fail(QHIP_RUNTIME, "schedule: no op could join a pass");
         plan.passes.push_back(P);
-        remaining.swap(deferred);
+        remaining.swap(rest);
     }
 }
 

@@ -160,6 +160,41 @@ def test_register_blocks_at_22_qubits(fusion, tile):
     assert np.abs(sim.statevector(cmds, n) - ref).max() < ATOL
 
 
+@pytest.mark.parametrize("precision,tile", [("complex128", 12), ("complex64", 13)])
+def test_largest_tiles(precision, tile):
+    """64-KiB tiles (2^12 complex128, 2^13 complex64 amplitudes): dense
+    k = 1..5 (matrix cores on and off in complex128), register blocks, the
+    adjoint's two-state passes, and a batched step whose small instances
+    share the largest tile array."""
+    n = 16
+    tol = ATOL if precision == "complex128" else 2e-5
+    cmds = W.brickwork_haar(n, 10, 17) + random_circuit(n, 50, 3)
+    for k in (1, 2, 3, 4, 5):
+        for mfma in ((-1, 0) if precision == "complex128" else (-1,)):
+            sim = gpu_sim(tile_qubits=tile, precision=precision, use_mfma=mfma)
+            sim.fusion_max_qubits, sim.fusion_min_qubits = k, 0
+            ref = O.statevector(pack_stream(cmds), n, k, 0)
+            assert np.abs(sim.statevector(cmds, n) - ref).max() < tol, (k, mfma)
+    w = W.c3_gradient(14, 3, 6, 40)
+    sim = gpu_sim(tile_qubits=min(tile, 14), precision=precision)
+    g = np.array(sim.run_gradient(w.commands, 14, w.observable, w.values, w.symbols))
+    idx = {s: i for i, s in enumerate(w.symbols)}
+    ref = O.run_gradient(pack_stream(w.commands, w.values, idx), 14, pack_observable(w.observable), len(idx))
+    assert np.abs(g - ref).max() < (1e-10 if precision == "complex128" else 1e-4)
+    inst = [(b.commands, b.n_qubits, b.observable, b.values, b.symbols) for b in W.batched_vqe(10, 2, 5, 14, 1, 4)]
+    sim = gpu_sim(tile_qubits=tile, precision=precision)
+    e, gr = sim.batch_energy_grad(inst)
+    for i, (c, m, o, v, s) in enumerate(inst):
+        ix = {q: j for j, q in enumerate(s)}
+        ps = pack_stream(c, values=v, param_index=ix)
+        sv = O.statevector(ps, m, 3, 12)
+        po = pack_observable(o)
+        scale = sum(abs(complex(x)) for _, x in o)
+        te, tg = (1e-12, 1e-10) if precision == "complex128" else (2e-5 * scale, 2e-5 * scale)
+        assert abs(e[i] - O.transition(sv, sv, m, po).real) < te
+        assert np.abs(gr[i] - O.run_gradient(ps, m, po, len(s))).max() < tg
+
+
 def test_wide_fusion_blocks_6_to_8():
     n = 11
     cmds = W.brickwork_haar(n, 8, 4)

@@ -72,3 +72,24 @@ def test_thirty_qubit_brickwork_schedule_shape():
         sched = inspect_schedule(cmds, 30, fusion_max_qubits=k, fusion_min_qubits=12)
         passes = max(p for p, _ in sched) + 1
         assert passes < len(sched)
+
+
+def test_long_circuit_beyond_the_candidate_window():
+    """More ops than the scheduler's candidate window (512): the passes still
+    reproduce the circuit."""
+    n = 9
+    cmds = random_circuit(n, 900, 77) + W.brickwork_haar(n, 40, 5)
+    ref = O.statevector(pack_stream(cmds), n, 2, 0)
+    got, sched = scheduled_state(cmds, n, fusion_max_qubits=2, fusion_min_qubits=0, tile_qubits=5, low_qubits=2)
+    assert len(sched) > 512
+    assert np.abs(got - ref).max() < 1e-12
+
+
+def test_frontier_scheduling_pass_counts():
+    """Best-fit pass selection over the dependency frontier on the 30-qubit
+    brickwork: at fusion width 2 and 2^12-amplitude tiles with 3 contiguous
+    low qubits it needs at most 45 passes for the 436 fused ops (taking ops
+    strictly in program order needed 62)."""
+    cmds = W.brickwork_haar(30, 30, 2)
+    sched = inspect_schedule(cmds, 30, fusion_max_qubits=2, fusion_min_qubits=0, tile_qubits=12, low_qubits=3)
+    assert max(p for p, _ in sched) + 1 <= 45
