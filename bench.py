@@ -95,12 +95,42 @@ def fused_apply(args, sim):
     achieved = bytes_per_pass / (kernel_ms * 1e-3) / 1e9
     norm = state.norm2()
     log(f"apply: {wall * 1e3:.2f} ms/step, {passes} passes, {value:.1f} GB/s, norm2 {norm:.15f}")
+    # Streaming floor: one tile pass carrying a single 1-qubit gate (same tile
+    # shape), and the driver's device-to-device copy of the state, as the
+    # read + write references for the per-pass bandwidth.
+    if args.no_floor:
+        return {
+            "state": state, "n": n, "wall_ms": wall * 1e3, "value": value, "passes": passes,
+            "fused_ops": int(cc.stats["fused_ops"]), "gates": len(cmds), "best": best,
+            "trials": tuned["trials"], "kernel_ms": kernel_ms, "achieved": achieved,
+            "gate_equiv_gbs": len(cmds) * bytes_per_pass / wall / 1e9, "norm2": norm,
+            "flops": float(cc.stats["flops"]), "pass_floor": None, "device_copy": None,
+        }
+    from openqarp_amd.ir import Command, GateType
+    one = CompiledCircuit(sim, [Command(GateType.H, 0)], n, fusion_max_qubits=best["fusion_max_qubits"],
+                          tile_qubits=best["tile_qubits"], low_qubits=best["low_qubits"])
+    one.apply(state)
+    floor_ms = float(np.concatenate([one.apply_timed(state) for _ in range(5)]).mean())
+    other = sim.new_state(n)
+    lib = _lib.load()
+    _lib.check(lib.qhip_state_copy(other._h, state._h))
+    t0 = time.perf_counter()
+    for _ in range(5):
+        _lib.check(lib.qhip_state_copy(other._h, state._h))
+    copy_ms = (time.perf_counter() - t0) / 5 * 1e3
+    del other
+    floor_gbs = bytes_per_pass / (floor_ms * 1e-3) / 1e9
+    copy_gbs = bytes_per_pass / (copy_ms * 1e-3) / 1e9
+    log(f"one-gate tile pass {floor_ms:.3f} ms = {floor_gbs:.1f} GB/s; device copy {copy_ms:.3f} ms = {copy_gbs:.1f} GB/s")
     return {
         "state": state, "n": n, "wall_ms": wall * 1e3, "value": value, "passes": passes,
         "fused_ops": int(cc.stats["fused_ops"]), "gates": len(cmds), "best": best,
         "trials": tuned["trials"], "kernel_ms": kernel_ms, "achieved": achieved,
         "gate_equiv_gbs": len(cmds) * bytes_per_pass / wall / 1e9, "norm2": norm,
         "flops": float(cc.stats["flops"]),
+        "pass_floor": {"ms": floor_ms, "GBps": floor_gbs, "frac": floor_gbs / HBM_PEAK_GBS,
+                       "what": "one tile pass carrying a single H gate, same tile shape"},
+        "device_copy": {"ms": copy_ms, "GBps": copy_gbs, "what": "qhip_state_copy (hipMemcpy device to device)"},
     }
 
 
@@ -313,6 +343,7 @@ def main(argv=None) -> int:
     ap.add_argument("--ps-points", type=int, default=0, help="time only this many parameter-shift points")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-qaoa", action="store_true")
+    ap.add_argument("--no-floor", action="store_true", help="skip the one-gate pass and device-copy references")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the backend is single-GPU; "
@@ -342,6 +373,7 @@ def main(argv=None) -> int:
                      "frac": round(fa["achieved"] / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
                      "kernel": "tile_pass_kernel", "mean_launch_ms": round(fa["kernel_ms"], 4),
                      "bytes_per_launch": 2 * 16 * (1 << fa["n"])},
+        "pass_floor": fa["pass_floor"], "device_copy": fa["device_copy"],
         "autotune": fa["trials"],
         "expectation": ex, "adjoint": adj, "batched_vqe": bat, "qaoa": qa, "cpu_baseline": cpu,
     }
