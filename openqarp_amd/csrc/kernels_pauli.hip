@@ -17,6 +17,7 @@
 
 #include <cstring>
 #include <memory>
+#include <type_traits>
 
 #include <unordered_map>
 
@@ -117,7 +118,12 @@ __device__ __forceinline__ double2 weight(uint64_t i, const GroupDev& G, const u
     return w;
 }
 
-template <typename A>
+// HERM (bra == ket, every coefficient real, so every group operator H_x is
+// Hermitian): with h the top bit of x != 0, the terms at i and i ^ x are
+// complex conjugates, so the real part is twice the sum over i with bit h
+// clear — half the partner reads.  Only the real part of the result is
+// meaningful then (the exact imaginary part is zero).
+template <typename A, bool HERM>
 __global__ __launch_bounds__(kT) void transition_kernel(const A* __restrict__ bra, const A* __restrict__ ket,
                                                         uint64_t dim, const GroupDev* __restrict__ groups,
                                                         int n_groups, const uint64_t* __restrict__ z,
@@ -134,12 +140,23 @@ __global__ __launch_bounds__(kT) void transition_kernel(const A* __restrict__ br
     }
     for (int g = 0; g < n_groups; ++g) {
         const GroupDev G = groups[g];
+        uint64_t hb = 0;   // HERM: top bit of x (indices with it set are skipped)
+        double sc = 1.0;
+        if (HERM && G.x) {
+            hb = uint64_t{1} << (63 - __clzll(G.x));
+            sc = 2.0;
+            if (base & hb) continue;   // a bit above the chunk: uniform over the workgroup
+        }
 #pragma unroll
         for (int r = 0; r < kPer; ++r) {
             const uint64_t i = base + threadIdx.x + static_cast<uint64_t>(r) * kT;
-            if (i >= dim) continue;
+            if (i >= dim || (HERM && (i & hb))) continue;
             const double2 b = load_amp(bra, i ^ G.x);
-            const double2 w = weight(i, G, z, c);
+            double2 w = weight(i, G, z, c);
+            if (HERM) {
+                w.x *= sc;
+                w.y *= sc;
+            }
             // conj(b) * w
             s[r].x += b.x * w.x + b.y * w.y;
             s[r].y += b.x * w.y - b.y * w.x;
@@ -247,18 +264,24 @@ cplx transition(const qhip_state* bra, const qhip_state* ket, const qhip_observa
     const uint64_t dim = ket->dim();
     const int64_t blocks = static_cast<int64_t>((dim + kChunk - 1) / kChunk);
     double2* part = static_cast<double2*>(C.red.ensure(sizeof(double2) * (blocks + 1)));
-    if (ket->precision == QHIP_C128)
-        hipLaunchKernelGGL(transition_kernel<double2>, dim3(static_cast<uint32_t>(blocks)), dim3(kT), 0, C.stream,
-                           static_cast<const double2*>(bra->d), static_cast<const double2*>(ket->d), dim, dg.g,
-                           dg.n_groups, dg.z, dg.c, part);
-    else
-        hipLaunchKernelGGL(transition_kernel<float2>, dim3(static_cast<uint32_t>(blocks)), dim3(kT), 0, C.stream,
-                           static_cast<const float2*>(bra->d), static_cast<const float2*>(ket->d), dim, dg.g,
-                           dg.n_groups, dg.z, dg.c, part);
+    bool herm = bra->d == ket->d;
+    for (int64_t t = 0; herm && t < obs->n_terms; ++t) herm = obs->terms[t].coeff_im == 0.0;
+    auto go = [&](auto herm_tag) {
+        constexpr bool H = decltype(herm_tag)::value;
+        if (ket->precision == QHIP_C128)
+            hipLaunchKernelGGL((transition_kernel<double2, H>), dim3(static_cast<uint32_t>(blocks)), dim3(kT), 0,
+                               C.stream, static_cast<const double2*>(bra->d), static_cast<const double2*>(ket->d), dim,
+                               dg.g, dg.n_groups, dg.z, dg.c, part);
+        else
+            hipLaunchKernelGGL((transition_kernel<float2, H>), dim3(static_cast<uint32_t>(blocks)), dim3(kT), 0,
+                               C.stream, static_cast<const float2*>(bra->d), static_cast<const float2*>(ket->d), dim,
+                               dg.g, dg.n_groups, dg.z, dg.c, part);
+    };
+    if (herm) go(std::true_type{});
+    else go(std::false_type{});
     QHIP_CHECK(hipGetLastError());
     const cplx r = finish(part, blocks);
-    // Keep the uploaded tables alive until the kernels have run.
-    return r;
+    return herm ? cplx(r.real(), 0.0) : r;
 }
 
 void apply_observable(const qhip_state* in, const qhip_observable* obs, qhip_state* out) {

@@ -292,6 +292,36 @@ def test_transition_random_sizes(n):
     assert abs(got - ref) < ATOL
 
 
+@pytest.mark.parametrize("precision", ["complex128", "complex64"])
+def test_hermitian_half_sum_expectation(precision):
+    """<psi|H|psi> with real coefficients sums half the index pairs per flip
+    mask (twice the real part): masks whose top bit falls in a lane bit, a
+    row bit and a bit above the 2^12 chunk, plus Z-only terms, all against the
+    oracle; complex coefficients take the full sum."""
+    n = 15
+    rng = np.random.default_rng(9)
+    sim = gpu_sim(precision=precision)
+    st = sim.new_state(n)
+    sim.apply(W.brickwork_haar(n, 5, 2), st)
+    amps = st.download().astype(np.complex128)
+    tol = ATOL if precision == "complex128" else 1e-6
+    obs = []
+    for top in (1, 2, 5, 7, 9, 11, 12, 14):
+        letters = [(top, "XY"[top % 2])] + [(int(q), "XYZ"[int(rng.integers(3))])
+                                             for q in rng.choice(top, min(top, 3), replace=False)]
+        obs.append((letters, float(rng.standard_normal())))
+    obs += [([(3, "Z"), (13, "Z")], 0.3), ([], -0.25)]
+    got = sim.expectation_state(st, obs)
+    ref = O.transition(amps, amps, n, pack_observable(obs))
+    assert abs(got - ref.real) < tol and abs(ref.imag) < 1e-12
+    other = sim.new_state(n)
+    other.upload(amps)
+    assert abs(sim.transition_state(other, st, obs) - ref) < tol      # two states: the full sum
+    cobs = [(p, c * (1 + 0.25j)) for p, c in obs]
+    want = O.transition(amps, amps, n, pack_observable(cobs))
+    assert abs(sim.transition_state(st, st, cobs) - want) < tol       # complex coefficients: the full sum
+
+
 def test_expectation_matches_reference_fixture_energy():
     f = F.load("hea16.npz")
     n = int(f["n"])
