@@ -131,12 +131,12 @@ __global__ __launch_bounds__(kT) void transition_kernel(const A* __restrict__ br
                                                         double2* __restrict__ partials) {
     __shared__ double2 red[kT / 64];
     const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kChunk;
-    double2 k[kPer], s[kPer];
+    double2 k[kPer];
+    double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
     for (int r = 0; r < kPer; ++r) {
         const uint64_t i = base + threadIdx.x + static_cast<uint64_t>(r) * kT;
         k[r] = i < dim ? load_amp(ket, i) : make_double2(0.0, 0.0);
-        s[r] = make_double2(0.0, 0.0);
     }
     for (int g = 0; g < n_groups; ++g) {
         const GroupDev G = groups[g];
@@ -157,16 +157,11 @@ __global__ __launch_bounds__(kT) void transition_kernel(const A* __restrict__ br
                 w.x *= sc;
                 w.y *= sc;
             }
-            // conj(b) * w
-            s[r].x += b.x * w.x + b.y * w.y;
-            s[r].y += b.x * w.y - b.y * w.x;
+            // conj(b) * w * ket[i]
+            const double vx = b.x * w.x + b.y * w.y, vy = b.x * w.y - b.y * w.x;
+            acc.x += vx * k[r].x - vy * k[r].y;
+            acc.y += vx * k[r].y + vy * k[r].x;
         }
-    }
-    double2 acc = make_double2(0.0, 0.0);
-#pragma unroll
-    for (int r = 0; r < kPer; ++r) {
-        acc.x += s[r].x * k[r].x - s[r].y * k[r].y;
-        acc.y += s[r].x * k[r].y + s[r].y * k[r].x;
     }
     const double2 tot = block_sum(acc, red);
     if (threadIdx.x == 0) partials[blockIdx.x] = tot;
