@@ -86,6 +86,12 @@ enum OpKind : int32_t {
     // Device-only: dense op (complex128, k = 3..5) on the matrix cores; coef
     // points at the per-lane MFMA fragments of its real 2^(k+1)-square form.
     OP_MFMA = 8,
+    // Device-only: diagonal op whose local operands (k <= 2, pos[], group
+    // positions as for dense ops) index the entries within a group, while its
+    // operands outside the tile (lmask bytes 0..5: global qubits, byte 7:
+    // their count) select a sub-table once per tile.  coef holds the 2^(k+kg)
+    // entries at index (global bits << k) | local bits.
+    OP_DIAGG = 9,
 };
 
 struct HostOp {

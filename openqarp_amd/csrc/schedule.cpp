@@ -173,13 +173,36 @@ static void emit_op(Plan& plan, const HostOp& op, const int* local_of, int n, in
             ++plan.dense_ops;
             break;
         }
-        case OP_DIAG: case OP_IP_DIAG:
-            for (int b = 0; b < op.k; ++b)
-                d.pos[b] = static_cast<int8_t>(local_of[op.q[b]] >= 0 ? local_of[op.q[b]]
-                                                                       : 32 + static_cast<int>(op.q[b]));
-            plan.coefs.insert(plan.coefs.end(), op.m.begin(), op.m.end());
+        case OP_DIAG: case OP_IP_DIAG: {
+            int lb[8], gb[8], kl = 0, kg = 0;   // operand indices inside / outside the tile
+            for (int b = 0; b < op.k; ++b) (local_of[op.q[b]] >= 0 ? lb[kl++] : gb[kg++]) = b;
+            if (op.kind == OP_DIAG && T > 0 && kl <= 2 && kg <= 6) {
+                // Grouped form: no per-amplitude gather of the entries.
+                d.kind = OP_DIAGG;
+                d.k = kl;
+                int sorted[2];
+                for (int i = 0; i < kl; ++i) sorted[i] = d.pos[i] = static_cast<int8_t>(local_of[op.q[lb[i]]]);
+                std::sort(sorted, sorted + kl);
+                for (int i = 0; i < kl; ++i) d.spos[i] = static_cast<int8_t>(sorted[i]);
+                if (kl > 0) group_positions(d, T);
+                for (int j = 0; j < kg; ++j) d.lmask |= static_cast<uint64_t>(op.q[gb[j]]) << (8 * j);
+                d.lmask |= static_cast<uint64_t>(kg) << 56;
+                for (uint32_t gi = 0; gi < (1u << kg); ++gi)
+                    for (uint32_t li = 0; li < (1u << kl); ++li) {
+                        uint32_t idx = 0;
+                        for (int i = 0; i < kl; ++i) idx |= ((li >> i) & 1u) << lb[i];
+                        for (int j = 0; j < kg; ++j) idx |= ((gi >> j) & 1u) << gb[j];
+                        plan.coefs.push_back(op.m[idx]);
+                    }
+            } else {
+                for (int b = 0; b < op.k; ++b)
+                    d.pos[b] = static_cast<int8_t>(local_of[op.q[b]] >= 0 ? local_of[op.q[b]]
+                                                                           : 32 + static_cast<int>(op.q[b]));
+                plan.coefs.insert(plan.coefs.end(), op.m.begin(), op.m.end());
+            }
             ++plan.diag_ops;
             break;
+        }
         case OP_MASKPHASE:
             for (int q = 0; q < n; ++q)
                 if ((op.mask >> q) & 1) {

@@ -450,6 +450,38 @@ __device__ __forceinline__ void block_apply(A* s, uint32_t tp, int G, const OpRe
     }
 }
 
+// Grouped diagonal over one state's tile: each thread scales its groups'
+// 2^K amplitudes by the entries D[x] (wave-uniform, through the scalar cache).
+template <typename A, int K>
+__device__ __forceinline__ void diag_groups(A* s, uint32_t tp, int G, const OpRegs& op, const A* __restrict__ Dg) {
+    constexpr int D = 1 << K;
+    constexpr int sh = Amp<A>::kShift;
+    if (G < 8 && threadIdx.x >= (1u << G)) return;
+    uint32_t so[D];
+    A d[D];
+#pragma unroll
+    for (int x = 0; x < D; ++x) {
+        so[x] = swz(local_offset<K>(x, op)) << sh;
+        d[x] = Dg[x];
+    }
+    const uint32_t hm = static_cast<uint32_t>(op.gmask);
+    const uint32_t iters = G > 8 ? 1u << (G - 8) : 1u;
+    uint32_t hi = 0;
+    for (uint32_t i = 0; i < iters; ++i) {
+        const uint32_t h = swz(hi) << sh;
+        hi = (hi - hm) & hm;
+        uint32_t a[D];
+        A v[D];
+#pragma unroll
+        for (int x = 0; x < D; ++x) {
+            a[x] = tp ^ (h ^ so[x]);
+            v[x] = at(s, a[x]);
+        }
+#pragma unroll
+        for (int x = 0; x < D; ++x) at(s, a[x]) = cmul(v[x], d[x]);
+    }
+}
+
 __device__ __forceinline__ uint32_t diag_index(uint32_t l, uint64_t gbase, const OpRegs& op, int k) {
     uint32_t idx = 0;
     for (int b = 0; b < k; ++b) {
@@ -548,6 +580,33 @@ __device__ __forceinline__ int run_ops(A* s, const PassDesc& P, uint64_t gbase, 
                         if (TWO) at(s, a | kPhi) = cmul(at(s, a | kPhi), d);
                     }
                 break;
+            case OP_DIAGG: {
+                const int kg = static_cast<int>(op.lmask >> 56);
+                uint32_t gsel = 0;   // the entries' sub-table for this tile's global bits
+                for (int j = 0; j < kg; ++j)
+                    gsel |= static_cast<uint32_t>((gbase >> ((op.lmask >> (8 * j)) & 0xffu)) & 1u) << j;
+                const A* Dg = U + (gsel << k);
+                if (k == 0) {
+                    const A ph = Dg[0];
+                    if (active)
+                        for (uint32_t r = 0; r < R; ++r) {
+                            const uint32_t a = te ^ (swz(r << 8) << sh);
+                            at(s, a) = cmul(at(s, a), ph);
+                            if (TWO) at(s, a | kPhi) = cmul(at(s, a | kPhi), ph);
+                        }
+                } else {
+                    const int G = P.T - k;
+                    const uint32_t tp = thread_part<A>(op, G);
+                    if (k == 1) {
+                        diag_groups<A, 1>(s, tp, G, op, Dg);
+                        if (TWO) diag_groups<A, 1>(s, tp | kPhi, G, op, Dg);
+                    } else {
+                        diag_groups<A, 2>(s, tp, G, op, Dg);
+                        if (TWO) diag_groups<A, 2>(s, tp | kPhi, G, op, Dg);
+                    }
+                }
+                break;
+            }
             case OP_MASKPHASE: {
                 const A ph = U[0];
                 const uint64_t lm = op.lmask, gm = op.gmask;
