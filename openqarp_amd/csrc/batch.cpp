@@ -25,8 +25,11 @@
 #include "common.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
+#include <exception>
 #include <memory>
+#include <thread>
 
 namespace qhip {
 
@@ -125,21 +128,49 @@ struct BatchSession {
         }
     }
 
+    void plan_one(Inst& I, int i) const {
+        I.n = n_qubits[i];
+        if (I.n < 0 || I.n > 34) fail(QHIP_INVALID, "batch: n_qubits out of range");
+        int64_t fused = 0;
+        I.fwd = plan_stream(cfg, &streams[i], I.n, precision, false, &fused);
+        if (want_grad) {
+            std::vector<HostOp> back;
+            build_backward(Stream{&streams[i]}, I.n, I.slots, back);
+            if (!I.slots.empty()) I.back = schedule(back, I.n, precision, schedule_options(cfg, true));
+        }
+        I.batched = batchable(I.fwd) && batchable(I.back);
+    }
+
+    // Host planning of every instance (fusion, lowering, pass selection,
+    // backward schedule) on a pool of host threads; instances are independent
+    // and the planning code shares no mutable state.  The first failure (in
+    // instance order) is rethrown.
     void plan_all(std::vector<Inst>& out) const {
         out.assign(n_inst, Inst{});
-        for (int i = 0; i < n_inst; ++i) {
-            Inst& I = out[i];
-            I.n = n_qubits[i];
-            if (I.n < 0 || I.n > 34) fail(QHIP_INVALID, "batch: n_qubits out of range");
-            int64_t fused = 0;
-            I.fwd = plan_stream(cfg, &streams[i], I.n, precision, false, &fused);
-            if (want_grad) {
-                std::vector<HostOp> back;
-                build_backward(Stream{&streams[i]}, I.n, I.slots, back);
-                if (!I.slots.empty()) I.back = schedule(back, I.n, precision, schedule_options(cfg, true));
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        const int workers = static_cast<int>(std::min<unsigned>(std::min(hw, 16u), static_cast<unsigned>(n_inst)));
+        std::vector<std::exception_ptr> err(n_inst);
+        std::atomic<int> next{0};
+        auto work = [&] {
+            for (int i = next++; i < n_inst; i = next++) {
+                try {
+                    plan_one(out[i], i);
+                } catch (...) {
+                    err[i] = std::current_exception();
+                }
             }
-            I.batched = batchable(I.fwd) && batchable(I.back);
+        };
+        if (workers <= 1) {
+            work();
+        } else {
+            std::vector<std::thread> pool;
+            pool.reserve(workers - 1);
+            for (int w = 1; w < workers; ++w) pool.emplace_back(work);
+            work();
+            for (auto& t : pool) t.join();
         }
+        for (const auto& e : err)
+            if (e) std::rethrow_exception(e);
     }
 
     void upload_coefs(const std::vector<cplx>& coefs) {
@@ -241,7 +272,7 @@ void BatchSession::build() {
             int n_ip = 0;
             for (int o = P.op_begin; o < P.op_end; ++o) {
                 const DevOp& d = plan.ops[o];
-                if (d.kind != OP_IP_DENSE && d.kind != OP_IP_DIAG) continue;
+                if (!is_ip_kind(d.kind)) continue;
                 S.jobs.push_back({nullptr, b.n_tiles, d_tot + I.tot + static_cast<size_t>(d.lmask)});
                 S.job_in.push_back(ip_off + sizeof(double2) * static_cast<size_t>(n_ip) * b.n_tiles);
                 ++n_ip;

@@ -88,11 +88,16 @@ enum OpKind : int32_t {
     OP_MFMA = 8,
     // Device-only: diagonal op whose local operands (k <= 2, pos[], group
     // positions as for dense ops) index the entries within a group, while its
-    // operands outside the tile (lmask bytes 0..5: global qubits, byte 7:
+    // operands outside the tile (spos bytes 0..5: global qubits, byte 7:
     // their count) select a sub-table once per tile.  coef holds the 2^(k+kg)
-    // entries at index (global bits << k) | local bits.
+    // entries at index (global bits << k) | local bits.  OP_IP_DIAGG is the
+    // inner-product form (lmask: accumulator slot).
     OP_DIAGG = 9,
+    OP_IP_DIAGG = 10,
 };
+
+// Device op kinds that accumulate an inner product.
+inline bool is_ip_kind(int32_t kind) { return kind == OP_IP_DENSE || kind == OP_IP_DIAG || kind == OP_IP_DIAGG; }
 
 struct HostOp {
     int32_t kind = OP_DENSE;

@@ -249,7 +249,7 @@ static void launch_passes(const Plan& plan, A* psi, A* phi, double2* ip_part, do
             if (phi) {
                 int n_ip = 0;
                 for (int o = P.op_begin; o < P.op_end; ++o)
-                    n_ip += plan.ops[o].kind == OP_IP_DENSE || plan.ops[o].kind == OP_IP_DIAG;
+                    n_ip += is_ip_kind(plan.ops[o].kind);
                 if (n_ip > 0) {
                     hipLaunchKernelGGL(reduce_pass_ip, dim3(n_ip), dim3(kThreads), 0, C.stream, ip_part, n_tiles,
                                        d_slots + slot_base, ip_tot);
@@ -279,7 +279,7 @@ static DeviceTables upload_tables(const Plan& plan, int precision, bool own) {
     for (const PassDesc& P : plan.passes) {
         int n_ip = 0;
         for (int o = P.op_begin; o < P.op_end; ++o)
-            if (plan.ops[o].kind == OP_IP_DENSE || plan.ops[o].kind == OP_IP_DIAG) {
+            if (is_ip_kind(plan.ops[o].kind)) {
                 slots.push_back(static_cast<int32_t>(plan.ops[o].lmask));
                 ++n_ip;
             }

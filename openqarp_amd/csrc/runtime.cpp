@@ -510,10 +510,10 @@ int qhip_schedule_inspect(const qhip_config* cfg, const qhip_stream* in, int32_t
                         break;
                     }
                     case qhip::OP_DIAGG: {   // local operands, then the global ones
-                        const int kg = static_cast<int>(d.lmask >> 56);
+                        const int kg = d.spos[7];
                         it.k = d.k + kg;
                         for (int b = 0; b < d.k; ++b) it.q[b] = global(d.pos[b]);
-                        for (int j = 0; j < kg; ++j) it.q[d.k + j] = static_cast<uint32_t>((d.lmask >> (8 * j)) & 0xff);
+                        for (int j = 0; j < kg; ++j) it.q[d.k + j] = static_cast<uint32_t>(d.spos[j]);
                         const size_t D = size_t{1} << it.k;
                         it.m.assign(D * D, qhip::cplx(0.0));
                         for (size_t y = 0; y < D; ++y) it.m[y * D + y] = plan.coefs[d.coef + y];

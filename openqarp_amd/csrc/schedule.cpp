@@ -176,17 +176,14 @@ static void emit_op(Plan& plan, const HostOp& op, const int* local_of, int n, in
         case OP_DIAG: case OP_IP_DIAG: {
             int lb[8], gb[8], kl = 0, kg = 0;   // operand indices inside / outside the tile
             for (int b = 0; b < op.k; ++b) (local_of[op.q[b]] >= 0 ? lb[kl++] : gb[kg++]) = b;
-            if (op.kind == OP_DIAG && T > 0 && kl <= 2 && kg <= 6) {
+            if (T > 0 && kl <= 2 && kg <= 6) {
                 // Grouped form: no per-amplitude gather of the entries.
-                d.kind = OP_DIAGG;
+                d.kind = op.kind == OP_DIAG ? OP_DIAGG : OP_IP_DIAGG;
                 d.k = kl;
-                int sorted[2];
-                for (int i = 0; i < kl; ++i) sorted[i] = d.pos[i] = static_cast<int8_t>(local_of[op.q[lb[i]]]);
-                std::sort(sorted, sorted + kl);
-                for (int i = 0; i < kl; ++i) d.spos[i] = static_cast<int8_t>(sorted[i]);
+                for (int i = 0; i < kl; ++i) d.pos[i] = static_cast<int8_t>(local_of[op.q[lb[i]]]);
                 if (kl > 0) group_positions(d, T);
-                for (int j = 0; j < kg; ++j) d.lmask |= static_cast<uint64_t>(op.q[gb[j]]) << (8 * j);
-                d.lmask |= static_cast<uint64_t>(kg) << 56;
+                for (int j = 0; j < kg; ++j) d.spos[j] = static_cast<int8_t>(op.q[gb[j]]);
+                d.spos[7] = static_cast<int8_t>(kg);
                 for (uint32_t gi = 0; gi < (1u << kg); ++gi)
                     for (uint32_t li = 0; li < (1u << kl); ++li) {
                         uint32_t idx = 0;

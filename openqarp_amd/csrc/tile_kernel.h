@@ -482,6 +482,48 @@ __device__ __forceinline__ void diag_groups(A* s, uint32_t tp, int G, const OpRe
     }
 }
 
+// <phi| D |psi> over one state pair's tile for a grouped diagonal (entries
+// D[x] for local operand bits x); tp0 / tp1 address psi's and phi's tiles.
+template <typename A, int K>
+__device__ __forceinline__ double2 ip_diag_groups(A* s, uint32_t tp0, uint32_t tp1, int G, const OpRegs& op,
+                                                  const A* __restrict__ Dg) {
+    constexpr int D = 1 << K;
+    constexpr int sh = Amp<A>::kShift;
+    double2 acc = make_double2(0.0, 0.0);
+    if (G < 8 && threadIdx.x >= (1u << G)) return acc;
+    uint32_t so[D];
+    double2 d[D];
+#pragma unroll
+    for (int x = 0; x < D; ++x) {
+        so[x] = swz(local_offset<K>(x, op)) << sh;
+        d[x] = make_double2(Dg[x].x, Dg[x].y);
+    }
+    const uint32_t hm = static_cast<uint32_t>(op.gmask);
+    const uint32_t iters = G > 8 ? 1u << (G - 8) : 1u;
+    uint32_t hi = 0;
+    for (uint32_t i = 0; i < iters; ++i) {
+        const uint32_t h = swz(hi) << sh;
+        hi = (hi - hm) & hm;
+#pragma unroll
+        for (int x = 0; x < D; ++x) {
+            const A xv = at(s, tp0 ^ (h ^ so[x])), f = at(s, tp1 ^ (h ^ so[x]));
+            const double gr = d[x].x * xv.x - d[x].y * xv.y, gi = d[x].x * xv.y + d[x].y * xv.x;
+            acc.x += static_cast<double>(f.x) * gr + static_cast<double>(f.y) * gi;
+            acc.y += static_cast<double>(f.x) * gi - static_cast<double>(f.y) * gr;
+        }
+    }
+    return acc;
+}
+
+// Sub-table of a grouped diagonal for a tile: the bits of its operands
+// outside the tile (global qubits in spos bytes 0..5, their count in byte 7).
+__device__ __forceinline__ uint32_t diag_sub_table(uint64_t gbase, const OpRegs& op) {
+    const int kg = static_cast<int>((op.spos >> 56) & 0xffu);
+    uint32_t gsel = 0;
+    for (int j = 0; j < kg; ++j) gsel |= static_cast<uint32_t>((gbase >> ((op.spos >> (8 * j)) & 0xffu)) & 1u) << j;
+    return gsel;
+}
+
 __device__ __forceinline__ uint32_t diag_index(uint32_t l, uint64_t gbase, const OpRegs& op, int k) {
     uint32_t idx = 0;
     for (int b = 0; b < k; ++b) {
@@ -581,11 +623,7 @@ __device__ __forceinline__ int run_ops(A* s, const PassDesc& P, uint64_t gbase, 
                     }
                 break;
             case OP_DIAGG: {
-                const int kg = static_cast<int>(op.lmask >> 56);
-                uint32_t gsel = 0;   // the entries' sub-table for this tile's global bits
-                for (int j = 0; j < kg; ++j)
-                    gsel |= static_cast<uint32_t>((gbase >> ((op.lmask >> (8 * j)) & 0xffu)) & 1u) << j;
-                const A* Dg = U + (gsel << k);
+                const A* Dg = U + (diag_sub_table(gbase, op) << k);
                 if (k == 0) {
                     const A ph = Dg[0];
                     if (active)
@@ -633,6 +671,7 @@ __device__ __forceinline__ int run_ops(A* s, const PassDesc& P, uint64_t gbase, 
             }
             case OP_IP_DENSE:
             case OP_IP_DIAG:
+            case OP_IP_DIAGG:
                 if constexpr (TWO) {
                     double2 acc = make_double2(0.0, 0.0);
                     if (kind == OP_IP_DENSE) {
@@ -640,7 +679,23 @@ __device__ __forceinline__ int run_ops(A* s, const PassDesc& P, uint64_t gbase, 
                         const uint32_t tp = thread_part<A>(op, G);
                         acc = k == 1 ? ip_dense<A, 1>(s, tp, tp | kPhi, G, op, U)
                                      : ip_dense<A, 2>(s, tp, tp | kPhi, G, op, U);
-                    } else if (active) {
+                    } else if (kind == OP_IP_DIAGG && k > 0) {
+                        const int G = P.T - k;
+                        const uint32_t tp = thread_part<A>(op, G);
+                        const A* Dg = U + (diag_sub_table(gbase, op) << k);
+                        acc = k == 1 ? ip_diag_groups<A, 1>(s, tp, tp | kPhi, G, op, Dg)
+                                     : ip_diag_groups<A, 2>(s, tp, tp | kPhi, G, op, Dg);
+                    } else if (kind == OP_IP_DIAGG && active) {   // no local operand: one entry per tile
+                        const A dv = U[diag_sub_table(gbase, op)];
+                        const double dx = dv.x, dy = dv.y;
+                        for (uint32_t r = 0; r < R; ++r) {
+                            const uint32_t a = te ^ (swz(r << 8) << sh);
+                            const A x = at(s, a), f = at(s, a | kPhi);
+                            const double gr = dx * x.x - dy * x.y, gi = dx * x.y + dy * x.x;
+                            acc.x += static_cast<double>(f.x) * gr + static_cast<double>(f.y) * gi;
+                            acc.y += static_cast<double>(f.x) * gi - static_cast<double>(f.y) * gr;
+                        }
+                    } else if (kind == OP_IP_DIAG && active) {
                         for (uint32_t r = 0; r < R; ++r) {
                             const uint32_t l = threadIdx.x + r * kThreads;
                             const uint32_t a = te ^ (swz(r << 8) << sh);
