@@ -94,10 +94,19 @@ enum OpKind : int32_t {
     // inner-product form (lmask: accumulator slot).
     OP_DIAGG = 9,
     OP_IP_DIAGG = 10,
+    // Device-only: register-block member that accumulates <phi| G |psi> for
+    // a generator on block bits (k: pattern as for OP_SUB, coef: its matrix,
+    // lmask: accumulator slot); two-state passes only.
+    OP_IPSUB = 11,
 };
 
 // Device op kinds that accumulate an inner product.
-inline bool is_ip_kind(int32_t kind) { return kind == OP_IP_DENSE || kind == OP_IP_DIAG || kind == OP_IP_DIAGG; }
+inline bool is_ip_kind(int32_t kind) {
+    return kind == OP_IP_DENSE || kind == OP_IP_DIAG || kind == OP_IP_DIAGG || kind == OP_IPSUB;
+}
+
+// Inner-product members per register block (per-thread accumulators).
+constexpr int kMaxBlockIp = 4;
 
 struct HostOp {
     int32_t kind = OP_DENSE;

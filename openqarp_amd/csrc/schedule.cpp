@@ -232,33 +232,48 @@ struct Unit {
     bool block_ok = false;
 };
 
-static std::vector<Unit> form_units(const std::vector<HostOp>& ops, const std::vector<size_t>& chosen, bool enable) {
+static bool is_ip_host(const HostOp& op) { return op.kind == OP_IP_DENSE || op.kind == OP_IP_DIAG; }
+
+// Members: dense ops, and diagonal ops whose operands are all tile qubits
+// (tile mask tq); in two-state passes also inner-product ops (at most
+// kMaxBlockIp per block), which a block evaluates on its registers.
+static std::vector<Unit> form_units(const std::vector<HostOp>& ops, const std::vector<size_t>& chosen, bool enable,
+                                    bool two, uint64_t tq) {
     std::vector<Unit> units;
+    std::vector<int> n_ip;
     for (size_t idx : chosen) {
         const HostOp& op = ops[idx];
         const uint64_t Q = op.qmask();
-        const bool mergeable = enable && op.kind == OP_DENSE && op.k >= 1 && op.k <= 2;
+        const bool ip = is_ip_host(op);
+        const bool kind_ok = op.kind == OP_DENSE || (op.kind == OP_DIAG && (Q & ~tq) == 0) ||
+                             (two && ip && (Q & ~tq) == 0);
+        const bool mergeable = enable && kind_ok && op.k >= 1 && op.k <= 2;
         int last = -1;
         for (int u = static_cast<int>(units.size()) - 1; u >= 0; --u)
             if (op.kind == OP_SCALAR || (units[u].qset & Q)) { last = u; break; }
         int target = -1;
+        auto fits = [&](int u) {
+            return units[u].block_ok && std::popcount(units[u].qset | Q) <= 3 && (!ip || n_ip[u] < kMaxBlockIp);
+        };
         if (mergeable) {
             if (last >= 0) {
-                if (units[last].block_ok && std::popcount(units[last].qset | Q) <= 3) target = last;
+                if (fits(last)) target = last;
             } else {
                 for (int u = static_cast<int>(units.size()) - 1; u >= 0; --u)
-                    if (units[u].block_ok && std::popcount(units[u].qset | Q) <= 3) { target = u; break; }
+                    if (fits(u)) { target = u; break; }
             }
         }
         if (target >= 0) {
             units[target].ops.push_back(idx);
             units[target].qset |= Q;
+            n_ip[target] += ip;
         } else {
             Unit u;
             u.ops.push_back(idx);
             u.qset = Q;
             u.block_ok = mergeable;
             units.push_back(std::move(u));
+            n_ip.push_back(ip);
         }
     }
     return units;
@@ -285,37 +300,53 @@ static void emit_block(Plan& plan, const std::vector<HostOp>& ops, const Unit& u
     group_positions(h, T);
     h.lmask = u.ops.size();
     h.coef = static_cast<int64_t>(plan.coefs.size());
+    const size_t hdr_index = plan.ops.size();   // spos[7] receives the inner-product member count
     plan.ops.push_back(h);
     auto bit_of = [&](uint32_t q) {
         for (int b = 0; b < 3; ++b)
             if (bp[b] == local_of[q]) return b;
         fail(QHIP_RUNTIME, "schedule: block operand outside the block");
     };
+    int n_ip = 0;
     for (size_t idx : u.ops) {
         const HostOp& op = ops[idx];
+        const bool diag = op.kind == OP_DIAG || op.kind == OP_IP_DIAG;
+        const size_t D = size_t{1} << op.k;
+        std::vector<cplx> M;   // member matrix over its operands (a diagonal expanded)
+        if (diag) {
+            M.assign(D * D, cplx(0.0));
+            for (size_t y = 0; y < D; ++y) M[y * D + y] = op.m[y];
+        } else {
+            M = op.m;
+        }
         DevOp d{};
-        d.kind = OP_SUB;
+        d.kind = is_ip_host(op) ? OP_IPSUB : OP_SUB;
         d.coef = static_cast<int64_t>(plan.coefs.size());
         if (op.k == 1) {
             d.k = bit_of(op.q[0]);
-            plan.coefs.insert(plan.coefs.end(), op.m.begin(), op.m.end());
+            plan.coefs.insert(plan.coefs.end(), M.begin(), M.end());
         } else {
             const int b0 = bit_of(op.q[0]), b1 = bit_of(op.q[1]);
             const int lo = std::min(b0, b1), hi = std::max(b0, b1);
             d.k = lo == 0 ? (hi == 1 ? 3 : 4) : 5;
             if (b0 < b1) {
-                plan.coefs.insert(plan.coefs.end(), op.m.begin(), op.m.end());
+                plan.coefs.insert(plan.coefs.end(), M.begin(), M.end());
             } else {   // swap the two operand bits of the 4x4 matrix
                 auto sw = [](int x) { return ((x & 1) << 1) | ((x >> 1) & 1); };
                 for (int y = 0; y < 4; ++y)
-                    for (int x = 0; x < 4; ++x) plan.coefs.push_back(op.m[sw(y) * 4 + sw(x)]);
+                    for (int x = 0; x < 4; ++x) plan.coefs.push_back(M[sw(y) * 4 + sw(x)]);
             }
         }
         for (int b = 0; b < op.k; ++b) d.pos[b] = static_cast<int8_t>(local_of[op.q[b]]);
+        if (d.kind == OP_IPSUB) {
+            d.lmask = static_cast<uint64_t>(op.slot);
+            ++n_ip;
+        }
         plan.ops.push_back(d);
-        ++plan.dense_ops;
+        ++(diag ? plan.diag_ops : plan.dense_ops);
         plan.flops += op_flops(op) * static_cast<double>(uint64_t{1} << n);
     }
+    plan.ops[hdr_index].spos[7] = static_cast<int8_t>(n_ip);
 }
 
 // Dense ops wider than the tile kernel handles (k > 5) run as passes of
@@ -488,7 +519,7 @@ static void schedule_segment(Plan& plan, const std::vector<HostOp>& ops, const s
         P.op_begin = static_cast<int32_t>(plan.ops.size());
         P.kmax = 0;
         const bool blocks = opt.register_blocks && T >= 3;
-        for (const Unit& u : form_units(ops, chosen, blocks)) {
+        for (const Unit& u : form_units(ops, chosen, blocks, opt.two_states, Q)) {
             if (u.ops.size() >= 2) {
                 emit_block(plan, ops, u, local_of, n, T);   // runs in every kernel variant
             } else {

@@ -524,6 +524,125 @@ __device__ __forceinline__ uint32_t diag_sub_table(uint64_t gbase, const OpRegs&
     return gsel;
 }
 
+// <f| G |v> over a block's 8 amplitudes for a generator on block bit B
+// (2x2) or on block bits B0 < B1 (4x4, operand 0 on B0); FP64 sums.
+template <typename A>
+__device__ __forceinline__ void ip_acc(double2& acc, const A& f, const A& g) {
+    acc.x += static_cast<double>(f.x) * g.x + static_cast<double>(f.y) * g.y;
+    acc.y += static_cast<double>(f.x) * g.y - static_cast<double>(f.y) * g.x;
+}
+
+template <typename A, int B>
+__device__ __forceinline__ double2 ip_single(const A* v, const A* f, const A* __restrict__ U) {
+    const A u00 = U[0], u01 = U[1], u10 = U[2], u11 = U[3];
+    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int x = 0; x < 8; ++x)
+        if (!((x >> B) & 1)) {
+            const A a = v[x], b = v[x | (1 << B)];
+            ip_acc(acc, f[x], cmad(cmul(u00, a), u01, b));
+            ip_acc(acc, f[x | (1 << B)], cmad(cmul(u10, a), u11, b));
+        }
+    return acc;
+}
+
+template <typename A, int B0, int B1>
+__device__ __forceinline__ double2 ip_pair(const A* v, const A* f, const A* __restrict__ U) {
+    constexpr int C = 3 - B0 - B1;
+    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const int i[4] = {r << C, (r << C) | (1 << B0), (r << C) | (1 << B1), (r << C) | (1 << B0) | (1 << B1)};
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+            A g;
+            g.x = 0;
+            g.y = 0;
+#pragma unroll
+            for (int x = 0; x < 4; ++x) g = cmad(g, U[y * 4 + x], v[i[x]]);
+            ip_acc(acc, f[i[y]], g);
+        }
+    }
+    return acc;
+}
+
+// One OP_BLOCK over both states of a two-state pass: a thread holds a
+// group's 8 amplitudes of psi (v) and phi (f); OP_SUB members act on both,
+// OP_IPSUB members add <phi| G |psi> to their accumulator.  The header's
+// spos[7] counts the OP_IPSUB members; their per-wave sums go to wave_ip
+// slots n_ip, n_ip + 1, ... in member order.  Returns that count.
+template <typename A>
+__device__ __forceinline__ int block_apply2(A* s, uint32_t tp, uint32_t kPhi, int G, const OpRegs& hdr,
+                                            const DevOp* __restrict__ subs, int nsub, const A* __restrict__ coefs,
+                                            double2* wave_ip, int n_ip) {
+    constexpr int sh = Amp<A>::kShift;
+    static_assert(kMaxBlockIp == 4, "four named accumulators");
+    const int nipb = static_cast<int>((hdr.spos >> 56) & 0xffu);
+    double2 acc0 = make_double2(0.0, 0.0), acc1 = acc0, acc2 = acc0, acc3 = acc0;
+    if (!(G < 8 && threadIdx.x >= (1u << G))) {
+        uint32_t so[8];
+#pragma unroll
+        for (int x = 0; x < 8; ++x) so[x] = swz(local_offset<3>(x, hdr)) << sh;
+        const uint32_t hm = static_cast<uint32_t>(hdr.gmask);
+        const uint32_t iters = G > 8 ? 1u << (G - 8) : 1u;
+        uint32_t hi = 0;
+        for (uint32_t i = 0; i < iters; ++i) {
+            const uint32_t h = swz(hi) << sh;
+            hi = (hi - hm) & hm;
+            uint32_t a[8];
+            A v[8], f[8];
+#pragma unroll
+            for (int x = 0; x < 8; ++x) {
+                a[x] = tp ^ (h ^ so[x]);
+                v[x] = at(s, a[x]);
+                f[x] = at(s, a[x] | kPhi);
+            }
+            int j_ip = 0;
+            for (int j = 0; j < nsub; ++j) {
+                const OpRegs sub = load_op(subs + j);
+                const A* U = coefs + sub.coef;
+                if (sub.kind == OP_IPSUB) {
+                    double2 r;
+                    switch (sub.k) {
+                        case 0: r = ip_single<A, 0>(v, f, U); break;
+                        case 1: r = ip_single<A, 1>(v, f, U); break;
+                        case 2: r = ip_single<A, 2>(v, f, U); break;
+                        case 3: r = ip_pair<A, 0, 1>(v, f, U); break;
+                        case 4: r = ip_pair<A, 0, 2>(v, f, U); break;
+                        default: r = ip_pair<A, 1, 2>(v, f, U); break;
+                    }
+                    double2& dst = j_ip == 0 ? acc0 : j_ip == 1 ? acc1 : j_ip == 2 ? acc2 : acc3;
+                    dst.x += r.x;
+                    dst.y += r.y;
+                    ++j_ip;
+                    continue;
+                }
+                switch (sub.k) {
+                    case 0: sub_single<A, 0>(v, U); sub_single<A, 0>(f, U); break;
+                    case 1: sub_single<A, 1>(v, U); sub_single<A, 1>(f, U); break;
+                    case 2: sub_single<A, 2>(v, U); sub_single<A, 2>(f, U); break;
+                    case 3: sub_pair<A, 0, 1>(v, U); sub_pair<A, 0, 1>(f, U); break;
+                    case 4: sub_pair<A, 0, 2>(v, U); sub_pair<A, 0, 2>(f, U); break;
+                    default: sub_pair<A, 1, 2>(v, U); sub_pair<A, 1, 2>(f, U); break;
+                }
+            }
+#pragma unroll
+            for (int x = 0; x < 8; ++x) {
+                at(s, a[x]) = v[x];
+                at(s, a[x] | kPhi) = f[x];
+            }
+        }
+    }
+    const double2 accs[4] = {acc0, acc1, acc2, acc3};
+#pragma unroll
+    for (int t = 0; t < kMaxBlockIp; ++t)
+        if (t < nipb) {
+            const double2 r = wave_sum(accs[t]);
+            if ((threadIdx.x & 63) == 0) wave_ip[(n_ip + t) * kWaves + (threadIdx.x >> 6)] = r;
+        }
+    return nipb;
+}
+
 __device__ __forceinline__ uint32_t diag_index(uint32_t l, uint64_t gbase, const OpRegs& op, int k) {
     uint32_t idx = 0;
     for (int b = 0; b < k; ++b) {
@@ -607,9 +726,9 @@ __device__ __forceinline__ int run_ops(A* s, const PassDesc& P, uint64_t gbase, 
                 const int nsub = static_cast<int>(op.lmask);
                 const int G = P.T - 3;
                 const uint32_t tp = thread_part<A>(op, G);
-                block_apply<A>(s, tp, G, op, ops + o + 1, nsub, coefs);
-                if (TWO) block_apply<A>(s, tp | kPhi, G, op, ops + o + 1, nsub, coefs);
-                o += nsub;   // the OP_SUB entries are consumed here
+                if constexpr (TWO) n_ip += block_apply2<A>(s, tp, kPhi, G, op, ops + o + 1, nsub, coefs, wave_ip, n_ip);
+                else block_apply<A>(s, tp, G, op, ops + o + 1, nsub, coefs);
+                o += nsub;   // the member entries are consumed here
                 break;
             }
             case OP_DIAG:
