@@ -369,6 +369,7 @@ def main(argv=None) -> int:
                    "fused_ops": fa["fused_ops"], "passes": fa["passes"], "parallelism": "single-gpu"},
         "pct_of_peak": round(100.0 * fa["value"] / HBM_PEAK_GBS, 2),
         "gate_equivalent_GBps": round(fa["gate_equiv_gbs"], 1),
+        "fused_gates_per_pass": round(fa["fused_ops"] / max(fa["passes"], 1), 2),
         "roofline": {"bound": "hbm", "achieved": round(fa["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(fa["achieved"] / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
                      "kernel": "tile_pass_kernel", "mean_launch_ms": round(fa["kernel_ms"], 4),
