@@ -126,7 +126,7 @@ typedef struct qhip_config {
     int32_t tile_qubits;         /* qubits held on chip per tile pass, 0=auto  */
     int32_t low_qubits;          /* contiguous low qubits per tile, 0=auto     */
     int32_t max_ops_per_pass;    /* 0 = auto                                   */
-    int32_t use_mfma;            /* -1 auto, 0 never: dense k=3..5 (complex128) on the matrix cores */
+    int32_t use_mfma;            /* -1 auto, 0 never: dense k=3..5 on the matrix cores (FP64 or FP32 MFMA) */
 } qhip_config;
 
 /* What one apply did (filled when the caller passes a non-NULL pointer). */

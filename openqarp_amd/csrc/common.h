@@ -232,8 +232,10 @@ struct ScheduleOptions {
 };
 Plan schedule(const std::vector<HostOp>& ops, int n, int precision, const ScheduleOptions& opt);
 // Per-lane fragments of a dense op for the matrix-core path, and back.
-void mfma_fragments(const std::vector<cplx>& U, int k, std::vector<cplx>& out);
-std::vector<cplx> mfma_matrix(const cplx* fragments, int k);
+// f32: fragments for v_mfma_f32_16x16x4 (complex64 states), whose result
+// rows are laid out differently from the FP64 instruction's (k = 3 only).
+void mfma_fragments(const std::vector<cplx>& U, int k, std::vector<cplx>& out, bool f32);
+std::vector<cplx> mfma_matrix(const cplx* fragments, int k, bool f32);
 int default_tile_qubits(int precision, bool two_states);
 
 // A schedule with its op tables resident on the device.

@@ -485,7 +485,7 @@ int qhip_schedule_inspect(const qhip_config* cfg, const qhip_stream* in, int32_t
                     case qhip::OP_MFMA:
                         it.k = d.k;
                         for (int b = 0; b < d.k; ++b) it.q[b] = global(d.pos[b]);
-                        it.m = qhip::mfma_matrix(plan.coefs.data() + d.coef, d.k);
+                        it.m = qhip::mfma_matrix(plan.coefs.data() + d.coef, d.k, precision != QHIP_C128);
                         break;
                     case qhip::OP_SUB:   // pos[] holds the member op's own operand positions
                     case qhip::OP_DENSE: {

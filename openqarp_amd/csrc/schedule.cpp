@@ -82,7 +82,7 @@ static void group_positions(DevOp& d, int T) {
 // 4(s >> 1) + c, part s & 1, so lane group c feeds whole amplitudes c,
 // c + 4, ....  Stored as complex pairs (steps 2t, 2t+1) at
 // [(mt * D/4 + t) * 64 + lane].
-void mfma_fragments(const std::vector<cplx>& U, int k, std::vector<cplx>& out) {
+void mfma_fragments(const std::vector<cplx>& U, int k, std::vector<cplx>& out, bool f32) {
     const int D = 1 << k;
     if (k >= 4) {   // three real products (Gauss), layout of the k = 4, 5 kernel
         const int NT = D / 16, HP = D / 8;
@@ -109,7 +109,7 @@ void mfma_fragments(const std::vector<cplx>& U, int k, std::vector<cplx>& out) {
     };
     auto frag = [&](int mt, int s, int lane) {
         const int i = lane % 16, c = lane / 16;
-        const int g = i % 4, r = i / 4;
+        const int g = f32 ? i / 4 : i % 4, r = f32 ? i % 4 : i / 4;   // result row i = 4g + r or 4r + g
         return M(8 * mt + 2 * g + (r >> 1), r & 1, 4 * (s >> 1) + c, s & 1);
     };
     for (int mt = 0; mt < NT; ++mt)
@@ -119,7 +119,7 @@ void mfma_fragments(const std::vector<cplx>& U, int k, std::vector<cplx>& out) {
 }
 
 // Inverse of mfma_fragments (schedule inspection).
-std::vector<cplx> mfma_matrix(const cplx* f, int k) {
+std::vector<cplx> mfma_matrix(const cplx* f, int k, bool f32) {
     const int D = 1 << k;
     std::vector<cplx> U(static_cast<size_t>(D) * D);
     if (k >= 4) {
@@ -140,7 +140,7 @@ std::vector<cplx> mfma_matrix(const cplx* f, int k) {
         for (int t = 0; t < S / 2; ++t)
             for (int lane = 0; lane < 64; ++lane) {
                 const cplx v = f[(mt * (S / 2) + t) * 64 + lane];
-                const int i = lane % 16, c = lane / 16, g = i % 4, r = i / 4;
+                const int i = lane % 16, c = lane / 16, g = f32 ? i / 4 : i % 4, r = f32 ? i % 4 : i / 4;
                 const int b = 8 * mt + 2 * g + (r >> 1), q = r & 1, a = 4 * t + c;   // step 2t: part 0
                 auto& u = U[static_cast<size_t>(b) * D + a];
                 if (q == 0) u.real(v.real());
@@ -162,10 +162,9 @@ static void emit_op(Plan& plan, const HostOp& op, const int* local_of, int n, in
             std::sort(sorted, sorted + op.k);
             for (int b = 0; b < op.k; ++b) d.spos[b] = static_cast<int8_t>(sorted[b]);
             group_positions(d, T);
-            if (op.kind == OP_DENSE && op.k >= 3 && op.k <= 5 && opt.mfma != 0 && opt.precision == QHIP_C128 &&
-                T - op.k >= 4) {
+            if (op.kind == OP_DENSE && op.k >= 3 && op.k <= 5 && opt.mfma != 0 && T - op.k >= 4) {
                 d.kind = OP_MFMA;
-                mfma_fragments(op.m, op.k, plan.coefs);
+                mfma_fragments(op.m, op.k, plan.coefs, opt.precision != QHIP_C128);
                 ++plan.mfma_ops;
             } else {
                 plan.coefs.insert(plan.coefs.end(), op.m.begin(), op.m.end());

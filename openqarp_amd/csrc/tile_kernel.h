@@ -265,100 +265,130 @@ __device__ __forceinline__ void dense_op(A* s, uint32_t tp, int G, const OpRegs&
     }
 }
 
-// Dense k-qubit op (k = 3..5, complex128) on the matrix cores.  Groups go
+// Dense k-qubit op (k = 3..5) on the matrix cores.  Groups go
 // 16 to a chunk (the MFMA's 16 columns, lane % 16); wave w takes chunks w,
 // w + 4, ....  Lane group c = lane / 16 loads input amplitudes c, c + 4, ...
 // of its group (re then im feed consecutive K-steps), the D/8 M-tiles of 16
 // real rows are accumulated over D/2 K-steps from per-lane fragments F,
 // and lane group g writes output amplitudes 8mt + 2g and 8mt + 2g + 1 of
 // its group.  A wave reads its whole chunk before writing any of it.
-typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+// FP64 (complex128) and FP32 (complex64) matrix-core steps.  Both read A
+// at lane (i = lane % 16, k = lane / 16) and B at lane (j = lane % 16,
+// k = lane / 16); D row i of the 16x16 result lands in register r of lane
+// group g = lane / 16 as i = 4r + g (f64) or i = 4g + r (f32), measured on
+// the device.
+template <typename A> struct Mfma;
+template <> struct Mfma<double2> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 step(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static constexpr bool kGroupRows = false;   // row = 4r + g
+};
+template <> struct Mfma<float2> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 step(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static constexpr bool kGroupRows = true;    // row = 4g + r
+};
 
-template <int K>
-__device__ __forceinline__ void dense_mfma(double2* s, uint32_t zoff, int G, const OpRegs& op,
-                                           const double2* __restrict__ F) {
+// k = 3, realified 16x16 matrix.  The host orders the fragment rows so that
+// lane group g receives output amplitudes 8mt + 2g and 8mt + 2g + 1 in
+// registers (0, 1) and (2, 3) for either row layout.
+template <typename A, int K>
+__device__ __forceinline__ void dense_mfma(A* s, uint32_t zoff, int G, const OpRegs& op, const A* __restrict__ F) {
+    using M = Mfma<A>;
+    constexpr int sh = Amp<A>::kShift;
     constexpr int D = 1 << K, NT = D / 8, H = D / 4;   // M-tiles; input amplitudes (= fragment pairs) per lane
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6, j = lane & 15u, c = lane >> 4;
     uint32_t lp = 0;
 #pragma unroll
     for (int b = 0; b < 4; ++b) lp |= ((j >> b) & 1u) << op.g(b);
-    lp = (swz(lp) << 4) | zoff;
+    lp = (swz(lp) << sh) | zoff;
     uint32_t so_in[H];
 #pragma unroll
-    for (int t = 0; t < H; ++t) so_in[t] = swz(local_offset<K>(c + 4u * t, op)) << 4;
+    for (int t = 0; t < H; ++t) so_in[t] = swz(local_offset<K>(c + 4u * t, op)) << sh;
     const uint32_t chunks = 1u << (G - 4);
     for (uint32_t ch = w; ch < chunks; ch += kWaves) {
         uint32_t up = 0;   // deposit of the group-number bits >= 4 (wave-uniform)
         for (int b = 4; b < G; ++b) up |= ((ch >> (b - 4)) & 1u) << op.g(b);
-        const uint32_t base = lp ^ (swz(up) << 4);
-        double2 x[H];
+        const uint32_t base = lp ^ (swz(up) << sh);
+        A x[H];
 #pragma unroll
         for (int t = 0; t < H; ++t) x[t] = at(s, base ^ so_in[t]);
-        // M-tiles one at a time (k = 5 keeps only one tile's fragments live).
-#pragma unroll(K < 5 ? NT : 1)
+#pragma unroll
         for (int mt = 0; mt < NT; ++mt) {
-            mfma_d4 acc = {0.0, 0.0, 0.0, 0.0};
+            typename M::v4 acc = {0, 0, 0, 0};
 #pragma unroll
             for (int t = 0; t < H; ++t) {
-                const double2 f = F[(mt * H + t) * 64 + lane];
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(f.x, x[t].x, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(f.y, x[t].y, acc, 0, 0, 0);
+                const A f = F[(mt * H + t) * 64 + lane];
+                acc = M::step(f.x, x[t].x, acc);
+                acc = M::step(f.y, x[t].y, acc);
             }
             const uint32_t b0 = 8u * static_cast<uint32_t>(mt) + 2u * c;
-            at(s, base ^ (swz(local_offset<K>(b0, op)) << 4)) = make_double2(acc[0], acc[1]);
-            at(s, base ^ (swz(local_offset<K>(b0 + 1u, op)) << 4)) = make_double2(acc[2], acc[3]);
+            A y0, y1;
+            y0.x = acc[0];
+            y0.y = acc[1];
+            y1.x = acc[2];
+            y1.y = acc[3];
+            at(s, base ^ (swz(local_offset<K>(b0, op)) << sh)) = y0;
+            at(s, base ^ (swz(local_offset<K>(b0 + 1u, op)) << sh)) = y1;
         }
     }
 }
 
 // k = 4, 5: the complex product as three real ones (Gauss):
 // P1 = Ur·xr, P2 = Ui·xi, P3 = (Ur + Ui)·(xr + xi); y = (P1 − P2) + i(P3 − P1 − P2).
-// Real 16-row M-tiles over output amplitudes (row i = 4r + g of tile mt is
-// amplitude 16mt + 4r + g), K-steps over input amplitudes (step s, column c
-// is amplitude 4s + c, so lane group c feeds amplitudes c, c + 4, ...): D/16
-// tiles x D/4 steps x 3 products = 3/4 of the MFMAs of the realified form.
-// Fragments: per tile and pair of steps, (Ur, Ui, Ur + Ui) as three 16-B
-// pairs per lane at [((mt * D/8 + t) * 3 + p) * 64 + lane].
-template <int K>
-__device__ __forceinline__ void dense_mfma3(double2* s, uint32_t zoff, int G, const OpRegs& op,
-                                            const double2* __restrict__ F) {
+// Real 16-row M-tiles over output amplitudes (row i of tile mt is amplitude
+// 16mt + i), K-steps over input amplitudes (step s, column c is amplitude
+// 4s + c, so lane group c feeds amplitudes c, c + 4, ...): D/16 tiles x D/4
+// steps x 3 products = 3/4 of the MFMAs of the realified form.  Fragments:
+// per tile and pair of steps, (Ur, Ui, Ur + Ui) as three pairs per lane at
+// [((mt * D/8 + t) * 3 + p) * 64 + lane].
+template <typename A, int K>
+__device__ __forceinline__ void dense_mfma3(A* s, uint32_t zoff, int G, const OpRegs& op, const A* __restrict__ F) {
+    using M = Mfma<A>;
+    constexpr int sh = Amp<A>::kShift;
     constexpr int D = 1 << K, NT = D / 16, H = D / 4, HP = H / 2;
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6, j = lane & 15u, c = lane >> 4;
     uint32_t lp = 0;
 #pragma unroll
     for (int b = 0; b < 4; ++b) lp |= ((j >> b) & 1u) << op.g(b);
-    lp = (swz(lp) << 4) | zoff;
+    lp = (swz(lp) << sh) | zoff;
     uint32_t so_in[H];
 #pragma unroll
-    for (int t = 0; t < H; ++t) so_in[t] = swz(local_offset<K>(c + 4u * t, op)) << 4;
+    for (int t = 0; t < H; ++t) so_in[t] = swz(local_offset<K>(c + 4u * t, op)) << sh;
     const uint32_t chunks = 1u << (G - 4);
     for (uint32_t ch = w; ch < chunks; ch += kWaves) {
         uint32_t up = 0;
         for (int b = 4; b < G; ++b) up |= ((ch >> (b - 4)) & 1u) << op.g(b);
-        const uint32_t base = lp ^ (swz(up) << 4);
-        double2 x[H];
+        const uint32_t base = lp ^ (swz(up) << sh);
+        A x[H];
 #pragma unroll
         for (int t = 0; t < H; ++t) x[t] = at(s, base ^ so_in[t]);
 #pragma unroll(K < 5 ? NT : 1)
         for (int mt = 0; mt < NT; ++mt) {
-            mfma_d4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = p1, p3 = p1;
+            typename M::v4 p1 = {0, 0, 0, 0}, p2 = p1, p3 = p1;
 #pragma unroll
             for (int t = 0; t < HP; ++t) {
-                const double2* f = F + ((mt * HP + t) * 3) * 64 + lane;
-                const double2 fr = f[0], fi = f[64], fs = f[128];
-                const double2 x0 = x[2 * t], x1 = x[2 * t + 1];
-                p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(fr.x, x0.x, p1, 0, 0, 0);
-                p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(fi.x, x0.y, p2, 0, 0, 0);
-                p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(fs.x, x0.x + x0.y, p3, 0, 0, 0);
-                p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(fr.y, x1.x, p1, 0, 0, 0);
-                p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(fi.y, x1.y, p2, 0, 0, 0);
-                p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(fs.y, x1.x + x1.y, p3, 0, 0, 0);
+                const A* f = F + ((mt * HP + t) * 3) * 64 + lane;
+                const A fr = f[0], fi = f[64], fs = f[128];
+                const A x0 = x[2 * t], x1 = x[2 * t + 1];
+                p1 = M::step(fr.x, x0.x, p1);
+                p2 = M::step(fi.x, x0.y, p2);
+                p3 = M::step(fs.x, x0.x + x0.y, p3);
+                p1 = M::step(fr.y, x1.x, p1);
+                p2 = M::step(fi.y, x1.y, p2);
+                p3 = M::step(fs.y, x1.x + x1.y, p3);
             }
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const uint32_t b = 16u * static_cast<uint32_t>(mt) + 4u * r + c;
-                at(s, base ^ (swz(local_offset<K>(b, op)) << 4)) =
-                    make_double2(p1[r] - p2[r], p3[r] - p1[r] - p2[r]);
+                const uint32_t b = 16u * static_cast<uint32_t>(mt) + (M::kGroupRows ? 4u * c + r : 4u * r + c);
+                A y;
+                y.x = p1[r] - p2[r];
+                y.y = p3[r] - p1[r] - p2[r];
+                at(s, base ^ (swz(local_offset<K>(b, op)) << sh)) = y;
             }
         }
     }
@@ -366,13 +396,11 @@ __device__ __forceinline__ void dense_mfma3(double2* s, uint32_t zoff, int G, co
 
 template <typename A>
 __device__ __forceinline__ void mfma_op(A* s, uint32_t zoff, int G, const OpRegs& op, const A* __restrict__ F) {
-    if constexpr (std::is_same_v<A, double2>) {
-        switch (op.k) {
-            case 3: dense_mfma<3>(s, zoff, G, op, F); break;
-            case 4: dense_mfma3<4>(s, zoff, G, op, F); break;
-            case 5: dense_mfma3<5>(s, zoff, G, op, F); break;
-            default: break;
-        }
+    switch (op.k) {
+        case 3: dense_mfma<A, 3>(s, zoff, G, op, F); break;
+        case 4: dense_mfma3<A, 4>(s, zoff, G, op, F); break;
+        case 5: dense_mfma3<A, 5>(s, zoff, G, op, F); break;
+        default: break;
     }
 }
 

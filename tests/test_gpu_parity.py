@@ -148,6 +148,25 @@ def test_matrix_core_path_matches_oracle(k):
     assert np.abs(got_off - ref).max() < ATOL
 
 
+@pytest.mark.parametrize("k", [3, 4, 5])
+def test_complex64_matrix_core_path(k):
+    """complex64 dense k = 3..5 on the FP32 matrix cores (the result-row
+    layout differs from FP64's) against the oracle and the VALU path."""
+    n = 15
+    cmds = W.brickwork_haar(n, 10, 60 + k)
+    ref = O.statevector(pack_stream(cmds), n, k, 0)
+    on = gpu_sim(precision="complex64")
+    on.fusion_max_qubits, on.fusion_min_qubits = k, 0
+    got = on.statevector(cmds, n)
+    assert on.last_stats["mfma_ops"] > 0
+    assert np.abs(got - ref).max() < 2e-5
+    off = gpu_sim(precision="complex64", use_mfma=0)
+    off.fusion_max_qubits, off.fusion_min_qubits = k, 0
+    got_off = off.statevector(cmds, n)
+    assert off.last_stats["mfma_ops"] == 0
+    assert np.abs(got_off - got).max() < 2e-5
+
+
 @pytest.mark.parametrize("fusion,tile", [(2, 11), (2, 12), (1, 11)])
 def test_register_blocks_at_22_qubits(fusion, tile):
     """2^22 amplitudes in 2^11 / 2^12 tiles, passes of register blocks and
