@@ -80,7 +80,8 @@ enum OpKind : int32_t {
     // ascending) whose 8 amplitudes a thread holds while the lmask following
     // OP_SUB entries act on them; an OP_SUB's k is its pattern (0-2: one
     // qubit on block bit k; 3, 4, 5: two qubits on block bits 01, 02, 12, the
-    // lower bit being matrix operand 0) and coef its matrix.
+    // lower bit being matrix operand 0; 6-11: the same for a diagonal) and
+    // coef its matrix (or its diagonal entries).
     OP_BLOCK = 6,
     OP_SUB = 7,
     // Device-only: dense op (complex128, k = 3..5) on the matrix cores; coef

@@ -489,16 +489,23 @@ int qhip_schedule_inspect(const qhip_config* cfg, const qhip_stream* in, int32_t
                         break;
                     case qhip::OP_SUB:   // pos[] holds the member op's own operand positions
                     case qhip::OP_DENSE: {
-                        it.k = d.kind == qhip::OP_SUB ? (d.k <= 2 ? 1 : 2) : d.k;
+                        const bool sub = d.kind == qhip::OP_SUB, diag = sub && d.k >= 6;
+                        const int pat = diag ? d.k - 6 : d.k;
+                        it.k = sub ? (pat <= 2 ? 1 : 2) : d.k;
                         const size_t D = size_t{1} << it.k;
-                        if (d.kind == qhip::OP_SUB && it.k == 2 && d.pos[0] > d.pos[1]) {
+                        if (sub && it.k == 2 && d.pos[0] > d.pos[1]) {
                             // stored in block-bit order: report it on ascending positions
                             it.q[0] = global(d.pos[1]);
                             it.q[1] = global(d.pos[0]);
                         } else {
                             for (int b = 0; b < it.k; ++b) it.q[b] = global(d.pos[b]);
                         }
-                        it.m.assign(plan.coefs.begin() + d.coef, plan.coefs.begin() + d.coef + D * D);
+                        if (diag) {
+                            it.m.assign(D * D, qhip::cplx(0.0));
+                            for (size_t y = 0; y < D; ++y) it.m[y * D + y] = plan.coefs[d.coef + y];
+                        } else {
+                            it.m.assign(plan.coefs.begin() + d.coef, plan.coefs.begin() + d.coef + D * D);
+                        }
                         break;
                     }
                     case qhip::OP_DIAG: {

@@ -310,28 +310,23 @@ static void emit_block(Plan& plan, const std::vector<HostOp>& ops, const Unit& u
     for (size_t idx : u.ops) {
         const HostOp& op = ops[idx];
         const bool diag = op.kind == OP_DIAG || op.kind == OP_IP_DIAG;
-        const size_t D = size_t{1} << op.k;
-        std::vector<cplx> M;   // member matrix over its operands (a diagonal expanded)
-        if (diag) {
-            M.assign(D * D, cplx(0.0));
-            for (size_t y = 0; y < D; ++y) M[y * D + y] = op.m[y];
-        } else {
-            M = op.m;
-        }
+        const std::vector<cplx>& M = op.m;   // D x D matrix, or D diagonal entries
+        auto sw = [](int x) { return ((x & 1) << 1) | ((x >> 1) & 1); };
         DevOp d{};
         d.kind = is_ip_host(op) ? OP_IPSUB : OP_SUB;
         d.coef = static_cast<int64_t>(plan.coefs.size());
         if (op.k == 1) {
-            d.k = bit_of(op.q[0]);
+            d.k = bit_of(op.q[0]) + (diag ? 6 : 0);
             plan.coefs.insert(plan.coefs.end(), M.begin(), M.end());
         } else {
             const int b0 = bit_of(op.q[0]), b1 = bit_of(op.q[1]);
             const int lo = std::min(b0, b1), hi = std::max(b0, b1);
-            d.k = lo == 0 ? (hi == 1 ? 3 : 4) : 5;
+            d.k = (lo == 0 ? (hi == 1 ? 3 : 4) : 5) + (diag ? 6 : 0);
             if (b0 < b1) {
                 plan.coefs.insert(plan.coefs.end(), M.begin(), M.end());
+            } else if (diag) {   // swap the two operand bits of the diagonal
+                for (int y = 0; y < 4; ++y) plan.coefs.push_back(M[sw(y)]);
             } else {   // swap the two operand bits of the 4x4 matrix
-                auto sw = [](int x) { return ((x & 1) << 1) | ((x >> 1) & 1); };
                 for (int y = 0; y < 4; ++y)
                     for (int x = 0; x < 4; ++x) plan.coefs.push_back(M[sw(y) * 4 + sw(x)]);
             }

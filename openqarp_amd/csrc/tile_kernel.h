@@ -438,6 +438,38 @@ __device__ __forceinline__ void sub_pair(A* v, const A* __restrict__ U) {
     }
 }
 
+template <typename A, int B0, int B1>   // diagonal on block bits B0 < B1 (B1 < 0: one bit)
+__device__ __forceinline__ void sub_diag(A* v, const A* __restrict__ Dg) {
+    constexpr int N = B1 < 0 ? 2 : 4;
+    A d[N];
+#pragma unroll
+    for (int e = 0; e < N; ++e) d[e] = Dg[e];
+#pragma unroll
+    for (int x = 0; x < 8; ++x) {
+        const int e = ((x >> B0) & 1) | (B1 < 0 ? 0 : ((x >> B1) & 1) << 1);
+        v[x] = cmul(v[x], d[e]);
+    }
+}
+
+template <typename A>
+__device__ __forceinline__ void sub_apply(A* v, int pattern, const A* __restrict__ U) {
+    switch (pattern) {
+        case 0: sub_single<A, 0>(v, U); break;
+        case 1: sub_single<A, 1>(v, U); break;
+        case 2: sub_single<A, 2>(v, U); break;
+        case 3: sub_pair<A, 0, 1>(v, U); break;
+        case 4: sub_pair<A, 0, 2>(v, U); break;
+        case 5: sub_pair<A, 1, 2>(v, U); break;
+        case 6: sub_diag<A, 0, -1>(v, U); break;
+        case 7: sub_diag<A, 1, -1>(v, U); break;
+        case 8: sub_diag<A, 2, -1>(v, U); break;
+        case 9: sub_diag<A, 0, 1>(v, U); break;
+        case 10: sub_diag<A, 0, 2>(v, U); break;
+        default: sub_diag<A, 1, 2>(v, U); break;
+    }
+}
+
+
 // One OP_BLOCK over one state's tile: load each group's 8 amplitudes, run
 // the nsub OP_SUB entries that follow the header, store.
 template <typename A>
@@ -464,14 +496,7 @@ __device__ __forceinline__ void block_apply(A* s, uint32_t tp, int G, const OpRe
         for (int j = 0; j < nsub; ++j) {
             const OpRegs sub = load_op(subs + j);
             const A* U = coefs + sub.coef;
-            switch (sub.k) {
-                case 0: sub_single<A, 0>(v, U); break;
-                case 1: sub_single<A, 1>(v, U); break;
-                case 2: sub_single<A, 2>(v, U); break;
-                case 3: sub_pair<A, 0, 1>(v, U); break;
-                case 4: sub_pair<A, 0, 2>(v, U); break;
-                default: sub_pair<A, 1, 2>(v, U); break;
-            }
+            sub_apply(v, sub.k, U);
         }
 #pragma unroll
         for (int x = 0; x < 8; ++x) at(s, a[x]) = v[x];
@@ -594,6 +619,21 @@ __device__ __forceinline__ double2 ip_pair(const A* v, const A* f, const A* __re
     return acc;
 }
 
+template <typename A, int B0, int B1>   // diagonal generator on block bits B0 < B1 (B1 < 0: one bit)
+__device__ __forceinline__ double2 ip_diag(const A* v, const A* f, const A* __restrict__ Dg) {
+    constexpr int N = B1 < 0 ? 2 : 4;
+    A d[N];
+#pragma unroll
+    for (int e = 0; e < N; ++e) d[e] = Dg[e];
+    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int x = 0; x < 8; ++x) {
+        const int e = ((x >> B0) & 1) | (B1 < 0 ? 0 : ((x >> B1) & 1) << 1);
+        ip_acc(acc, f[x], cmul(d[e], v[x]));
+    }
+    return acc;
+}
+
 // One OP_BLOCK over both states of a two-state pass: a thread holds a
 // group's 8 amplitudes of psi (v) and phi (f); OP_SUB members act on both,
 // OP_IPSUB members add <phi| G |psi> to their accumulator.  The header's
@@ -637,7 +677,13 @@ __device__ __forceinline__ int block_apply2(A* s, uint32_t tp, uint32_t kPhi, in
                         case 2: r = ip_single<A, 2>(v, f, U); break;
                         case 3: r = ip_pair<A, 0, 1>(v, f, U); break;
                         case 4: r = ip_pair<A, 0, 2>(v, f, U); break;
-                        default: r = ip_pair<A, 1, 2>(v, f, U); break;
+                        case 5: r = ip_pair<A, 1, 2>(v, f, U); break;
+                        case 6: r = ip_diag<A, 0, -1>(v, f, U); break;
+                        case 7: r = ip_diag<A, 1, -1>(v, f, U); break;
+                        case 8: r = ip_diag<A, 2, -1>(v, f, U); break;
+                        case 9: r = ip_diag<A, 0, 1>(v, f, U); break;
+                        case 10: r = ip_diag<A, 0, 2>(v, f, U); break;
+                        default: r = ip_diag<A, 1, 2>(v, f, U); break;
                     }
                     double2& dst = j_ip == 0 ? acc0 : j_ip == 1 ? acc1 : j_ip == 2 ? acc2 : acc3;
                     dst.x += r.x;
@@ -645,14 +691,8 @@ __device__ __forceinline__ int block_apply2(A* s, uint32_t tp, uint32_t kPhi, in
                     ++j_ip;
                     continue;
                 }
-                switch (sub.k) {
-                    case 0: sub_single<A, 0>(v, U); sub_single<A, 0>(f, U); break;
-                    case 1: sub_single<A, 1>(v, U); sub_single<A, 1>(f, U); break;
-                    case 2: sub_single<A, 2>(v, U); sub_single<A, 2>(f, U); break;
-                    case 3: sub_pair<A, 0, 1>(v, U); sub_pair<A, 0, 1>(f, U); break;
-                    case 4: sub_pair<A, 0, 2>(v, U); sub_pair<A, 0, 2>(f, U); break;
-                    default: sub_pair<A, 1, 2>(v, U); sub_pair<A, 1, 2>(f, U); break;
-                }
+                sub_apply(v, sub.k, U);
+                sub_apply(f, sub.k, U);
             }
 #pragma unroll
             for (int x = 0; x < 8; ++x) {
