@@ -38,8 +38,8 @@
 namespace qhip {
 
 constexpr int kST = 256;
-constexpr int kPerS = 8;
-constexpr uint64_t kSChunk = kST * kPerS;   // 2048 amplitudes
+constexpr int kPerS = 32;
+constexpr uint64_t kSChunk = kST * kPerS;   // 8192 amplitudes per chunk
 constexpr int kSerial = -100000;
 
 __device__ __forceinline__ double prob(const double2* s, uint64_t i) {
@@ -95,12 +95,18 @@ __global__ __launch_bounds__(kST) void chunk_increments(const A* __restrict__ s,
             const double x = ldexp(v, 52 - e);          // exact power-of-two scaling
             // A chunk that stays in the binade has every term below 2^e, i.e.
             // x < 2^52; anything larger (or NaN) sends the chunk to the serial
-            // path and keeps the 2048-term sum below 2^63.
+            // path.  Its whole sum is below 2^52 too, so a thread whose
+            // partial reaches 2^53 flags the chunk as well: the 256 partials
+            // then stay below 2^61 and the block sum cannot wrap.
             if (!(x < 4503599627370496.0)) { flag |= 2; continue; }
             const double fl = floor(x);
             const double fr = x - fl;                   // exact (Sterbenz)
             acc += static_cast<unsigned long long>(fl) + (fr > 0.5 ? 1u : 0u);
             if (fr == 0.5) flag |= 1;
+            if (acc >= (1ULL << 53)) {
+                flag |= 2;
+                acc = 0;
+            }
         }
     }
     rr[threadIdx.x] = acc;
