@@ -794,8 +794,16 @@ __device__ __forceinline__ int run_ops(A* s, const PassDesc& P, uint64_t gbase, 
                 const int nsub = static_cast<int>(op.lmask);
                 const int G = P.T - 3;
                 const uint32_t tp = thread_part<A>(op, G);
-                if constexpr (TWO) n_ip += block_apply2<A>(s, tp, kPhi, G, op, ops + o + 1, nsub, coefs, wave_ip, n_ip);
-                else block_apply<A>(s, tp, G, op, ops + o + 1, nsub, coefs);
+                if constexpr (TWO) {
+                    if ((op.spos >> 56) & 0xffu) {   // inner-product members: both states in registers
+                        n_ip += block_apply2<A>(s, tp, kPhi, G, op, ops + o + 1, nsub, coefs, wave_ip, n_ip);
+                    } else {
+                        block_apply<A>(s, tp, G, op, ops + o + 1, nsub, coefs);
+                        block_apply<A>(s, tp | kPhi, G, op, ops + o + 1, nsub, coefs);
+                    }
+                } else {
+                    block_apply<A>(s, tp, G, op, ops + o + 1, nsub, coefs);
+                }
                 o += nsub;   // the member entries are consumed here
                 break;
             }
@@ -1016,8 +1024,13 @@ __device__ __forceinline__ void tile_body(A* s, A* __restrict__ psi, A* __restri
     }
 }
 
+// Two-state tiles of the common kind (dense ops of at most two qubits) ask
+// the register allocator for two waves per SIMD (VGPRs + AGPRs <= 256), the
+// occupancy their LDS allows; the other variants are left unconstrained.
+constexpr int two_state_waves(bool two, int kmax) { return two && kmax == 2 ? 2 : 1; }
+
 template <typename A, bool TWO, int KMAX, int TB>
-__global__ __launch_bounds__(kThreads) void tile_pass_kernel(
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(two_state_waves(TWO, KMAX)))) void tile_pass_kernel(
     A* __restrict__ psi, A* __restrict__ phi, const PassDesc P, const DevOp* __restrict__ ops,
     const A* __restrict__ coefs, double2* __restrict__ ip_out, int64_t n_tiles) {
     __shared__ __align__(16) A s[TileLds<A, TWO, TB>::kElems];
@@ -1028,7 +1041,7 @@ __global__ __launch_bounds__(kThreads) void tile_pass_kernel(
 // of item j, where first[j] <= b < first[j + 1] (first[] is the exclusive
 // prefix sum of the items' tile counts, found by bisection; wave-uniform).
 template <typename A, bool TWO, int KMAX, int TB>
-__global__ __launch_bounds__(kThreads) void tile_batch_kernel(const BatchItem* __restrict__ items,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(two_state_waves(TWO, KMAX)))) void tile_batch_kernel(const BatchItem* __restrict__ items,
                                                               const int64_t* __restrict__ first, int n_items) {
     __shared__ __align__(16) A s[TileLds<A, TWO, TB>::kElems];
     const int64_t b = blockIdx.x;
