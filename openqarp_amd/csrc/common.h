@@ -80,8 +80,11 @@ enum OpKind : int32_t {
     // ascending) whose 8 amplitudes a thread holds while the lmask following
     // OP_SUB entries act on them; an OP_SUB's k is its pattern (0-2: one
     // qubit on block bit k; 3, 4, 5: two qubits on block bits 01, 02, 12, the
-    // lower bit being matrix operand 0; 6-11: the same for a diagonal) and
-    // coef its matrix (or its diagonal entries).
+    // lower bit being matrix operand 0; 6-11: the same for a diagonal, the
+    // lower block bit indexing entry bit 0; 12: a diagonal with no operand
+    // on the block) and coef its matrix (or its diagonal entries).  A
+    // diagonal member's operands outside the tile (spos bytes, count in
+    // spos[7]) select a sub-table of its entries once per tile.
     OP_BLOCK = 6,
     OP_SUB = 7,
     // Device-only: dense op (complex128, k = 3..5) on the matrix cores; coef

@@ -490,8 +490,17 @@ int qhip_schedule_inspect(const qhip_config* cfg, const qhip_stream* in, int32_t
                     case qhip::OP_SUB:   // pos[] holds the member op's own operand positions
                     case qhip::OP_DENSE: {
                         const bool sub = d.kind == qhip::OP_SUB, diag = sub && d.k >= 6;
-                        const int pat = diag ? d.k - 6 : d.k;
-                        it.k = sub ? (pat <= 2 ? 1 : 2) : d.k;
+                        if (diag) {   // local operands (ascending), then the ones outside the tile
+                            const int kl = d.k == 12 ? 0 : d.k <= 8 ? 1 : 2, kg = d.spos[7];
+                            it.k = kl + kg;
+                            for (int b = 0; b < kl; ++b) it.q[b] = global(d.pos[b]);
+                            for (int j = 0; j < kg; ++j) it.q[kl + j] = static_cast<uint32_t>(d.spos[j]);
+                            const size_t D = size_t{1} << it.k;
+                            it.m.assign(D * D, qhip::cplx(0.0));
+                            for (size_t y = 0; y < D; ++y) it.m[y * D + y] = plan.coefs[d.coef + y];
+                            break;
+                        }
+                        it.k = sub ? (d.k <= 2 ? 1 : 2) : d.k;
                         const size_t D = size_t{1} << it.k;
                         if (sub && it.k == 2 && d.pos[0] > d.pos[1]) {
                             // stored in block-bit order: report it on ascending positions
@@ -500,12 +509,7 @@ int qhip_schedule_inspect(const qhip_config* cfg, const qhip_stream* in, int32_t
                         } else {
                             for (int b = 0; b < it.k; ++b) it.q[b] = global(d.pos[b]);
                         }
-                        if (diag) {
-                            it.m.assign(D * D, qhip::cplx(0.0));
-                            for (size_t y = 0; y < D; ++y) it.m[y * D + y] = plan.coefs[d.coef + y];
-                        } else {
-                            it.m.assign(plan.coefs.begin() + d.coef, plan.coefs.begin() + d.coef + D * D);
-                        }
+                        it.m.assign(plan.coefs.begin() + d.coef, plan.coefs.begin() + d.coef + D * D);
                         break;
                     }
                     case qhip::OP_DIAG: {

@@ -227,32 +227,36 @@ static void emit_op(Plan& plan, const HostOp& op, const int* local_of, int n, in
 // past disjoint ops only); an op touching no unit may join any block.
 struct Unit {
     std::vector<size_t> ops;
-    uint64_t qset = 0;
+    uint64_t qset = 0;   // tile qubits (a block's bits)
+    uint64_t qall = 0;   // every qubit its ops touch (ordering)
     bool block_ok = false;
 };
 
 static bool is_ip_host(const HostOp& op) { return op.kind == OP_IP_DENSE || op.kind == OP_IP_DIAG; }
 
-// Members: dense ops, and diagonal ops whose operands are all tile qubits
-// (tile mask tq); in two-state passes also inner-product ops (at most
-// kMaxBlockIp per block), which a block evaluates on its registers.
+// Members: dense ops on at most two qubits, and diagonal ops with at most
+// two operands on tile qubits (tile mask tq) and up to four outside the
+// tile, whose bits select a sub-table once per tile; in two-state passes
+// also the inner-product forms of both (at most kMaxBlockIp per block),
+// which a block evaluates on its registers.
 static std::vector<Unit> form_units(const std::vector<HostOp>& ops, const std::vector<size_t>& chosen, bool enable,
                                     bool two, uint64_t tq) {
     std::vector<Unit> units;
     std::vector<int> n_ip;
     for (size_t idx : chosen) {
         const HostOp& op = ops[idx];
-        const uint64_t Q = op.qmask();
+        const uint64_t Q = op.qmask(), Ql = Q & tq;
         const bool ip = is_ip_host(op);
-        const bool kind_ok = op.kind == OP_DENSE || (op.kind == OP_DIAG && (Q & ~tq) == 0) ||
-                             (two && ip && (Q & ~tq) == 0);
-        const bool mergeable = enable && kind_ok && op.k >= 1 && op.k <= 2;
+        const bool diag = op.kind == OP_DIAG || (two && op.kind == OP_IP_DIAG);
+        const bool kind_ok = (op.kind == OP_DENSE && op.k <= 2) || (two && op.kind == OP_IP_DENSE && op.k <= 2) ||
+                             (diag && std::popcount(Ql) <= 2 && std::popcount(Q & ~tq) <= 4);
+        const bool mergeable = enable && kind_ok && op.k >= 1;
         int last = -1;
         for (int u = static_cast<int>(units.size()) - 1; u >= 0; --u)
-            if (op.kind == OP_SCALAR || (units[u].qset & Q)) { last = u; break; }
+            if (op.kind == OP_SCALAR || (units[u].qall & Q)) { last = u; break; }
         int target = -1;
         auto fits = [&](int u) {
-            return units[u].block_ok && std::popcount(units[u].qset | Q) <= 3 && (!ip || n_ip[u] < kMaxBlockIp);
+            return units[u].block_ok && std::popcount(units[u].qset | Ql) <= 3 && (!ip || n_ip[u] < kMaxBlockIp);
         };
         if (mergeable) {
             if (last >= 0) {
@@ -264,12 +268,14 @@ static std::vector<Unit> form_units(const std::vector<HostOp>& ops, const std::v
         }
         if (target >= 0) {
             units[target].ops.push_back(idx);
-            units[target].qset |= Q;
+            units[target].qset |= Ql;
+            units[target].qall |= Q;
             n_ip[target] += ip;
         } else {
             Unit u;
             u.ops.push_back(idx);
-            u.qset = Q;
+            u.qset = Ql;
+            u.qall = Q;
             u.block_ok = mergeable;
             units.push_back(std::move(u));
             n_ip.push_back(ip);
@@ -307,31 +313,50 @@ static void emit_block(Plan& plan, const std::vector<HostOp>& ops, const Unit& u
         fail(QHIP_RUNTIME, "schedule: block operand outside the block");
     };
     int n_ip = 0;
+    auto sw = [](int x) { return ((x & 1) << 1) | ((x >> 1) & 1); };
     for (size_t idx : u.ops) {
         const HostOp& op = ops[idx];
         const bool diag = op.kind == OP_DIAG || op.kind == OP_IP_DIAG;
-        const std::vector<cplx>& M = op.m;   // D x D matrix, or D diagonal entries
-        auto sw = [](int x) { return ((x & 1) << 1) | ((x >> 1) & 1); };
         DevOp d{};
         d.kind = is_ip_host(op) ? OP_IPSUB : OP_SUB;
         d.coef = static_cast<int64_t>(plan.coefs.size());
-        if (op.k == 1) {
-            d.k = bit_of(op.q[0]) + (diag ? 6 : 0);
-            plan.coefs.insert(plan.coefs.end(), M.begin(), M.end());
+        if (diag) {
+            // Local operands index the entries within a block group (lower
+            // block bit first), global ones pick a sub-table per tile.
+            int lb[2], gb[4], kl = 0, kg = 0;
+            for (int b = 0; b < op.k; ++b) (local_of[op.q[b]] >= 0 ? lb[kl++] : gb[kg++]) = b;
+            if (kl == 2 && bit_of(op.q[lb[0]]) > bit_of(op.q[lb[1]])) std::swap(lb[0], lb[1]);
+            int bits[2] = {0, 0};
+            for (int i = 0; i < kl; ++i) bits[i] = bit_of(op.q[lb[i]]);
+            d.k = kl == 0 ? 12 : kl == 1 ? 6 + bits[0] : 9 + (bits[0] == 0 ? (bits[1] == 1 ? 0 : 1) : 2);
+            for (int i = 0; i < kl; ++i) d.pos[i] = static_cast<int8_t>(local_of[op.q[lb[i]]]);
+            for (int j = 0; j < kg; ++j) d.spos[j] = static_cast<int8_t>(op.q[gb[j]]);
+            d.spos[7] = static_cast<int8_t>(kg);
+            for (uint32_t gi = 0; gi < (1u << kg); ++gi)
+                for (uint32_t li = 0; li < (1u << kl); ++li) {
+                    uint32_t e = 0;
+                    for (int i = 0; i < kl; ++i) e |= ((li >> i) & 1u) << lb[i];
+                    for (int j = 0; j < kg; ++j) e |= ((gi >> j) & 1u) << gb[j];
+                    plan.coefs.push_back(op.m[e]);
+                }
         } else {
-            const int b0 = bit_of(op.q[0]), b1 = bit_of(op.q[1]);
-            const int lo = std::min(b0, b1), hi = std::max(b0, b1);
-            d.k = (lo == 0 ? (hi == 1 ? 3 : 4) : 5) + (diag ? 6 : 0);
-            if (b0 < b1) {
+            const std::vector<cplx>& M = op.m;
+            if (op.k == 1) {
+                d.k = bit_of(op.q[0]);
                 plan.coefs.insert(plan.coefs.end(), M.begin(), M.end());
-            } else if (diag) {   // swap the two operand bits of the diagonal
-                for (int y = 0; y < 4; ++y) plan.coefs.push_back(M[sw(y)]);
-            } else {   // swap the two operand bits of the 4x4 matrix
-                for (int y = 0; y < 4; ++y)
-                    for (int x = 0; x < 4; ++x) plan.coefs.push_back(M[sw(y) * 4 + sw(x)]);
+            } else {
+                const int b0 = bit_of(op.q[0]), b1 = bit_of(op.q[1]);
+                const int lo = std::min(b0, b1), hi = std::max(b0, b1);
+                d.k = lo == 0 ? (hi == 1 ? 3 : 4) : 5;
+                if (b0 < b1) {
+                    plan.coefs.insert(plan.coefs.end(), M.begin(), M.end());
+                } else {   // swap the two operand bits of the 4x4 matrix
+                    for (int y = 0; y < 4; ++y)
+                        for (int x = 0; x < 4; ++x) plan.coefs.push_back(M[sw(y) * 4 + sw(x)]);
+                }
             }
+            for (int b = 0; b < op.k; ++b) d.pos[b] = static_cast<int8_t>(local_of[op.q[b]]);
         }
-        for (int b = 0; b < op.k; ++b) d.pos[b] = static_cast<int8_t>(local_of[op.q[b]]);
         if (d.kind == OP_IPSUB) {
             d.lmask = static_cast<uint64_t>(op.slot);
             ++n_ip;

@@ -163,6 +163,19 @@ __device__ __forceinline__ uint32_t local_offset(uint32_t x, const OpRegs& op) {
     return o;
 }
 
+// Sub-table of a grouped diagonal for a tile: the bits of its operands
+// outside the tile (global qubits in spos bytes 0..5, their count in byte 7).
+__device__ __forceinline__ uint32_t diag_sub_table(uint64_t gbase, const OpRegs& op) {
+    const int kg = static_cast<int>((op.spos >> 56) & 0xffu);
+    uint32_t gsel = 0;
+    for (int j = 0; j < kg; ++j) gsel |= static_cast<uint32_t>((gbase >> ((op.spos >> (8 * j)) & 0xffu)) & 1u) << j;
+    return gsel;
+}
+
+// Entries of a register-block member: a diagonal member (pattern >= 6) with
+// operands outside the tile uses the sub-table its tile selects.
+__device__ __forceinline__ int member_local_bits(int pattern) { return pattern == 12 ? 0 : pattern <= 8 ? 1 : 2; }
+
 // Byte offset (swizzled) of this thread's part of a dense op's group number:
 // thread bit b of the group number sits at local position gpos[b].
 template <typename A>
@@ -465,7 +478,13 @@ __device__ __forceinline__ void sub_apply(A* v, int pattern, const A* __restrict
         case 8: sub_diag<A, 2, -1>(v, U); break;
         case 9: sub_diag<A, 0, 1>(v, U); break;
         case 10: sub_diag<A, 0, 2>(v, U); break;
-        default: sub_diag<A, 1, 2>(v, U); break;
+        case 11: sub_diag<A, 1, 2>(v, U); break;
+        default: {   // 12: no operand on the block
+            const A d = U[0];
+#pragma unroll
+            for (int x = 0; x < 8; ++x) v[x] = cmul(v[x], d);
+            break;
+        }
     }
 }
 
@@ -474,7 +493,8 @@ __device__ __forceinline__ void sub_apply(A* v, int pattern, const A* __restrict
 // the nsub OP_SUB entries that follow the header, store.
 template <typename A>
 __device__ __forceinline__ void block_apply(A* s, uint32_t tp, int G, const OpRegs& hdr,
-                                            const DevOp* __restrict__ subs, int nsub, const A* __restrict__ coefs) {
+                                            const DevOp* __restrict__ subs, int nsub, const A* __restrict__ coefs,
+                                            uint64_t gbase) {
     constexpr int sh = Amp<A>::kShift;
     if (G < 8 && threadIdx.x >= (1u << G)) return;
     uint32_t so[8];
@@ -496,6 +516,7 @@ __device__ __forceinline__ void block_apply(A* s, uint32_t tp, int G, const OpRe
         for (int j = 0; j < nsub; ++j) {
             const OpRegs sub = load_op(subs + j);
             const A* U = coefs + sub.coef;
+            if (sub.k >= 6) U += diag_sub_table(gbase, sub) << member_local_bits(sub.k);
             sub_apply(v, sub.k, U);
         }
 #pragma unroll
@@ -568,15 +589,6 @@ __device__ __forceinline__ double2 ip_diag_groups(A* s, uint32_t tp0, uint32_t t
     return acc;
 }
 
-// Sub-table of a grouped diagonal for a tile: the bits of its operands
-// outside the tile (global qubits in spos bytes 0..5, their count in byte 7).
-__device__ __forceinline__ uint32_t diag_sub_table(uint64_t gbase, const OpRegs& op) {
-    const int kg = static_cast<int>((op.spos >> 56) & 0xffu);
-    uint32_t gsel = 0;
-    for (int j = 0; j < kg; ++j) gsel |= static_cast<uint32_t>((gbase >> ((op.spos >> (8 * j)) & 0xffu)) & 1u) << j;
-    return gsel;
-}
-
 // <f| G |v> over a block's 8 amplitudes for a generator on block bit B
 // (2x2) or on block bits B0 < B1 (4x4, operand 0 on B0); FP64 sums.
 template <typename A>
@@ -642,7 +654,7 @@ __device__ __forceinline__ double2 ip_diag(const A* v, const A* f, const A* __re
 template <typename A>
 __device__ __forceinline__ int block_apply2(A* s, uint32_t tp, uint32_t kPhi, int G, const OpRegs& hdr,
                                             const DevOp* __restrict__ subs, int nsub, const A* __restrict__ coefs,
-                                            double2* wave_ip, int n_ip) {
+                                            double2* wave_ip, int n_ip, uint64_t gbase) {
     constexpr int sh = Amp<A>::kShift;
     static_assert(kMaxBlockIp == 4, "four named accumulators");
     const int nipb = static_cast<int>((hdr.spos >> 56) & 0xffu);
@@ -669,6 +681,7 @@ __device__ __forceinline__ int block_apply2(A* s, uint32_t tp, uint32_t kPhi, in
             for (int j = 0; j < nsub; ++j) {
                 const OpRegs sub = load_op(subs + j);
                 const A* U = coefs + sub.coef;
+                if (sub.k >= 6) U += diag_sub_table(gbase, sub) << member_local_bits(sub.k);
                 if (sub.kind == OP_IPSUB) {
                     double2 r;
                     switch (sub.k) {
@@ -683,7 +696,14 @@ __device__ __forceinline__ int block_apply2(A* s, uint32_t tp, uint32_t kPhi, in
                         case 8: r = ip_diag<A, 2, -1>(v, f, U); break;
                         case 9: r = ip_diag<A, 0, 1>(v, f, U); break;
                         case 10: r = ip_diag<A, 0, 2>(v, f, U); break;
-                        default: r = ip_diag<A, 1, 2>(v, f, U); break;
+                        case 11: r = ip_diag<A, 1, 2>(v, f, U); break;
+                        default: {   // 12: no operand on the block
+                            const A d = U[0];
+                            r = make_double2(0.0, 0.0);
+#pragma unroll
+                            for (int x = 0; x < 8; ++x) ip_acc(r, f[x], cmul(d, v[x]));
+                            break;
+                        }
                     }
                     double2& dst = j_ip == 0 ? acc0 : j_ip == 1 ? acc1 : j_ip == 2 ? acc2 : acc3;
                     dst.x += r.x;
@@ -796,13 +816,13 @@ __device__ __forceinline__ int run_ops(A* s, const PassDesc& P, uint64_t gbase, 
                 const uint32_t tp = thread_part<A>(op, G);
                 if constexpr (TWO) {
                     if ((op.spos >> 56) & 0xffu) {   // inner-product members: both states in registers
-                        n_ip += block_apply2<A>(s, tp, kPhi, G, op, ops + o + 1, nsub, coefs, wave_ip, n_ip);
+                        n_ip += block_apply2<A>(s, tp, kPhi, G, op, ops + o + 1, nsub, coefs, wave_ip, n_ip, gbase);
                     } else {
-                        block_apply<A>(s, tp, G, op, ops + o + 1, nsub, coefs);
-                        block_apply<A>(s, tp | kPhi, G, op, ops + o + 1, nsub, coefs);
+                        block_apply<A>(s, tp, G, op, ops + o + 1, nsub, coefs, gbase);
+                        block_apply<A>(s, tp | kPhi, G, op, ops + o + 1, nsub, coefs, gbase);
                     }
                 } else {
-                    block_apply<A>(s, tp, G, op, ops + o + 1, nsub, coefs);
+                    block_apply<A>(s, tp, G, op, ops + o + 1, nsub, coefs, gbase);
                 }
                 o += nsub;   // the member entries are consumed here
                 break;
@@ -1026,11 +1046,19 @@ __device__ __forceinline__ void tile_body(A* s, A* __restrict__ psi, A* __restri
 
 // Two-state tiles of the common kind (dense ops of at most two qubits) ask
 // the register allocator for two waves per SIMD (VGPRs + AGPRs <= 256), the
-// occupancy their LDS allows; the other variants are left unconstrained.
+// occupancy their LDS allows (VGPRs and AGPRs share one register file).
 constexpr int two_state_waves(bool two, int kmax) { return two && kmax == 2 ? 2 : 1; }
+// Single-state 32-KiB tiles of the common kind in the one-circuit kernel:
+// four waves per SIMD (VGPRs + AGPRs <= 128); their LDS would allow five.
+// (The batched kernel's instance lookup does not fit that budget.)
+template <typename A, bool TWO, int KMAX, int TB, bool BATCH>
+constexpr int tile_waves() {
+    if (TWO) return two_state_waves(TWO, KMAX);
+    return (!BATCH && KMAX == 2 && TB + Amp<A>::kShift == 15) ? 4 : 1;
+}
 
 template <typename A, bool TWO, int KMAX, int TB>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(two_state_waves(TWO, KMAX)))) void tile_pass_kernel(
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(tile_waves<A, TWO, KMAX, TB, false>()))) void tile_pass_kernel(
     A* __restrict__ psi, A* __restrict__ phi, const PassDesc P, const DevOp* __restrict__ ops,
     const A* __restrict__ coefs, double2* __restrict__ ip_out, int64_t n_tiles) {
     __shared__ __align__(16) A s[TileLds<A, TWO, TB>::kElems];
@@ -1041,7 +1069,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(two_st
 // of item j, where first[j] <= b < first[j + 1] (first[] is the exclusive
 // prefix sum of the items' tile counts, found by bisection; wave-uniform).
 template <typename A, bool TWO, int KMAX, int TB>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(two_state_waves(TWO, KMAX)))) void tile_batch_kernel(const BatchItem* __restrict__ items,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(tile_waves<A, TWO, KMAX, TB, true>()))) void tile_batch_kernel(const BatchItem* __restrict__ items,
                                                               const int64_t* __restrict__ first, int n_items) {
     __shared__ __align__(16) A s[TileLds<A, TWO, TB>::kElems];
     const int64_t b = blockIdx.x;
