@@ -89,6 +89,26 @@ def _param_values(cmd: Command, values: Optional[Mapping[str, float]]) -> list[f
     return out
 
 
+def _param_values_fast(cmd: Command, values: Optional[Mapping[str, float]]) -> list[float]:
+    """_param_values with the affine case inlined (same arithmetic as
+    Param.evaluate: offset + sum of coefficient x value)."""
+    out = []
+    for p in cmd.params:
+        if p._fn is not None:
+            return _param_values(cmd, values)
+        t = p._terms
+        if not t:
+            out.append(p._offset)
+            continue
+        if values is None:
+            raise RuntimeError(f"gate '{cmd.gate.name}' has unresolved symbolic parameter")
+        try:
+            out.append(p._offset + sum(c * float(values[s]) for s, c in t.items()))
+        except KeyError:
+            return _param_values(cmd, values)   # raises with the unbound symbols named
+    return out
+
+
 def pack_stream(commands: Sequence[Command], values: Optional[Mapping[str, float]] = None,
                 param_index: Optional[Mapping[str, int]] = None) -> PackedStream:
     """Flatten commands into qhip_op records.
@@ -117,7 +137,7 @@ def pack_stream(commands: Sequence[Command], values: Optional[Mapping[str, float
         qubits.extend(qs)
         ps = cmd.params
         if ps:
-            vals = _param_values(cmd, values)
+            vals = _param_values_fast(cmd, values)
             if len(vals) > 4:
                 raise ValueError(f"gate '{g.name}' carries {len(vals)} parameters")
             npar[i] = len(vals)
@@ -135,12 +155,13 @@ def pack_stream(commands: Sequence[Command], values: Optional[Mapping[str, float
         if param_index is not None and ps:
             nd = 0
             for p in ps:
-                if not p.is_symbolic():
-                    continue
-                try:
-                    coeffs = affine_coefficients(p)
-                except CapabilityError as e:
-                    raise CapabilityError(f"run_gradient: {e}") from None
+                if p._fn is None:
+                    coeffs = p._terms   # affine: the coefficients are the terms
+                else:
+                    try:
+                        coeffs = affine_coefficients(p)
+                    except CapabilityError as e:
+                        raise CapabilityError(f"run_gradient: {e}") from None
                 for name, alpha in coeffs.items():
                     if name in param_index:
                         dep_idx.append(int(param_index[name]))
