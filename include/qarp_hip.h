@@ -119,7 +119,7 @@ typedef struct qhip_observable {
 /* Simulator knobs.  fusion_max_qubits / fusion_min_qubits have the meaning
  * of QarpSimulator::fusion_max_qubits / fusion_min_qubits
  * (simulator/qarp_simulator.h:41-61).  The rest steer the device schedule:
- * 0 (or -1 for use_mfma) selects the tuned default. */
+ * 0 (or -1 for use_mfma and relabel_qubits) selects the tuned default. */
 typedef struct qhip_config {
     int32_t fusion_max_qubits;   /* 0..8                                       */
     int32_t fusion_min_qubits;   /* registers below this get 1q fusion only    */
@@ -127,6 +127,9 @@ typedef struct qhip_config {
     int32_t low_qubits;          /* contiguous low qubits per tile, 0=auto     */
     int32_t max_ops_per_pass;    /* 0 = auto                                   */
     int32_t use_mfma;            /* -1 auto, 0 never: dense k=3..5 on the matrix cores (FP64 or FP32 MFMA) */
+    int32_t relabel_qubits;      /* -1 auto, 0 never: a pass's store may move qubits into the contiguous low
+                                    tile positions for the next pass (the layout is restored before a
+                                    schedule ends) */
 } qhip_config;
 
 /* What one apply did (filled when the caller passes a non-NULL pointer). */

@@ -49,7 +49,8 @@ class QhipObservable(C.Structure):
 class QhipConfig(C.Structure):
     _fields_ = [("fusion_max_qubits", C.c_int32), ("fusion_min_qubits", C.c_int32),
                 ("tile_qubits", C.c_int32), ("low_qubits", C.c_int32),
-                ("max_ops_per_pass", C.c_int32), ("use_mfma", C.c_int32)]
+                ("max_ops_per_pass", C.c_int32), ("use_mfma", C.c_int32),
+                ("relabel_qubits", C.c_int32)]
 
 
 class QhipApplyStats(C.Structure):
@@ -229,6 +230,7 @@ def pack_observable(observable: Observable) -> PackedObservable:
 
 
 def config_struct(fusion_max_qubits: int, fusion_min_qubits: int, tile_qubits: int = 0,
-                  low_qubits: int = 0, max_ops_per_pass: int = 0, use_mfma: int = -1) -> QhipConfig:
+                  low_qubits: int = 0, max_ops_per_pass: int = 0, use_mfma: int = -1,
+                  relabel_qubits: int = -1) -> QhipConfig:
     return QhipConfig(fusion_max_qubits, fusion_min_qubits, tile_qubits, low_qubits,
-                      max_ops_per_pass, use_mfma)
+                      max_ops_per_pass, use_mfma, relabel_qubits)

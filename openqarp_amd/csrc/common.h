@@ -203,6 +203,11 @@ struct PassDesc {
     uint64_t tmask;      // global bits that are tile-local
     uint64_t hmask;      // global bits of local positions >= 8
     int32_t qpos[40];    // qpos[l] = global qubit of local position l
+    // Store permutation: the amplitude written at output local index m is
+    // the one at input local index sum_i bit_i(m) << sperm[i] (identity:
+    // sperm[i] = i).  It relabels which qubit lives at which tile position
+    // for the passes that follow.
+    int8_t sperm[16];
 };
 
 // One instance's pass in a batched launch (tile_batch_kernel).
@@ -232,6 +237,7 @@ struct ScheduleOptions {
     bool two_states = false; // adjoint passes hold psi and phi
     bool register_blocks = true;  // group k <= 2 dense ops into OP_BLOCKs
     int mfma = -1;           // dense k >= 3 on the matrix cores: -1 auto, 0 never, 1 where possible
+    bool relabel = true;     // passes may move the next pass's qubits into the contiguous low positions
     int precision = QHIP_C128;
 };
 Plan schedule(const std::vector<HostOp>& ops, int n, int precision, const ScheduleOptions& opt);

@@ -121,6 +121,7 @@ static ScheduleOptions options(const qhip_config* cfg, bool two) {
         o.low_qubits = cfg->low_qubits;
         o.max_ops = cfg->max_ops_per_pass;
         o.mfma = cfg->use_mfma;
+        o.relabel = cfg->relabel_qubits != 0;
     }
     return o;
 }
@@ -547,6 +548,28 @@ int qhip_schedule_inspect(const qhip_config* cfg, const qhip_stream* in, int32_t
                     default:
                         qhip::fail(QHIP_RUNTIME, "qhip_schedule_inspect: unexpected op kind");
                 }
+                f->items.push_back(std::move(it));
+                f->src_ops.push_back(qhip_op{});
+                f->src_qubits.emplace_back();
+                f->src_mats.emplace_back();
+            }
+            // The store permutation as SWAPs of the pass's global bits:
+            // position i ends up holding what position sperm[i] held.
+            int held[16];
+            for (int i = 0; i < 16; ++i) held[i] = i;
+            for (int i = 0; i < P.T && P.T <= 16; ++i) {
+                int j = i;
+                while (j < P.T && held[j] != P.sperm[i]) ++j;
+                if (j == P.T) qhip::fail(QHIP_RUNTIME, "qhip_schedule_inspect: store permutation is not a permutation");
+                if (j == i) continue;
+                std::swap(held[i], held[j]);
+                qhip::FusedItem it;
+                it.pass = static_cast<int32_t>(pi);
+                it.k = 2;
+                it.q[0] = static_cast<uint32_t>(P.qpos[i]);
+                it.q[1] = static_cast<uint32_t>(P.qpos[j]);
+                it.m.assign(16, qhip::cplx(0.0));
+                for (int x = 0; x < 4; ++x) it.m[(((x & 1) << 1) | (x >> 1)) * 4 + x] = 1.0;
                 f->items.push_back(std::move(it));
                 f->src_ops.push_back(qhip_op{});
                 f->src_qubits.emplace_back();

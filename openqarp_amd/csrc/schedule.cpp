@@ -149,7 +149,10 @@ std::vector<cplx> mfma_matrix(const cplx* f, int k, bool f32) {
     return U;
 }
 
-static void emit_op(Plan& plan, const HostOp& op, const int* local_of, int n, int T, const ScheduleOptions& opt) {
+// local_of[q]: tile position of qubit q (-1 outside the tile); pos[q]: the
+// global bit that holds qubit q (the identity unless passes relabel).
+static void emit_op(Plan& plan, const HostOp& op, const int* local_of, const int* pos, int n, int T,
+                    const ScheduleOptions& opt) {
     DevOp d{};
     d.kind = op.kind;
     d.k = op.k;
@@ -181,7 +184,7 @@ static void emit_op(Plan& plan, const HostOp& op, const int* local_of, int n, in
                 d.k = kl;
                 for (int i = 0; i < kl; ++i) d.pos[i] = static_cast<int8_t>(local_of[op.q[lb[i]]]);
                 if (kl > 0) group_positions(d, T);
-                for (int j = 0; j < kg; ++j) d.spos[j] = static_cast<int8_t>(op.q[gb[j]]);
+                for (int j = 0; j < kg; ++j) d.spos[j] = static_cast<int8_t>(pos[op.q[gb[j]]]);
                 d.spos[7] = static_cast<int8_t>(kg);
                 for (uint32_t gi = 0; gi < (1u << kg); ++gi)
                     for (uint32_t li = 0; li < (1u << kl); ++li) {
@@ -193,7 +196,7 @@ static void emit_op(Plan& plan, const HostOp& op, const int* local_of, int n, in
             } else {
                 for (int b = 0; b < op.k; ++b)
                     d.pos[b] = static_cast<int8_t>(local_of[op.q[b]] >= 0 ? local_of[op.q[b]]
-                                                                           : 32 + static_cast<int>(op.q[b]));
+                                                                           : 32 + pos[op.q[b]]);
                 plan.coefs.insert(plan.coefs.end(), op.m.begin(), op.m.end());
             }
             ++plan.diag_ops;
@@ -203,7 +206,7 @@ static void emit_op(Plan& plan, const HostOp& op, const int* local_of, int n, in
             for (int q = 0; q < n; ++q)
                 if ((op.mask >> q) & 1) {
                     if (local_of[q] >= 0) d.lmask |= uint64_t{1} << local_of[q];
-                    else d.gmask |= uint64_t{1} << q;
+                    else d.gmask |= uint64_t{1} << pos[q];
                 }
             plan.coefs.push_back(op.phase);
             ++plan.diag_ops;
@@ -286,7 +289,8 @@ static std::vector<Unit> form_units(const std::vector<HostOp>& ops, const std::v
 
 // Emits a block of >= 2 member ops: the OP_BLOCK header, then one OP_SUB per
 // member with its matrix in block-bit order.
-static void emit_block(Plan& plan, const std::vector<HostOp>& ops, const Unit& u, const int* local_of, int n, int T) {
+static void emit_block(Plan& plan, const std::vector<HostOp>& ops, const Unit& u, const int* local_of, const int* pos,
+                       int n, int T) {
     int bp[3];
     int m = 0;
     for (int q = 0; q < n; ++q)
@@ -330,7 +334,7 @@ static void emit_block(Plan& plan, const std::vector<HostOp>& ops, const Unit& u
             for (int i = 0; i < kl; ++i) bits[i] = bit_of(op.q[lb[i]]);
             d.k = kl == 0 ? 12 : kl == 1 ? 6 + bits[0] : 9 + (bits[0] == 0 ? (bits[1] == 1 ? 0 : 1) : 2);
             for (int i = 0; i < kl; ++i) d.pos[i] = static_cast<int8_t>(local_of[op.q[lb[i]]]);
-            for (int j = 0; j < kg; ++j) d.spos[j] = static_cast<int8_t>(op.q[gb[j]]);
+            for (int j = 0; j < kg; ++j) d.spos[j] = static_cast<int8_t>(pos[op.q[gb[j]]]);
             d.spos[7] = static_cast<int8_t>(kg);
             for (uint32_t gi = 0; gi < (1u << kg); ++gi)
                 for (uint32_t li = 0; li < (1u << kl); ++li) {
@@ -470,15 +474,165 @@ PassPick pick_pass(const SegmentDag& dag, const std::vector<size_t>& seg, const 
 }
 }  // namespace
 
+// Qubit relabeling.  A tile keeps its low qubits at the lowest global bits,
+// so every thread's loads and stores are contiguous runs.  Which qubits sit
+// there is free: a pass may store its tile through a permutation of tile
+// positions (PassDesc::sperm) that moves the qubits the next pass wants to
+// the low bits, so that pass need not spend tile positions on the forced
+// ones.  Layouts are transpositions of low slot i with the home bit of the
+// qubit that occupies it; a relabel is taken when every bit it changes is a
+// bit of the current tile.  The identity layout is restored by the last pass
+// of the segment (or by one pass of its own), so a plan starts and ends on
+// the caller's qubit order.
+namespace {
+struct Layout {
+    int low = 0;
+    int at[64];    // at[b] = qubit held by global bit b
+    int pos[64];   // pos[q] = global bit holding qubit q
+    explicit Layout(int l) : low(l) {
+        for (int q = 0; q < 64; ++q) at[q] = pos[q] = q;
+    }
+    // Low slot i holds a[i] (a[i] == i or a[i] >= low, all distinct).
+    static Layout of(const int* a, int low) {
+        Layout L(low);
+        for (int i = 0; i < low; ++i)
+            if (a[i] != i) {
+                L.at[i] = a[i];
+                L.at[a[i]] = i;
+                L.pos[a[i]] = i;
+                L.pos[i] = a[i];
+            }
+        return L;
+    }
+    uint64_t low_qubits() const {
+        uint64_t m = 0;
+        for (int i = 0; i < low; ++i) m |= uint64_t{1} << at[i];
+        return m;
+    }
+    bool identity() const {
+        for (int i = 0; i < low; ++i)
+            if (at[i] != i) return false;
+        return true;
+    }
+    // Global bits whose qubit differs between this layout and `to`.
+    uint64_t changed(const Layout& to) const {
+        uint64_t m = 0;
+        for (int i = 0; i < low; ++i) {
+            if (at[i] != to.at[i]) m |= uint64_t{1} << i;
+            for (int x : {at[i], to.at[i]})
+                if (at[x] != to.at[x]) m |= uint64_t{1} << x;
+        }
+        return m;
+    }
+};
+
+// The low-slot assignment that holds as many of `want` (qubits in the order
+// the next pass uses them, first m of them) as the slots take, keeping
+// qubits that already sit in a slot.  Returns false when it is not a valid
+// assignment.
+bool assign_low(const Layout& cur, const std::vector<int>& want, size_t m, int* a) {
+    const int low = cur.low;
+    std::vector<int> rest;
+    for (int i = 0; i < low; ++i) a[i] = -1;
+    for (size_t j = 0; j < m; ++j) {
+        const int x = want[j];
+        if (x < low) a[x] = x;
+        else rest.push_back(x);
+    }
+    for (int i = 0; i < low; ++i)
+        if (a[i] < 0) {
+            auto it = std::find(rest.begin(), rest.end(), cur.at[i]);
+            if (it != rest.end()) {
+                a[i] = *it;
+                rest.erase(it);
+            }
+        }
+    for (int i = 0; i < low; ++i)
+        if (a[i] < 0 && !rest.empty()) {
+            a[i] = rest.front();
+            rest.erase(rest.begin());
+        }
+    // Free slots keep a displaced occupant where they can (no move), else
+    // take their own qubit back.
+    for (int i = 0; i < low; ++i)
+        if (a[i] < 0) {
+            const int x = cur.at[i];
+            a[i] = (x >= low && std::find(a, a + low, x) == a + low) ? x : i;
+        }
+    for (int i = 0; i < low; ++i)
+        for (int j = 0; j < i; ++j)
+            if (a[i] == a[j]) return false;
+    return true;
+}
+}  // namespace
+
 static void schedule_segment(Plan& plan, const std::vector<HostOp>& ops, const std::vector<size_t>& seg, int n,
                              int T, int low, const ScheduleOptions& opt) {
-    const uint64_t low_mask = low >= 64 ? ~uint64_t{0} : ((uint64_t{1} << low) - 1);
     const uint64_t all_mask = n >= 64 ? ~uint64_t{0} : ((uint64_t{1} << n) - 1);
     SegmentDag dag(ops, seg);
     std::vector<int> remaining(seg.size());
     for (size_t i = 0; i < seg.size(); ++i) remaining[i] = static_cast<int>(i);
-    while (!remaining.empty()) {
-        PassPick best = pick_pass(dag, seg, remaining, low_mask, T, -1, opt);
+    Layout cur(low);
+    const bool relabel = opt.relabel && low > 0;
+
+    // One pass over tile bits `tile` (global bits) applying `chosen`, whose
+    // store leaves layout `next`.
+    auto emit_pass = [&](uint64_t tile, const std::vector<size_t>& chosen, const Layout& next) {
+        PassDesc P{};
+        P.n = n;
+        P.T = T;
+        P.tmask = tile;
+        int local_of[64];
+        for (int q = 0; q < 64; ++q) local_of[q] = -1;
+        int l = 0;
+        for (int b = 0; b < n; ++b)
+            if ((tile >> b) & 1) {
+                P.qpos[l] = b;
+                local_of[cur.at[b]] = l;
+                if (l >= 8) P.hmask |= uint64_t{1} << b;
+                ++l;
+            }
+        for (int i = 0; i < 16; ++i) P.sperm[i] = static_cast<int8_t>(i);
+        for (int i = 0; i < l; ++i) {
+            const int src = local_of[next.at[P.qpos[i]]];
+            if (src < 0) fail(QHIP_RUNTIME, "schedule: relabel moves a qubit from outside the tile");
+            P.sperm[i] = static_cast<int8_t>(src);
+        }
+        P.op_begin = static_cast<int32_t>(plan.ops.size());
+        P.kmax = 0;
+        uint64_t Q = 0;   // tile qubits
+        for (int b = 0; b < n; ++b)
+            if ((tile >> b) & 1) Q |= uint64_t{1} << cur.at[b];
+        const bool blocks = opt.register_blocks && T >= 3;
+        for (const Unit& u : form_units(ops, chosen, blocks, opt.two_states, Q)) {
+            if (u.ops.size() >= 2) {
+                emit_block(plan, ops, u, local_of, cur.pos, n, T);   // runs in every kernel variant
+            } else {
+                emit_op(plan, ops[u.ops[0]], local_of, cur.pos, n, T, opt);
+                // kmax sizes the VALU dense path; matrix-core ops run in every variant.
+                if (ops[u.ops[0]].needs_local() && plan.ops.back().kind != OP_MFMA)
+                    P.kmax = std::max(P.kmax, ops[u.ops[0]].k);
+            }
+        }
+        P.op_end = static_cast<int32_t>(plan.ops.size());
+        plan.passes.push_back(P);
+        cur = next;
+    };
+    // Tile bits of qubits Q plus `extra` bits, filled up to T with the
+    // lowest free bits; false when Q and extra need more than T.
+    auto tile_bits = [&](uint64_t Q, uint64_t extra, uint64_t& tile) {
+        tile = extra;
+        for (int q = 0; q < n; ++q)
+            if ((Q >> q) & 1) tile |= uint64_t{1} << cur.pos[q];
+        if (std::popcount(tile) > T) return false;
+        for (int b = 0; b < n && std::popcount(tile) < T; ++b) tile |= uint64_t{1} << b;
+        tile &= all_mask;
+        return true;
+    };
+
+    // The pass applying the most ops from tile qubits Q0 over the seeds.
+    auto best_pick = [&](uint64_t Q0) {
+        PassPick best = pick_pass(dag, seg, remaining, Q0, T, -1, opt);
         int seeds = 0;
         const size_t W = std::min(remaining.size(), kWindow);
         for (size_t r = 0; r < W && seeds < kSeeds; ++r) {
@@ -489,20 +643,24 @@ static void schedule_segment(Plan& plan, const std::vector<HostOp>& ops, const s
             for (int p : dag.preds[i]) ready = ready && dag.done[p];
             if (!ready) continue;
             ++seeds;
-            if (std::popcount(low_mask | op.qmask()) > T) continue;
-            PassPick pk = pick_pass(dag, seg, remaining, low_mask, T, static_cast<int>(r), opt);
+            if (std::popcount(Q0 | op.qmask()) > T) continue;
+            PassPick pk = pick_pass(dag, seg, remaining, Q0, T, static_cast<int>(r), opt);
             if (pk.chosen.size() > best.chosen.size()) best = std::move(pk);
         }
+        return best;
+    };
+
+    while (!remaining.empty()) {
+        PassPick best = best_pick(cur.low_qubits());
         if (best.chosen.empty()) {
             // The first remaining op (always ready) needs more than the free
             // tile qubits: it displaces forced low qubits (the tile is then
             // less coalesced, never wrong), so every pass makes progress.
             const uint64_t qm = ops[seg[remaining.front()]].qmask();
             uint64_t Q0 = qm;
-            for (int q = 0; q < n && std::popcount(Q0) < T && q < low; ++q) Q0 |= uint64_t{1} << q;
+            for (int i = 0; i < low && std::popcount(Q0) < T; ++i) Q0 |= uint64_t{1} << cur.at[i];
             best = pick_pass(dag, seg, remaining, Q0, T, 0, opt);
         }
-        uint64_t Q = best.Q;
         std::vector<size_t> chosen;
         chosen.reserve(best.chosen.size());
         std::vector<int> rest;
@@ -517,41 +675,54 @@ static void schedule_segment(Plan& plan, const std::vector<HostOp>& ops, const s
                 rest.push_back(remaining[r]);
             }
         }
-        // Fill the tile up to T qubits with the lowest unused ones.
-        for (int q = 0; q < n && std::popcount(Q) < T; ++q) Q |= uint64_t{1} << q;
-        Q &= all_mask;
+        if (chosen.empty()) fail(QHIP_RUNTIME, "schedule: no op could join a pass");
+        remaining.swap(rest);
 
-        PassDesc P{};
-        P.n = n;
-        P.T = T;
-        P.tmask = Q;
-        int local_of[64];
-        for (int q = 0; q < 64; ++q) local_of[q] = -1;
-        int l = 0;
-        for (int q = 0; q < n; ++q)
-            if ((Q >> q) & 1) {
-                P.qpos[l] = q;
-                local_of[q] = l;
-                if (l >= 8) P.hmask |= uint64_t{1} << q;
-                ++l;
+        uint64_t tile = 0;
+        if (!tile_bits(best.Q, 0, tile)) fail(QHIP_RUNTIME, "schedule: pass wider than the tile");
+        Layout next = cur;
+        if (relabel && !remaining.empty()) {
+            // Look ahead: the qubits of the pass that would follow, in the
+            // order its ops use them, go to the low slots as far as the
+            // current tile can carry the move.
+            const PassPick nx = best_pick(0);
+            std::vector<int> want;
+            for (size_t r : nx.chosen) {
+                uint64_t qm = ops[seg[remaining[r]]].qmask();
+                while (qm) {
+                    const int q = std::countr_zero(qm);
+                    qm &= qm - 1;
+                    if (std::find(want.begin(), want.end(), q) == want.end()) want.push_back(q);
+                }
             }
-        P.op_begin = static_cast<int32_t>(plan.ops.size());
-        P.kmax = 0;
-        const bool blocks = opt.register_blocks && T >= 3;
-        for (const Unit& u : form_units(ops, chosen, blocks, opt.two_states, Q)) {
-            if (u.ops.size() >= 2) {
-                emit_block(plan, ops, u, local_of, n, T);   // runs in every kernel variant
-            } else {
-                emit_op(plan, ops[u.ops[0]], local_of, n, T, opt);
-                // kmax sizes the VALU dense path; matrix-core ops run in every variant.
-                if (ops[u.ops[0]].needs_local() && plan.ops.back().kind != OP_MFMA)
-                    P.kmax = std::max(P.kmax, ops[u.ops[0]].k);
+            for (size_t m = std::min<size_t>(low, want.size()); m > 0; --m) {
+                int a[64];
+                if (!assign_low(cur, want, m, a)) continue;
+                const Layout cand = Layout::of(a, low);
+                uint64_t t2;
+                if (tile_bits(best.Q, cur.changed(cand), t2)) {
+                    tile = t2;
+                    next = cand;
+                    break;
+                }
+            }
+        } else if (relabel && !cur.identity()) {
+            // Last pass of the segment: restore the identity if it fits.
+            const Layout id(low);
+            uint64_t t2;
+            if (tile_bits(best.Q, cur.changed(id), t2)) {
+                tile = t2;
+                next = id;
             }
         }
-        P.op_end = static_cast<int32_t>(plan.ops.size());
-        if (chosen.empty()) fail(QHIP_RUNTIME, "schedule: no op could join a pass");
-        plan.passes.push_back(P);
-        remaining.swap(rest);
+        emit_pass(tile, chosen, next);
+    }
+    if (!cur.identity()) {
+        // A pass of its own that only moves the qubits home.
+        const Layout id(low);
+        uint64_t tile;
+        if (!tile_bits(0, cur.changed(id), tile)) fail(QHIP_RUNTIME, "schedule: relabel restore wider than the tile");
+        emit_pass(tile, {}, id);
     }
 }
 
@@ -581,10 +752,11 @@ Plan schedule(const std::vector<HostOp>& ops, int n, int precision, const Schedu
             P.n = n;
             P.T = 0;   // wide pass
             P.kmax = op.k;
+            for (int i = 0; i < 16; ++i) P.sperm[i] = static_cast<int8_t>(i);
             int local_of[64];
             for (int q = 0; q < 64; ++q) local_of[q] = q;   // positions are global qubits
             P.op_begin = static_cast<int32_t>(plan.ops.size());
-            emit_op(plan, op, local_of, n, 0, opt);   // no tile: the wide kernel reads spos
+            emit_op(plan, op, local_of, local_of, n, 0, opt);   // no tile: the wide kernel reads spos
             P.op_end = static_cast<int32_t>(plan.ops.size());
             plan.passes.push_back(P);
             continue;

@@ -1014,13 +1014,33 @@ __device__ __forceinline__ void tile_body(A* s, A* __restrict__ psi, A* __restri
         ip_out[static_cast<int64_t>(k) * n_tiles + static_cast<int64_t>(t)] = v;
     }
 
+    // Store through the pass's permutation of tile positions (the identity
+    // unless the pass relabels qubits): output local index m reads input
+    // index perm(m) = sum_i bit_i(m) << sperm[i].  perm and the swizzle are
+    // linear over XOR, so the LDS offset is the thread's part XOR one
+    // uniform offset per set row bit.
+    static_assert(TB <= 13, "tile rows carry at most five permuted bits");
+    uint32_t tperm = 0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+        if (b < P.T) tperm |= ((threadIdx.x >> b) & 1u) << P.sperm[b];
+    const uint32_t to = swz(tperm) << sh;
+    uint32_t rbit[5];
+#pragma unroll
+    for (int b = 0; b < 5; ++b) rbit[b] = 8 + b < P.T ? swz(1u << P.sperm[8 + b]) << sh : 0u;
+    auto row_perm = [&](uint32_t r) {
+        uint32_t p = 0;
+#pragma unroll
+        for (int b = 0; b < 5; ++b) p ^= ((r >> b) & 1u) ? rbit[b] : 0u;
+        return p;
+    };
     if (R >= kBatch) {
         uint64_t dr = 0;
         for (uint32_t r0 = 0; r0 < R; r0 += kBatch) {
             A v0[kBatch], v1[kBatch];
 #pragma unroll
             for (int i = 0; i < kBatch; ++i) {
-                const uint32_t a = te ^ (swz((r0 + i) << 8) << sh);
+                const uint32_t a = to ^ row_perm(r0 + i);
                 v0[i] = at(s, a);
                 if (TWO) v1[i] = at(s, a | kPhi);
             }
@@ -1036,7 +1056,7 @@ __device__ __forceinline__ void tile_body(A* s, A* __restrict__ psi, A* __restri
         uint64_t dr = 0;
         for (uint32_t r = 0; r < R; ++r) {
             const uint64_t g = gbase | dj | dr;
-            const uint32_t a = te ^ (swz(r << 8) << sh);
+            const uint32_t a = to ^ row_perm(r);
             psi[g] = at(s, a);
             if (TWO) phi[g] = at(s, a | kPhi);
             dr = (dr - P.hmask) & P.hmask;

@@ -130,7 +130,7 @@ class QarpSimulator:
     DEFAULT_FUSION_MIN_QUBITS = 12
 
     def __init__(self, *, precision: str = "complex128", tile_qubits: int = 0, low_qubits: int = 0,
-                 max_ops_per_pass: int = 0, use_mfma: int = -1):
+                 max_ops_per_pass: int = 0, use_mfma: int = -1, relabel_qubits: int = -1):
         if precision not in ("complex128", "complex64"):
             raise ValueError(f"precision must be complex128 or complex64, got {precision!r}")
         _lib.load()
@@ -141,6 +141,7 @@ class QarpSimulator:
         self.low_qubits = int(low_qubits)
         self.max_ops_per_pass = int(max_ops_per_pass)
         self.use_mfma = int(use_mfma)   # dense k = 3..5 on the matrix cores: -1 auto, 0 never
+        self.relabel_qubits = int(relabel_qubits)   # passes move qubits into the low tile positions: -1 auto, 0 never
         self.last_stats: Optional[dict] = None
         self._pool: dict = {}
 
@@ -167,7 +168,7 @@ class QarpSimulator:
 
     def _config(self):
         return config_struct(self._fusion_max, self._fusion_min, self.tile_qubits, self.low_qubits,
-                             self.max_ops_per_pass, self.use_mfma)
+                             self.max_ops_per_pass, self.use_mfma, self.relabel_qubits)
 
     # ── device-resident primitives ──
     def new_state(self, n_qubits: int, initial_state=None, where: str = "statevector") -> DeviceState:
@@ -466,16 +467,18 @@ class CompiledCircuit:
     def __init__(self, sim: "QarpSimulator", commands: Sequence[Command], n_qubits: int, *,
                  fusion_max_qubits: Optional[int] = None, tile_qubits: Optional[int] = None,
                  low_qubits: Optional[int] = None, use_mfma: Optional[int] = None,
-                 values: Optional[Mapping[str, float]] = None):
+                 relabel_qubits: Optional[int] = None, values: Optional[Mapping[str, float]] = None):
         self.n_qubits = int(n_qubits)
         self.precision = sim.precision
         cfg = config_struct(sim.fusion_max_qubits if fusion_max_qubits is None else int(fusion_max_qubits),
                             sim.fusion_min_qubits,
                             sim.tile_qubits if tile_qubits is None else int(tile_qubits),
                             sim.low_qubits if low_qubits is None else int(low_qubits),
-                            sim.max_ops_per_pass, sim.use_mfma if use_mfma is None else int(use_mfma))
+                            sim.max_ops_per_pass, sim.use_mfma if use_mfma is None else int(use_mfma),
+                            sim.relabel_qubits if relabel_qubits is None else int(relabel_qubits))
         self.config = {"fusion_max_qubits": cfg.fusion_max_qubits, "tile_qubits": cfg.tile_qubits,
-                       "low_qubits": cfg.low_qubits, "use_mfma": cfg.use_mfma}
+                       "low_qubits": cfg.low_qubits, "use_mfma": cfg.use_mfma,
+                       "relabel_qubits": cfg.relabel_qubits}
         self._cfg = cfg
         self._commands = list(commands)
         ps = pack_stream(self._commands, values=values)
@@ -529,12 +532,15 @@ class CompiledCircuit:
 
 def inspect_schedule(commands: Sequence[Command], n_qubits: int, *, fusion_max_qubits: int = 3,
                      fusion_min_qubits: int = 12, tile_qubits: int = 0, low_qubits: int = 0,
-                     precision: str = "complex128") -> list:
+                     precision: str = "complex128", relabel_qubits: int = -1) -> list:
     """The device schedule of ``commands`` as [(pass, Command)] in execution
     order, computed on the host only (qhip_schedule_inspect): dense and
-    diagonal blocks as Custom, mask phases as MCZ, scalar phases as GPhase."""
+    diagonal blocks as Custom, mask phases as MCZ, scalar phases as GPhase,
+    on the qubit positions the kernels address.  A pass that relabels qubits
+    ends with the SWAP gates that realise its store permutation, so applying
+    the list in order reproduces the circuit's state."""
     lib = _lib.load()
-    cfg = config_struct(fusion_max_qubits, fusion_min_qubits, tile_qubits, low_qubits, 0, -1)
+    cfg = config_struct(fusion_max_qubits, fusion_min_qubits, tile_qubits, low_qubits, 0, -1, relabel_qubits)
     ps = pack_stream(commands)
     h = C.c_void_p()
     _lib.check(lib.qhip_schedule_inspect(C.byref(cfg), ps.ref, int(n_qubits),

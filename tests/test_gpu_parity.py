@@ -214,6 +214,47 @@ def test_largest_tiles(precision, tile):
         assert np.abs(gr[i] - O.run_gradient(ps, m, po, len(s))).max() < tg
 
 
+@pytest.mark.parametrize("precision", ["complex128", "complex64"])
+def test_relabeling_store_permutations(precision):
+    """Passes whose stores permute tile positions (moving the next pass's
+    qubits into the contiguous low positions) against the same circuits
+    without relabeling and the oracle: one-state passes at fusion widths
+    2..5, the adjoint's two-state passes and a batched step."""
+    from openqarp_amd.simulator import inspect_schedule
+    n, tile, low = 14, 9, 3
+    tol = ATOL if precision == "complex128" else 2e-5
+    cmds = W.brickwork_haar(n, 8, 3) + random_circuit(n, 40, 9)
+    for k in (2, 3, 4, 5):
+        sched = inspect_schedule(cmds, n, fusion_max_qubits=k, fusion_min_qubits=0, tile_qubits=tile,
+                                 low_qubits=low, precision=precision)
+        swap = np.eye(4)[[0, 2, 1, 3]]
+        assert any(c.unitary is not None and c.unitary.shape == (4, 4) and np.array_equal(c.unitary, swap)
+                   for _, c in sched), k
+        ref = O.statevector(pack_stream(cmds), n, k, 0)
+        for relabel in (-1, 0):
+            sim = gpu_sim(tile_qubits=tile, low_qubits=low, precision=precision, relabel_qubits=relabel)
+            sim.fusion_max_qubits, sim.fusion_min_qubits = k, 0
+            assert np.abs(sim.statevector(cmds, n) - ref).max() < tol, (k, relabel)
+    w = W.c3_gradient(14, 3, 6, 40)
+    idx = {s: i for i, s in enumerate(w.symbols)}
+    ref = O.run_gradient(pack_stream(w.commands, w.values, idx), 14, pack_observable(w.observable), len(idx))
+    sim = gpu_sim(tile_qubits=8, low_qubits=low, precision=precision)
+    g = np.array(sim.run_gradient(w.commands, 14, w.observable, w.values, w.symbols))
+    assert np.abs(g - ref).max() < (1e-10 if precision == "complex128" else 1e-4)
+    inst = [(b.commands, b.n_qubits, b.observable, b.values, b.symbols) for b in W.batched_vqe(12, 2, 5, 14, 1, 3)]
+    sim = gpu_sim(tile_qubits=8, low_qubits=low, precision=precision)
+    e, gr = sim.batch_energy_grad(inst)
+    for i, (c, m, o, v, s) in enumerate(inst):
+        ix = {q: j for j, q in enumerate(s)}
+        ps = pack_stream(c, values=v, param_index=ix)
+        sv = O.statevector(ps, m, 3, 12)
+        po = pack_observable(o)
+        scale = sum(abs(complex(x)) for _, x in o)
+        te, tg = (1e-12, 1e-10) if precision == "complex128" else (2e-5 * scale, 2e-5 * scale)
+        assert abs(e[i] - O.transition(sv, sv, m, po).real) < te
+        assert np.abs(gr[i] - O.run_gradient(ps, m, po, len(s))).max() < tg
+
+
 def test_wide_fusion_blocks_6_to_8():
     n = 11
     cmds = W.brickwork_haar(n, 8, 4)

@@ -93,3 +93,39 @@ def test_frontier_scheduling_pass_counts():
     cmds = W.brickwork_haar(30, 30, 2)
     sched = inspect_schedule(cmds, 30, fusion_max_qubits=2, fusion_min_qubits=0, tile_qubits=12, low_qubits=3)
     assert max(p for p, _ in sched) + 1 <= 45
+
+
+def _swaps(sched):
+    swap = np.eye(4)[[0, 2, 1, 3]]
+    return sum(1 for _, c in sched if c.unitary is not None and c.unitary.shape == (4, 4)
+               and np.array_equal(c.unitary, swap))
+
+
+@pytest.mark.parametrize("n,tile,low,fusion", [(12, 7, 3, 3), (12, 8, 3, 2), (14, 9, 3, 4), (10, 6, 2, 2),
+                                               (12, 6, 4, 2)])
+def test_relabeling_passes_reproduce_the_circuit(n, tile, low, fusion):
+    """Passes whose stores move the next pass's qubits into the contiguous
+    low tile positions (reported as SWAPs of the pass's global bits after its
+    ops): replayed in order they give the circuit's state, so the layout is
+    back to the identity when the schedule ends.  Without relabeling no pass
+    permutes."""
+    cmds = W.brickwork_haar(n, 8, 3) + random_circuit(n, 40, 9)
+    ref = O.statevector(pack_stream(cmds), n, fusion, 0)
+    kw = dict(fusion_max_qubits=fusion, fusion_min_qubits=0, tile_qubits=tile, low_qubits=low)
+    got, sched = scheduled_state(cmds, n, **kw)
+    assert _swaps(sched) > 0
+    assert np.abs(got - ref).max() < 1e-12
+    got0, sched0 = scheduled_state(cmds, n, relabel_qubits=0, **kw)
+    assert np.abs(got0 - ref).max() < 1e-12
+
+
+def test_relabeling_saves_passes_on_the_brickwork():
+    """30-qubit brickwork, fusion width 4, 3 contiguous low qubits: with
+    relabeling 2^11-amplitude tiles need 56 passes instead of 62."""
+    cmds = W.brickwork_haar(30, 30, 2)
+    kw = dict(fusion_max_qubits=4, fusion_min_qubits=0, tile_qubits=11, low_qubits=3)
+    on = inspect_schedule(cmds, 30, **kw)
+    off = inspect_schedule(cmds, 30, relabel_qubits=0, **kw)
+    assert max(p for p, _ in on) + 1 <= 56
+    assert max(p for p, _ in off) + 1 == 62
+    assert _swaps(off) == 0
