@@ -26,6 +26,9 @@ COMMON = [
     "-O3", "-std=c++20", "-fPIC", f"--offload-arch={ARCH}",
     # No packed-FP32 instructions anywhere in device code.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # Matrix-core accumulators in ordinary VGPRs: the results feed VALU adds
+    # and LDS stores directly instead of through v_accvgpr_read copies.
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-command-line-argument",
 ]
 
