@@ -314,7 +314,8 @@ __device__ __forceinline__ void dense_mfma(A* s, uint32_t zoff, int G, const OpR
     using M = Mfma<A>;
     constexpr int sh = Amp<A>::kShift;
     constexpr int D = 1 << K, NT = D / 8, H = D / 4;   // M-tiles; input amplitudes (= fragment pairs) per lane
-    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6, j = lane & 15u, c = lane >> 4;
+    const uint32_t lane = threadIdx.x & 63u, j = lane & 15u, c = lane >> 4;
+    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform: chunk loop in SALU
     uint32_t lp = 0;
 #pragma unroll
     for (int b = 0; b < 4; ++b) lp |= ((j >> b) & 1u) << op.g(b);
@@ -364,7 +365,8 @@ __device__ __forceinline__ void dense_mfma3(A* s, uint32_t zoff, int G, const Op
     using M = Mfma<A>;
     constexpr int sh = Amp<A>::kShift;
     constexpr int D = 1 << K, NT = D / 16, H = D / 4, HP = H / 2;
-    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6, j = lane & 15u, c = lane >> 4;
+    const uint32_t lane = threadIdx.x & 63u, j = lane & 15u, c = lane >> 4;
+    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform: chunk loop in SALU
     uint32_t lp = 0;
 #pragma unroll
     for (int b = 0; b < 4; ++b) lp |= ((j >> b) & 1u) << op.g(b);
