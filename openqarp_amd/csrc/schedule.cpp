@@ -480,8 +480,10 @@ PassPick pick_pass(const SegmentDag& dag, const std::vector<size_t>& seg, const 
 // positions (PassDesc::sperm) that moves the qubits the next pass wants to
 // the low bits, so that pass need not spend tile positions on the forced
 // ones.  Layouts are transpositions of low slot i with the home bit of the
-// qubit that occupies it; a relabel is taken when every bit it changes is a
-// bit of the current tile.  The identity layout is restored by the last pass
+// qubit that occupies it; a relabel is possible when every bit it changes is
+// a bit of the current tile, and among the possible ones (moving the first
+// m qubits the next pass uses, for each m) the one whose low qubits let the
+// following pass apply the most ops is taken.  The identity layout is restored by the last pass
 // of the segment (or by one pass of its own), so a plan starts and ends on
 // the caller's qubit order.
 namespace {
@@ -695,15 +697,22 @@ static void schedule_segment(Plan& plan, const std::vector<HostOp>& ops, const s
                     if (std::find(want.begin(), want.end(), q) == want.end()) want.push_back(q);
                 }
             }
+            // Each feasible assignment is scored by the ops the following
+            // pass would apply from it; ties keep the larger move.
+            size_t best_score = best_pick(cur.low_qubits()).chosen.size();
+            bool moved = false;
             for (size_t m = std::min<size_t>(low, want.size()); m > 0; --m) {
                 int a[64];
                 if (!assign_low(cur, want, m, a)) continue;
                 const Layout cand = Layout::of(a, low);
                 uint64_t t2;
-                if (tile_bits(best.Q, cur.changed(cand), t2)) {
+                if (!tile_bits(best.Q, cur.changed(cand), t2)) continue;
+                const size_t score = best_pick(cand.low_qubits()).chosen.size();
+                if (score > best_score || (score == best_score && !moved)) {
+                    best_score = score;
                     tile = t2;
                     next = cand;
-                    break;
+                    moved = true;
                 }
             }
         } else if (relabel && !cur.identity()) {
