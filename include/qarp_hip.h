@@ -211,6 +211,27 @@ int qhip_fused_pass(const qhip_fused* f, int64_t i, int32_t* pass);
 int qhip_apply(const qhip_config* cfg, const qhip_stream* stream, qhip_state* s,
                qhip_apply_stats* stats);
 
+/* ── structured-program kernels ──────────────────────────────────────────────
+ * The non-gate kernels of qx.Program run by QarpSimulator::execute_program
+ * (src/qarpx/src/simulator/qarp_simulator.cpp:789-808); gates and dense
+ * kernels go through qhip_apply.
+ *
+ * Stands in for apply_permutation (src/qarpx/src/simulator/program.cpp:610-646):
+ * |x> -> |table[x]> on `qubits` (local bit b <-> qubits[b]), k = 1..26 distinct
+ * qubits, table a bijection of [0, 2^k).  Out of place through a library-owned
+ * scratch state, then copied back. */
+int qhip_state_permute(qhip_state* s, const uint32_t* qubits, int32_t k, const uint64_t* table);
+
+/* Stands in for apply_controlled_powers (program.cpp:648-712): matrix^exponents[j]
+ * on `targets` wherever control j is |1> (the powers of one matrix commute).
+ * matrix: 2^m x 2^m, row-major, complex128 (re, im) interleaved; the powers
+ * are formed on the host along one squaring chain as the reference does, and
+ * each controlled power runs as a Custom op on (targets..., control) through
+ * the fused apply, so m <= 7 (QHIP_CAPABILITY above). */
+int qhip_state_controlled_powers(const qhip_config* cfg, qhip_state* s, const uint32_t* controls,
+                                 const uint64_t* exponents, int32_t n_controls, const uint32_t* targets,
+                                 int32_t m, const double* matrix);
+
 /* Compiled stream: the fusion pass and the device schedule of qhip_apply done
  * once, op tables resident on the device, applied to any state of the given
  * width and precision.  The counterpart of compiling a circuit once and

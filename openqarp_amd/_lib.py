@@ -24,7 +24,8 @@ EXPORTS = [
     "qhip_schedule_inspect", "qhip_fused_pass",
     "qhip_plan_create", "qhip_plan_apply", "qhip_plan_apply_timed", "qhip_plan_apply_graph",
     "qhip_plan_update", "qhip_plan_destroy",
-    "qhip_apply", "qhip_apply_terminal", "qhip_transition", "qhip_apply_observable",
+    "qhip_apply", "qhip_apply_terminal", "qhip_state_permute", "qhip_state_controlled_powers",
+    "qhip_transition", "qhip_apply_observable",
     "qhip_sample", "qhip_sample_uniforms", "qhip_probabilities", "qhip_exact",
     "qhip_run_gradient", "qhip_run_gradient_phi", "qhip_batch_energy_grad",
     "qhip_batch_create", "qhip_batch_run", "qhip_batch_update", "qhip_batch_destroy",
@@ -62,6 +63,8 @@ def _declare(lib):
         "qhip_plan_destroy": [vp],
         "qhip_apply": [vp, vp, vp, vp],
         "qhip_apply_terminal": [vp, vp, vp, vp],
+        "qhip_state_permute": [vp, vp, i32, vp],
+        "qhip_state_controlled_powers": [vp, vp, vp, vp, i32, vp, i32, vp],
         "qhip_transition": [vp, vp, vp, vp],
         "qhip_apply_observable": [vp, vp, vp],
         "qhip_sample": [vp, i64, u32, vp],

@@ -50,6 +50,7 @@ struct Context {
     DeviceBuffer red;       // reduction partials
     DeviceBuffer misc;      // sampling / adjoint scratch
     DeviceBuffer misc2;
+    DeviceBuffer perm;      // out-of-place permutation target (one state)
     DeviceBuffer pinned_dummy;
 };
 Context& ctx();   // initialises on first use (throws QHIP_DEVICE without a GPU)
@@ -320,6 +321,10 @@ void reduce_ip(const void* d_partials, int64_t n_tiles, int n_slots, cplx* host_
 
 void apply_stream(const qhip_config* cfg, const qhip_stream* stream, qhip_state* s,
                   qhip_apply_stats* stats, bool terminal);
+// Structured-program kernels (kernels_program.hip).
+void permute_state(qhip_state* s, const uint32_t* qubits, int k, const uint64_t* table);
+void controlled_powers(const qhip_config* cfg, qhip_state* s, const uint32_t* controls, const uint64_t* exponents,
+                       int n_controls, const uint32_t* targets, int m, const double* matrix);
 Plan plan_stream(const qhip_config* cfg, const qhip_stream* stream, int n, int precision,
                  bool terminal, int64_t* fused_ops);
 // Adjoint pieces shared by run_gradient and the batched driver: the reverse

@@ -644,6 +644,23 @@ int qhip_apply(const qhip_config* cfg, const qhip_stream* stream, qhip_state* s,
     });
 }
 
+int qhip_state_permute(qhip_state* s, const uint32_t* qubits, int32_t k, const uint64_t* table) {
+    return guarded([&] {
+        need(s && (k <= 0 || (qubits && table)), "qhip_state_permute: NULL argument");
+        qhip::permute_state(s, qubits, k, table);
+    });
+}
+
+int qhip_state_controlled_powers(const qhip_config* cfg, qhip_state* s, const uint32_t* controls,
+                                 const uint64_t* exponents, int32_t n_controls, const uint32_t* targets,
+                                 int32_t m, const double* matrix) {
+    return guarded([&] {
+        need(s && (n_controls <= 0 || (controls && exponents)) && (m <= 0 || (targets && matrix)),
+             "qhip_state_controlled_powers: NULL argument");
+        qhip::controlled_powers(cfg, s, controls, exponents, n_controls, targets, m, matrix);
+    });
+}
+
 int qhip_plan_create(const qhip_config* cfg, const qhip_stream* stream, int32_t n_qubits, int32_t precision,
                      qhip_plan** out) {
     return guarded([&] {

@@ -22,6 +22,7 @@ from typing import Mapping, Optional, Sequence
 import numpy as np
 
 from . import _lib
+from .program import Program, measurements_terminal, needs_trajectory
 from ._abi import (QHIP_C64, QHIP_C128, PackedStream, QhipApplyStats, config_struct,
                    pack_observable, pack_stream)
 from .ir import Command, GateType, SamplingResult, cbit_register_width
@@ -276,6 +277,10 @@ class QarpSimulator:
         (qarp_simulator.cpp:1017-1036) and sample_terminal (:751-783)."""
         st = self.new_state(n_qubits, initial_state, "run")
         self.apply(commands, st, terminal=True)
+        return self._terminal_result(st, commands, n_qubits, n_shots, seed)
+
+    def _terminal_result(self, st: DeviceState, commands: Sequence[Command], n_qubits: int, n_shots: int,
+                         seed: Optional[int]) -> SamplingResult:
         outcomes = self.sample_state(st, n_shots, seed)
         res = SamplingResult(n_qubits=int(n_qubits), n_shots=int(n_shots))
         vals, cnt = np.unique(outcomes, return_counts=True)
@@ -292,6 +297,72 @@ class QarpSimulator:
                 hist.append(reg)
             res.cbit_history = hist
         return res
+
+    # ── structured programs (QarpSimulator::execute_program / program_statevector /
+    #    program_run, qarp_simulator.cpp:789-895) ──
+    def _check_program_width(self, program: Program, n_qubits: int, where: str) -> None:
+        w = program.min_register_width
+        if int(n_qubits) < 0 or w > int(n_qubits):
+            raise ValueError(f"QarpSimulator::{where}: a kernel acts on qubit {w - 1} of a "
+                             f"{int(n_qubits)}-qubit register")
+
+    def execute_program(self, program: Program, state: DeviceState, terminal_last: bool = False) -> None:
+        """Run the program's kernels in order on a device state (the last gates
+        kernel on the terminal-measurement path when ``terminal_last``)."""
+        lib = _lib.load()
+        kernels = program.kernels()
+        for i, k in enumerate(kernels):
+            if k[0] == "gates":
+                self.apply(k[1], state, terminal=terminal_last and i + 1 == len(kernels))
+            elif k[0] == "permutation":
+                qs = np.ascontiguousarray(k[1], dtype=np.uint32)
+                table = np.ascontiguousarray(k[2], dtype=np.uint64)
+                _lib.check(lib.qhip_state_permute(state.handle, qs.ctypes.data, qs.size, table.ctypes.data))
+            elif k[0] == "dense":
+                self.apply([Command(GateType.Custom, qubits=k[1], unitary=k[2])], state)
+            else:
+                cs = np.ascontiguousarray(k[1], dtype=np.uint32)
+                es = np.ascontiguousarray(k[2], dtype=np.uint64)
+                ts = np.ascontiguousarray(k[3], dtype=np.uint32)
+                m = np.ascontiguousarray(k[4], dtype=np.complex128)
+                cfg = self._config()
+                _lib.check(lib.qhip_state_controlled_powers(C.byref(cfg), state.handle, cs.ctypes.data,
+                                                            es.ctypes.data, cs.size, ts.ctypes.data, ts.size,
+                                                            m.ctypes.data))
+
+    def program_statevector(self, program: Program, n_qubits: int, *, initial_state=None) -> np.ndarray:
+        """statevector() for a structured program (qarp_simulator.cpp:822-849)."""
+        self._check_program_width(program, n_qubits, "program_statevector")
+        for k in program.kernels():
+            if k[0] == "gates" and any(needs_trajectory(c) for c in k[1]):
+                raise RuntimeError("QarpSimulator::program_statevector: program contains mid-circuit "
+                                   "measurement, Reset, or classical condition — statevector is not "
+                                   "deterministic.")
+        st = self.new_state(n_qubits, initial_state, "program_statevector")
+        self.execute_program(program, st)
+        return st.download()
+
+    def program_run(self, program: Program, n_qubits: int, n_shots: int, seed: Optional[int] = None, *,
+                    initial_state=None) -> SamplingResult:
+        """run()'s terminal sample-once path for a structured program
+        (qarp_simulator.cpp:851-888)."""
+        self._check_program_width(program, n_qubits, "program_run")
+        kernels = program.kernels()
+        last: Sequence[Command] = []
+        for i, k in enumerate(kernels):
+            if k[0] != "gates":
+                continue
+            if i + 1 == len(kernels):
+                last = k[1]
+            elif any(needs_trajectory(c) for c in k[1]):
+                raise ValueError("QarpSimulator::program_run: a measurement or classical condition "
+                                 "outside the program's last gates kernel")
+        if last and not measurements_terminal(last):
+            raise ValueError("QarpSimulator::program_run: the last gates kernel's measurements are "
+                             "not terminal")
+        st = self.new_state(n_qubits, initial_state, "program_run")
+        self.execute_program(program, st, terminal_last=True)
+        return self._terminal_result(st, last, n_qubits, n_shots, seed)
 
     def batch_run(self, commands: Sequence[Command], n_qubits: int, n_shots: int,
                   param_sets: Sequence[Mapping[str, float]], seed: Optional[int] = None) -> list:

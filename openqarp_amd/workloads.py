@@ -212,3 +212,27 @@ def batched_vqe(n_instances: int = 512, seed: int = 4, n_min: int = 12, n_max: i
         w.observable = tfim_chain(n, h, 1.0)
         out.append(w)
     return out
+
+
+def structured_program(n: int = 12, seed: int = 8):
+    """A seeded structured program exercising every kernel kind: brickwork
+    gates, two basis permutations (the second folds into the first), a dense
+    3-qubit block, QPE-style controlled powers of a 2-qubit Haar unitary
+    (exponents 2^j on the top qubits, plus a zero and a multi-bit exponent),
+    then more gates."""
+    from .program import Program
+    rng = np.random.default_rng(seed)
+    p = Program()
+    p.add_gates(brickwork_haar(n, 3, seed))
+    q1 = [int(q) for q in rng.choice(n, size=min(n, 5), replace=False)]
+    p.add_permutation(q1, rng.permutation(1 << len(q1)))
+    q2 = [int(q) for q in rng.choice(n, size=min(n, 3), replace=False)]
+    p.add_permutation(q2, rng.permutation(1 << len(q2)))
+    p.add_dense([int(q) for q in rng.choice(n, size=3, replace=False)], haar_unitary(8, rng))
+    controls = list(range(n - 1, max(n - 6, 2) - 1, -1))
+    exps = [1 << j for j in range(len(controls))]
+    exps[1] = 0
+    exps[-1] = 11
+    p.add_controlled_powers(controls, exps, [0, 1], haar_unitary(4, rng))
+    p.add_gates(brickwork_haar(n, 2, seed + 1))
+    return p

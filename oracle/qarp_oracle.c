@@ -1135,3 +1135,104 @@ int orc_run_gradient_phi(const qhip_stream* s, int n, const double* phi, int n_p
                          const double* init, double* grad) {
     return gradient_core(s, n, n_params, init, NULL, phi, grad);
 }
+
+/* ── structured-program kernels (program.cpp:610-712) ──────────────────── */
+
+/* apply_permutation (program.cpp:610-646), scatter form: out[(i & keep) |
+ * deposit(table[extract(i)])] = in[i].  The caller validated the table. */
+int orc_permute(const double* in, int n, const uint32_t* qubits, int k, const uint64_t* table,
+                double* out) {
+    if (k < 1 || k > 26) return fail(QHIP_INVALID, "orc_permute: 1..26 qubits required");
+    const uint64_t dim = (uint64_t)1 << n;
+    uint64_t keep = ~(uint64_t)0;
+    for (int b = 0; b < k; ++b) keep &= ~((uint64_t)1 << qubits[b]);
+    for (uint64_t i = 0; i < dim; ++i) {
+        uint64_t x = 0;
+        for (int b = 0; b < k; ++b) x |= ((i >> qubits[b]) & 1u) << b;
+        const uint64_t y = table[x];
+        uint64_t j = i & keep;
+        for (int b = 0; b < k; ++b) j |= ((y >> b) & 1u) << qubits[b];
+        out[2 * j] = in[2 * i];
+        out[2 * j + 1] = in[2 * i + 1];
+    }
+    return 0;
+}
+
+static void mat_mul(const cplx* a, const cplx* b, cplx* c, uint64_t d) {
+    for (uint64_t y = 0; y < d; ++y)
+        for (uint64_t x = 0; x < d; ++x) {
+            cplx s = {0, 0};
+            for (uint64_t t = 0; t < d; ++t) {
+                const cplx p = cmul(a[y * d + t], b[t * d + x]);
+                s.re += p.re;
+                s.im += p.im;
+            }
+            c[y * d + x] = s;
+        }
+}
+
+/* matrix^e by the squaring chain of apply_controlled_powers: squares of the
+ * matrix, multiplied from the lowest set bit of e up. */
+static void mat_power(const cplx* m, uint64_t e, cplx* out, uint64_t d) {
+    cplx* sq = malloc(sizeof(cplx) * d * d);
+    cplx* tmp = malloc(sizeof(cplx) * d * d);
+    memcpy(sq, m, sizeof(cplx) * d * d);
+    int first = 1;
+    for (int b = 0; b < 64 && (e >> b); ++b) {
+        if (b > 0) {
+            mat_mul(sq, sq, tmp, d);
+            memcpy(sq, tmp, sizeof(cplx) * d * d);
+        }
+        if (!((e >> b) & 1)) continue;
+        if (first) {
+            memcpy(out, sq, sizeof(cplx) * d * d);
+            first = 0;
+        } else {
+            mat_mul(out, sq, tmp, d);
+            memcpy(out, tmp, sizeof(cplx) * d * d);
+        }
+    }
+    free(sq);
+    free(tmp);
+}
+
+/* apply_controlled_powers (program.cpp:648-712): for each control j in order
+ * with e_j != 0, P = matrix^e_j applied to the target sub-vector of every
+ * basis state whose control bit is set. */
+int orc_controlled_powers(double* state, int n, const uint32_t* controls, const uint64_t* exponents,
+                          int n_controls, const uint32_t* targets, int m, const double* matrix) {
+    if (m < 1 || m > 12) return fail(QHIP_INVALID, "orc_controlled_powers: 1..12 targets");
+    const uint64_t dim = (uint64_t)1 << n, d = (uint64_t)1 << m;
+    uint64_t tmask = 0;
+    for (int b = 0; b < m; ++b) tmask |= (uint64_t)1 << targets[b];
+    cplx* M = malloc(sizeof(cplx) * d * d);
+    cplx* P = malloc(sizeof(cplx) * d * d);
+    cplx* col = malloc(sizeof(cplx) * d);
+    uint64_t* off = malloc(sizeof(uint64_t) * d);
+    for (uint64_t i = 0; i < d * d; ++i) M[i] = (cplx){matrix[2 * i], matrix[2 * i + 1]};
+    for (uint64_t l = 0; l < d; ++l) {
+        off[l] = 0;
+        for (int b = 0; b < m; ++b) off[l] |= ((l >> b) & 1u) << targets[b];
+    }
+    for (int j = 0; j < n_controls; ++j) {
+        if (exponents[j] == 0) continue;
+        mat_power(M, exponents[j], P, d);
+        const uint64_t cbit = (uint64_t)1 << controls[j];
+        for (uint64_t base = 0; base < dim; ++base) {
+            if ((base & tmask) || !(base & cbit)) continue;
+            for (uint64_t l = 0; l < d; ++l) col[l] = (cplx){state[2 * (base | off[l])], state[2 * (base | off[l]) + 1]};
+            for (uint64_t y = 0; y < d; ++y) {
+                cplx s = {0, 0};
+                for (uint64_t x = 0; x < d; ++x) {
+                    const cplx p = cmul(P[y * d + x], col[x]);
+                    s.re += p.re;
+                    s.im += p.im;
+                }
+                state[2 * (base | off[y])] = s.re;
+                state[2 * (base | off[y]) + 1] = s.im;
+            }
+        }
+    }
+    free(M); free(P); free(col); free(off);
+    return 0;
+}

@@ -64,6 +64,14 @@ int orc_run_gradient(const qhip_stream* s, int n, const qhip_observable* obs, in
 int orc_run_gradient_phi(const qhip_stream* s, int n, const double* phi, int n_params,
                          const double* init, double* grad);
 
+/* apply_permutation (program.cpp:610-646): |x> -> |table[x]> on `qubits`
+ * (local bit b <-> qubits[b]), out of place. */
+int orc_permute(const double* in, int n, const uint32_t* qubits, int k, const uint64_t* table,
+                double* out);
+/* apply_controlled_powers (program.cpp:648-712), in place. */
+int orc_controlled_powers(double* state, int n, const uint32_t* controls, const uint64_t* exponents,
+                          int n_controls, const uint32_t* targets, int m, const double* matrix);
+
 #ifdef __cplusplus
 }
 #endif

@@ -15,6 +15,7 @@
 #include "qarpx/core/errors.h"
 #include "qarpx/simulator/fusion.h"
 #include "qarpx/simulator/pauli_expectation.h"
+#include "qarpx/simulator/program.h"
 #include "qarpx/simulator/qarp_simulator.h"
 #include "qarpx/transpiler/fusion.h"
 
@@ -282,6 +283,55 @@ int ref_run_gradient_phi(const qhip_stream* s, int n, const double* phi, int n_p
         return 0;
     } catch (const std::invalid_argument& e) { return catch_all(e, QHIP_INVALID);
     } catch (const capability_error& e) { return catch_all(e, QHIP_CAPABILITY);
+    } catch (const std::exception& e) { return catch_all(e, QHIP_RUNTIME); }
+}
+
+// One kernel of a structured program, as the tests describe it.
+struct ref_kernel {
+    int32_t            kind;        // 0 gates, 1 permutation, 2 dense, 3 controlled powers
+    int32_t            k;           // permutation / dense qubits, or targets
+    int32_t            n_controls;
+    int32_t            unused;
+    const uint32_t*    qubits;      // permutation / dense qubits, or targets
+    const uint64_t*    table;       // permutation
+    const double*      matrix;      // dense / controlled powers, row-major (re, im)
+    const uint32_t*    controls;
+    const uint64_t*    exponents;
+    const qhip_stream* gates;
+};
+
+// QarpSimulator::program_statevector over a Program built with the
+// reference's own add_* methods (permutations fold as they do there).
+int ref_program_statevector(const ref_kernel* ks, int nk, int n, int fusion_max, int fusion_min,
+                            const double* init, double* out) {
+    try {
+        auto sim = make_sim(fusion_max, fusion_min);
+        Program prog;
+        auto matrix = [](const double* m, int k) {
+            const int D = 1 << k;
+            Eigen::MatrixXcd M(D, D);
+            for (int y = 0; y < D; ++y)
+                for (int x = 0; x < D; ++x) M(y, x) = cd(m[2 * (y * D + x)], m[2 * (y * D + x) + 1]);
+            return M;
+        };
+        for (int i = 0; i < nk; ++i) {
+            const ref_kernel& K = ks[i];
+            std::vector<uint32_t> qs(K.qubits, K.qubits + (K.kind == 0 ? 0 : K.k));
+            switch (K.kind) {
+                case 0: prog.add_gates(to_commands(K.gates)); break;
+                case 1: prog.add_permutation(qs, std::vector<uint64_t>(K.table, K.table + (uint64_t{1} << K.k)));
+                        break;
+                case 2: prog.add_dense(qs, matrix(K.matrix, K.k)); break;
+                default:
+                    prog.add_controlled_powers(std::vector<uint32_t>(K.controls, K.controls + K.n_controls),
+                                               std::vector<uint64_t>(K.exponents, K.exponents + K.n_controls),
+                                               qs, matrix(K.matrix, K.k));
+            }
+        }
+        auto sv = sim.program_statevector(prog, n, to_state(init, n));
+        std::memcpy(out, sv.data(), sizeof(cd) * sv.size());
+        return 0;
+    } catch (const std::invalid_argument& e) { return catch_all(e, QHIP_INVALID);
     } catch (const std::exception& e) { return catch_all(e, QHIP_RUNTIME); }
 }
 

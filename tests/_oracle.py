@@ -50,6 +50,8 @@ def oracle():
         lib.orc_fused_get.argtypes = [vp, i64, C.POINTER(i32), C.POINTER(i32), vp, vp, i64]
         lib.orc_fused_free.argtypes = [vp]
         lib.orc_gate_matrix.argtypes = [vp, i64, vp, i64]
+        lib.orc_permute.argtypes = [vp, i32, vp, i32, vp, vp]
+        lib.orc_controlled_powers.argtypes = [vp, i32, vp, vp, i32, vp, i32, vp]
         _orc = lib
     return _orc
 
@@ -73,6 +75,7 @@ def reference():
         lib.ref_discrete.argtypes = [vp, i64, C.c_uint32, i64, vp]
         lib.ref_run_gradient.argtypes = [vp, i32, vp, i32, vp, vp, vp]
         lib.ref_run_gradient_phi.argtypes = [vp, i32, vp, i32, vp, vp, vp]
+        lib.ref_program_statevector.argtypes = [vp, i32, i32, i32, i32, vp, vp]
         _ref = lib
     return _ref
 
@@ -235,4 +238,94 @@ def ref_fuse(packed, n, fusion_max, fusion_min, cap=100000):
             m = mats[i, : d * d].reshape(d, d).copy()
         out.append((int(gates[i]), tuple(int(q) for q in qubits[i, : min(k, 8)]),
                     tuple(params[i]), m))
+    return out
+
+
+# ── structured programs ────────────────────────────────────────────────────
+
+def permute(psi, n, qubits, table):
+    """apply_permutation restated (orc_permute)."""
+    inp = np.ascontiguousarray(psi, dtype=np.complex128)
+    out = np.zeros(1 << n, dtype=np.complex128)
+    qs = np.ascontiguousarray(qubits, dtype=np.uint32)
+    t = np.ascontiguousarray(table, dtype=np.uint64)
+    L = oracle()
+    _check(L, L.orc_permute(inp.ctypes.data, n, qs.ctypes.data, qs.size, t.ctypes.data, out.ctypes.data))
+    return out
+
+
+def controlled_powers(psi, n, controls, exponents, targets, matrix):
+    """apply_controlled_powers restated (orc_controlled_powers)."""
+    st = np.array(psi, dtype=np.complex128)
+    cs = np.ascontiguousarray(controls, dtype=np.uint32)
+    es = np.ascontiguousarray(exponents, dtype=np.uint64)
+    ts = np.ascontiguousarray(targets, dtype=np.uint32)
+    m = np.ascontiguousarray(matrix, dtype=np.complex128)
+    L = oracle()
+    _check(L, L.orc_controlled_powers(st.ctypes.data, n, cs.ctypes.data, es.ctypes.data, cs.size, ts.ctypes.data,
+                                      ts.size, m.ctypes.data))
+    return st
+
+
+def program_statevector(kernels, n, init=None, fusion_max=3, fusion_min=12, lib="orc"):
+    """A structured program's statevector: the oracle composes its kernels
+    in order (gates: orc_statevector from the current state; dense: the
+    matrix as one Custom op; permutation / controlled powers: the restated
+    kernels); lib="ref" runs QarpSimulator::program_statevector of the
+    reference build.  kernels: the tuples of openqarp_amd.program.Program."""
+    from openqarp_amd._abi import pack_stream
+    from openqarp_amd.ir import Command, GateType
+    psi = np.zeros(1 << n, dtype=np.complex128)
+    if init is None:
+        psi[0] = 1.0
+    else:
+        psi[:] = init
+    if lib == "orc":
+        for k in kernels:
+            if k[0] == "gates":
+                psi = statevector(pack_stream(k[1]), n, fusion_max, fusion_min, init=psi)
+            elif k[0] == "dense":
+                psi = statevector(pack_stream([Command(GateType.Custom, qubits=k[1], unitary=k[2])]), n,
+                                  fusion_max, fusion_min, init=psi)
+            elif k[0] == "permutation":
+                psi = permute(psi, n, k[1], k[2])
+            else:
+                psi = controlled_powers(psi, n, k[1], k[2], k[3], k[4])
+        return psi
+
+    class K(C.Structure):
+        _fields_ = [("kind", C.c_int32), ("k", C.c_int32), ("n_controls", C.c_int32), ("unused", C.c_int32),
+                    ("qubits", C.c_void_p), ("table", C.c_void_p), ("matrix", C.c_void_p),
+                    ("controls", C.c_void_p), ("exponents", C.c_void_p), ("gates", C.c_void_p)]
+    keep = []
+    arr = (K * max(len(kernels), 1))()
+
+    def hold(a):
+        keep.append(a)
+        return a.ctypes.data
+    for i, k in enumerate(kernels):
+        e = arr[i]
+        if k[0] == "gates":
+            ps = pack_stream(k[1])
+            keep.append(ps)
+            e.kind, e.gates = 0, C.addressof(ps.struct)
+        elif k[0] == "permutation":
+            e.kind, e.k = 1, len(k[1])
+            e.qubits = hold(np.ascontiguousarray(k[1], dtype=np.uint32))
+            e.table = hold(np.ascontiguousarray(k[2], dtype=np.uint64))
+        elif k[0] == "dense":
+            e.kind, e.k = 2, len(k[1])
+            e.qubits = hold(np.ascontiguousarray(k[1], dtype=np.uint32))
+            e.matrix = hold(np.ascontiguousarray(k[2], dtype=np.complex128))
+        else:
+            e.kind, e.k, e.n_controls = 3, len(k[3]), len(k[1])
+            e.controls = hold(np.ascontiguousarray(k[1], dtype=np.uint32))
+            e.exponents = hold(np.ascontiguousarray(k[2], dtype=np.uint64))
+            e.qubits = hold(np.ascontiguousarray(k[3], dtype=np.uint32))
+            e.matrix = hold(np.ascontiguousarray(k[4], dtype=np.complex128))
+    out = np.zeros(1 << n, dtype=np.complex128)
+    L = reference()
+    init_c = np.ascontiguousarray(psi)
+    _check(L, L.ref_program_statevector(C.cast(arr, C.c_void_p), len(kernels), n, fusion_max, fusion_min,
+                                        init_c.ctypes.data, out.ctypes.data), "ref")
     return out

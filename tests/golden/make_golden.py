@@ -122,6 +122,15 @@ def sample_fixture(name, n, seed, shots, shot_seed):
     np.savez_compressed(HERE / name, n=n, state=amp, shots=shots, shot_seed=shot_seed, samples=out)
 
 
+def program_fixture(name, n, seed):
+    """A structured program (openqarp_amd.workloads.structured_program) run
+    through QarpSimulator::program_statevector of the reference build, with
+    the reference's own permutation folding."""
+    prog = W.structured_program(n, seed)
+    state = O.program_statevector(prog.kernels(), n, lib="ref")
+    np.savez_compressed(HERE / name, n=n, seed=seed, kinds=np.array(prog.kinds()), state=state)
+
+
 def main():
     if O.reference() is None:
         raise SystemExit("oracle/_ref/libqarp_ref.so is not built (make -C oracle)")
@@ -134,6 +143,7 @@ def main():
     qaoa_fixture("qaoa10.npz", 10, 2, 5)
     rng_fixture("mt19937.npz")
     sample_fixture("sample10.npz", 10, 21, 5000, 77)
+    program_fixture("program10.npz", 10, 8)
     print("golden fixtures written to", HERE)
 
 
