@@ -281,6 +281,43 @@ def qaoa(n=34, p=4, seed=5, shots=100_000, n_adj=33, n_cmp=32, fusion=2):
             "c64_grad_abs_err": float(np.abs(g64 - g128).max())}
 
 
+def structured(sim, state, n):
+    """Structured-program kernels on the apply leg's state: basis
+    permutations of the 20 lowest qubits (gathered reads inside 2^20-amplitude
+    blocks) and of the 10 highest (whole 16-MiB runs move), and QPE-style
+    controlled powers (8 controls with exponents 2^j, 2 targets).  Bytes per
+    permutation: one out-of-place pass, 2 x 16 B x 2^n (the state takes the
+    permuted buffer)."""
+    from openqarp_amd import _lib
+    from openqarp_amd.program import Program
+    from openqarp_amd.workloads import haar_unitary
+    rng = np.random.default_rng(9)
+    out = {}
+    for name, qs in (("permutation_low20", list(range(20))), ("permutation_high10", list(range(n - 10, n)))):
+        p = Program()
+        p.add_permutation(qs, rng.permutation(1 << len(qs)))
+        sim.execute_program(p, state)
+        t0 = time.perf_counter()
+        for _ in range(3):
+            sim.execute_program(p, state)
+        ms = (time.perf_counter() - t0) / 3 * 1e3
+        gbs = 2 * 16 * (1 << n) / (ms * 1e-3) / 1e9
+        out[name] = {"ms": round(ms, 3), "GBps": round(gbs, 1)}
+        log(f"{name}: {ms:.2f} ms = {gbs:.0f} GB/s (one out-of-place pass)")
+    p = Program()
+    controls = list(range(n - 1, n - 9, -1))
+    p.add_controlled_powers(controls, [1 << j for j in range(8)], [0, 1], haar_unitary(4, rng))
+    sim.execute_program(p, state)
+    _lib.check(_lib.load().qhip_synchronize())
+    t0 = time.perf_counter()
+    sim.execute_program(p, state)
+    _lib.check(_lib.load().qhip_synchronize())
+    ms = (time.perf_counter() - t0) * 1e3
+    out["controlled_powers_8x2"] = {"ms": round(ms, 3)}
+    log(f"controlled powers (8 controls, 2 targets): {ms:.2f} ms")
+    return out
+
+
 def cpu_baseline(n=24, layers=30, fusion=3):
     """The reference simulator (oracle/_ref) on brickwork n=24 — the same
     workload family at 1/64 the state size — in the metric's unit."""
@@ -343,6 +380,7 @@ def main(argv=None) -> int:
     ap.add_argument("--ps-points", type=int, default=0, help="time only this many parameter-shift points")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-qaoa", action="store_true")
+    ap.add_argument("--no-structured", action="store_true", help="skip the structured-program kernels")
     ap.add_argument("--no-floor", action="store_true", help="skip the one-gate pass and device-copy references")
     args = ap.parse_args(argv)
     if args.gpus != 1:
@@ -354,6 +392,7 @@ def main(argv=None) -> int:
     sim = QarpSimulator()
     fa = fused_apply(args, sim)
     ex = None if args.no_expectation else expectation(sim, fa["state"], fa["n"])
+    st = None if args.no_structured else structured(sim, fa["state"], fa["n"])
     fa["state"] = None
     adj = None if args.no_adjoint else adjoint(sim, points=args.ps_points or None)
     bat = None if args.no_batch else batched()
@@ -376,7 +415,8 @@ def main(argv=None) -> int:
                      "bytes_per_launch": 2 * 16 * (1 << fa["n"])},
         "pass_floor": fa["pass_floor"], "device_copy": fa["device_copy"],
         "autotune": fa["trials"],
-        "expectation": ex, "adjoint": adj, "batched_vqe": bat, "qaoa": qa, "cpu_baseline": cpu,
+        "expectation": ex, "structured_program": st, "adjoint": adj, "batched_vqe": bat, "qaoa": qa,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
     return 0

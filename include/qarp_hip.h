@@ -219,7 +219,9 @@ int qhip_apply(const qhip_config* cfg, const qhip_stream* stream, qhip_state* s,
  * Stands in for apply_permutation (src/qarpx/src/simulator/program.cpp:610-646):
  * |x> -> |table[x]> on `qubits` (local bit b <-> qubits[b]), k = 1..26 distinct
  * qubits, table a bijection of [0, 2^k).  Out of place through a library-owned
- * scratch state, then copied back. */
+ * scratch state: a state created by qhip_state_create takes the permuted
+ * buffer (its device pointer changes; re-read qhip_state_device_ptr), a state
+ * wrapping caller memory gets the result copied back. */
 int qhip_state_permute(qhip_state* s, const uint32_t* qubits, int32_t k, const uint64_t* table);
 
 /* Stands in for apply_controlled_powers (program.cpp:648-712): matrix^exponents[j]

@@ -27,18 +27,43 @@ struct PermQubits {
 
 // Gather form of |x> -> |table[x]>: out[j] = in[(j & keep) | deposit(inv[x(j)])],
 // so each thread's write is coalesced and its read is a gathered 8/16-B load.
-template <typename A>
+// A thread takes kUnroll outputs kBlock apart per step and issues all their
+// table loads, then all their amplitude loads, before any store: the two
+// dependent loads of one output cost one latency per step, not per output.
+constexpr int kUnroll = 8;
+
+// KB: the table width rounded up (8, 16 or 26); positions past k hold 63,
+// whose bit is 0 in every index and table entry, so the bit loops have a
+// fixed trip count and the positions stay in scalar registers.
+template <typename A, int KB>
 __global__ __launch_bounds__(kBlock) void permute_kernel(const A* __restrict__ in, A* __restrict__ out,
-                                                           uint64_t dim, const uint32_t* __restrict__ inv,
-                                                           const PermQubits Q, int k, uint64_t keep) {
-    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
-    for (uint64_t j = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; j < dim; j += stride) {
-        uint32_t x = 0;
-        for (int b = 0; b < k; ++b) x |= static_cast<uint32_t>((j >> Q.q[b]) & 1u) << b;
-        const uint32_t s = inv[x];
-        uint64_t i = j & keep;
-        for (int b = 0; b < k; ++b) i |= static_cast<uint64_t>((s >> b) & 1u) << Q.q[b];
-        out[j] = in[i];
+                                                         uint64_t dim, const uint32_t* __restrict__ inv,
+                                                         const PermQubits Q, uint64_t keep) {
+    const uint64_t step = static_cast<uint64_t>(gridDim.x) * kBlock * kUnroll;
+    for (uint64_t j0 = static_cast<uint64_t>(blockIdx.x) * kBlock * kUnroll + threadIdx.x; j0 < dim; j0 += step) {
+        uint32_t s[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const uint64_t j = j0 + static_cast<uint64_t>(u) * kBlock;
+            uint32_t x = 0;
+#pragma unroll
+            for (int b = 0; b < KB; ++b) x |= static_cast<uint32_t>((j >> Q.q[b]) & 1u) << b;
+            s[u] = j < dim ? inv[x] : 0u;
+        }
+        A v[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const uint64_t j = j0 + static_cast<uint64_t>(u) * kBlock;
+            uint64_t i = j & keep;
+#pragma unroll
+            for (int b = 0; b < KB; ++b) i |= static_cast<uint64_t>((s[u] >> b) & 1u) << Q.q[b];
+            if (j < dim) v[u] = in[i];
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const uint64_t j = j0 + static_cast<uint64_t>(u) * kBlock;
+            if (j < dim) out[j] = v[u];
+        }
     }
 }
 
@@ -47,10 +72,19 @@ void launch_permute(const qhip_state* s, void* scratch, const uint32_t* d_inv, c
                     uint64_t keep) {
     auto& C = ctx();
     const uint64_t dim = s->dim();
-    const uint64_t want = (dim + kBlock - 1) / kBlock;
-    const int blocks = static_cast<int>(std::min<uint64_t>(want, static_cast<uint64_t>(C.n_cu) * 16));
-    hipLaunchKernelGGL(permute_kernel<A>, dim3(std::max(blocks, 1)), dim3(kBlock), 0, C.stream,
-                       static_cast<const A*>(s->d), static_cast<A*>(scratch), dim, d_inv, Q, k, keep);
+    const uint64_t want = (dim + kBlock * kUnroll - 1) / (kBlock * kUnroll);
+    const int blocks = std::max(1, static_cast<int>(std::min<uint64_t>(want, static_cast<uint64_t>(C.n_cu) * 16)));
+    const A* in = static_cast<const A*>(s->d);
+    A* out = static_cast<A*>(scratch);
+    if (k <= 8)
+        hipLaunchKernelGGL((permute_kernel<A, 8>), dim3(blocks), dim3(kBlock), 0, C.stream, in, out, dim, d_inv, Q,
+                           keep);
+    else if (k <= 16)
+        hipLaunchKernelGGL((permute_kernel<A, 16>), dim3(blocks), dim3(kBlock), 0, C.stream, in, out, dim, d_inv, Q,
+                           keep);
+    else
+        hipLaunchKernelGGL((permute_kernel<A, kMaxTableQubits>), dim3(blocks), dim3(kBlock), 0, C.stream, in, out,
+                           dim, d_inv, Q, keep);
     QHIP_CHECK(hipGetLastError());
 }
 
@@ -92,6 +126,7 @@ void permute_state(qhip_state* s, const uint32_t* qubits, int k, const uint64_t*
     }
     auto& C = ctx();
     PermQubits Q{};
+    for (int b = 0; b < 32; ++b) Q.q[b] = 63;   // padding: bit 63 is clear in every index and entry
     uint64_t keep = ~uint64_t{0};
     for (int b = 0; b < k; ++b) {
         Q.q[b] = static_cast<uint8_t>(qs[b]);
@@ -103,9 +138,17 @@ void permute_state(qhip_state* s, const uint32_t* qubits, int k, const uint64_t*
     QHIP_CHECK(hipMemcpyAsync(d_inv, inv.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, C.stream));
     if (s->precision == QHIP_C128) launch_permute<double2>(s, scratch, d_inv, Q, k, keep);
     else launch_permute<float2>(s, scratch, d_inv, Q, k, keep);
-    // Back into the caller's buffer (a state may wrap memory it does not own).
-    QHIP_CHECK(hipMemcpyAsync(s->d, scratch, bytes, hipMemcpyDeviceToDevice, C.stream));
-    QHIP_CHECK(hipStreamSynchronize(C.stream));
+    if (s->owned) {
+        // The state takes the permuted buffer; its old one becomes the scratch.
+        QHIP_CHECK(hipStreamSynchronize(C.stream));
+        C.perm.ptr = s->d;
+        C.perm.bytes = bytes;
+        s->d = scratch;
+    } else {
+        // A state wrapping caller memory keeps its buffer: copy back.
+        QHIP_CHECK(hipMemcpyAsync(s->d, scratch, bytes, hipMemcpyDeviceToDevice, C.stream));
+        QHIP_CHECK(hipStreamSynchronize(C.stream));
+    }
 }
 
 void controlled_powers(const qhip_config* cfg, qhip_state* s, const uint32_t* controls, const uint64_t* exponents,
