@@ -246,17 +246,47 @@ class PauliAveraging:
         return self.ket + basis
 
     def run(self, sim: QarpSimulator, params=None, seeds: Optional[Sequence[Optional[int]]] = None) -> float:
+        """The ket runs once into a device state; each group copies it,
+        applies only its basis change and samples the copy (the reference runs
+        ket + basis change per group, pauli_averaging.py:119-205: the same
+        state up to rounding, with the same per-group seeds)."""
         energy = self.constant
+        if not self.groups:
+            return float(energy)
+        ket = bind(self.ket, params)
+        shared = not any(c.gate == GateType.Reset or c.condition_bits or (c.gate == GateType.Measure and c.cbits)
+                         or c.gate in (GateType.BranchBegin, GateType.BranchElse, GateType.BranchEnd)
+                         for c in ket)
+        if shared:
+            base = sim.new_state(self.n_qubits)
+            sim.apply([c for c in ket if c.gate not in (GateType.Measure, GateType.Barrier)], base)
+            work = sim.new_state(self.n_qubits)
         for g, (letters, idx) in enumerate(self.groups):
             seed = None if seeds is None else seeds[g]
-            res = Sampler(self.group_circuit(letters), self.n_qubits, self.n_shots).raw(sim, params, seed)
-            outcomes, weights = outcome_arrays(res)
-            masks = np.array([sum(1 << q for q in self._terms[t][0]) for t in idx], dtype=np.int64)
-            coeffs = np.array([self._terms[t][1] for t in idx])
-            parity = _parity(outcomes[:, None] & masks[None, :])
-            e_terms = (weights @ (1.0 - 2.0 * parity)) / res.n_shots
-            energy += float(coeffs @ e_terms)
+            if not shared:   # trajectory commands: the per-group circuit raises as run() does
+                res = Sampler(self.group_circuit(letters), self.n_qubits, self.n_shots).raw(sim, params, seed)
+            else:
+                work.copy_from(base)
+                basis = self.group_circuit(letters)[len(self.ket):]
+                if basis:
+                    sim.apply(basis, work)
+                if self.n_shots is EXACT:
+                    keys, probs = sim.exact_state(work, EXACT_PRUNE_TOL)
+                    res = ExactResult(self.n_qubits, keys, probs)
+                else:
+                    res = sim._terminal_result(work, [], self.n_qubits, int(self.n_shots), seed)
+            energy += self.group_estimate(res, idx)
         return float(energy)
+
+    def group_estimate(self, res, idx: Sequence[int]) -> float:
+        """Σ_t c_t ⟨P_t⟩ of one group from its counts (or exact distribution):
+        each term's parity over its qubits, weighted by the outcome counts."""
+        outcomes, weights = outcome_arrays(res)
+        masks = np.array([sum(1 << q for q in self._terms[t][0]) for t in idx], dtype=np.int64)
+        coeffs = np.array([self._terms[t][1] for t in idx])
+        parity = _parity(outcomes[:, None] & masks[None, :])
+        e_terms = (weights @ (1.0 - 2.0 * parity)) / res.n_shots
+        return float(coeffs @ e_terms)
 
 
 def require_capability(prim, method: str) -> None:

@@ -105,6 +105,23 @@ def test_pauli_averaging_exact_equals_expectation_and_sampling_is_close():
     assert abs(est - want) < 0.05 * scale
 
 
+def test_pauli_averaging_shares_the_ket_state():
+    """The ket runs once and each group samples a copy after its basis
+    change: the estimate equals the one from running ket + basis change per
+    group with the same seeds (the reference's per-group circuits)."""
+    from openqarp_amd.simulator import QarpSimulator
+    n = 9
+    ket = W.brickwork_haar(n, 3, 4)
+    obs = W.random_pauli_sum(n, 20, 6)
+    sim = QarpSimulator()
+    pa = PauliAveraging(ket, n, obs, 4000)
+    seeds = list(range(100, 100 + pa.n_groups))
+    want = pa.constant
+    for g, (letters, idx) in enumerate(pa.groups):
+        want += pa.group_estimate(Sampler(pa.group_circuit(letters), n, 4000).raw(sim, None, seeds[g]), idx)
+    assert abs(pa.run(sim, None, seeds) - want) < 1e-12
+
+
 def test_gradient_methods_agree():
     n = 4
     cmds, obs, syms, params = _gradient_circuit(n)
