@@ -234,6 +234,13 @@ int qhip_state_controlled_powers(const qhip_config* cfg, qhip_state* s, const ui
                                  const uint64_t* exponents, int32_t n_controls, const uint32_t* targets,
                                  int32_t m, const double* matrix);
 
+/* Batched basis states for transition columns: the state is read as blocks
+ * of `stride` amplitudes (a power of two, at most 2^n); block b is set
+ * to |labels[b]> for b < n_labels and to zero beyond.  Stands in for
+ * set_basis_states (src/qarpx/src/simulator/transition_columns.cu:88-93)
+ * after its memset of the batch. */
+int qhip_state_set_basis(qhip_state* s, const uint64_t* labels, int64_t n_labels, uint64_t stride);
+
 /* Compiled stream: the fusion pass and the device schedule of qhip_apply done
  * once, op tables resident on the device, applied to any state of the given
  * width and precision.  The counterpart of compiling a circuit once and

@@ -25,6 +25,7 @@ EXPORTS = [
     "qhip_plan_create", "qhip_plan_apply", "qhip_plan_apply_timed", "qhip_plan_apply_graph",
     "qhip_plan_update", "qhip_plan_destroy",
     "qhip_apply", "qhip_apply_terminal", "qhip_state_permute", "qhip_state_controlled_powers",
+    "qhip_state_set_basis",
     "qhip_transition", "qhip_apply_observable",
     "qhip_sample", "qhip_sample_uniforms", "qhip_probabilities", "qhip_exact",
     "qhip_run_gradient", "qhip_run_gradient_phi", "qhip_batch_energy_grad",
@@ -64,6 +65,7 @@ def _declare(lib):
         "qhip_apply": [vp, vp, vp, vp],
         "qhip_apply_terminal": [vp, vp, vp, vp],
         "qhip_state_permute": [vp, vp, i32, vp],
+        "qhip_state_set_basis": [vp, vp, i64, u64],
         "qhip_state_controlled_powers": [vp, vp, vp, vp, i32, vp, i32, vp],
         "qhip_transition": [vp, vp, vp, vp],
         "qhip_apply_observable": [vp, vp, vp],

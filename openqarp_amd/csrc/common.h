@@ -323,6 +323,7 @@ void apply_stream(const qhip_config* cfg, const qhip_stream* stream, qhip_state*
                   qhip_apply_stats* stats, bool terminal);
 // Structured-program kernels (kernels_program.hip).
 void permute_state(qhip_state* s, const uint32_t* qubits, int k, const uint64_t* table);
+void set_basis_states(qhip_state* s, const uint64_t* labels, int64_t n_labels, uint64_t stride);
 void controlled_powers(const qhip_config* cfg, qhip_state* s, const uint32_t* controls, const uint64_t* exponents,
                        int n_controls, const uint32_t* targets, int m, const double* matrix);
 Plan plan_stream(const qhip_config* cfg, const qhip_stream* stream, int n, int precision,

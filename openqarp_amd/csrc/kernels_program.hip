@@ -88,6 +88,22 @@ void launch_permute(const qhip_state* s, void* scratch, const uint32_t* d_inv, c
     QHIP_CHECK(hipGetLastError());
 }
 
+// Zero the state, then amplitude b * stride + labels[b] = 1 for each b.
+template <typename A>
+__global__ __launch_bounds__(kBlock) void set_basis_kernel(A* s, uint64_t dim, const uint64_t* __restrict__ labels,
+                                                           int64_t n_labels, int log_stride) {
+    const uint64_t step = static_cast<uint64_t>(gridDim.x) * kBlock;
+    const uint64_t low = (uint64_t{1} << log_stride) - 1;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < dim; i += step) {
+        A v;
+        v.x = 0;
+        v.y = 0;
+        const uint64_t b = i >> log_stride;
+        if (b < static_cast<uint64_t>(n_labels) && labels[b] == (i & low)) v.x = 1;
+        s[i] = v;
+    }
+}
+
 using Mat = std::vector<cplx>;   // row-major d x d
 
 Mat matmul(const Mat& a, const Mat& b, size_t d) {
@@ -149,6 +165,31 @@ void permute_state(qhip_state* s, const uint32_t* qubits, int k, const uint64_t*
         QHIP_CHECK(hipMemcpyAsync(s->d, scratch, bytes, hipMemcpyDeviceToDevice, C.stream));
         QHIP_CHECK(hipStreamSynchronize(C.stream));
     }
+}
+
+void set_basis_states(qhip_state* s, const uint64_t* labels, int64_t n_labels, uint64_t stride) {
+    const uint64_t dim = s->dim();
+    if (stride == 0 || stride > dim || (stride & (stride - 1)) != 0)
+        fail(QHIP_INVALID, "qhip_state_set_basis: stride must be a power of two <= 2^n");
+    const int log_stride = std::countr_zero(stride);
+    if (n_labels < 0 || static_cast<uint64_t>(n_labels) > dim / stride)
+        fail(QHIP_INVALID, "qhip_state_set_basis: more labels than blocks of the state");
+    for (int64_t b = 0; b < n_labels; ++b)
+        if (labels[b] >= stride) fail(QHIP_INVALID, "qhip_state_set_basis: label outside its block");
+    auto& C = ctx();
+    auto* d_labels = static_cast<uint64_t*>(C.misc2.ensure(sizeof(uint64_t) * std::max<int64_t>(n_labels, 1)));
+    if (n_labels > 0)
+        QHIP_CHECK(hipMemcpyAsync(d_labels, labels, sizeof(uint64_t) * n_labels, hipMemcpyHostToDevice, C.stream));
+    const int blocks = std::max(1, static_cast<int>(std::min<uint64_t>((dim + kBlock - 1) / kBlock,
+                                                                       static_cast<uint64_t>(C.n_cu) * 32)));
+    if (s->precision == QHIP_C128)
+        hipLaunchKernelGGL(set_basis_kernel<double2>, dim3(blocks), dim3(kBlock), 0, C.stream,
+                           static_cast<double2*>(s->d), dim, d_labels, n_labels, log_stride);
+    else
+        hipLaunchKernelGGL(set_basis_kernel<float2>, dim3(blocks), dim3(kBlock), 0, C.stream,
+                           static_cast<float2*>(s->d), dim, d_labels, n_labels, log_stride);
+    QHIP_CHECK(hipGetLastError());
+    QHIP_CHECK(hipStreamSynchronize(C.stream));
 }
 
 void controlled_powers(const qhip_config* cfg, qhip_state* s, const uint32_t* controls, const uint64_t* exponents,

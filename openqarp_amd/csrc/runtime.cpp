@@ -661,6 +661,13 @@ int qhip_state_controlled_powers(const qhip_config* cfg, qhip_state* s, const ui
     });
 }
 
+int qhip_state_set_basis(qhip_state* s, const uint64_t* labels, int64_t n_labels, uint64_t stride) {
+    return guarded([&] {
+        need(s && (n_labels <= 0 || labels), "qhip_state_set_basis: NULL argument");
+        qhip::set_basis_states(s, labels, n_labels, stride);
+    });
+}
+
 int qhip_plan_create(const qhip_config* cfg, const qhip_stream* stream, int32_t n_qubits, int32_t precision,
                      qhip_plan** out) {
     return guarded([&] {

@@ -364,6 +364,13 @@ class QarpSimulator:
         self.execute_program(program, st, terminal_last=True)
         return self._terminal_result(st, last, n_qubits, n_shots, seed)
 
+    def make_transition_column_provider(self, U: Sequence[Command], n: int, observable,
+                                        initial_batch_capacity: int = 1, sector_target_labels=None):
+        """Columns U† H U |s> for batches of basis states
+        (CudaqSimulator.make_transition_column_provider, bindings_montecarlo.cpp:375-381)."""
+        from .columns import TransitionColumnProvider
+        return TransitionColumnProvider(self, U, n, observable, initial_batch_capacity, sector_target_labels)
+
     def batch_run(self, commands: Sequence[Command], n_qubits: int, n_shots: int,
                   param_sets: Sequence[Mapping[str, float]], seed: Optional[int] = None) -> list:
         out = []
