@@ -318,6 +318,29 @@ def structured(sim, state, n):
     return out
 
 
+def columns(n=20, capacity=32, n_sources=128, n_terms=400):
+    """Transition columns U† H U |s> (TransitionColumnProvider): a 20-qubit
+    brickwork U, a 400-term local Pauli sum, 128 sources in batches of 32
+    (one 25-qubit register per batch)."""
+    from openqarp_amd import workloads as W
+    from openqarp_amd.simulator import QarpSimulator
+    sim = QarpSimulator()
+    U = W.brickwork_haar(n, 10, 6)
+    obs = [(t, float(np.real(c))) for t, c in W.random_local_pauli_sum(n, n_terms, 7)]
+    prov = sim.make_transition_column_provider(U, n, obs, capacity)
+    src = np.arange(0, n_sources * 4099, 4099, dtype=np.uint64) % np.uint64(1 << n)
+    prov.columns(src[:capacity])
+    t0 = time.perf_counter()
+    prov.compact_columns(src, 1e-12)
+    s = time.perf_counter() - t0
+    log(f"transition columns: {n_sources} columns of 2^{n} in {s:.3f} s ({n_sources / s:.0f} columns/s)")
+    d = prov.diagnostics
+    return {"n_qubits": n, "sources": n_sources, "batch": capacity, "terms": n_terms, "s": round(s, 4),
+            "columns_per_s": round(n_sources / s, 1),
+            "u_s": round(d["u_wall_seconds"], 4), "h_s": round(d["hamiltonian_wall_seconds"], 4),
+            "u_dagger_s": round(d["u_dagger_wall_seconds"], 4), "transfer_s": round(d["transfer_wall_seconds"], 4)}
+
+
 def cpu_baseline(n=24, layers=30, fusion=3):
     """The reference simulator (oracle/_ref) on brickwork n=24 — the same
     workload family at 1/64 the state size — in the metric's unit."""
@@ -380,7 +403,8 @@ def main(argv=None) -> int:
     ap.add_argument("--ps-points", type=int, default=0, help="time only this many parameter-shift points")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-qaoa", action="store_true")
-    ap.add_argument("--no-structured", action="store_true", help="skip the structured-program kernels")
+    ap.add_argument("--no-structured", action="store_true",
+                    help="skip the structured-program kernels and transition columns")
     ap.add_argument("--no-floor", action="store_true", help="skip the one-gate pass and device-copy references")
     args = ap.parse_args(argv)
     if args.gpus != 1:
@@ -397,6 +421,7 @@ def main(argv=None) -> int:
     adj = None if args.no_adjoint else adjoint(sim, points=args.ps_points or None)
     bat = None if args.no_batch else batched()
     qa = None if args.no_qaoa else qaoa()
+    cols = None if args.no_structured else columns()
     cpu = None if args.no_cpu else cpu_baseline()
     line = {
         "metric": METRIC, "value": round(fa["value"], 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
@@ -415,7 +440,8 @@ def main(argv=None) -> int:
                      "bytes_per_launch": 2 * 16 * (1 << fa["n"])},
         "pass_floor": fa["pass_floor"], "device_copy": fa["device_copy"],
         "autotune": fa["trials"],
-        "expectation": ex, "structured_program": st, "adjoint": adj, "batched_vqe": bat, "qaoa": qa,
+        "expectation": ex, "structured_program": st, "transition_columns": cols, "adjoint": adj,
+        "batched_vqe": bat, "qaoa": qa,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -176,33 +176,29 @@ class TransitionColumnProvider:
             raise ValueError("TransitionColumnProvider requires a finite, non-negative compaction tolerance")
         src = self._check_sources(source_labels)
         gather = self._sector.size > 0
-        offsets = [0]
-        labels, values = [], []
-        if src.size:
-            rows = self._rows(src, gather)
-            t = time.perf_counter()
-            targets = self._sector if gather else np.arange(self._dim, dtype=np.uint64)
-            mag = np.hypot(rows.real, rows.imag)
-            if not np.all(np.isfinite(mag)):
-                raise RuntimeError("Exact column contains a nonfinite coefficient")
-            keep = mag > tol
-            pruned = np.where(keep, 0.0, mag)
-            self._diag["max_pruned_amplitude"] = max(self._diag["max_pruned_amplitude"],
-                                                     float(pruned.max()) if pruned.size else 0.0)
-            for j in range(src.size):
-                idx = np.nonzero(keep[j])[0]
-                labels.append(targets[idx])
-                values.append(rows[j, idx])
-                offsets.append(offsets[-1] + idx.size)
-            n_kept = offsets[-1]
-            self._diag["compact_entries"] += n_kept
-            self._diag["pruned_entries"] += int(src.size) * rows.shape[1] - n_kept
-            if gather:
-                self._diag["sector_omitted_entries"] += int(src.size) * (self._dim - rows.shape[1])
-            self._diag["compaction_wall_seconds"] += time.perf_counter() - t
-        lab = np.concatenate(labels).astype(np.uint64) if labels else np.zeros(0, np.uint64)
-        val = np.concatenate(values).astype(np.complex128) if values else np.zeros(0, np.complex128)
-        return np.asarray(offsets, dtype=np.uint64), lab, val
+        if src.size == 0:
+            return np.zeros(1, np.uint64), np.zeros(0, np.uint64), np.zeros(0, np.complex128)
+        rows = self._rows(src, gather)
+        t = time.perf_counter()
+        targets = self._sector if gather else np.arange(self._dim, dtype=np.uint64)
+        mag = np.abs(rows)
+        if not np.isfinite(mag).all():
+            raise RuntimeError("Exact column contains a nonfinite coefficient")
+        keep = mag > tol
+        r, c = np.nonzero(keep)          # row-major: per source, targets ascending
+        offsets = np.zeros(src.size + 1, dtype=np.uint64)
+        offsets[1:] = np.cumsum(np.bincount(r, minlength=src.size))
+        labels = targets[c].astype(np.uint64)
+        values = rows[r, c].astype(np.complex128)
+        pruned = mag[~keep]
+        d = self._diag
+        d["max_pruned_amplitude"] = max(d["max_pruned_amplitude"], float(pruned.max()) if pruned.size else 0.0)
+        d["compact_entries"] += int(labels.size)
+        d["pruned_entries"] += int(src.size) * rows.shape[1] - int(labels.size)
+        if gather:
+            d["sector_omitted_entries"] += int(src.size) * (self._dim - rows.shape[1])
+        d["compaction_wall_seconds"] += time.perf_counter() - t
+        return offsets, labels, values
 
 
 def make_transition_column_provider(sim, U: Sequence[Command], n: int, observable,
