@@ -241,6 +241,30 @@ int qhip_state_controlled_powers(const qhip_config* cfg, qhip_state* s, const ui
  * after its memset of the batch. */
 int qhip_state_set_basis(qhip_state* s, const uint64_t* labels, int64_t n_labels, uint64_t stride);
 
+/* ── shot-batched trajectories ───────────────────────────────────────────────
+ * The per-shot loop of QarpSimulator::run (src/qarpx/src/simulator/
+ * qarp_simulator.cpp:1038-1211) with mid-circuit Measure, Reset, classical
+ * conditions and branches (apply_command_trajectory, :600-689).  A batch of
+ * shots lives in one state of n + b qubits, shot j in the block of `stride`
+ * = 2^n amplitudes starting at j * stride; unitary runs go through qhip_apply
+ * on that state (their qubits are < n).
+ *
+ * Every block := the n-qubit `src` (the shared unitary prefix). */
+int qhip_shots_broadcast(qhip_state* dst, const qhip_state* src);
+/* Blocks j with active[j] == 0 := the same blocks of `saved` (shots a
+ * conditional run skipped; active has dim / stride entries). */
+int qhip_shots_restore(qhip_state* dst, const qhip_state* saved, const uint8_t* active, uint64_t stride);
+/* p0[j] = sum of |a|^2 over block j's amplitudes with `qubit` clear (M0_prob,
+ * csim stat_ops_probability.cpp:11-30), FP64. */
+int qhip_shots_prob0(const qhip_state* s, int32_t qubit, uint64_t stride, double* p0);
+/* Per block: action -1 untouched; 0 keep |0> of `qubit` and scale; 1 keep |1>
+ * and scale; 2 move |1> to |0> and scale (Reset after outcome 1).  P0_gate /
+ * P1_gate, normalize(p) = scale by sqrt(1/p), X (qarp_simulator.cpp:636-685). */
+int qhip_shots_project(qhip_state* s, int32_t qubit, uint64_t stride, const int8_t* action, const double* scale);
+/* n draws of std::uniform_real_distribution<double>(0, 1) from
+ * std::mt19937(seed) — the per-shot stream mt19937(base_s + shot) of run(). */
+int qhip_mt_uniforms(uint32_t seed, int64_t n, double* out);
+
 /* Compiled stream: the fusion pass and the device schedule of qhip_apply done
  * once, op tables resident on the device, applied to any state of the given
  * width and precision.  The counterpart of compiling a circuit once and

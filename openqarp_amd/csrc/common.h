@@ -324,6 +324,14 @@ void apply_stream(const qhip_config* cfg, const qhip_stream* stream, qhip_state*
 // Structured-program kernels (kernels_program.hip).
 void permute_state(qhip_state* s, const uint32_t* qubits, int k, const uint64_t* table);
 void set_basis_states(qhip_state* s, const uint64_t* labels, int64_t n_labels, uint64_t stride);
+// Shot-batched trajectories (kernels_traj.hip).
+void shots_broadcast(qhip_state* dst, const qhip_state* src);
+void shots_restore(qhip_state* dst, const qhip_state* saved, const uint8_t* active, uint64_t stride);
+void shots_prob0(const qhip_state* s, int q, uint64_t stride, double* p0);
+void shots_project(qhip_state* s, int q, uint64_t stride, const int8_t* action, const double* scale);
+// generate_canonical<double, 53> over std::mt19937(seed), which is also
+// uniform_real_distribution<double>(0, 1) (kernels_sample.hip).
+void mt_uniforms(uint32_t seed, int64_t n, double* out);
 void controlled_powers(const qhip_config* cfg, qhip_state* s, const uint32_t* controls, const uint64_t* exponents,
                        int n_controls, const uint32_t* targets, int m, const double* matrix);
 Plan plan_stream(const qhip_config* cfg, const qhip_stream* stream, int n, int precision,

@@ -668,6 +668,41 @@ int qhip_state_set_basis(qhip_state* s, const uint64_t* labels, int64_t n_labels
     });
 }
 
+int qhip_shots_broadcast(qhip_state* dst, const qhip_state* src) {
+    return guarded([&] {
+        need(dst && src, "qhip_shots_broadcast: NULL argument");
+        qhip::shots_broadcast(dst, src);
+    });
+}
+
+int qhip_shots_restore(qhip_state* dst, const qhip_state* saved, const uint8_t* active, uint64_t stride) {
+    return guarded([&] {
+        need(dst && saved && active, "qhip_shots_restore: NULL argument");
+        qhip::shots_restore(dst, saved, active, stride);
+    });
+}
+
+int qhip_shots_prob0(const qhip_state* s, int32_t qubit, uint64_t stride, double* p0) {
+    return guarded([&] {
+        need(s && p0, "qhip_shots_prob0: NULL argument");
+        qhip::shots_prob0(s, qubit, stride, p0);
+    });
+}
+
+int qhip_shots_project(qhip_state* s, int32_t qubit, uint64_t stride, const int8_t* action, const double* scale) {
+    return guarded([&] {
+        need(s && action && scale, "qhip_shots_project: NULL argument");
+        qhip::shots_project(s, qubit, stride, action, scale);
+    });
+}
+
+int qhip_mt_uniforms(uint32_t seed, int64_t n, double* out) {
+    return guarded([&] {
+        need(n <= 0 || out, "qhip_mt_uniforms: NULL argument");
+        qhip::mt_uniforms(seed, n, out);
+    });
+}
+
 int qhip_plan_create(const qhip_config* cfg, const qhip_stream* stream, int32_t n_qubits, int32_t precision,
                      qhip_plan** out) {
     return guarded([&] {

@@ -274,7 +274,12 @@ class QarpSimulator:
     def run(self, commands: Sequence[Command], n_qubits: int, n_shots: int, seed: Optional[int] = None, *,
             initial_state=None) -> SamplingResult:
         """Terminal-measurement fast path of QarpSimulator::run
-        (qarp_simulator.cpp:1017-1036) and sample_terminal (:751-783)."""
+        (qarp_simulator.cpp:1017-1036) and sample_terminal (:751-783); circuits
+        with mid-circuit measurement, Reset, conditions or branches take the
+        shot-batched trajectory path (trajectory.py, :1038-1211)."""
+        if not measurements_terminal(commands):
+            from .trajectory import run_trajectories
+            return run_trajectories(self, commands, n_qubits, n_shots, seed, initial_state)
         st = self.new_state(n_qubits, initial_state, "run")
         self.apply(commands, st, terminal=True)
         return self._terminal_result(st, commands, n_qubits, n_shots, seed)

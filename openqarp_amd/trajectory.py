@@ -1,0 +1,220 @@
+"""Shot-batched trajectories: QarpSimulator::run for circuits with mid-circuit
+Measure, Reset, classical conditions and branches
+(src/qarpx/src/simulator/qarp_simulator.cpp:1038-1211; per command,
+apply_command_trajectory :600-689; branches, build_branch_table :298-350).
+
+The unitary prefix before the first trajectory command runs once.  Shots then
+run in batches of B = 2^b inside one (n + b)-qubit device state, shot j in
+the amplitude block [j·2^n, (j+1)·2^n):
+
+- a run of unconditional gates is one fused apply on the whole state (its
+  qubits are < n, so every shot advances together); when only some shots
+  execute it (inside a branch, or a conditional gate), the others are
+  restored from a copy afterwards (qhip_shots_restore);
+- Measure / Reset: the probability of |0> per shot on the device
+  (qhip_shots_prob0), one uniform per executing shot from its own stream
+  std::mt19937(base_s + shot) (qhip_mt_uniforms), outcome = u >= p0, then
+  the per-shot projection and renormalisation on the device
+  (qhip_shots_project);
+- the final draw per shot is the reference's inline CDF walk: probabilities
+  |a|^2 = re*re + im*im, their running sum in index order, threshold =
+  u · total, first index whose running sum reaches it.
+
+Random numbers are consumed per shot exactly as the reference does (one per
+executed Measure or Reset, one for the final draw), so outcomes agree with
+the reference's except where a comparison lands within rounding of a
+probability."""
+from __future__ import annotations
+
+import ctypes as C
+import secrets
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from .ir import Command, GateType, SamplingResult, cbit_register_width
+from .program import needs_trajectory
+
+MAX_BATCH_AMPS = 1 << 24        # amplitudes of one batched state
+
+
+def _parse(cmds: Sequence[Command], i: int = 0, stop=()) -> tuple:
+    """Items of cmds[i:] up to a stop marker: ("gates", [..]) runs,
+    ("op", cmd) for Measure / Reset / conditional commands, and
+    ("branch", begin, then_items, else_items)."""
+    items = []
+    run: list = []
+
+    def flush():
+        if run:
+            items.append(("gates", list(run)))
+            run.clear()
+    while i < len(cmds):
+        c = cmds[i]
+        if c.gate == GateType.BranchBegin:
+            flush()
+            then, i = _parse(cmds, i + 1, (GateType.BranchElse, GateType.BranchEnd))
+            els = []
+            if cmds[i].gate == GateType.BranchElse:
+                els, i = _parse(cmds, i + 1, (GateType.BranchEnd,))
+            items.append(("branch", c, then, els))
+            i += 1
+        elif c.gate in (GateType.BranchElse, GateType.BranchEnd):
+            if c.gate in stop:
+                flush()
+                return items, i
+            raise RuntimeError(f"branch table: {c.gate.name} without matching BranchBegin")
+        elif c.gate in (GateType.Measure, GateType.Reset) or c.condition_bits:
+            flush()
+            items.append(("op", c))
+            i += 1
+        elif c.gate == GateType.Barrier:
+            i += 1
+        else:
+            run.append(c)
+            i += 1
+    if stop:
+        raise RuntimeError("branch table: BranchBegin without matching BranchEnd")
+    flush()
+    return items, i
+
+
+def _draws(items) -> int:
+    n = 0
+    for it in items:
+        if it[0] == "op" and it[1].gate in (GateType.Measure, GateType.Reset):
+            n += 1
+        elif it[0] == "branch":
+            n += _draws(it[2]) + _draws(it[3])
+    return n
+
+
+def _holds(bits, values, regs: np.ndarray) -> np.ndarray:
+    ok = np.ones(regs.shape[0], dtype=bool)
+    for b, v in zip(bits, values):
+        ok &= regs[:, b] == bool(v)
+    return ok
+
+
+class _Batch:
+    def __init__(self, sim, n: int, b: int, n_cbits: int, uniforms: np.ndarray):
+        from .simulator import DeviceState
+        self.sim = sim
+        self.n, self.stride, self.B = n, 1 << n, 1 << b
+        self.state = DeviceState(n + b, sim.precision)
+        self.saved: Optional[DeviceState] = None
+        self.regs = np.zeros((self.B, n_cbits), dtype=bool)
+        self.u = uniforms                       # (B, draws + 1)
+        self.ptr = np.zeros(self.B, dtype=np.int64)
+        self.lib = _lib.load()
+
+    def gates(self, cmds, active: np.ndarray) -> None:
+        if not active.any():
+            return
+        if active.all():
+            self.sim.apply(cmds, self.state)
+            return
+        if self.saved is None:
+            from .simulator import DeviceState
+            self.saved = DeviceState(self.state.n_qubits, self.sim.precision)
+        self.saved.copy_from(self.state)
+        self.sim.apply(cmds, self.state)
+        act = np.ascontiguousarray(active, dtype=np.uint8)
+        _lib.check(self.lib.qhip_shots_restore(self.state.handle, self.saved.handle, act.ctypes.data, self.stride))
+
+    def collapse(self, q: int, cbits, reset: bool, active: np.ndarray) -> None:
+        if not active.any():
+            return
+        p0 = np.zeros(self.B)
+        _lib.check(self.lib.qhip_shots_prob0(self.state.handle, q, self.stride, p0.ctypes.data))
+        idx = np.nonzero(active)[0]
+        u = self.u[idx, self.ptr[idx]]
+        self.ptr[idx] += 1
+        one = u >= p0[idx]
+        with np.errstate(divide="ignore"):
+            p = np.where(one, np.maximum(1.0 - p0[idx], 0.0), np.maximum(p0[idx], 0.0))
+            f = np.sqrt(1.0 / p)
+        action = np.full(self.B, -1, dtype=np.int8)
+        action[idx] = np.where(one, 2 if reset else 1, 0)
+        scale = np.ones(self.B)
+        scale[idx] = f
+        _lib.check(self.lib.qhip_shots_project(self.state.handle, q, self.stride, action.ctypes.data,
+                                               scale.ctypes.data))
+        if cbits and not reset:
+            self.regs[idx, cbits[0]] = one
+
+    def run(self, items, active: np.ndarray) -> None:
+        for it in items:
+            if it[0] == "gates":
+                self.gates(it[1], active)
+            elif it[0] == "branch":
+                hold = _holds(it[1].condition_bits, it[1].condition_values, self.regs)
+                self.run(it[2], active & hold)
+                if it[3]:
+                    self.run(it[3], active & ~hold)
+            else:
+                c = it[1]
+                act = active & _holds(c.condition_bits, c.condition_values, self.regs) if c.condition_bits \
+                    else active
+                if c.gate in (GateType.Measure, GateType.Reset):
+                    if not c.qubits:
+                        raise RuntimeError(f"trajectory: {c.gate.name} requires a qubit")
+                    self.collapse(c.qubits[0], list(c.cbits), c.gate == GateType.Reset, act)
+                else:
+                    bare = Command(c.gate, qubits=c.qubits, params=c.params, unitary=c.unitary)
+                    self.gates([bare], act)
+
+    def sample(self, count: int) -> np.ndarray:
+        amps = self.state.download().reshape(self.B, self.stride)[:count]
+        probs = amps.real * amps.real + amps.imag * amps.imag
+        cum = np.add.accumulate(probs, axis=1)
+        out = np.empty(count, dtype=np.uint64)
+        for s in range(count):
+            threshold = self.u[s, self.ptr[s]] * cum[s, -1]
+            k = int(np.searchsorted(cum[s], threshold, side="left"))
+            out[s] = min(k, self.stride - 1)
+        return out
+
+
+def run_trajectories(sim, commands: Sequence[Command], n_qubits: int, n_shots: int,
+                     seed: Optional[int] = None, initial_state=None) -> SamplingResult:
+    """QarpSimulator::run's trajectory path on the device (shots in batches)."""
+    n = int(n_qubits)
+    n_shots = int(n_shots)
+    if n_shots < 0:
+        raise ValueError("QarpSimulator::run: n_shots must be non-negative")
+    base_s = (secrets.randbits(32) if seed is None else int(seed)) & 0xFFFFFFFF
+    commands = list(commands)
+    split = next((i for i, c in enumerate(commands) if needs_trajectory(c)), len(commands))
+    items, _ = _parse(commands[split:])
+    n_draws = _draws(items) + 1
+    n_cbits = cbit_register_width(commands)
+    base = sim.new_state(n, initial_state, "run")
+    sim.apply(commands[:split], base)
+    res = SamplingResult(n_qubits=n, n_shots=n_shots)
+    res.n_cbits = n_cbits
+    counts: dict = {}
+    history: list = []
+    lib = _lib.load()
+    # Shots per batch: enough for all of them, within MAX_BATCH_AMPS amplitudes.
+    b = min(max(n_shots - 1, 0).bit_length(), max(0, MAX_BATCH_AMPS.bit_length() - 1 - n))
+    for shot0 in range(0, n_shots, 1 << b):
+        count = min(1 << b, n_shots - shot0)
+        u = np.zeros((1 << b, n_draws))
+        for s in range(count):
+            _lib.check(lib.qhip_mt_uniforms(C.c_uint32((base_s + shot0 + s) & 0xFFFFFFFF), n_draws,
+                                            u[s].ctypes.data))
+        batch = _Batch(sim, n, b, n_cbits, u)
+        _lib.check(lib.qhip_shots_broadcast(batch.state.handle, base.handle))
+        active = np.zeros(1 << b, dtype=bool)
+        active[:count] = True
+        batch.run(items, active)
+        outcomes = batch.sample(count)
+        vals, cnt = np.unique(outcomes, return_counts=True)
+        for v, c in zip(vals.tolist(), cnt.tolist()):
+            counts[int(v)] = counts.get(int(v), 0) + int(c)
+        history.extend([list(map(bool, r)) for r in batch.regs[:count]])
+    res.counts = counts
+    res.cbit_history = history
+    return res

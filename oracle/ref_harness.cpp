@@ -21,6 +21,7 @@
 
 #include <Eigen/Dense>
 
+#include <algorithm>
 #include <complex>
 #include <cstring>
 #include <memory>
@@ -229,6 +230,41 @@ int ref_run_counts(const qhip_stream* s, int n, int n_shots, uint32_t seed, int 
             counts[i] = v;
             ++i;
         }
+        return 0;
+    } catch (const std::invalid_argument& e) { return catch_all(e, QHIP_INVALID);
+    } catch (const std::exception& e) { return catch_all(e, QHIP_RUNTIME); }
+}
+
+// QarpSimulator::run on any circuit (the trajectory path when measurements are
+// not terminal): counts and the per-shot classical registers (shots x n_cbits
+// bytes, capacity hist_cap).
+// Conditions ride beside the stream: cond_count[i] (bit, value) pairs per op,
+// flattened in op order.
+int ref_run_trajectory(const qhip_stream* s, const int32_t* cond_count, const uint32_t* cond_bits,
+                       const uint8_t* cond_values, int n, int n_shots, uint32_t seed, int64_t cap, int64_t* n_keys,
+                       uint64_t* keys, int32_t* counts, int32_t* n_cbits, uint8_t* hist, int64_t hist_cap) {
+    try {
+        auto sim = make_sim(3, 12);
+        auto cmds = to_commands(s);
+        int64_t k0 = 0;
+        for (size_t i = 0; i < cmds.size(); ++i)
+            for (int j = 0; j < cond_count[i]; ++j, ++k0) {
+                cmds[i].condition_bits.push_back(cond_bits[k0]);
+                cmds[i].condition_values.push_back(cond_values[k0] != 0);
+            }
+        auto res = sim.run(cmds, n, n_shots, seed);
+        *n_keys = static_cast<int64_t>(res.counts.size());
+        std::vector<std::pair<uint64_t, int>> kv(res.counts.begin(), res.counts.end());
+        std::sort(kv.begin(), kv.end());
+        for (int64_t i = 0; i < static_cast<int64_t>(kv.size()) && i < cap; ++i) {
+            keys[i] = kv[i].first;
+            counts[i] = kv[i].second;
+        }
+        *n_cbits = res.n_cbits;
+        int64_t k = 0;
+        for (const auto& reg : res.cbit_history)
+            for (bool bit : reg)
+                if (k < hist_cap) hist[k++] = bit ? 1 : 0;
         return 0;
     } catch (const std::invalid_argument& e) { return catch_all(e, QHIP_INVALID);
     } catch (const std::exception& e) { return catch_all(e, QHIP_RUNTIME); }

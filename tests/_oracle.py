@@ -76,6 +76,7 @@ def reference():
         lib.ref_run_gradient.argtypes = [vp, i32, vp, i32, vp, vp, vp]
         lib.ref_run_gradient_phi.argtypes = [vp, i32, vp, i32, vp, vp, vp]
         lib.ref_program_statevector.argtypes = [vp, i32, i32, i32, i32, vp, vp]
+        lib.ref_run_trajectory.argtypes = [vp, vp, vp, vp, i32, i32, C.c_uint32, i64, vp, vp, vp, vp, vp, i64]
         _ref = lib
     return _ref
 
@@ -329,3 +330,118 @@ def program_statevector(kernels, n, init=None, fusion_max=3, fusion_min=12, lib=
     _check(L, L.ref_program_statevector(C.cast(arr, C.c_void_p), len(kernels), n, fusion_max, fusion_min,
                                         init_c.ctypes.data, out.ctypes.data), "ref")
     return out
+
+
+# ── trajectories (QarpSimulator::run with mid-circuit measurement) ───────────
+
+def run_trajectory(commands, n, n_shots, seed, lib="orc"):
+    """(counts dict, cbit history list) of QarpSimulator::run.  lib="ref":
+    the reference build.  lib="orc": a per-shot restatement of the
+    reference's trajectory loop (qarp_simulator.cpp:1038-1211,
+    apply_command_trajectory :600-689): the unitary prefix once, then per
+    shot std::mt19937(seed + shot) (orc_uniforms), each Measure / Reset draws
+    u, outcome = u >= P(0) (sum of |a|^2 in index order), projection and
+    scaling by sqrt(1 / p), conditions and branches on the shot's register,
+    and the final inline CDF walk."""
+    from openqarp_amd._abi import pack_stream
+    from openqarp_amd.ir import GateType, cbit_register_width, Command
+    from openqarp_amd.program import needs_trajectory
+    if lib == "ref":
+        L = reference()
+        cap = 1 << n
+        keys = np.zeros(cap, np.uint64)
+        cnt = np.zeros(cap, np.int32)
+        nk = C.c_int64()
+        nc = C.c_int32()
+        width = cbit_register_width(commands)
+        hist = np.zeros(max(1, n_shots * width), np.uint8)
+        ps = pack_stream(commands)
+        ccount = np.array([len(c.condition_bits) for c in commands], dtype=np.int32)
+        cbits = np.array([b for c in commands for b in c.condition_bits] or [0], dtype=np.uint32)
+        cvals = np.array([bool(v) for c in commands for v in c.condition_values] or [0], dtype=np.uint8)
+        _check(L, L.ref_run_trajectory(ps.ref, ccount.ctypes.data, cbits.ctypes.data, cvals.ctypes.data, n,
+                                       n_shots, seed, cap, C.byref(nk), keys.ctypes.data, cnt.ctypes.data,
+                                       C.byref(nc), hist.ctypes.data, hist.size), "ref")
+        counts = {int(keys[i]): int(cnt[i]) for i in range(nk.value)}
+        h = hist[: n_shots * nc.value].reshape(n_shots, nc.value).astype(bool)
+        return counts, [list(map(bool, r)) for r in h]
+    cmds = list(commands)
+    split = next((i for i, c in enumerate(cmds) if needs_trajectory(c)), len(cmds))
+    base = statevector(pack_stream(cmds[:split]), n, 3, 12)
+    suffix = cmds[split:]
+    width = cbit_register_width(cmds)
+    dim = 1 << n
+    ends = {}
+    stack = []
+    for i, c in enumerate(suffix):
+        if c.gate == GateType.BranchBegin:
+            stack.append([i, None])
+        elif c.gate == GateType.BranchElse:
+            stack[-1][1] = i
+        elif c.gate == GateType.BranchEnd:
+            b, e = stack.pop()
+            ends[b] = (e, i)
+    counts, hist = {}, []
+    for shot in range(n_shots):
+        u = uniforms((seed + shot) & 0xFFFFFFFF, 4 * len(suffix) + 2)
+        k = [0]
+        reg = [False] * width
+        psi = base.copy()
+
+        def draw():
+            v = u[k[0]]
+            k[0] += 1
+            return v
+
+        def holds(c):
+            return all(reg[b] == bool(v) for b, v in zip(c.condition_bits, c.condition_values))
+
+        def run(lo, hi):
+            nonlocal psi
+            i = lo
+            while i < hi:
+                c = suffix[i]
+                if c.gate == GateType.BranchBegin:
+                    e, end = ends[i]
+                    if holds(c):
+                        run(i + 1, e if e is not None else end)
+                    elif e is not None:
+                        run(e + 1, end)
+                    i = end + 1
+                    continue
+                i += 1
+                if c.condition_bits and not holds(c):
+                    continue
+                if c.gate in (GateType.Measure, GateType.Reset):
+                    q = c.qubits[0]
+                    idx = np.arange(dim)
+                    zero = ((idx >> q) & 1) == 0
+                    p0 = 0.0
+                    for a in psi[zero]:
+                        p0 += a.real * a.real + a.imag * a.imag
+                    one = draw() >= p0
+                    p = max(1.0 - p0, 0.0) if one else max(p0, 0.0)
+                    f = np.sqrt(1.0 / p)
+                    psi = np.where(zero != one, psi * f, 0)
+                    if c.gate == GateType.Reset and one:
+                        psi = psi[idx ^ (1 << q)]
+                    elif c.gate == GateType.Measure and c.cbits:
+                        reg[c.cbits[0]] = bool(one)
+                elif c.gate != GateType.Barrier:
+                    bare = Command(c.gate, qubits=c.qubits, params=c.params, unitary=c.unitary)
+                    psi = statevector(pack_stream([bare]), n, 0, 0, init=psi)
+        run(0, len(suffix))
+        probs = psi.real * psi.real + psi.imag * psi.imag
+        total = 0.0
+        for pr in probs:
+            total += pr
+        threshold = draw() * total
+        acc, out = 0.0, dim - 1
+        for i, pr in enumerate(probs):
+            acc += pr
+            if acc >= threshold:
+                out = i
+                break
+        counts[out] = counts.get(out, 0) + 1
+        hist.append(list(reg))
+    return counts, hist

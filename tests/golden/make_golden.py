@@ -131,6 +131,18 @@ def program_fixture(name, n, seed):
     np.savez_compressed(HERE / name, n=n, seed=seed, kinds=np.array(prog.kinds()), state=state)
 
 
+def trajectory_fixture(name, n, seed, shots, shot_seed):
+    """QarpSimulator::run of the reference build on a circuit with mid-circuit
+    measurements, resets and nested branches (tests/test_trajectory.py
+    circuits(n, seed)["branches"]): counts and every shot's register."""
+    from test_trajectory import circuits
+    counts, hist = O.run_trajectory(circuits(n, seed)["branches"], n, shots, shot_seed, lib="ref")
+    keys = np.array(sorted(counts), dtype=np.int64)
+    np.savez_compressed(HERE / name, n=n, seed=seed, shots=shots, shot_seed=shot_seed, keys=keys,
+                        counts=np.array([counts[k] for k in keys.tolist()], dtype=np.int64),
+                        history=np.array(hist, dtype=bool))
+
+
 def main():
     if O.reference() is None:
         raise SystemExit("oracle/_ref/libqarp_ref.so is not built (make -C oracle)")
@@ -144,6 +156,7 @@ def main():
     rng_fixture("mt19937.npz")
     sample_fixture("sample10.npz", 10, 21, 5000, 77)
     program_fixture("program10.npz", 10, 8)
+    trajectory_fixture("trajectory6.npz", 6, 2, 400, 555)
     print("golden fixtures written to", HERE)
 
 
