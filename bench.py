@@ -341,6 +341,28 @@ def columns(n=20, capacity=32, n_sources=128, n_terms=400):
             "u_dagger_s": round(d["u_dagger_wall_seconds"], 4), "transfer_s": round(d["transfer_wall_seconds"], 4)}
 
 
+def trajectories(n=16, shots=2048):
+    """Trajectory path: a 16-qubit brickwork, mid-circuit measurements with
+    a conditional correction and a reset, more brickwork, 2048 shots in
+    batches of 256 (one 24-qubit state per batch)."""
+    from openqarp_amd import workloads as W
+    from openqarp_amd.ir import Command, GateType
+    from openqarp_amd.simulator import QarpSimulator
+    sim = QarpSimulator()
+    cmds = W.brickwork_haar(n, 4, 11) + [
+        Command(GateType.Measure, 0, cbits=[0]),
+        Command(GateType.X, 1, condition_bits=[0], condition_values=[True]),
+        Command(GateType.Reset, 2),
+    ] + W.brickwork_haar(n, 4, 12) + [Command(GateType.Measure, 3, cbits=[1])]
+    sim.run(cmds, n, 64, seed=1)
+    t0 = time.perf_counter()
+    res = sim.run(cmds, n, shots, seed=2)
+    s = time.perf_counter() - t0
+    log(f"trajectories: {shots} shots of a {n}-qubit circuit with mid-circuit measurement in {s:.3f} s")
+    return {"n_qubits": n, "shots": shots, "s": round(s, 4), "shots_per_s": round(shots / s, 1),
+            "distinct_outcomes": len(res.counts)}
+
+
 def cpu_baseline(n=24, layers=30, fusion=3):
     """The reference simulator (oracle/_ref) on brickwork n=24 — the same
     workload family at 1/64 the state size — in the metric's unit."""
@@ -404,7 +426,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-qaoa", action="store_true")
     ap.add_argument("--no-structured", action="store_true",
-                    help="skip the structured-program kernels and transition columns")
+                    help="skip the structured-program kernels, transition columns and trajectories")
     ap.add_argument("--no-floor", action="store_true", help="skip the one-gate pass and device-copy references")
     args = ap.parse_args(argv)
     if args.gpus != 1:
@@ -422,6 +444,7 @@ def main(argv=None) -> int:
     bat = None if args.no_batch else batched()
     qa = None if args.no_qaoa else qaoa()
     cols = None if args.no_structured else columns()
+    traj = None if args.no_structured else trajectories()
     cpu = None if args.no_cpu else cpu_baseline()
     line = {
         "metric": METRIC, "value": round(fa["value"], 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
@@ -440,7 +463,8 @@ def main(argv=None) -> int:
                      "bytes_per_launch": 2 * 16 * (1 << fa["n"])},
         "pass_floor": fa["pass_floor"], "device_copy": fa["device_copy"],
         "autotune": fa["trials"],
-        "expectation": ex, "structured_program": st, "transition_columns": cols, "adjoint": adj,
+        "expectation": ex, "structured_program": st, "transition_columns": cols, "trajectories": traj,
+        "adjoint": adj,
         "batched_vqe": bat, "qaoa": qa,
         "cpu_baseline": cpu,
     }
