@@ -261,6 +261,11 @@ int qhip_shots_prob0(const qhip_state* s, int32_t qubit, uint64_t stride, double
  * and scale; 2 move |1> to |0> and scale (Reset after outcome 1).  P0_gate /
  * P1_gate, normalize(p) = scale by sqrt(1/p), X (qarp_simulator.cpp:636-685). */
 int qhip_shots_project(qhip_state* s, int32_t qubit, uint64_t stride, const int8_t* action, const double* scale);
+/* Each shot's final draw (run()'s inline CDF walk, qarp_simulator.cpp:1151-1163):
+ * for j < n_shots, out[j] = the first index of block j whose running sum of
+ * |a|^2 (re*re + im*im, index order) reaches u[j] times the block's total,
+ * else stride - 1.  One device thread per shot, sums rounded term by term. */
+int qhip_shots_sample(const qhip_state* s, uint64_t stride, const double* u, int64_t n_shots, uint64_t* out);
 /* n draws of std::uniform_real_distribution<double>(0, 1) from
  * std::mt19937(seed) — the per-shot stream mt19937(base_s + shot) of run(). */
 int qhip_mt_uniforms(uint32_t seed, int64_t n, double* out);

@@ -5,8 +5,8 @@
 // every shot at once through the ordinary apply, and these kernels do the
 // per-shot parts — broadcasting the shared prefix state, restoring shots a
 // conditional run must not touch, the probability of |0> on a qubit per
-// shot (M0_prob), and the per-shot projection + renormalisation of Measure
-// and Reset (P0/P1_gate, normalize, X).
+// shot (M0_prob), the per-shot projection + renormalisation of Measure and
+// Reset (P0/P1_gate, normalize, X), and each shot's final draw.
 #include "common.h"
 
 #include <algorithm>
@@ -96,6 +96,36 @@ __global__ __launch_bounds__(kBlk) void project_kernel(A* __restrict__ s, int q,
     }
 }
 
+// One thread per shot: the reference's final draw, sequentially — total =
+// sum of p_i = re*re + im*im in index order, threshold = u * total, first
+// index whose running sum reaches it (else the last).  The products and sums
+// are rounded one by one (no fused multiply-add), as the host code does.
+template <typename A>
+__global__ __launch_bounds__(64) void sample_blocks_kernel(const A* __restrict__ s, uint64_t stride,
+                                                           const double* __restrict__ u, int64_t n_shots,
+                                                           uint64_t* __restrict__ out) {
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * 64 + threadIdx.x;
+    if (j >= n_shots) return;
+    const A* b = s + static_cast<uint64_t>(j) * stride;
+    double total = 0.0;
+    for (uint64_t i = 0; i < stride; ++i) {
+        const double x = b[i].x, y = b[i].y;
+        total = __dadd_rn(total, __dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)));
+    }
+    const double threshold = __dmul_rn(u[j], total);
+    double acc = 0.0;
+    uint64_t pick = stride - 1;
+    for (uint64_t i = 0; i < stride; ++i) {
+        const double x = b[i].x, y = b[i].y;
+        acc = __dadd_rn(acc, __dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)));
+        if (acc >= threshold) {
+            pick = i;
+            break;
+        }
+    }
+    out[j] = pick;
+}
+
 void check_blocks(const qhip_state* s, uint64_t stride, const char* what) {
     if (stride == 0 || stride > s->dim() || (stride & (stride - 1)) != 0)
         fail(QHIP_INVALID, std::string(what) + ": shot stride must be a power of two <= 2^n");
@@ -161,6 +191,28 @@ void shots_prob0(const qhip_state* s, int q, uint64_t stride, double* p0) {
         for (uint64_t c = 0; c < chunks; ++c) t += part[j * chunks + c];
         p0[j] = t;
     }
+}
+
+void shots_sample(const qhip_state* s, uint64_t stride, const double* u, int64_t n_shots, uint64_t* out) {
+    check_blocks(s, stride, "qhip_shots_sample");
+    if (n_shots < 0 || static_cast<uint64_t>(n_shots) > s->dim() / stride)
+        fail(QHIP_INVALID, "qhip_shots_sample: more shots than blocks of the state");
+    if (n_shots == 0) return;
+    auto& C = ctx();
+    char* base = static_cast<char*>(C.misc2.ensure(16 * static_cast<size_t>(n_shots) + 64));
+    auto* d_u = reinterpret_cast<double*>(base);
+    auto* d_out = reinterpret_cast<uint64_t*>(base + 8 * n_shots);
+    QHIP_CHECK(hipMemcpyAsync(d_u, u, sizeof(double) * n_shots, hipMemcpyHostToDevice, C.stream));
+    const dim3 grid(static_cast<uint32_t>((n_shots + 63) / 64));
+    if (s->precision == QHIP_C128)
+        hipLaunchKernelGGL(sample_blocks_kernel<double2>, grid, dim3(64), 0, C.stream,
+                           static_cast<const double2*>(s->d), stride, d_u, n_shots, d_out);
+    else
+        hipLaunchKernelGGL(sample_blocks_kernel<float2>, grid, dim3(64), 0, C.stream,
+                           static_cast<const float2*>(s->d), stride, d_u, n_shots, d_out);
+    QHIP_CHECK(hipGetLastError());
+    QHIP_CHECK(hipMemcpyAsync(out, d_out, sizeof(uint64_t) * n_shots, hipMemcpyDeviceToHost, C.stream));
+    QHIP_CHECK(hipStreamSynchronize(C.stream));
 }
 
 void shots_project(qhip_state* s, int q, uint64_t stride, const int8_t* action, const double* scale) {

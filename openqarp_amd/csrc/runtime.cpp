@@ -696,6 +696,13 @@ int qhip_shots_project(qhip_state* s, int32_t qubit, uint64_t stride, const int8
     });
 }
 
+int qhip_shots_sample(const qhip_state* s, uint64_t stride, const double* u, int64_t n_shots, uint64_t* out) {
+    return guarded([&] {
+        need(s && (n_shots <= 0 || (u && out)), "qhip_shots_sample: NULL argument");
+        qhip::shots_sample(s, stride, u, n_shots, out);
+    });
+}
+
 int qhip_mt_uniforms(uint32_t seed, int64_t n, double* out) {
     return guarded([&] {
         need(n <= 0 || out, "qhip_mt_uniforms: NULL argument");

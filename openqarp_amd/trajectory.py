@@ -16,9 +16,10 @@ the amplitude block [j·2^n, (j+1)·2^n):
   std::mt19937(base_s + shot) (qhip_mt_uniforms), outcome = u >= p0, then
   the per-shot projection and renormalisation on the device
   (qhip_shots_project);
-- the final draw per shot is the reference's inline CDF walk: probabilities
-  |a|^2 = re*re + im*im, their running sum in index order, threshold =
-  u · total, first index whose running sum reaches it.
+- the final draw per shot is the reference's inline CDF walk, one device
+  thread per shot (qhip_shots_sample): probabilities |a|^2 = re*re + im*im,
+  their running sum in index order, threshold = u · total, first index whose
+  running sum reaches it.
 
 Random numbers are consumed per shot exactly as the reference does (one per
 executed Measure or Reset, one for the final draw), so outcomes agree with
@@ -98,9 +99,10 @@ def _holds(bits, values, regs: np.ndarray) -> np.ndarray:
 
 
 class _Batch:
-    def __init__(self, sim, n: int, b: int, n_cbits: int, uniforms: np.ndarray):
+    def __init__(self, sim, n: int, b: int, n_cbits: int, uniforms: np.ndarray, plans: dict):
         from .simulator import DeviceState
         self.sim = sim
+        self.plans = plans   # per call: id(run) -> (run, CompiledCircuit on n + b qubits), bare conditionals
         self.n, self.stride, self.B = n, 1 << n, 1 << b
         self.state = DeviceState(n + b, sim.precision)
         self.saved: Optional[DeviceState] = None
@@ -109,17 +111,35 @@ class _Batch:
         self.ptr = np.zeros(self.B, dtype=np.int64)
         self.lib = _lib.load()
 
+    def apply(self, cmds) -> None:
+        """One fused, scheduled apply of a command run, compiled once per run
+        for every batch (all batches share the register width)."""
+        from .simulator import CompiledCircuit
+        plan = self.plans.get(id(cmds))
+        if plan is None:
+            plan = self.plans[id(cmds)] = (cmds, CompiledCircuit(self.sim, cmds, self.state.n_qubits))
+        plan[1].apply(self.state)
+
+    def bare(self, item) -> list:
+        """The command of a conditional item without its condition, as a run
+        (one list per item for the whole call, so its plan is compiled once)."""
+        key = ("bare", id(item))
+        if key not in self.plans:
+            c = item[1]
+            self.plans[key] = (item, [Command(c.gate, qubits=c.qubits, params=c.params, unitary=c.unitary)])
+        return self.plans[key][1]
+
     def gates(self, cmds, active: np.ndarray) -> None:
         if not active.any():
             return
         if active.all():
-            self.sim.apply(cmds, self.state)
+            self.apply(cmds)
             return
         if self.saved is None:
             from .simulator import DeviceState
             self.saved = DeviceState(self.state.n_qubits, self.sim.precision)
         self.saved.copy_from(self.state)
-        self.sim.apply(cmds, self.state)
+        self.apply(cmds)
         act = np.ascontiguousarray(active, dtype=np.uint8)
         _lib.check(self.lib.qhip_shots_restore(self.state.handle, self.saved.handle, act.ctypes.data, self.stride))
 
@@ -162,18 +182,15 @@ class _Batch:
                         raise RuntimeError(f"trajectory: {c.gate.name} requires a qubit")
                     self.collapse(c.qubits[0], list(c.cbits), c.gate == GateType.Reset, act)
                 else:
-                    bare = Command(c.gate, qubits=c.qubits, params=c.params, unitary=c.unitary)
-                    self.gates([bare], act)
+                    self.gates(self.bare(it), act)
 
     def sample(self, count: int) -> np.ndarray:
-        amps = self.state.download().reshape(self.B, self.stride)[:count]
-        probs = amps.real * amps.real + amps.imag * amps.imag
-        cum = np.add.accumulate(probs, axis=1)
-        out = np.empty(count, dtype=np.uint64)
-        for s in range(count):
-            threshold = self.u[s, self.ptr[s]] * cum[s, -1]
-            k = int(np.searchsorted(cum[s], threshold, side="left"))
-            out[s] = min(k, self.stride - 1)
+        """Each shot's final draw on the device (qhip_shots_sample), with the
+        shot's next uniform."""
+        u = np.ascontiguousarray(self.u[np.arange(count), self.ptr[:count]])
+        out = np.zeros(count, dtype=np.uint64)
+        _lib.check(self.lib.qhip_shots_sample(self.state.handle, self.stride, u.ctypes.data, count,
+                                              out.ctypes.data))
         return out
 
 
@@ -197,6 +214,7 @@ def run_trajectories(sim, commands: Sequence[Command], n_qubits: int, n_shots: i
     counts: dict = {}
     history: list = []
     lib = _lib.load()
+    plans: dict = {}
     # Shots per batch: enough for all of them, within MAX_BATCH_AMPS amplitudes.
     b = min(max(n_shots - 1, 0).bit_length(), max(0, MAX_BATCH_AMPS.bit_length() - 1 - n))
     for shot0 in range(0, n_shots, 1 << b):
@@ -205,7 +223,7 @@ def run_trajectories(sim, commands: Sequence[Command], n_qubits: int, n_shots: i
         for s in range(count):
             _lib.check(lib.qhip_mt_uniforms(C.c_uint32((base_s + shot0 + s) & 0xFFFFFFFF), n_draws,
                                             u[s].ctypes.data))
-        batch = _Batch(sim, n, b, n_cbits, u)
+        batch = _Batch(sim, n, b, n_cbits, u, plans)
         _lib.check(lib.qhip_shots_broadcast(batch.state.handle, base.handle))
         active = np.zeros(1 << b, dtype=bool)
         active[:count] = True
