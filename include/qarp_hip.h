@@ -241,6 +241,15 @@ int qhip_state_controlled_powers(const qhip_config* cfg, qhip_state* s, const ui
  * after its memset of the batch. */
 int qhip_state_set_basis(qhip_state* s, const uint64_t* labels, int64_t n_labels, uint64_t stride);
 
+/* Compaction of transition columns (the loop of compact_columns,
+ * transition_columns.cu:262-291, on the device): among the first n_amps
+ * amplitudes, the indices whose magnitude hypot(re, im) exceeds `tolerance`
+ * and the amplitudes (complex128, (re, im)), in ascending index order.  With
+ * keys or values NULL only *n_out = the count is returned; otherwise up to
+ * `capacity` entries are written (QHIP_INVALID if more are retained). */
+int qhip_compact(const qhip_state* s, uint64_t n_amps, double tolerance, int64_t* keys, double* values,
+                 int64_t capacity, int64_t* n_out);
+
 /* ── shot-batched trajectories ───────────────────────────────────────────────
  * The per-shot loop of QarpSimulator::run (src/qarpx/src/simulator/
  * qarp_simulator.cpp:1038-1211) with mid-circuit Measure, Reset, classical

@@ -25,7 +25,7 @@ EXPORTS = [
     "qhip_plan_create", "qhip_plan_apply", "qhip_plan_apply_timed", "qhip_plan_apply_graph",
     "qhip_plan_update", "qhip_plan_destroy",
     "qhip_apply", "qhip_apply_terminal", "qhip_state_permute", "qhip_state_controlled_powers",
-    "qhip_state_set_basis", "qhip_shots_broadcast", "qhip_shots_restore", "qhip_shots_prob0",
+    "qhip_state_set_basis", "qhip_compact", "qhip_shots_broadcast", "qhip_shots_restore", "qhip_shots_prob0",
     "qhip_shots_project", "qhip_shots_sample", "qhip_mt_uniforms",
     "qhip_transition", "qhip_apply_observable",
     "qhip_sample", "qhip_sample_uniforms", "qhip_probabilities", "qhip_exact",
@@ -67,6 +67,7 @@ def _declare(lib):
         "qhip_apply_terminal": [vp, vp, vp, vp],
         "qhip_state_permute": [vp, vp, i32, vp],
         "qhip_state_set_basis": [vp, vp, i64, u64],
+        "qhip_compact": [vp, u64, C.c_double, vp, vp, i64, C.POINTER(i64)],
         "qhip_shots_broadcast": [vp, vp],
         "qhip_shots_restore": [vp, vp, vp, u64],
         "qhip_shots_prob0": [vp, i32, u64, vp],

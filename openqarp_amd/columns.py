@@ -12,6 +12,7 @@ U, H into a second state, U†, and one copy of the B columns to the host.
 """
 from __future__ import annotations
 
+import ctypes as C
 import math
 import time
 from typing import Optional, Sequence
@@ -123,40 +124,47 @@ class TransitionColumnProvider:
             raise ValueError("TransitionColumnProvider source label is outside the configured sector")
         return src
 
-    def _rows(self, src: np.ndarray, gather: bool) -> np.ndarray:
+    def _evaluate(self, part: np.ndarray) -> None:
+        """U† H U on the basis states `part` (at most one batch): the result
+        is left in the second state, block j = column of part[j]."""
         lib = _lib.load()
+        d = self._diag
+        t = time.perf_counter()
+        _lib.check(lib.qhip_state_set_basis(self._psi.handle, part.ctypes.data, part.size, self._dim))
+        t1 = time.perf_counter()
+        self._u.apply(self._psi)
+        _lib.check(lib.qhip_synchronize())
+        t2 = time.perf_counter()
+        if self._has_terms:
+            _lib.check(lib.qhip_apply_observable(self._psi.handle, self._obs.ref, self._phi.handle))
+        else:
+            _lib.check(lib.qhip_state_set_basis(self._phi.handle, None, 0, self._dim))
+        _lib.check(lib.qhip_synchronize())
+        t3 = time.perf_counter()
+        self._ud.apply(self._phi)
+        _lib.check(lib.qhip_synchronize())
+        t4 = time.perf_counter()
+        d["basis_wall_seconds"] += t1 - t
+        d["u_wall_seconds"] += t2 - t1
+        d["hamiltonian_wall_seconds"] += t3 - t2
+        d["u_dagger_wall_seconds"] += t4 - t3
+        d["column_batches"] += 1
+
+    def _rows(self, src: np.ndarray, gather: bool) -> np.ndarray:
         width = len(self._sector) if gather else self._dim
         out = np.empty((src.size, width), dtype=np.complex128)
         d = self._diag
         for start in range(0, src.size, self._batch):
             part = np.ascontiguousarray(src[start:start + self._batch])
-            t = time.perf_counter()
-            _lib.check(lib.qhip_state_set_basis(self._psi.handle, part.ctypes.data, part.size, self._dim))
-            t1 = time.perf_counter()
-            self._u.apply(self._psi)
-            _lib.check(lib.qhip_synchronize())
-            t2 = time.perf_counter()
-            if self._has_terms:
-                _lib.check(lib.qhip_apply_observable(self._psi.handle, self._obs.ref, self._phi.handle))
-            else:
-                _lib.check(lib.qhip_state_set_basis(self._phi.handle, None, 0, self._dim))
-            _lib.check(lib.qhip_synchronize())
-            t3 = time.perf_counter()
-            self._ud.apply(self._phi)
-            _lib.check(lib.qhip_synchronize())
+            self._evaluate(part)
             t4 = time.perf_counter()
             rows = self._phi.download().reshape(self._batch, self._dim)[:part.size]
             t5 = time.perf_counter()
             out[start:start + part.size] = rows[:, self._sector.astype(np.int64)] if gather else rows
             t6 = time.perf_counter()
-            d["basis_wall_seconds"] += t1 - t
-            d["u_wall_seconds"] += t2 - t1
-            d["hamiltonian_wall_seconds"] += t3 - t2
-            d["u_dagger_wall_seconds"] += t4 - t3
             d["transfer_wall_seconds"] += t5 - t4
             d["sector_gather_wall_seconds"] += t6 - t5 if gather else 0.0
             d["transferred_bytes"] += self._batch * self._dim * 16
-            d["column_batches"] += 1
         d["columns"] += int(src.size)
         return out
 
@@ -166,6 +174,41 @@ class TransitionColumnProvider:
         if src.size == 0:
             return np.zeros((0, self._dim), dtype=np.complex128)
         return self._rows(src, False)
+
+    def _compact_on_device(self, src: np.ndarray, tol: float):
+        """Full columns compacted on the device (qhip_compact): only the kept
+        entries cross to the host."""
+        lib = _lib.load()
+        d = self._diag
+        keys_all, vals_all = [], []
+        for start in range(0, src.size, self._batch):
+            part = np.ascontiguousarray(src[start:start + self._batch])
+            self._evaluate(part)
+            t = time.perf_counter()
+            n_amps = part.size * self._dim
+            cnt = C.c_int64()
+            _lib.check(lib.qhip_compact(self._phi.handle, n_amps, tol, None, None, 0, C.byref(cnt)))
+            keys = np.zeros(cnt.value, dtype=np.int64)
+            vals = np.zeros(cnt.value, dtype=np.complex128)
+            if cnt.value:
+                _lib.check(lib.qhip_compact(self._phi.handle, n_amps, tol, keys.ctypes.data, vals.ctypes.data,
+                                            cnt.value, C.byref(cnt)))
+            keys += start * self._dim
+            keys_all.append(keys)
+            vals_all.append(vals)
+            d["transferred_bytes"] += 24 * cnt.value
+            d["compaction_wall_seconds"] += time.perf_counter() - t
+        keys = np.concatenate(keys_all)
+        vals = np.concatenate(vals_all)
+        if not np.all(np.isfinite(vals)):
+            raise RuntimeError("Exact column contains a nonfinite coefficient")
+        row = keys >> self._n
+        offsets = np.zeros(src.size + 1, dtype=np.uint64)
+        offsets[1:] = np.cumsum(np.bincount(row, minlength=src.size))
+        d["columns"] += int(src.size)
+        d["compact_entries"] += int(keys.size)
+        d["pruned_entries"] += int(src.size) * self._dim - int(keys.size)
+        return offsets, (keys & (self._dim - 1)).astype(np.uint64), vals
 
     def compact_columns(self, source_labels, absolute_tolerance: float):
         """(offsets, labels, values): per source the entries of magnitude above
@@ -178,6 +221,8 @@ class TransitionColumnProvider:
         gather = self._sector.size > 0
         if src.size == 0:
             return np.zeros(1, np.uint64), np.zeros(0, np.uint64), np.zeros(0, np.complex128)
+        if not gather:
+            return self._compact_on_device(src, tol)
         rows = self._rows(src, gather)
         t = time.perf_counter()
         targets = self._sector if gather else np.arange(self._dim, dtype=np.uint64)

@@ -330,6 +330,10 @@ void shots_restore(qhip_state* dst, const qhip_state* saved, const uint8_t* acti
 void shots_prob0(const qhip_state* s, int q, uint64_t stride, double* p0);
 void shots_project(qhip_state* s, int q, uint64_t stride, const int8_t* action, const double* scale);
 void shots_sample(const qhip_state* s, uint64_t stride, const double* u, int64_t n_shots, uint64_t* out);
+// Entries of the first n_amps amplitudes with hypot(re, im) > tol, ascending
+// (kernels_sample.hip); returns the count, fills keys / values when given.
+int64_t compact_values(const qhip_state* s, uint64_t n_amps, double tol, int64_t* keys, double* values,
+                       int64_t capacity);
 // generate_canonical<double, 53> over std::mt19937(seed), which is also
 // uniform_real_distribution<double>(0, 1) (kernels_sample.hip).
 void mt_uniforms(uint32_t seed, int64_t n, double* out);

@@ -244,6 +244,60 @@ __global__ __launch_bounds__(kST) void exact_scatter(const A* __restrict__ s, ui
     }
 }
 
+// Value compaction (transition columns): indices whose amplitude magnitude
+// hypot(re, im) exceeds a tolerance, with the amplitudes, in ascending index
+// order — the same count / scan / ballot-ranked scatter as the EXACT readout.
+template <typename A>
+__global__ __launch_bounds__(kST) void compact_count(const A* __restrict__ s, uint64_t dim, double tol,
+                                                     uint64_t* __restrict__ counts) {
+    __shared__ uint32_t wsum[kST / 64];
+    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kXChunk;
+    uint32_t n = 0;
+    for (int r = 0; r < 16; ++r) {
+        const uint64_t i = base + static_cast<uint64_t>(r) * kST + threadIdx.x;
+        if (i < dim && hypot(static_cast<double>(s[i].x), static_cast<double>(s[i].y)) > tol) ++n;
+    }
+    for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+        for (int w = 0; w < kST / 64; ++w) t += wsum[w];
+        counts[blockIdx.x] = t;
+    }
+}
+
+template <typename A>
+__global__ __launch_bounds__(kST) void compact_scatter(const A* __restrict__ s, uint64_t dim, double tol,
+                                                       const uint64_t* __restrict__ offsets,
+                                                       int64_t* __restrict__ keys, double2* __restrict__ values) {
+    __shared__ uint32_t wcount[kST / 64];
+    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kXChunk;
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint64_t pos = offsets[blockIdx.x];
+    for (int r = 0; r < 16; ++r) {
+        const uint64_t i = base + static_cast<uint64_t>(r) * kST + threadIdx.x;
+        double2 v = {0.0, 0.0};
+        if (i < dim) v = {static_cast<double>(s[i].x), static_cast<double>(s[i].y)};
+        const bool keep = i < dim && hypot(v.x, v.y) > tol;
+        const uint64_t ballot = __ballot(keep);
+        const uint32_t rank = __popcll(ballot & ((1ull << lane) - 1ull));
+        if (lane == 0) wcount[w] = __popcll(ballot);
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+        for (int u = 0; u < kST / 64; ++u) {
+            before += u < static_cast<int>(w) ? wcount[u] : 0u;
+            total += wcount[u];
+        }
+        if (keep) {
+            keys[pos + before + rank] = static_cast<int64_t>(i);
+            values[pos + before + rank] = v;
+        }
+        pos += total;
+        __syncthreads();
+    }
+}
+
 // ── host side ───────────────────────────────────────────────────────────────
 
 static int binade(double v) {      // e with v in [2^e, 2^(e+1)), v > 0 finite
@@ -442,6 +496,48 @@ int64_t exact(const qhip_state* s, double threshold, int64_t* keys, double* prob
     QHIP_CHECK(hipGetLastError());
     QHIP_CHECK(hipMemcpyAsync(keys, d_keys, 8 * total, hipMemcpyDeviceToHost, C.stream));
     QHIP_CHECK(hipMemcpyAsync(probs, d_probs, 8 * total, hipMemcpyDeviceToHost, C.stream));
+    QHIP_CHECK(hipStreamSynchronize(C.stream));
+    return static_cast<int64_t>(total);
+}
+
+int64_t compact_values(const qhip_state* s, uint64_t n_amps, double tol, int64_t* keys, double* values,
+                       int64_t capacity) {
+    auto& C = ctx();
+    if (n_amps > s->dim()) fail(QHIP_INVALID, "qhip_compact: more amplitudes than the state holds");
+    if (!(tol >= 0.0) || tol == HUGE_VAL) fail(QHIP_INVALID, "qhip_compact: tolerance must be finite and >= 0");
+    const uint64_t dim = n_amps;
+    if (dim == 0) return 0;
+    const uint64_t nb = (dim + kXChunk - 1) / kXChunk;
+    uint64_t* counts = static_cast<uint64_t*>(C.red.ensure(sizeof(uint64_t) * (2 * nb + 1)));
+    uint64_t* offsets = counts + nb;
+    if (s->precision == QHIP_C128)
+        hipLaunchKernelGGL(compact_count<double2>, dim3(static_cast<uint32_t>(nb)), dim3(kST), 0, C.stream,
+                           static_cast<const double2*>(s->d), dim, tol, counts);
+    else
+        hipLaunchKernelGGL(compact_count<float2>, dim3(static_cast<uint32_t>(nb)), dim3(kST), 0, C.stream,
+                           static_cast<const float2*>(s->d), dim, tol, counts);
+    hipLaunchKernelGGL(exclusive_scan_u64, dim3(1), dim3(kST), 0, C.stream, counts, nb, offsets);
+    QHIP_CHECK(hipGetLastError());
+    uint64_t total = 0;
+    QHIP_CHECK(hipMemcpyAsync(&total, offsets + nb, sizeof(uint64_t), hipMemcpyDeviceToHost, C.stream));
+    QHIP_CHECK(hipStreamSynchronize(C.stream));
+    if (!keys || !values) return static_cast<int64_t>(total);
+    if (static_cast<uint64_t>(capacity) < total)
+        fail(QHIP_INVALID, "qhip_compact: capacity " + std::to_string(capacity) + " is below the " +
+                               std::to_string(total) + " retained entries");
+    if (total == 0) return 0;
+    char* out = static_cast<char*>(C.misc.ensure(24 * total));
+    int64_t* d_keys = reinterpret_cast<int64_t*>(out);
+    double2* d_vals = reinterpret_cast<double2*>(out + 8 * total);
+    if (s->precision == QHIP_C128)
+        hipLaunchKernelGGL(compact_scatter<double2>, dim3(static_cast<uint32_t>(nb)), dim3(kST), 0, C.stream,
+                           static_cast<const double2*>(s->d), dim, tol, offsets, d_keys, d_vals);
+    else
+        hipLaunchKernelGGL(compact_scatter<float2>, dim3(static_cast<uint32_t>(nb)), dim3(kST), 0, C.stream,
+                           static_cast<const float2*>(s->d), dim, tol, offsets, d_keys, d_vals);
+    QHIP_CHECK(hipGetLastError());
+    QHIP_CHECK(hipMemcpyAsync(keys, d_keys, 8 * total, hipMemcpyDeviceToHost, C.stream));
+    QHIP_CHECK(hipMemcpyAsync(values, d_vals, 16 * total, hipMemcpyDeviceToHost, C.stream));
     QHIP_CHECK(hipStreamSynchronize(C.stream));
     return static_cast<int64_t>(total);
 }

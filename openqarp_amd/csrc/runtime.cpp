@@ -668,6 +668,14 @@ int qhip_state_set_basis(qhip_state* s, const uint64_t* labels, int64_t n_labels
     });
 }
 
+int qhip_compact(const qhip_state* s, uint64_t n_amps, double tolerance, int64_t* keys, double* values,
+                 int64_t capacity, int64_t* n_out) {
+    return guarded([&] {
+        need(s && n_out, "qhip_compact: NULL argument");
+        *n_out = qhip::compact_values(s, n_amps, tolerance, keys, values, capacity);
+    });
+}
+
 int qhip_shots_broadcast(qhip_state* dst, const qhip_state* src) {
     return guarded([&] {
         need(dst && src, "qhip_shots_broadcast: NULL argument");

@@ -127,3 +127,25 @@ def test_compact_columns_with_sector():
         prov.compact_columns(src, -1.0)
     empty = sim.make_transition_column_provider(U, n, [], 2)
     assert not np.any(empty.columns(np.array([3], dtype=np.uint64)))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tol", [0.0, 0.05])
+def test_compact_columns_on_device(tol):
+    """Without a sector the columns are compacted on the device
+    (qhip_compact): labels ascending per source, values to 1e-12, offsets
+    per source, across batches."""
+    from openqarp_amd.simulator import QarpSimulator
+    n = 7
+    U, obs = workload(n, 4)
+    sim = QarpSimulator()
+    prov = sim.make_transition_column_provider(U, n, [(t, complex(c)) for t, c in obs], 2)
+    src = np.array([3, 100, 7, 64, 127], dtype=np.uint64)
+    offsets, labels, values = prov.compact_columns(src, tol)
+    want = oracle_columns(U, n, obs, src)
+    assert len(offsets) == len(src) + 1 and offsets[0] == 0
+    for j in range(len(src)):
+        keep = np.abs(want[j]) > tol
+        lo, hi = int(offsets[j]), int(offsets[j + 1])
+        assert np.array_equal(labels[lo:hi], np.nonzero(keep)[0].astype(np.uint64))
+        assert np.abs(values[lo:hi] - want[j][keep]).max() < 1e-12
