@@ -104,23 +104,46 @@ template <typename A>
 __global__ __launch_bounds__(64) void sample_blocks_kernel(const A* __restrict__ s, uint64_t stride,
                                                            const double* __restrict__ u, int64_t n_shots,
                                                            uint64_t* __restrict__ out) {
+    // Loads run kAhead elements ahead of the sequential sum, so a thread waits
+    // one memory latency per kAhead terms; the additions stay in index order.
+    constexpr int kAhead = 16;
     const int64_t j = static_cast<int64_t>(blockIdx.x) * 64 + threadIdx.x;
     if (j >= n_shots) return;
     const A* b = s + static_cast<uint64_t>(j) * stride;
+    auto terms = [&](uint64_t i0, double* p) {
+#pragma unroll
+        for (int t = 0; t < kAhead; ++t) {
+            double x = 0.0, y = 0.0;
+            if (i0 + t < stride) {
+                x = b[i0 + t].x;
+                y = b[i0 + t].y;
+            }
+            p[t] = __dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y));
+        }
+    };
     double total = 0.0;
-    for (uint64_t i = 0; i < stride; ++i) {
-        const double x = b[i].x, y = b[i].y;
-        total = __dadd_rn(total, __dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)));
+    for (uint64_t i0 = 0; i0 < stride; i0 += kAhead) {
+        double p[kAhead];
+        terms(i0, p);
+#pragma unroll
+        for (int t = 0; t < kAhead; ++t) total = __dadd_rn(total, p[t]);   // + 0 past the end: exact
     }
     const double threshold = __dmul_rn(u[j], total);
     double acc = 0.0;
     uint64_t pick = stride - 1;
-    for (uint64_t i = 0; i < stride; ++i) {
-        const double x = b[i].x, y = b[i].y;
-        acc = __dadd_rn(acc, __dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)));
-        if (acc >= threshold) {
-            pick = i;
-            break;
+    bool found = false;
+    for (uint64_t i0 = 0; i0 < stride && !found; i0 += kAhead) {
+        double p[kAhead];
+        terms(i0, p);
+#pragma unroll
+        for (int t = 0; t < kAhead; ++t) {
+            if (!found && i0 + t < stride) {
+                acc = __dadd_rn(acc, p[t]);
+                if (acc >= threshold) {
+                    pick = i0 + t;
+                    found = true;
+                }
+            }
         }
     }
     out[j] = pick;
