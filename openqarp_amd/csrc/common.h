@@ -330,6 +330,9 @@ void shots_restore(qhip_state* dst, const qhip_state* saved, const uint8_t* acti
 void shots_prob0(const qhip_state* s, int q, uint64_t stride, double* p0);
 void shots_project(qhip_state* s, int q, uint64_t stride, const int8_t* action, const double* scale);
 void shots_sample(const qhip_state* s, uint64_t stride, const double* u, int64_t n_shots, uint64_t* out);
+// run()'s inline CDF walk over a whole state with uniform u, exact in
+// parallel (kernels_sample.hip).
+uint64_t walk_draw(const qhip_state* s, double u);
 // Entries of the first n_amps amplitudes with hypot(re, im) > tol, ascending
 // (kernels_sample.hip); returns the count, fills keys / values when given.
 int64_t compact_values(const qhip_state* s, uint64_t n_amps, double tol, int64_t* keys, double* values,

@@ -441,6 +441,46 @@ static void sample_impl(const qhip_state* st, const double* uniforms, int64_t n,
     QHIP_CHECK(hipStreamSynchronize(C.stream));
 }
 
+// run()'s inline CDF walk over one state (qarp_simulator.cpp:1151-1163):
+// total = the sequential sum of p_i from 0.0, threshold = u * total, the first
+// index whose sequential running sum reaches it (else the last).  The exact
+// chunk ends of that running sum come from resolve(); only the chunk that
+// holds the threshold is walked term by term (on the host, from its exact
+// start).
+template <typename A>
+static uint64_t walk_impl(const qhip_state* st, double u) {
+    auto& C = ctx();
+    const uint64_t dim = st->dim();
+    const int64_t nc = static_cast<int64_t>((dim + kSChunk - 1) / kSChunk);
+    double* d_approx = static_cast<double*>(C.red.ensure(sizeof(double) * nc));
+    hipLaunchKernelGGL(chunk_sums<A>, dim3(static_cast<uint32_t>(nc)), dim3(kST), 0, C.stream,
+                       static_cast<const A*>(st->d), dim, d_approx);
+    QHIP_CHECK(hipGetLastError());
+    std::vector<double> approx(nc);
+    QHIP_CHECK(hipMemcpyAsync(approx.data(), d_approx, sizeof(double) * nc, hipMemcpyDeviceToHost, C.stream));
+    QHIP_CHECK(hipStreamSynchronize(C.stream));
+    const ChunkScan run = resolve<A>(st, approx, 1.0, false, false);
+    const double threshold = u * run.ends.back();
+    int64_t c = 0;
+    while (c < nc && run.ends[c] < threshold) ++c;
+    if (c == nc) return dim - 1;
+    const uint64_t b = static_cast<uint64_t>(c) * kSChunk;
+    const uint64_t e = std::min(b + kSChunk, dim);
+    std::vector<char> raw((e - b) * st->amp_bytes());
+    QHIP_CHECK(hipMemcpy(raw.data(), static_cast<const char*>(st->d) + b * st->amp_bytes(), raw.size(),
+                         hipMemcpyDeviceToHost));
+    double acc = run.starts[c];
+    for (uint64_t i = b; i < e; ++i) {
+        acc = acc + host_prob(st, raw, i - b);
+        if (acc >= threshold) return i;
+    }
+    return dim - 1;
+}
+
+uint64_t walk_draw(const qhip_state* s, double u) {
+    return s->precision == QHIP_C128 ? walk_impl<double2>(s, u) : walk_impl<float2>(s, u);
+}
+
 void sample(const qhip_state* s, const double* uniforms, int64_t n, uint64_t* out) {
     if (s->precision == QHIP_C128) sample_impl<double2>(s, uniforms, n, out);
     else sample_impl<float2>(s, uniforms, n, out);

@@ -19,6 +19,9 @@ namespace {
 
 constexpr int kBlk = 256;
 constexpr uint64_t kChunk = uint64_t{1} << 14;   // amplitude pairs per partial of prob0
+// Shots of at least this many amplitudes take the chunk-scan walk for their
+// final draw; smaller ones a thread each.
+constexpr uint64_t kWalkBlock = uint64_t{1} << 20;
 
 int blocks_for(uint64_t n, int cap_per_cu = 32) {
     return static_cast<int>(std::max<uint64_t>(
@@ -221,6 +224,19 @@ void shots_sample(const qhip_state* s, uint64_t stride, const double* u, int64_t
     if (n_shots < 0 || static_cast<uint64_t>(n_shots) > s->dim() / stride)
         fail(QHIP_INVALID, "qhip_shots_sample: more shots than blocks of the state");
     if (n_shots == 0) return;
+    if (stride >= kWalkBlock) {
+        // Wide shots: each block's walk through the exact chunk scan of the
+        // terminal sampler (one view of the block per shot).
+        for (int64_t j = 0; j < n_shots; ++j) {
+            qhip_state view{};
+            view.n = std::countr_zero(stride);
+            view.precision = s->precision;
+            view.d = static_cast<char*>(s->d) + static_cast<uint64_t>(j) * stride * s->amp_bytes();
+            view.owned = false;
+            out[j] = walk_draw(&view, u[j]);
+        }
+        return;
+    }
     auto& C = ctx();
     char* base = static_cast<char*>(C.misc2.ensure(16 * static_cast<size_t>(n_shots) + 64));
     auto* d_u = reinterpret_cast<double*>(base);

@@ -416,9 +416,8 @@ def run_trajectory(commands, n, n_shots, seed, lib="orc"):
                     q = c.qubits[0]
                     idx = np.arange(dim)
                     zero = ((idx >> q) & 1) == 0
-                    p0 = 0.0
-                    for a in psi[zero]:
-                        p0 += a.real * a.real + a.imag * a.imag
+                    pz = psi[zero]
+                    p0 = float(np.add.accumulate(pz.real * pz.real + pz.imag * pz.imag)[-1])
                     one = draw() >= p0
                     p = max(1.0 - p0, 0.0) if one else max(p0, 0.0)
                     f = np.sqrt(1.0 / p)
@@ -432,16 +431,9 @@ def run_trajectory(commands, n, n_shots, seed, lib="orc"):
                     psi = statevector(pack_stream([bare]), n, 0, 0, init=psi)
         run(0, len(suffix))
         probs = psi.real * psi.real + psi.imag * psi.imag
-        total = 0.0
-        for pr in probs:
-            total += pr
-        threshold = draw() * total
-        acc, out = 0.0, dim - 1
-        for i, pr in enumerate(probs):
-            acc += pr
-            if acc >= threshold:
-                out = i
-                break
+        run = np.add.accumulate(probs)             # the sequential running sum from 0.0
+        threshold = draw() * run[-1]
+        out = min(int(np.searchsorted(run, threshold, side="left")), dim - 1)
         counts[out] = counts.get(out, 0) + 1
         hist.append(list(reg))
     return counts, hist

@@ -107,3 +107,17 @@ def test_device_trajectories_match_fixture_and_batch_split(monkeypatch):
     res = QarpSimulator().run(cmds, n, int(d["shots"]), seed=int(d["shot_seed"]))
     assert res.counts == dict(zip(d["keys"].tolist(), d["counts"].tolist()))
     assert np.array_equal(np.array(res.cbit_history, dtype=bool), d["history"])
+
+
+@pytest.mark.gpu
+def test_wide_shots_take_the_exact_chunk_walk():
+    """21-qubit shots (2^21 amplitudes each, 8 per batch): the final draw of
+    every shot runs through the terminal sampler's exact chunk scan and
+    equals the oracle's sequential walk."""
+    from openqarp_amd.simulator import QarpSimulator
+    n = 21
+    cmds = circuits(n, 4)["teleport"]
+    res = QarpSimulator().run(cmds, n, 12, seed=31)
+    counts, hist = O.run_trajectory(cmds, n, 12, 31)
+    assert res.counts == counts
+    assert res.cbit_history == hist
