@@ -285,7 +285,8 @@ def structured(sim, state, n):
     """Structured-program kernels on the apply leg's state: basis
     permutations of the 20 lowest qubits (gathered reads inside 2^20-amplitude
     blocks) and of the 10 highest (whole 16-MiB runs move), and QPE-style
-    controlled powers (8 controls with exponents 2^j, 2 targets).  Bytes per
+    controlled powers (8 controls with exponents 2^j, 2 targets; one control,
+    10 targets through the controlled dense kernel).  Bytes per
     permutation: one out-of-place pass, 2 x 16 B x 2^n (the state takes the
     permuted buffer)."""
     from openqarp_amd import _lib
@@ -315,6 +316,19 @@ def structured(sim, state, n):
     ms = (time.perf_counter() - t0) * 1e3
     out["controlled_powers_8x2"] = {"ms": round(ms, 3)}
     log(f"controlled powers (8 controls, 2 targets): {ms:.2f} ms")
+    # Ten targets: wider than one tile-pass op, so the controlled dense
+    # kernel runs; 8 flops per complex multiply-add, 2^(n-1) x 2^10 of them.
+    p = Program()
+    p.add_controlled_powers([n - 1], [1], list(range(10)), haar_unitary(1 << 10, rng))
+    sim.execute_program(p, state)
+    _lib.check(_lib.load().qhip_synchronize())
+    t0 = time.perf_counter()
+    sim.execute_program(p, state)
+    _lib.check(_lib.load().qhip_synchronize())
+    ms = (time.perf_counter() - t0) * 1e3
+    tf = 8.0 * 2.0 ** (n - 1) * 1024 / (ms * 1e-3) / 1e12
+    out["controlled_power_1x10"] = {"ms": round(ms, 3), "tflops_f64": round(tf, 2)}
+    log(f"controlled power (1 control, 10 targets): {ms:.2f} ms = {tf:.1f} TFLOP/s f64")
     return out
 
 

@@ -3,7 +3,8 @@
 //  - a basis permutation |x> -> |table[x]> of a qubit subset, applied out of
 //    place through a scratch state (apply_permutation, program.cpp:610-646);
 //  - controlled matrix powers matrix^e_j on the targets wherever control j
-//    is |1>, lowered to controlled dense ops of the ordinary tile pass
+//    is |1>, lowered to controlled dense ops of the ordinary tile pass on up
+//    to 7 targets, and run by a controlled dense kernel on 8..12
 //    (apply_controlled_powers, program.cpp:648-712).
 #include "common.h"
 
@@ -116,6 +117,125 @@ Mat matmul(const Mat& a, const Mat& b, size_t d) {
     return c;
 }
 
+// ── controlled powers on 8..12 targets ─────────────────────────────────────
+// Too wide for one tile-pass op (targets + control > 8 operands): per
+// control, a kernel applies P on the targets of every group whose control
+// bit is set (apply_controlled_powers, program.cpp:676-706).  A group is the
+// 2^m amplitudes base | deposit(l); the groups with the control set are
+// enumerated directly (zero bits inserted at the sorted targets and control,
+// then the control bit set), so no group is visited and skipped.
+constexpr int kMaxDenseTargets = 12;    // 2^12 amplitudes per group
+constexpr int kCtlBlock = 256;
+constexpr int kCtlAmps = 4096;          // LDS amplitudes per block: G groups x d rows
+
+struct CtlGeom {
+    uint8_t tpos[kMaxDenseTargets];      // local bit b <-> global bit tpos[b]
+    uint8_t ins[kMaxDenseTargets + 1];   // targets and control, ascending
+};
+
+__device__ __forceinline__ double2 cmac(double2 acc, double2 u, double2 x) {
+    acc.x = __fma_rn(u.x, x.x, acc.x);
+    acc.x = __fma_rn(-u.y, x.y, acc.x);
+    acc.y = __fma_rn(u.x, x.y, acc.y);
+    acc.y = __fma_rn(u.y, x.x, acc.y);
+    return acc;
+}
+__device__ __forceinline__ float2 cmac(float2 acc, float2 u, float2 x) {
+    acc.x = __fmaf_rn(u.x, x.x, acc.x);
+    acc.x = __fmaf_rn(-u.y, x.y, acc.x);
+    acc.y = __fmaf_rn(u.x, x.y, acc.y);
+    acc.y = __fmaf_rn(u.y, x.x, acc.y);
+    return acc;
+}
+
+// d = 256 R rows (m = 8 + log2 R); each thread owns rows threadIdx.x + 256 r
+// of G = 4096 / d groups staged in LDS.  PT is P transposed (PT[k d + row]):
+// for a fixed k the block reads one contiguous row of PT, and every group's
+// amplitude k is an LDS broadcast.  PT (d^2 entries, at most 256 MiB) stays
+// in L2 / HBM and is read once per G groups.
+template <typename A, int R>
+__global__ __launch_bounds__(kCtlBlock) void controlled_dense_kernel(A* __restrict__ s, const A* __restrict__ PT,
+                                                                     const CtlGeom geo, int m, uint64_t cbit,
+                                                                     uint64_t groups) {
+    constexpr int d = kCtlBlock * R;
+    constexpr int G = kCtlAmps / d;
+    __shared__ A col[kCtlAmps];
+    uint64_t off[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const uint32_t l = threadIdx.x + kCtlBlock * r;
+        uint64_t o = 0;
+        for (int b = 0; b < m; ++b) o |= static_cast<uint64_t>((l >> b) & 1u) << geo.tpos[b];
+        off[r] = o;
+    }
+    auto group_base = [&](uint64_t gi) {
+        uint64_t base = gi;
+        for (int b = 0; b <= m; ++b) {
+            const uint64_t p = geo.ins[b];
+            base = ((base >> p) << (p + 1)) | (base & ((uint64_t{1} << p) - 1u));
+        }
+        return base | cbit;
+    };
+    const uint64_t supers = (groups + G - 1) / G;
+    for (uint64_t sg = blockIdx.x; sg < supers; sg += gridDim.x) {
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const uint64_t gi = sg * G + g;
+            if (gi < groups) {
+                const uint64_t base = group_base(gi);
+#pragma unroll
+                for (int r = 0; r < R; ++r) col[g * d + threadIdx.x + kCtlBlock * r] = s[base | off[r]];
+            }
+        }
+        __syncthreads();
+        A acc[R][G];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int g = 0; g < G; ++g) { acc[r][g].x = 0; acc[r][g].y = 0; }
+#pragma unroll 2
+        for (int k = 0; k < d; ++k) {
+            A p[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) p[r] = PT[static_cast<size_t>(k) * d + threadIdx.x + kCtlBlock * r];
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const A c = col[g * d + k];
+#pragma unroll
+                for (int r = 0; r < R; ++r) acc[r][g] = cmac(acc[r][g], p[r], c);
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const uint64_t gi = sg * G + g;
+            if (gi < groups) {
+                const uint64_t base = group_base(gi);
+#pragma unroll
+                for (int r = 0; r < R; ++r) s[base | off[r]] = acc[r][g];
+            }
+        }
+        __syncthreads();   // col is rewritten by the next super-group
+    }
+}
+
+template <typename A>
+void launch_controlled_dense(qhip_state* s, const void* PT, const CtlGeom& geo, int m, uint64_t cbit) {
+    auto& C = ctx();
+    const uint64_t groups = uint64_t{1} << (s->n - m - 1);
+    const uint64_t supers = (groups + (kCtlAmps >> m) - 1) / (kCtlAmps >> m);
+    const int blocks = static_cast<int>(std::min<uint64_t>(supers, static_cast<uint64_t>(C.n_cu) * 4));
+    auto* st = static_cast<A*>(s->d);
+    auto* pt = static_cast<const A*>(PT);
+    switch (m) {
+        case 8: hipLaunchKernelGGL((controlled_dense_kernel<A, 1>), dim3(blocks), dim3(kCtlBlock), 0, C.stream, st, pt, geo, m, cbit, groups); break;
+        case 9: hipLaunchKernelGGL((controlled_dense_kernel<A, 2>), dim3(blocks), dim3(kCtlBlock), 0, C.stream, st, pt, geo, m, cbit, groups); break;
+        case 10: hipLaunchKernelGGL((controlled_dense_kernel<A, 4>), dim3(blocks), dim3(kCtlBlock), 0, C.stream, st, pt, geo, m, cbit, groups); break;
+        case 11: hipLaunchKernelGGL((controlled_dense_kernel<A, 8>), dim3(blocks), dim3(kCtlBlock), 0, C.stream, st, pt, geo, m, cbit, groups); break;
+        default: hipLaunchKernelGGL((controlled_dense_kernel<A, 16>), dim3(blocks), dim3(kCtlBlock), 0, C.stream, st, pt, geo, m, cbit, groups); break;
+    }
+    QHIP_CHECK(hipGetLastError());
+}
+
 void check_distinct(std::vector<uint32_t> q, const char* what) {
     std::sort(q.begin(), q.end());
     if (std::adjacent_find(q.begin(), q.end()) != q.end())
@@ -202,9 +322,9 @@ void controlled_powers(const qhip_config* cfg, qhip_state* s, const uint32_t* co
         if (q >= static_cast<uint32_t>(s->n))
             fail(QHIP_INVALID, "qhip_state_controlled_powers: qubit " + std::to_string(q) + " outside the " +
                                    std::to_string(s->n) + "-qubit register");
-    if (m > kMaxPowerTargets)
-        fail(QHIP_CAPABILITY, "qhip_state_controlled_powers: at most " + std::to_string(kMaxPowerTargets) +
-                                  " targets (a controlled power runs as one dense op on targets + control)");
+    if (m > kMaxDenseTargets)
+        fail(QHIP_CAPABILITY, "qhip_state_controlled_powers: at most " + std::to_string(kMaxDenseTargets) +
+                                  " targets (one group of 2^m amplitudes is staged in LDS)");
     const size_t d = size_t{1} << m;
     Mat M(d * d);
     for (size_t i = 0; i < d * d; ++i) M[i] = cplx(matrix[2 * i], matrix[2 * i + 1]);
@@ -225,6 +345,43 @@ void controlled_powers(const qhip_config* cfg, qhip_state* s, const uint32_t* co
         for (uint64_t x = e & (e - 1); x != 0; x &= x - 1)
             r = matmul(r, squares[static_cast<size_t>(std::countr_zero(x))], d);
         products.emplace(e, std::move(r));
+    }
+
+    if (m > kMaxPowerTargets) {
+        // Per control, the controlled dense kernel with P transposed in the
+        // state's precision; the stream is drained before the host buffer is
+        // refilled for the next control.
+        auto& C = ctx();
+        CtlGeom geo{};
+        std::vector<uint32_t> ins(targets, targets + m);
+        for (int b = 0; b < m; ++b) geo.tpos[b] = static_cast<uint8_t>(targets[b]);
+        const bool c128 = s->precision == QHIP_C128;
+        void* dPT = C.misc2.ensure(d * d * s->amp_bytes());
+        std::vector<double> h64(c128 ? 2 * d * d : 0);
+        std::vector<float> h32(c128 ? 0 : 2 * d * d);
+        for (int j = 0; j < n_controls; ++j) {
+            const uint64_t e = exponents[j];
+            if (e == 0) continue;
+            const Mat& P = std::has_single_bit(e) ? squares[static_cast<size_t>(std::countr_zero(e))] : products.at(e);
+            for (size_t row = 0; row < d; ++row)
+                for (size_t k = 0; k < d; ++k) {
+                    const cplx v = P[row * d + k];
+                    const size_t t = 2 * (k * d + row);
+                    if (c128) { h64[t] = v.real(); h64[t + 1] = v.imag(); }
+                    else { h32[t] = static_cast<float>(v.real()); h32[t + 1] = static_cast<float>(v.imag()); }
+                }
+            std::vector<uint32_t> srt(ins);
+            srt.push_back(controls[j]);
+            std::sort(srt.begin(), srt.end());
+            for (int b = 0; b <= m; ++b) geo.ins[b] = static_cast<uint8_t>(srt[b]);
+            QHIP_CHECK(hipMemcpyAsync(dPT, c128 ? static_cast<const void*>(h64.data()) : h32.data(),
+                                      d * d * s->amp_bytes(), hipMemcpyHostToDevice, C.stream));
+            const uint64_t cbit = uint64_t{1} << controls[j];
+            if (c128) launch_controlled_dense<double2>(s, dPT, geo, m, cbit);
+            else launch_controlled_dense<float2>(s, dPT, geo, m, cbit);
+            QHIP_CHECK(hipStreamSynchronize(C.stream));
+        }
+        return;
     }
 
     // One Custom op per control with a non-zero exponent: operands

@@ -184,7 +184,7 @@ def test_program_statevector_matches_reference_fixture(precision):
 def test_program_kernels_against_oracle(seed):
     """Every kernel kind at 14 qubits, tiles of 2^9 amplitudes (several
     passes): permutations over subsets and the whole register, dense blocks
-    of 1..8 qubits, controlled powers on 1..7 targets."""
+    of 1..8 qubits, controlled powers on 1..7 targets (tile-pass ops)."""
     rng = np.random.default_rng(200 + seed)
     n = 14
     p = Program()
@@ -278,7 +278,38 @@ def test_program_errors():
     q.add_permutation([1], [1, 0])
     with pytest.raises(ValueError, match="outside the program's last gates kernel"):
         sim.program_run(q, 2, 10)
-    r = Program()
-    r.add_controlled_powers([8], [1], list(range(8)), np.eye(256))
-    with pytest.raises(CapabilityError):
-        sim.program_statevector(r, 9)
+    # 13 targets: rejected by the library before the matrix is read.
+    import ctypes as C
+    from openqarp_amd import _lib
+    from openqarp_amd.simulator import DeviceState
+    st = DeviceState(14, "complex128")
+    cs, es, ts = np.array([13], np.uint32), np.array([1], np.uint64), np.arange(13, dtype=np.uint32)
+    cfg = sim._config()
+    with pytest.raises(CapabilityError, match="at most 12 targets"):
+        _lib.check(_lib.load().qhip_state_controlled_powers(C.byref(cfg), st.handle, cs.ctypes.data, es.ctypes.data,
+                                                            1, ts.ctypes.data, 13, np.zeros(2).ctypes.data))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("m,n,precision,exps", [
+    (8, 12, "complex128", [5, 0, 6, 1]), (9, 12, "complex64", [3, 1]), (10, 13, "complex128", [1]),
+    (11, 13, "complex64", [1]), (12, 13, "complex128", [1])])
+def test_wide_controlled_powers_against_oracle(m, n, precision, exps):
+    """Controlled powers on 8..12 targets (wider than one tile-pass op) run
+    in the controlled dense kernel: scattered targets and controls, zero
+    exponents skipped, products of squares (m = 8, 9), against the oracle."""
+    rng = np.random.default_rng(40 + m)
+    qs = [int(q) for q in rng.permutation(n)[: m + len(exps)]]
+    d = 1 << m
+    if m <= 10:
+        M = haar(d, rng)
+    else:   # any square matrix is a valid operand; Gaussian rows are cheap at this size
+        M = (rng.standard_normal((d, d)) + 1j * rng.standard_normal((d, d))) / np.sqrt(2 * d)
+    p = Program()
+    p.add_gates(random_circuit(n, 20, m))
+    p.add_controlled_powers(qs[m:], exps, qs[:m], M)
+    init = random_state(n, rng)
+    want = O.program_statevector(p.kernels(), n, init=init)
+    got = _sim(precision=precision).program_statevector(p, n, initial_state=init)
+    tol = ATOL if precision == "complex128" else 2e-6 * max(1.0, float(np.abs(want).max()) * np.sqrt(1 << n))
+    assert np.abs(got - want).max() < tol
