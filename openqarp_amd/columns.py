@@ -158,9 +158,14 @@ class TransitionColumnProvider:
             part = np.ascontiguousarray(src[start:start + self._batch])
             self._evaluate(part)
             t4 = time.perf_counter()
-            rows = self._phi.download().reshape(self._batch, self._dim)[:part.size]
-            t5 = time.perf_counter()
-            out[start:start + part.size] = rows[:, self._sector.astype(np.int64)] if gather else rows
+            if not gather and part.size == self._batch:
+                # A full batch lands in its output rows directly.
+                self._phi.download(out[start:start + part.size].reshape(-1))
+                t5 = time.perf_counter()
+            else:
+                rows = self._phi.download().reshape(self._batch, self._dim)[:part.size]
+                t5 = time.perf_counter()
+                out[start:start + part.size] = rows[:, self._sector.astype(np.int64)] if gather else rows
             t6 = time.perf_counter()
             d["transfer_wall_seconds"] += t5 - t4
             d["sector_gather_wall_seconds"] += t6 - t5 if gather else 0.0
@@ -177,38 +182,43 @@ class TransitionColumnProvider:
 
     def _compact_on_device(self, src: np.ndarray, tol: float):
         """Full columns compacted on the device (qhip_compact): only the kept
-        entries cross to the host."""
+        entries cross to the host, straight into output arrays sized for
+        every entry (only the used prefix is touched).  Per batch: one
+        qhip_compact call, row offsets from the ascending keys by binary
+        search, labels masked in place.  Non-finite entries: the batch's
+        FP64 norm is finite whenever every amplitude is (short of overflow);
+        otherwise the batch is downloaded and checked entry by entry, pruned
+        entries included, as the reference's loop does
+        (transition_columns.cu:266-276)."""
         lib = _lib.load()
         d = self._diag
-        keys_all, vals_all = [], []
+        keys = np.empty(src.size * self._dim, dtype=np.int64)
+        vals = np.empty(src.size * self._dim, dtype=np.complex128)
+        offsets = np.zeros(src.size + 1, dtype=np.uint64)
+        pos = 0
         for start in range(0, src.size, self._batch):
             part = np.ascontiguousarray(src[start:start + self._batch])
             self._evaluate(part)
             t = time.perf_counter()
+            if not math.isfinite(self._phi.norm2()):
+                rows = self._phi.download()[:part.size * self._dim]
+                if not np.isfinite(np.hypot(rows.real, rows.imag)).all():
+                    raise RuntimeError("Exact column contains a nonfinite coefficient")
             n_amps = part.size * self._dim
             cnt = C.c_int64()
-            _lib.check(lib.qhip_compact(self._phi.handle, n_amps, tol, None, None, 0, C.byref(cnt)))
-            keys = np.zeros(cnt.value, dtype=np.int64)
-            vals = np.zeros(cnt.value, dtype=np.complex128)
-            if cnt.value:
-                _lib.check(lib.qhip_compact(self._phi.handle, n_amps, tol, keys.ctypes.data, vals.ctypes.data,
-                                            cnt.value, C.byref(cnt)))
-            keys += start * self._dim
-            keys_all.append(keys)
-            vals_all.append(vals)
+            _lib.check(lib.qhip_compact(self._phi.handle, n_amps, tol, keys[pos:].ctypes.data,
+                                        vals[pos:].ctypes.data, n_amps, C.byref(cnt)))
+            k = keys[pos:pos + cnt.value]
+            ends = np.arange(1, part.size + 1, dtype=np.int64) << self._n
+            offsets[start + 1:start + part.size + 1] = pos + np.searchsorted(k, ends)
+            np.bitwise_and(k, self._dim - 1, out=k)
+            pos += cnt.value
             d["transferred_bytes"] += 24 * cnt.value
             d["compaction_wall_seconds"] += time.perf_counter() - t
-        keys = np.concatenate(keys_all)
-        vals = np.concatenate(vals_all)
-        if not np.all(np.isfinite(vals)):
-            raise RuntimeError("Exact column contains a nonfinite coefficient")
-        row = keys >> self._n
-        offsets = np.zeros(src.size + 1, dtype=np.uint64)
-        offsets[1:] = np.cumsum(np.bincount(row, minlength=src.size))
         d["columns"] += int(src.size)
-        d["compact_entries"] += int(keys.size)
-        d["pruned_entries"] += int(src.size) * self._dim - int(keys.size)
-        return offsets, (keys & (self._dim - 1)).astype(np.uint64), vals
+        d["compact_entries"] += pos
+        d["pruned_entries"] += int(src.size) * self._dim - pos
+        return offsets, keys[:pos].view(np.uint64), vals[:pos]
 
     def compact_columns(self, source_labels, absolute_tolerance: float):
         """(offsets, labels, values): per source the entries of magnitude above

@@ -80,8 +80,13 @@ class DeviceState:
         _lib.check(_lib.load().qhip_state_upload_c128(self._h, a.ctypes.data, a.size))
         return self
 
-    def download(self) -> np.ndarray:
-        out = np.empty(self.dim, dtype=np.complex128)
+    def download(self, out: Optional[np.ndarray] = None) -> np.ndarray:
+        """The amplitudes as complex128, into `out` when given (a C-contiguous
+        complex128 array of 2^n entries, written in place)."""
+        if out is None:
+            out = np.empty(self.dim, dtype=np.complex128)
+        elif out.dtype != np.complex128 or out.size != self.dim or not out.flags.c_contiguous:
+            raise ValueError("DeviceState.download: out must be C-contiguous complex128 with 2^n entries")
         _lib.check(_lib.load().qhip_state_download_c128(self._h, out.ctypes.data, out.size))
         return out
 

@@ -149,3 +149,18 @@ def test_compact_columns_on_device(tol):
         lo, hi = int(offsets[j]), int(offsets[j + 1])
         assert np.array_equal(labels[lo:hi], np.nonzero(keep)[0].astype(np.uint64))
         assert np.abs(values[lo:hi] - want[j][keep]).max() < 1e-12
+
+
+@pytest.mark.gpu
+def test_compaction_rejects_nonfinite_columns():
+    """Two finite coefficients whose sum overflows make the column non-finite
+    (inf, then NaN through U†): the device compaction raises, as the
+    reference does for every entry, pruned ones included."""
+    from openqarp_amd.simulator import QarpSimulator
+    n = 6
+    U, _ = workload(n, 2)
+    obs = [([(0, "Z")], 1e308), ([(1, "Z")], 1e308)]
+    prov = QarpSimulator().make_transition_column_provider(U, n, obs, 2)
+    for tol in (0.0, 1e300):
+        with pytest.raises(RuntimeError, match="nonfinite coefficient"):
+            prov.compact_columns(np.array([0, 5], dtype=np.uint64), tol)
