@@ -465,6 +465,26 @@ def test_uniforms_at_every_cdf_boundary():
     assert (gpu_sim().sample_uniforms(st, u) == O.sample_uniforms(amp, n, u)).all()
 
 
+def test_batch_run_seeds_each_set_with_seed_plus_index():
+    """batch_run substitutes each parameter set and samples it with seed + i
+    in uint32 arithmetic (qarp_simulator.cpp:1257-1266): the counts equal the
+    oracle's sampler on the oracle's statevector with that seed, across the
+    2^32 wrap."""
+    n = 9
+    a, b = Param.symbol("a"), Param.symbol("b")
+    cmds = W.brickwork_haar(n, 3, 5) + [Command(GateType.Rx, 2, params=[a]), Command(GateType.Ry, 6, params=[b]),
+                                        Command(GateType.CX, 2, 6)]
+    sets = [{"a": 0.3, "b": -1.1}, {"a": 2.0, "b": 0.4}, {"a": -0.7, "b": 3.0}]
+    seed, shots = 0xFFFFFFFE, 3000
+    res = gpu_sim().batch_run(cmds + [Command(GateType.Measure, q, cbits=[q]) for q in range(n)], n, shots,
+                              sets, seed=seed)
+    assert len(res) == len(sets)
+    for i, (r, ps) in enumerate(zip(res, sets)):
+        sv = O.statevector(pack_stream([c.substitute(ps) for c in cmds]), n, 3, 12)
+        vals, cnt = np.unique(O.sample(sv, n, shots, (seed + i) & 0xFFFFFFFF), return_counts=True)
+        assert r.counts == {int(v): int(c) for v, c in zip(vals, cnt)}, i
+
+
 def test_run_counts_and_cbit_history():
     f = F.load("hea12.npz")
     n = int(f["n"])
