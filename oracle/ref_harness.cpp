@@ -138,6 +138,35 @@ int ref_expectation(const qhip_stream* s, int n, int fusion_max, int fusion_min,
     } catch (const std::exception& e) { return catch_all(e, QHIP_RUNTIME); }
 }
 
+// QarpSimulator::statevector followed by pauli_transition on the result,
+// reduced to a fingerprint so full-size states (16 GiB at 30 qubits) need no
+// second host copy: the amplitudes at idx[0..n_idx), the FP64 sum of |a|^2 in
+// index order, <Z_q> for every qubit (stats[1..n]), and <obs> (stats[n+1],
+// skipped when obs is NULL).  stats[0] is the norm.
+int ref_statevector_fingerprint(const qhip_stream* s, int n, int fusion_max, int fusion_min,
+                                const uint64_t* idx, int64_t n_idx, double* amps,
+                                const qhip_observable* obs, double* stats) {
+    try {
+        auto sim = make_sim(fusion_max, fusion_min);
+        const auto sv = sim.statevector(to_commands(s), n);
+        for (int64_t j = 0; j < n_idx; ++j) {
+            amps[2 * j] = sv[idx[j]].real();
+            amps[2 * j + 1] = sv[idx[j]].imag();
+        }
+        const uint64_t dim = uint64_t{1} << n;
+        std::vector<double> z(static_cast<size_t>(n) + 1, 0.0);
+        for (uint64_t i = 0; i < dim; ++i) {
+            const double p = std::norm(sv[i]);
+            z[0] += p;
+            for (int q = 0; q < n; ++q) z[q + 1] += ((i >> q) & 1) ? -p : p;
+        }
+        for (int q = 0; q <= n; ++q) stats[q] = z[q];
+        if (obs) stats[n + 1] = pauli_transition(sv.data(), sv.data(), n, to_observable(obs)).real();
+        return 0;
+    } catch (const std::invalid_argument& e) { return catch_all(e, QHIP_INVALID);
+    } catch (const std::exception& e) { return catch_all(e, QHIP_RUNTIME); }
+}
+
 // pauli_transition
 int ref_transition(const double* bra, const double* ket, int n, const qhip_observable* obs,
                    double out[2]) {

@@ -77,6 +77,7 @@ def reference():
         lib.ref_run_gradient_phi.argtypes = [vp, i32, vp, i32, vp, vp, vp]
         lib.ref_program_statevector.argtypes = [vp, i32, i32, i32, i32, vp, vp]
         lib.ref_run_trajectory.argtypes = [vp, vp, vp, vp, i32, i32, C.c_uint32, i64, vp, vp, vp, vp, vp, i64]
+        lib.ref_statevector_fingerprint.argtypes = [vp, i32, i32, i32, vp, i64, vp, vp, vp]
         _ref = lib
     return _ref
 
@@ -102,6 +103,30 @@ def statevector(packed, n, fusion_max=3, fusion_min=12, init=None, lib="orc"):
     fn = L.orc_statevector if lib == "orc" else L.ref_statevector
     _check(L, fn(packed.ref, n, fusion_max, fusion_min, _cptr(init), out.ctypes.data), lib)
     return out
+
+
+def fingerprint_indices(n, count, seed):
+    """Seeded distinct amplitude indices (ascending) for a state fingerprint;
+    index 0 is always included."""
+    rng = np.random.default_rng(seed)
+    dim = 1 << n
+    if count >= dim:
+        return np.arange(dim, dtype=np.uint64)
+    idx = np.unique(rng.integers(0, dim, size=count, dtype=np.uint64))
+    return np.union1d(idx, np.array([0], np.uint64)).astype(np.uint64)
+
+
+def ref_fingerprint(packed, n, fusion_max, fusion_min, idx, pobs=None):
+    """Reference statevector reduced to (amplitudes at idx, norm, <Z_q> for
+    every q, <obs> or None) without a host copy of the state."""
+    idx = np.ascontiguousarray(idx, dtype=np.uint64)
+    amps = np.zeros(idx.size, dtype=np.complex128)
+    stats = np.zeros(n + 2)
+    L = reference()
+    _check(L, L.ref_statevector_fingerprint(packed.ref, n, fusion_max, fusion_min, idx.ctypes.data,
+                                            idx.size, amps.ctypes.data,
+                                            None if pobs is None else pobs.ref, stats.ctypes.data), "ref")
+    return amps, float(stats[0]), stats[1:n + 1].copy(), (None if pobs is None else float(stats[n + 1]))
 
 
 def transition(bra, ket, n, pobs, lib="orc"):

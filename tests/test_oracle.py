@@ -428,3 +428,87 @@ def test_qaoa_fixture_state_energy_and_gradient():
     assert np.abs(sv - f["state"]).max() < 1e-12
     assert O.transition(sv, sv, n, po).real == pytest.approx(float(f["energy"]), abs=1e-12)
     assert np.abs(O.run_gradient(ps, n, po, len(idx)) - f["grad"]).max() < 1e-12
+
+
+def _generator_circuit():
+    """Every differentiable gate type once, with affine and compound angles."""
+    t = [Param.symbol(f"t{i}") for i in range(12)]
+    cmds = [Command(GateType.H, 0), Command(GateType.H, 1), Command(GateType.H, 2),
+            Command(GateType.Rx, 0, t[0]), Command(GateType.Ry, 1, t[1] * 2.0 + 0.3),
+            Command(GateType.Rz, 2, t[2]), Command(GateType.P, 0, t[3]),
+            Command(GateType.CRx, 0, 1, t[4]), Command(GateType.CRy, 1, 2, t[5]),
+            Command(GateType.CRz, 2, 0, t[6]), Command(GateType.CP, 0, 2, t[7]),
+            Command(GateType.RXX, 0, 1, t[8]), Command(GateType.RYY, 1, 2, t[9]),
+            Command(GateType.RZZ, 0, 2, t[10] - t[0]), Command(GateType.GPhase, params=[t[11]]),
+            Command(GateType.S, 1), Command(GateType.SX, 2), Command(GateType.iSWAP, 0, 1)]
+    return cmds, [f"t{i}" for i in range(12)]
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_gradient_phi_is_the_derivative_of_2re_phi_psi(seed):
+    """run_gradient_phi with a fixed |phi> returns d/dtheta 2 Re<phi|psi(theta)>
+    (run_gradient_core with the caller's phi, qarp_simulator.cpp:1626-1639):
+    central differences of that closed form, from |0> and from an initial
+    state; the same function equals the reference build where present."""
+    n = 3
+    cmds, names = _generator_circuit()
+    rng = np.random.default_rng(seed)
+    vals = dict(zip(names, rng.uniform(-2, 2, len(names))))
+    index = {s: i for i, s in enumerate(names)}
+    phi = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
+    init = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
+    init /= np.linalg.norm(init)
+    for start in (None, init):
+        g = O.run_gradient_phi(pack_stream(cmds, vals, index), n, phi, len(names), init=start)
+
+        def f(v):
+            sv = O.statevector(pack_stream([c.substitute(v) for c in cmds]), n, 0, 0, init=start)
+            return 2.0 * np.vdot(phi, sv).real
+
+        h = 1e-6
+        fd = []
+        for s in names:
+            vp = dict(vals); vp[s] += h
+            vm = dict(vals); vm[s] -= h
+            fd.append((f(vp) - f(vm)) / (2 * h))
+        assert np.abs(g - np.array(fd)).max() < 1e-7
+        if REF:
+            gr = O.run_gradient_phi(pack_stream(cmds, vals, index), n, phi, len(names),
+                                    values=np.array([vals[s] for s in names]), init=start, lib="ref")
+            assert np.abs(g - gr).max() < 1e-12
+
+
+@needs_ref
+def test_gradient_phi_matches_reference_build_on_random_ansatz():
+    """A 9-qubit, 2-layer RY/RZ + CZ ansatz with phi = <bra|psi> |bra> (the
+    OVERLAP adjoint's phi, _gradients.py:522-544): oracle equals the reference."""
+    n = 9
+    w = W.c3_ansatz(n, 2, 17)
+    index = {s: i for i, s in enumerate(w.symbols)}
+    ps = pack_stream(w.commands, w.values, index)
+    rng = np.random.default_rng(5)
+    bra = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
+    bra /= np.linalg.norm(bra)
+    psi = O.statevector(pack_stream([c.substitute(w.values) for c in w.commands]), n, 0, 0)
+    phi = np.vdot(bra, psi) * bra
+    vals = np.array([w.values[s] for s in w.symbols])
+    g = O.run_gradient_phi(ps, n, phi, len(vals))
+    gr = O.run_gradient_phi(ps, n, phi, len(vals), values=vals, lib="ref")
+    assert np.abs(g - gr).max() < 1e-12
+
+
+def test_c1_full_size_fixture_against_oracle():
+    """The 20-qubit HEA fixture (reference build, full size): the oracle's
+    state at the fingerprint indices, <H>, <Z_q> and the 4000 shots."""
+    f = F.load("c1_hea20.npz")
+    n = int(f["n"])
+    w = W.c1_hea(n, int(f["layers"]), int(f["seed"]))
+    sv = O.statevector(pack_stream(w.commands), n, 3, 12)
+    assert np.abs(sv[f["idx"].astype(np.int64)] - f["amps"]).max() < 1e-12
+    p = np.abs(sv) ** 2
+    bits = np.arange(1 << n)
+    zq = np.array([p[(bits >> q) & 1 == 0].sum() - p[(bits >> q) & 1 == 1].sum() for q in range(n)])
+    assert np.abs(zq - f["zq"]).max() < 1e-12
+    assert O.transition(sv, sv, n, pack_observable(w.observable)).real == pytest.approx(float(f["energy"]),
+                                                                                         abs=1e-12)
+    assert np.array_equal(O.sample(sv, n, int(f["shots"]), int(f["shot_seed"])), f["samples"])
