@@ -321,6 +321,18 @@ int qhip_apply_terminal(const qhip_config* cfg, const qhip_stream* stream, qhip_
 int qhip_transition(const qhip_state* bra, const qhip_state* ket,
                     const qhip_observable* obs, double out[2]);
 
+/* qhip_transition with no device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * group table and reduction partials live in a fixed 725,248-byte device
+ * array per GPU (held until the process ends), so the observable may have at
+ * most 4096 distinct X/Y flip masks and 16384 terms (QHIP_INVALID beyond).
+ * Same arguments, errors and result (bit for bit) as qhip_transition.  Calls
+ * share the array: the library serialises its own calls; concurrent use on one
+ * GPU from outside that lock races. */
+int qhip_transition_noalloc(const qhip_state* bra, const qhip_state* ket,
+                            const qhip_observable* obs, double out[2]);
+
 /* |out> = H |in> (apply_observable, qarp_simulator.cpp:1459-1479). */
 int qhip_apply_observable(const qhip_state* in, const qhip_observable* obs, qhip_state* out);
 

@@ -27,7 +27,7 @@ EXPORTS = [
     "qhip_apply", "qhip_apply_terminal", "qhip_state_permute", "qhip_state_controlled_powers",
     "qhip_state_set_basis", "qhip_compact", "qhip_shots_broadcast", "qhip_shots_restore", "qhip_shots_prob0",
     "qhip_shots_project", "qhip_shots_sample", "qhip_mt_uniforms",
-    "qhip_transition", "qhip_apply_observable",
+    "qhip_transition", "qhip_transition_noalloc", "qhip_apply_observable",
     "qhip_sample", "qhip_sample_uniforms", "qhip_probabilities", "qhip_exact",
     "qhip_run_gradient", "qhip_run_gradient_phi", "qhip_batch_energy_grad",
     "qhip_batch_create", "qhip_batch_run", "qhip_batch_update", "qhip_batch_destroy",
@@ -76,6 +76,7 @@ def _declare(lib):
         "qhip_mt_uniforms": [u32, i64, vp],
         "qhip_state_controlled_powers": [vp, vp, vp, vp, i32, vp, i32, vp],
         "qhip_transition": [vp, vp, vp, vp],
+        "qhip_transition_noalloc": [vp, vp, vp, vp],
         "qhip_apply_observable": [vp, vp, vp],
         "qhip_sample": [vp, i64, u32, vp],
         "qhip_sample_uniforms": [vp, vp, i64, vp],

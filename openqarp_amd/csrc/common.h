@@ -299,6 +299,9 @@ void state_convert_download(const qhip_state* s, double* host, uint64_t n);
 
 struct PauliTables;   // grouped observable on the device
 cplx transition(const qhip_state* bra, const qhip_state* ket, const qhip_observable* obs);
+// transition() with its group table and partials in a fixed module-scope
+// device array (no device allocation); bit-equal to transition().
+cplx transition_noalloc(const qhip_state* bra, const qhip_state* ket, const qhip_observable* obs);
 void apply_observable(const qhip_state* in, const qhip_observable* obs, qhip_state* out);
 cplx inner(const qhip_state* a, const qhip_state* b);   // <a|b> in FP64
 // Grouped observables of many instances in one device allocation, and

@@ -15,6 +15,7 @@
 // per-chunk partials in index order — the result does not depend on timing.
 #include "common.h"
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <type_traits>
@@ -128,9 +129,9 @@ __global__ __launch_bounds__(kT) void transition_kernel(const A* __restrict__ br
                                                         uint64_t dim, const GroupDev* __restrict__ groups,
                                                         int n_groups, const uint64_t* __restrict__ z,
                                                         const double2* __restrict__ c,
-                                                        double2* __restrict__ partials) {
+                                                        double2* __restrict__ partials, uint64_t chunk0) {
     __shared__ double2 red[kT / 64];
-    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kChunk;
+    const uint64_t base = (chunk0 + blockIdx.x) * kChunk;
     double2 k[kPer];
     double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
@@ -266,17 +267,135 @@ cplx transition(const qhip_state* bra, const qhip_state* ket, const qhip_observa
         if (ket->precision == QHIP_C128)
             hipLaunchKernelGGL((transition_kernel<double2, H>), dim3(static_cast<uint32_t>(blocks)), dim3(kT), 0,
                                C.stream, static_cast<const double2*>(bra->d), static_cast<const double2*>(ket->d), dim,
-                               dg.g, dg.n_groups, dg.z, dg.c, part);
+                               dg.g, dg.n_groups, dg.z, dg.c, part, uint64_t{0});
         else
             hipLaunchKernelGGL((transition_kernel<float2, H>), dim3(static_cast<uint32_t>(blocks)), dim3(kT), 0,
                                C.stream, static_cast<const float2*>(bra->d), static_cast<const float2*>(ket->d), dim,
-                               dg.g, dg.n_groups, dg.z, dg.c, part);
+                               dg.g, dg.n_groups, dg.z, dg.c, part, uint64_t{0});
     };
     if (herm) go(std::true_type{});
     else go(std::false_type{});
     QHIP_CHECK(hipGetLastError());
     const cplx r = finish(part, blocks);
     return herm ? cplx(r.real(), 0.0) : r;
+}
+
+// ── <bra|H|ket> with no device allocation ───────────────────────────────────
+// For callers that cannot allocate (graph capture, serving loops): the group
+// table and the reduction partials live in a fixed module-scope device array.
+// The ket runs in slices of kSliceChunks chunks; after each slice
+// accumulate_partials adds its chunk partials into 256 running sums, thread t
+// taking chunks t, t + 256, ... in increasing order — the additions
+// sum_partials performs over all chunks at once — so the result is bit-equal
+// to transition().  Every call in the process shares the array (the library
+// lock serialises them); callers that bypass that lock on one GPU race.
+
+constexpr int64_t kSliceChunks = 16384;              // multiple of kT
+constexpr int kNoAllocMaxGroups = 4096;
+constexpr int kNoAllocMaxTerms = 16384;
+constexpr size_t kNaGroupsOff = 0;
+constexpr size_t kNaZOff = kNaGroupsOff + sizeof(GroupDev) * kNoAllocMaxGroups;
+constexpr size_t kNaCOff = kNaZOff + sizeof(uint64_t) * kNoAllocMaxTerms;
+constexpr size_t kNaPartOff = kNaCOff + sizeof(double2) * kNoAllocMaxTerms;
+constexpr size_t kNaAccOff = kNaPartOff + sizeof(double2) * kSliceChunks;
+constexpr size_t kNaOutOff = kNaAccOff + sizeof(double2) * kT;
+constexpr size_t kNoAllocBytes = kNaOutOff + 256;    // 725,248 bytes
+static_assert(kSliceChunks % kT == 0, "slices must keep chunk residues mod kT");
+static_assert(kNaPartOff % 256 == 0 && kNaAccOff % 256 == 0 && kNaOutOff % 256 == 0, "aligned sections");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// The array's device address per device, from hipGetSymbolAddress on the
+// first call with that device current (the name itself is a host-side stand-in).
+constexpr int kMaxDevices = 64;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+static unsigned char* noalloc_scratch() {
+    const int dev = ctx().device;
+    if (dev < 0 || dev >= kMaxDevices) fail(QHIP_DEVICE, "qhip_transition_noalloc: device index out of range");
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            fail(QHIP_DEVICE, std::string("qhip_transition_noalloc: hipGetSymbolAddress(scratch) failed: ") +
+                                  hipGetErrorString(e));
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    return static_cast<unsigned char*>(g_scratch_ptr[dev]);
+}
+
+__global__ __launch_bounds__(kT) void accumulate_partials(const double2* __restrict__ in, int64_t n,
+                                                          double2* __restrict__ acc, int first) {
+    double ax = 0.0, ay = 0.0;
+    if (!first) {
+        ax = acc[threadIdx.x].x;
+        ay = acc[threadIdx.x].y;
+    }
+    for (int64_t i = threadIdx.x; i < n; i += kT) { ax += in[i].x; ay += in[i].y; }
+    acc[threadIdx.x] = make_double2(ax, ay);
+}
+
+cplx transition_noalloc(const qhip_state* bra, const qhip_state* ket, const qhip_observable* obs) {
+    const int n = ket->n;
+    const Grouped gr = group_terms(obs, n, "pauli_transition");
+    if (gr.groups.empty()) return cplx(0.0, 0.0);
+    if (gr.groups.size() > static_cast<size_t>(kNoAllocMaxGroups) || gr.z.size() > static_cast<size_t>(kNoAllocMaxTerms))
+        fail(QHIP_INVALID, "qhip_transition_noalloc: the observable has " + std::to_string(gr.groups.size()) +
+                               " flip-mask groups and " + std::to_string(gr.z.size()) +
+                               " terms; the fixed scratch holds at most " + std::to_string(kNoAllocMaxGroups) +
+                               " groups and " + std::to_string(kNoAllocMaxTerms) + " terms");
+    auto& C = ctx();
+    unsigned char* base = noalloc_scratch();
+    auto* d_groups = reinterpret_cast<GroupDev*>(base + kNaGroupsOff);
+    auto* d_z = reinterpret_cast<uint64_t*>(base + kNaZOff);
+    auto* d_c = reinterpret_cast<double2*>(base + kNaCOff);
+    auto* d_part = reinterpret_cast<double2*>(base + kNaPartOff);
+    auto* d_acc = reinterpret_cast<double2*>(base + kNaAccOff);
+    auto* d_out = reinterpret_cast<double2*>(base + kNaOutOff);
+    QHIP_CHECK(hipMemcpyAsync(d_groups, gr.groups.data(), gr.groups.size() * sizeof(GroupDev),
+                              hipMemcpyHostToDevice, C.stream));
+    QHIP_CHECK(hipMemcpyAsync(d_z, gr.z.data(), gr.z.size() * sizeof(uint64_t), hipMemcpyHostToDevice, C.stream));
+    QHIP_CHECK(hipMemcpyAsync(d_c, gr.c.data(), gr.c.size() * sizeof(double2), hipMemcpyHostToDevice, C.stream));
+    const uint64_t dim = ket->dim();
+    const int64_t chunks = static_cast<int64_t>((dim + kChunk - 1) / kChunk);
+    bool herm = bra->d == ket->d;
+    for (int64_t t = 0; herm && t < obs->n_terms; ++t) herm = obs->terms[t].coeff_im == 0.0;
+    const int ng = static_cast<int>(gr.groups.size());
+    for (int64_t c0 = 0; c0 < chunks; c0 += kSliceChunks) {
+        const int64_t nb = std::min<int64_t>(kSliceChunks, chunks - c0);
+        const dim3 grid(static_cast<uint32_t>(nb));
+        const uint64_t off = static_cast<uint64_t>(c0);
+        if (ket->precision == QHIP_C128) {
+            const auto* b = static_cast<const double2*>(bra->d);
+            const auto* k = static_cast<const double2*>(ket->d);
+            if (herm)
+                hipLaunchKernelGGL((transition_kernel<double2, true>), grid, dim3(kT), 0, C.stream, b, k, dim, d_groups,
+                                   ng, d_z, d_c, d_part, off);
+            else
+                hipLaunchKernelGGL((transition_kernel<double2, false>), grid, dim3(kT), 0, C.stream, b, k, dim,
+                                   d_groups, ng, d_z, d_c, d_part, off);
+        } else {
+            const auto* b = static_cast<const float2*>(bra->d);
+            const auto* k = static_cast<const float2*>(ket->d);
+            if (herm)
+                hipLaunchKernelGGL((transition_kernel<float2, true>), grid, dim3(kT), 0, C.stream, b, k, dim, d_groups,
+                                   ng, d_z, d_c, d_part, off);
+            else
+                hipLaunchKernelGGL((transition_kernel<float2, false>), grid, dim3(kT), 0, C.stream, b, k, dim,
+                                   d_groups, ng, d_z, d_c, d_part, off);
+        }
+        hipLaunchKernelGGL(accumulate_partials, dim3(1), dim3(kT), 0, C.stream, d_part, nb, d_acc,
+                           c0 == 0 ? 1 : 0);
+        QHIP_CHECK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(sum_partials, dim3(1), dim3(kT), 0, C.stream, d_acc, static_cast<int64_t>(kT), d_out);
+    QHIP_CHECK(hipGetLastError());
+    double2 r;
+    QHIP_CHECK(hipMemcpyAsync(&r, d_out, sizeof(double2), hipMemcpyDeviceToHost, C.stream));
+    QHIP_CHECK(hipStreamSynchronize(C.stream));
+    return herm ? cplx(r.x, 0.0) : cplx(r.x, r.y);
 }
 
 void apply_observable(const qhip_state* in, const qhip_observable* obs, qhip_state* out) {

@@ -800,6 +800,20 @@ int qhip_transition(const qhip_state* bra, const qhip_state* ket, const qhip_obs
     });
 }
 
+int qhip_transition_noalloc(const qhip_state* bra, const qhip_state* ket, const qhip_observable* obs,
+                            double out[2]) {
+    return guarded([&] {
+        need(bra && ket && obs && out, "qhip_transition_noalloc: NULL argument");
+        if (ket->n < 0 || ket->n > 62)
+            qhip::fail(QHIP_INVALID, "pauli_transition: n_qubits must be in [0, 62], got " + std::to_string(ket->n));
+        need(bra->n == ket->n && bra->precision == ket->precision,
+             "QarpSimulator::transition: bra/ket lengths do not match");
+        const qhip::cplx v = qhip::transition_noalloc(bra, ket, obs);
+        out[0] = v.real();
+        out[1] = v.imag();
+    });
+}
+
 int qhip_apply_observable(const qhip_state* in, const qhip_observable* obs, qhip_state* out) {
     return guarded([&] {
         need(in && obs && out, "qhip_apply_observable: NULL argument");
