@@ -290,32 +290,55 @@ def concrete(value: float) -> Param:
     return Param(value)
 
 
-def affine_coefficients(param: Union[Param, float]) -> dict[str, float]:
+def affine_coefficients(param) -> dict[str, float]:
     """d(param)/d(symbol) for every free symbol (core/param.cpp:462-497).
 
-    Affine parameters answer exactly; compound ones are probed at 0, 1, 2 per
-    symbol and all-ones, as the reference does, and a mismatch raises
+    Affine parameters of this package answer with their coefficients;
+    compound ones and the reference's own Param objects are probed as the
+    reference does: at 0, then 1 and 2 per symbol (coefficient = f(1) - f(0)),
+    then all-ones, each check within 1e-9 of the running scale; an
+    evaluation that fails or is not finite, or a failed check, raises
     CapabilityError."""
-    p = Param._coerce(param)
+    p = as_param(param)
     if p.is_concrete():
         return {}
     if p._fn is None:
         return dict(p._terms)
+
+    def probe(at):
+        try:
+            v = p.evaluate(at)
+        except (RuntimeError, ZeroDivisionError, ValueError, OverflowError) as e:
+            raise CapabilityError(f"parameter {p!r} cannot be evaluated at a probe point ({e}): gradient "
+                                  "rules are exact only for affine gate angles") from None
+        if not math.isfinite(v):
+            raise CapabilityError(f"parameter {p!r} evaluates to a non-finite value at a probe point")
+        return v
+
     syms = p.free_symbols()
-    zero = {s: 0.0 for s in syms}
-    f0 = p.evaluate(zero)
-    coeffs = {}
+    at = {s: 0.0 for s in syms}
+    v0 = probe(at)
+    scale = max(1.0, abs(v0))
+    coeffs, total = {}, 0.0
     for s in syms:
-        v1 = p.evaluate({**zero, s: 1.0})
-        v2 = p.evaluate({**zero, s: 2.0})
-        c = v1 - f0
-        tol = 1e-9 * max(1.0, abs(v2))
-        if abs((v2 - f0) - 2.0 * c) > tol:
-            raise CapabilityError(f"parameter {p!r} is not affine in '{s}'")
+        at[s] = 1.0
+        v1 = probe(at)
+        at[s] = 2.0
+        v2 = probe(at)
+        at[s] = 0.0
+        c = v1 - v0
+        scale = max(scale, abs(v1), abs(v2))
+        if abs((v2 - v0) - 2.0 * c) > 1e-9 * scale:
+            raise CapabilityError(f"parameter {p!r} enters non-linearly in symbol '{s}': gradient rules are "
+                                  "exact only for affine gate angles")
         coeffs[s] = c
-    ones = p.evaluate({s: 1.0 for s in syms})
-    if abs(ones - (f0 + sum(coeffs.values()))) > 1e-9 * max(1.0, abs(ones)):
-        raise CapabilityError(f"parameter {p!r} is not affine")
+        total += c
+    for s in syms:
+        at[s] = 1.0
+    v_all = probe(at)
+    if abs((v_all - v0) - total) > 1e-9 * max(scale, abs(v_all)):
+        raise CapabilityError(f"parameter {p!r} is not affine in its symbols: gradient rules are exact only "
+                              "for affine gate angles")
     return coeffs
 
 
@@ -427,6 +450,62 @@ class Command:
     def __repr__(self) -> str:
         ps = f", params={list(self.params)}" if self.params else ""
         return f"Command({self.gate.name}, qubits={list(self.qubits)}{ps})"
+
+
+# ── the reference's own objects at the boundary ────────────────────────────
+# QarpEngine hands the simulator qx.Command objects (src/qarpx/python/
+# qarpx.pyi:576-620: gate, qubits, params, cbits, condition_bits,
+# condition_values) whose params are qx.Param (:507-555: is_concrete, value,
+# free_symbols, evaluate).  They are read through that public surface only.
+
+def as_param(p) -> Param:
+    """A parameter of this package from a float, a Param, or an object with
+    qx.Param's surface.  A symbolic foreign parameter keeps its own evaluate()
+    (so bound values equal the reference's) and answers affine_coefficients
+    by the reference's probes (core/param.cpp:462-497)."""
+    if isinstance(p, Param):
+        return p
+    if isinstance(p, (float, int, np.floating, np.integer)):
+        return Param(float(p))
+    is_concrete = getattr(p, "is_concrete", None)
+    if is_concrete is None:
+        return Param(float(p))
+    if is_concrete():
+        return Param(float(p.value()))
+    evaluate = p.evaluate
+    return Param.compound(lambda values: float(evaluate(dict(values))), list(p.free_symbols()), repr(p))
+
+
+def _gate_of(g) -> GateType:
+    if isinstance(g, GateType):
+        return g
+    v = getattr(g, "value", g)
+    try:
+        return GateType(int(v))
+    except (TypeError, ValueError):
+        return GateType[str(getattr(g, "name", g))]
+
+
+def as_command(cmd) -> Command:
+    """A Command of this package from one of ours or an object with
+    qx.Command's surface (a ``unitary`` attribute is read when present)."""
+    if isinstance(cmd, Command):
+        return cmd
+    return Command(_gate_of(cmd.gate), qubits=list(cmd.qubits), params=[as_param(p) for p in cmd.params],
+                   cbits=list(getattr(cmd, "cbits", ()) or ()),
+                   condition_bits=list(getattr(cmd, "condition_bits", ()) or ()),
+                   condition_values=list(getattr(cmd, "condition_values", ()) or ()),
+                   unitary=getattr(cmd, "unitary", None))
+
+
+def as_commands(commands) -> list:
+    """``commands`` as Commands of this package; a list of them is returned
+    as it is."""
+    cmds = commands if isinstance(commands, list) else list(commands)
+    for c in cmds:
+        if not isinstance(c, Command):
+            return [as_command(x) for x in cmds]
+    return cmds
 
 
 def substitute_all(commands: Sequence[Command], values: Mapping[str, float]) -> list[Command]:

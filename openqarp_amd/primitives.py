@@ -34,7 +34,7 @@ from typing import Mapping, Optional, Sequence, Union
 import numpy as np
 
 from .errors import CapabilityError
-from .ir import Command, GateType
+from .ir import Command, GateType, as_commands
 from .results import (EXACT, EXACT_PRUNE_TOL, ExactResult, SamplingDistribution, Shots,
                       distribution_from_result, outcome_arrays)
 from .simulator import QarpSimulator
@@ -102,8 +102,8 @@ class StateVector:
     def __init__(self, ket: Sequence[Command], n_qubits: int, observable: Optional[Observable] = None, *,
                  bra: Optional[Sequence[Command]] = None,
                  target: Target = Target.EXPECTATION_VALUE, initial_state=None):
-        self.ket = list(ket)
-        self.bra = list(bra) if bra is not None else None
+        self.ket = as_commands(list(ket))
+        self.bra = as_commands(list(bra)) if bra is not None else None
         self.n_qubits = int(n_qubits)
         self.observable = observable
         self.target = Target(target)
@@ -154,7 +154,7 @@ class Sampler:
 
     def __init__(self, circuit: Sequence[Command], n_qubits: int, n_shots: Union[int, Shots] = 1024,
                  measured: Optional[Sequence[int]] = None, initial_state=None):
-        self.circuit = list(circuit)
+        self.circuit = as_commands(list(circuit))
         self.n_qubits = int(n_qubits)
         if not (n_shots is EXACT or (isinstance(n_shots, (int, np.integer)) and n_shots >= 0)):
             raise ValueError(f"n_shots must be a non-negative int or EXACT, got {n_shots!r}")
@@ -222,7 +222,7 @@ class PauliAveraging:
 
     def __init__(self, ket: Sequence[Command], n_qubits: int, observable: Observable,
                  n_shots: Union[int, Shots] = 1024):
-        self.ket = list(ket)
+        self.ket = as_commands(list(ket))
         self.n_qubits = int(n_qubits)
         self.observable = list(observable)
         self.n_shots = n_shots

@@ -16,7 +16,7 @@ from typing import Mapping, Sequence
 
 import numpy as np
 
-from .ir import Command, GateType, substitute_all
+from .ir import Command, GateType, as_commands, substitute_all
 
 MAX_TABLE_QUBITS = 26        # program.cpp:35
 MAX_DENSE_QUBITS = 8         # dense_kernel.h:13
@@ -68,7 +68,7 @@ class Program:
         self._kernels: list = []
 
     def add_gates(self, commands: Sequence[Command]) -> None:
-        self._kernels.append(("gates", list(commands)))
+        self._kernels.append(("gates", as_commands(list(commands))))
 
     def add_permutation(self, qubits: Sequence[int], table) -> None:
         """|x> -> |table[x]> on qubits (local bit b <-> qubits[b]); table is a

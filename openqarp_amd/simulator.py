@@ -25,7 +25,7 @@ from . import _lib
 from .program import Program, measurements_terminal, needs_trajectory
 from ._abi import (QHIP_C64, QHIP_C128, PackedStream, QhipApplyStats, config_struct,
                    pack_observable, pack_stream)
-from .ir import Command, GateType, SamplingResult, cbit_register_width
+from .ir import Command, GateType, SamplingResult, as_commands, cbit_register_width
 
 Observable = Sequence[tuple]
 
@@ -187,7 +187,7 @@ class QarpSimulator:
               packed: Optional[PackedStream] = None) -> dict:
         """Run the command stream on a device state in place (the gate loop of
         QarpSimulator::statevector); returns the schedule statistics."""
-        ps = packed if packed is not None else pack_stream(commands)
+        ps = packed if packed is not None else pack_stream(as_commands(commands))
         cfg = self._config()
         stats = QhipApplyStats()
         fn = _lib.load().qhip_apply_terminal if terminal else _lib.load().qhip_apply
@@ -239,12 +239,14 @@ class QarpSimulator:
 
     # ── reference surface ──
     def statevector(self, commands: Sequence[Command], n_qubits: int, *, initial_state=None) -> np.ndarray:
+        commands = as_commands(commands)
         st = self.new_state(n_qubits, initial_state, "statevector")
         self.apply(commands, st)
         return st.download()
 
     def expectation(self, commands: Sequence[Command], n_qubits: int, observable: Observable, *,
                     initial_state=None) -> float:
+        commands = as_commands(commands)
         st = self.new_state(n_qubits, initial_state, "expectation")
         self.apply(commands, st)
         return self.expectation_state(st, observable)
@@ -265,6 +267,7 @@ class QarpSimulator:
 
     def batch_expectation(self, commands: Sequence[Command], n_qubits: int, observable: Observable,
                           param_sets: Sequence[Mapping[str, float]]) -> list:
+        commands = as_commands(commands)
         st = DeviceState(n_qubits, self.precision)
         po = pack_observable(observable)
         out = []
@@ -282,6 +285,7 @@ class QarpSimulator:
         (qarp_simulator.cpp:1017-1036) and sample_terminal (:751-783); circuits
         with mid-circuit measurement, Reset, conditions or branches take the
         shot-batched trajectory path (trajectory.py, :1038-1211)."""
+        commands = as_commands(commands)
         if not measurements_terminal(commands):
             from .trajectory import run_trajectories
             return run_trajectories(self, commands, n_qubits, n_shots, seed, initial_state)
@@ -383,6 +387,7 @@ class QarpSimulator:
 
     def batch_run(self, commands: Sequence[Command], n_qubits: int, n_shots: int,
                   param_sets: Sequence[Mapping[str, float]], seed: Optional[int] = None) -> list:
+        commands = as_commands(commands)
         out = []
         for i, params in enumerate(param_sets):
             s = None if seed is None else (int(seed) + i) & 0xFFFFFFFF
@@ -400,6 +405,7 @@ class QarpSimulator:
     def run_gradient_energy(self, commands, n_qubits, observable, params, param_order, *,
                             initial_state=None, state: Optional[DeviceState] = None):
         """(gradient array, energy) in one adjoint sweep."""
+        commands = as_commands(commands)
         index = {name: i for i, name in enumerate(param_order)}
         ps = pack_stream(commands, values=params, param_index=index)
         po = pack_observable(observable)
@@ -413,6 +419,7 @@ class QarpSimulator:
 
     def run_gradient_phi(self, commands: Sequence[Command], n_qubits: int, phi, params: Mapping[str, float],
                          param_order: Sequence[str], *, initial_state=None) -> list:
+        commands = as_commands(commands)
         index = {name: i for i, name in enumerate(param_order)}
         ps = pack_stream(commands, values=params, param_index=index)
         st = self.new_state(n_qubits, initial_state, "run_gradient")
@@ -443,7 +450,7 @@ class QarpSimulator:
 
     def fused_stream(self, commands: Sequence[Command], n_qubits: int) -> list:
         """The fusion pass output: list of (gate id, qubits, params, matrix-or-None)."""
-        ps = pack_stream(commands)
+        ps = pack_stream(as_commands(commands))
         cfg = self._config()
         lib = _lib.load()
         h = C.c_void_p()
@@ -475,7 +482,7 @@ class _PackedBatch:
 
     def __init__(self, instances: Sequence):
         from ._abi import QhipObservable
-        self.instances = list(instances)
+        self.instances = [(as_commands(c), n, o, p, order) for c, n, o, p, order in instances]
         self.m = len(self.instances)
         self.nq = np.zeros(self.m, dtype=np.int32)
         self.npar = np.zeros(self.m, dtype=np.int32)
@@ -568,7 +575,7 @@ class CompiledCircuit:
                        "low_qubits": cfg.low_qubits, "use_mfma": cfg.use_mfma,
                        "relabel_qubits": cfg.relabel_qubits}
         self._cfg = cfg
-        self._commands = list(commands)
+        self._commands = as_commands(list(commands))
         ps = pack_stream(self._commands, values=values)
         self._h = None
         h = C.c_void_p()
@@ -629,7 +636,7 @@ def inspect_schedule(commands: Sequence[Command], n_qubits: int, *, fusion_max_q
     the list in order reproduces the circuit's state."""
     lib = _lib.load()
     cfg = config_struct(fusion_max_qubits, fusion_min_qubits, tile_qubits, low_qubits, 0, -1, relabel_qubits)
-    ps = pack_stream(commands)
+    ps = pack_stream(as_commands(commands))
     h = C.c_void_p()
     _lib.check(lib.qhip_schedule_inspect(C.byref(cfg), ps.ref, int(n_qubits),
                                          QHIP_C128 if precision == "complex128" else QHIP_C64, C.byref(h)))
