@@ -324,9 +324,10 @@ int qhip_transition(const qhip_state* bra, const qhip_state* ket,
 /* qhip_transition with no device allocation.  No counterpart in the
  * reference: for callers that cannot allocate device memory (code under
  * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
- * group table and reduction partials live in a fixed 725,248-byte device
+ * group table and reduction partials live in a fixed 1,081,856-byte device
  * array per GPU (held until the process ends), so the observable may have at
- * most 4096 distinct X/Y flip masks and 16384 terms (QHIP_INVALID beyond).
+ * most 4096 distinct X/Y flip masks and 16384 terms, and the state at most
+ * 2^38 amplitudes (QHIP_INVALID beyond).
  * Same arguments, errors and result (bit for bit) as qhip_transition.  Calls
  * share the array: the library serialises its own calls; concurrent use on one
  * GPU from outside that lock races. */

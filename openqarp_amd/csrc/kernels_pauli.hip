@@ -9,7 +9,13 @@
 // Device layout: each workgroup owns a contiguous chunk of 2^12 ket indices
 // (16 per thread, kept in registers) and sweeps every group over it; for a
 // group the partner chunk is the chunk at (chunk ^ x_high), read as one
-// contiguous block with the low bits of x permuting lanes.  Sums are FP64
+// contiguous block with the low bits of x permuting lanes.  A thread's 16
+// indices differ only in bits 8..11 (r = 0..15), so a group's terms are split
+// on the host by their z bits there (the pattern p): per pattern the thread
+// sums c_t (-1)^popcount(i_0 & z_t) once at its r = 0 index i_0, and the
+// weight at index r is the sum over patterns of (-1)^popcount(r & p) times
+// that sum — one popcount per term per thread instead of one per term per
+// amplitude, with the r-dependent signs wave-uniform.  Sums are FP64
 // (complex64 states are widened on load): per-thread sums, a lane-shuffle
 // wave reduction, a fixed-order block reduction, then one block sums the
 // per-chunk partials in index order — the result does not depend on timing.
@@ -27,14 +33,24 @@ namespace qhip {
 constexpr int kT = 256;
 constexpr int kPer = 16;                 // ket amplitudes per thread
 constexpr uint64_t kChunk = kT * kPer;   // per workgroup
+constexpr int kRShift = 8;               // a thread's amplitudes differ in bits 8..11
+static_assert(kT == 1 << kRShift && kPer == 16, "r occupies index bits 8..11");
 
 struct GroupDev {
     uint64_t x;
+    int32_t s_begin, s_end;              // its patterns in the SubDev table
+    int32_t t_begin, t_end;              // its terms (contiguous across its patterns)
+};
+
+struct SubDev {
+    int32_t rpat;                        // z bits 8..11 shared by the terms
     int32_t t_begin, t_end;
+    int32_t unused;
 };
 
 struct Grouped {
     std::vector<GroupDev> groups;
+    std::vector<SubDev> subs;
     std::vector<uint64_t> z;
     std::vector<double2> c;
 };
@@ -74,15 +90,31 @@ static Grouped group_terms(const qhip_observable* obs, int n, const char* who) {
     }
     std::vector<uint64_t> xs(per.size());
     for (const auto& [x, i] : index) xs[i] = x;
+    // The diagonal group (x = 0) first: the expectation kernel treats it apart.
+    if (auto it = index.find(0); it != index.end() && it->second != 0) {
+        std::swap(xs[0], xs[it->second]);
+        std::swap(per[0], per[it->second]);
+    }
     for (size_t g = 0; g < per.size(); ++g) {
         GroupDev d;
         d.x = xs[g];
-        d.t_begin = static_cast<int32_t>(out.z.size());
-        for (const auto& [z, c] : per[g]) {
-            out.z.push_back(z);
-            out.c.push_back(make_double2(c.real(), c.imag()));
+        d.s_begin = static_cast<int32_t>(out.subs.size());
+        // Terms by pattern, first appearance order kept within a pattern.
+        auto& terms = per[g];
+        std::stable_sort(terms.begin(), terms.end(), [](const auto& a, const auto& b) {
+            return ((a.first >> kRShift) & 15) < ((b.first >> kRShift) & 15);
+        });
+        for (size_t t = 0; t < terms.size(); ++t) {
+            const int32_t pat = static_cast<int32_t>((terms[t].first >> kRShift) & 15);
+            if (t == 0 || pat != out.subs.back().rpat)
+                out.subs.push_back(SubDev{pat, static_cast<int32_t>(out.z.size()), 0, 0});
+            out.z.push_back(terms[t].first);
+            out.c.push_back(make_double2(terms[t].second.real(), terms[t].second.imag()));
+            out.subs.back().t_end = static_cast<int32_t>(out.z.size());
         }
-        d.t_end = static_cast<int32_t>(out.z.size());
+        d.s_end = static_cast<int32_t>(out.subs.size());
+        d.t_begin = out.subs[d.s_begin].t_begin;
+        d.t_end = out.subs.back().t_end;
         out.groups.push_back(d);
     }
     return out;
@@ -107,71 +139,135 @@ __device__ __forceinline__ double2 block_sum(double2 v, double2* red) {
     return r;
 }
 
-// Weight of group g at index i.
-__device__ __forceinline__ double2 weight(uint64_t i, const GroupDev& G, const uint64_t* __restrict__ z,
-                                          const double2* __restrict__ c) {
-    double2 w = make_double2(0.0, 0.0);
-    for (int t = G.t_begin; t < G.t_end; ++t) {
-        const double s = (__popcll(i & z[t]) & 1) ? -1.0 : 1.0;
-        w.x += s * c[t].x;
-        w.y += s * c[t].y;
+// Sum over the terms of pattern S of c_t (-1)^popcount(i0 & z_t), i0 with
+// bits 8..11 clear.  The weight of the group at index i0 | (r << 8) is then
+// the sum over its patterns p of (-1)^popcount(r & p) times this sum.
+__device__ __forceinline__ double2 pattern_sum(uint64_t i0, const SubDev& S, const uint64_t* __restrict__ z,
+                                               const double2* __restrict__ c) {
+    double ax = 0.0, ay = 0.0;
+#pragma unroll 4
+    for (int t = S.t_begin; t < S.t_end; ++t) {
+        const double sg = (__popcll(i0 & z[t]) & 1) ? -1.0 : 1.0;
+        ax += sg * c[t].x;
+        ay += sg * c[t].y;
     }
-    return w;
+    return make_double2(ax, ay);
 }
+
+// (-1)^popcount(r & p) for a compile-time r and a wave-uniform p.
+__device__ __forceinline__ double rsign(int r, int p) { return (__popc(r & p) & 1) ? -1.0 : 1.0; }
 
 // HERM (bra == ket, every coefficient real, so every group operator H_x is
 // Hermitian): with h the top bit of x != 0, the terms at i and i ^ x are
 // complex conjugates, so the real part is twice the sum over i with bit h
 // clear — half the partner reads.  Only the real part of the result is
 // meaningful then (the exact imaginary part is zero).
-template <typename A, bool HERM>
+// FULL: the state has at least one whole chunk (n >= 12), so no index needs a
+// bounds check.  Indices are formed from i_0 and the wave-uniform r bits, so
+// no per-r 64-bit index stays live across the group loop.
+template <typename A, bool HERM, bool FULL>
 __global__ __launch_bounds__(kT) void transition_kernel(const A* __restrict__ bra, const A* __restrict__ ket,
                                                         uint64_t dim, const GroupDev* __restrict__ groups,
-                                                        int n_groups, const uint64_t* __restrict__ z,
+                                                        int n_groups, const SubDev* __restrict__ subs,
+                                                        const uint64_t* __restrict__ z,
                                                         const double2* __restrict__ c,
-                                                        double2* __restrict__ partials, uint64_t chunk0) {
+                                                        double2* __restrict__ partials, uint64_t chunk0, int same) {
     __shared__ double2 red[kT / 64];
     const uint64_t base = (chunk0 + blockIdx.x) * kChunk;
+    const uint64_t i0 = base + threadIdx.x;
+    constexpr uint64_t rmask = uint64_t{15} << kRShift;
     double2 k[kPer];
     double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
     for (int r = 0; r < kPer; ++r) {
-        const uint64_t i = base + threadIdx.x + static_cast<uint64_t>(r) * kT;
-        k[r] = i < dim ? load_amp(ket, i) : make_double2(0.0, 0.0);
+        const uint32_t lid = threadIdx.x + static_cast<uint32_t>(r) * kT;
+        k[r] = (FULL || lid < dim) ? load_amp(ket, i0 | (static_cast<uint64_t>(r) << kRShift))
+                                   : make_double2(0.0, 0.0);
     }
-    for (int g = 0; g < n_groups; ++g) {
+    int g0 = 0;
+    if (same && n_groups > 0 && groups[0].x == 0) {
+        // The diagonal group (placed first by the host) with bra == ket: its
+        // partner is the ket amplitude itself, u[r] = |ket[i_r]|^2 is real, and
+        // every pattern sum is formed once.  Often the group with most terms.
+        const GroupDev G = groups[0];
+        double u[kPer];
+#pragma unroll
+        for (int r = 0; r < kPer; ++r) u[r] = k[r].x * k[r].x + k[r].y * k[r].y;
+        for (int si = G.s_begin; si < G.s_end; ++si) {
+            const SubDev S = subs[si];
+            const double2 a = pattern_sum(i0, S, z, c);
+            double sx = 0.0;
+#pragma unroll
+            for (int r = 0; r < kPer; ++r) sx += rsign(r, S.rpat) * u[r];
+            acc.x += a.x * sx;
+            acc.y += a.y * sx;
+        }
+        g0 = 1;
+    }
+    for (int g = g0; g < n_groups; ++g) {
         const GroupDev G = groups[g];
         uint64_t hb = 0;   // HERM: top bit of x (indices with it set are skipped)
-        double sc = 1.0;
         if (HERM && G.x) {
             hb = uint64_t{1} << (63 - __clzll(G.x));
-            sc = 2.0;
             if (base & hb) continue;   // a bit above the chunk: uniform over the workgroup
         }
+        const bool lane_skip = HERM && (i0 & hb);        // h among the thread bits
+        const uint64_t p0 = (i0 ^ G.x) & ~rmask;         // partner of r, bits 8..11 cleared
+        const int xr = static_cast<int>((G.x >> kRShift) & 15);
+        const int hr = static_cast<int>((hb >> kRShift) & 15);   // h among the r bits
+        // u[r] = conj(bra[i_r ^ x]) ket[i_r]; the group adds sum_r w(i_r) u[r],
+        // in two halves of eight r so that few registers stay live (the
+        // pattern sums are formed once per half).
+        double2 gacc = make_double2(0.0, 0.0);
 #pragma unroll
-        for (int r = 0; r < kPer; ++r) {
-            const uint64_t i = base + threadIdx.x + static_cast<uint64_t>(r) * kT;
-            if (i >= dim || (HERM && (i & hb))) continue;
-            const double2 b = load_amp(bra, i ^ G.x);
-            double2 w = weight(i, G, z, c);
-            if (HERM) {
-                w.x *= sc;
-                w.y *= sc;
+        for (int h = 0; h < kPer; h += kPer / 2) {
+            double2 u[kPer / 2];
+#pragma unroll
+            for (int q = 0; q < kPer / 2; ++q) {
+                const int r = h + q;
+                const uint32_t lid = threadIdx.x + static_cast<uint32_t>(r) * kT;
+                if ((!FULL && lid >= dim) || (HERM && (lane_skip || (r & hr)))) {
+                    u[q] = make_double2(0.0, 0.0);
+                    continue;
+                }
+                const double2 b = load_amp(bra, p0 | (static_cast<uint64_t>(r ^ xr) << kRShift));
+                u[q] = make_double2(b.x * k[r].x + b.y * k[r].y, b.x * k[r].y - b.y * k[r].x);
             }
-            // conj(b) * w * ket[i]
-            const double vx = b.x * w.x + b.y * w.y, vy = b.x * w.y - b.y * w.x;
-            acc.x += vx * k[r].x - vy * k[r].y;
-            acc.y += vx * k[r].y + vy * k[r].x;
+            for (int si = G.s_begin; si < G.s_end; ++si) {
+                const SubDev S = subs[si];
+                const double2 a = pattern_sum(i0, S, z, c);
+                double sx = 0.0, sy = 0.0;
+#pragma unroll
+                for (int q = 0; q < kPer / 2; ++q) {
+                    const double sg = rsign(h + q, S.rpat);
+                    sx += sg * u[q].x;
+                    sy += sg * u[q].y;
+                }
+                gacc.x += a.x * sx - a.y * sy;
+                gacc.y += a.x * sy + a.y * sx;
+            }
         }
+        if (HERM && G.x) {   // the skipped half is the conjugate of the visited one
+            gacc.x *= 2.0;
+            gacc.y *= 2.0;
+        }
+        acc.x += gacc.x;
+        acc.y += gacc.y;
     }
     const double2 tot = block_sum(acc, red);
     if (threadIdx.x == 0) partials[blockIdx.x] = tot;
 }
 
+// out[j] = sum_g w_g(j ^ x_g) in[j ^ x_g], one output per thread and pass of
+// a grid-stride loop: the partner reads of every group are gathers that the
+// caches mostly serve, so this kernel keeps many waves in flight (few
+// registers) rather than many outputs per thread.  The group's terms are
+// contiguous across its patterns.
 template <typename A>
 __global__ __launch_bounds__(kT) void observable_kernel(const A* __restrict__ in, A* __restrict__ out,
                                                         uint64_t dim, const GroupDev* __restrict__ groups,
-                                                        int n_groups, const uint64_t* __restrict__ z,
+                                                        int n_groups, const SubDev* __restrict__ subs,
+                                                        const uint64_t* __restrict__ z,
                                                         const double2* __restrict__ c) {
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kT;
     for (uint64_t j = static_cast<uint64_t>(blockIdx.x) * kT + threadIdx.x; j < dim; j += stride) {
@@ -180,9 +276,14 @@ __global__ __launch_bounds__(kT) void observable_kernel(const A* __restrict__ in
             const GroupDev G = groups[g];
             const uint64_t src = j ^ G.x;
             const double2 v = load_amp(in, src);
-            const double2 w = weight(src, G, z, c);
-            acc.x += w.x * v.x - w.y * v.y;
-            acc.y += w.x * v.y + w.y * v.x;
+            double wx = 0.0, wy = 0.0;
+            for (int t = G.t_begin; t < G.t_end; ++t) {
+                const double sg = (__popcll(src & z[t]) & 1) ? -1.0 : 1.0;
+                wx += sg * c[t].x;
+                wy += sg * c[t].y;
+            }
+            acc.x += wx * v.x - wy * v.y;
+            acc.y += wx * v.y + wy * v.x;
         }
         A o;
         o.x = acc.x;
@@ -221,24 +322,123 @@ __global__ __launch_bounds__(kT) void sum_partials(const double2* __restrict__ i
     if (threadIdx.x == 0) out[0] = make_double2(rx[0], ry[0]);
 }
 
+// The chunk partials of <bra|H|ket> are summed in slices of kSliceChunks:
+// block b adds chunks [b * kSliceChunks, (b + 1) * kSliceChunks) in the order
+// of sum_partials (thread t takes t, t + 256, ... then a fixed tree), and
+// sum_partials then adds the slice sums.  The no-allocation path reduces its
+// slices one launch at a time with the same kernel, so both agree bit for bit.
+constexpr int64_t kSliceChunks = 16384;
+static_assert(kSliceChunks % kT == 0, "slices keep chunk residues mod kT");
+
+__global__ __launch_bounds__(kT) void reduce_slices(const double2* __restrict__ in, int64_t n,
+                                                    double2* __restrict__ out) {
+    __shared__ double rx[kT], ry[kT];
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kSliceChunks;
+    const int64_t m = n - base < kSliceChunks ? n - base : kSliceChunks;
+    double ax = 0.0, ay = 0.0;
+    for (int64_t i = threadIdx.x; i < m; i += kT) { ax += in[base + i].x; ay += in[base + i].y; }
+    rx[threadIdx.x] = ax;
+    ry[threadIdx.x] = ay;
+    __syncthreads();
+    for (int w = kT / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) { rx[threadIdx.x] += rx[threadIdx.x + w]; ry[threadIdx.x] += ry[threadIdx.x + w]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = make_double2(rx[0], ry[0]);
+}
+
 struct DeviceGroups {
     const GroupDev* g;
+    const SubDev* s;
     const uint64_t* z;
     const double2* c;
     int n_groups;
 };
 
-static DeviceGroups upload_groups(const Grouped& gr) {
+// Byte offsets of the four tables packed in one buffer (16-byte aligned).
+struct TableLayout {
+    size_t gb, sb, zb, cb;
+    size_t so() const { return gb; }
+    size_t zo() const { return gb + sb; }
+    size_t co() const { return (gb + sb + zb + 15) & ~size_t{15}; }
+    size_t total() const { return co() + cb; }
+};
+
+static TableLayout layout_of(const Grouped& gr) {
+    return {gr.groups.size() * sizeof(GroupDev), gr.subs.size() * sizeof(SubDev), gr.z.size() * sizeof(uint64_t),
+            gr.c.size() * sizeof(double2)};
+}
+
+static DeviceGroups view_groups(char* base, const Grouped& gr) {
+    const TableLayout L = layout_of(gr);
+    return {reinterpret_cast<const GroupDev*>(base), reinterpret_cast<const SubDev*>(base + L.so()),
+            reinterpret_cast<const uint64_t*>(base + L.zo()), reinterpret_cast<const double2*>(base + L.co()),
+            static_cast<int>(gr.groups.size())};
+}
+
+static void pack_tables(char* host, const Grouped& gr) {
+    const TableLayout L = layout_of(gr);
+    std::memcpy(host, gr.groups.data(), L.gb);
+    std::memcpy(host + L.so(), gr.subs.data(), L.sb);
+    std::memcpy(host + L.zo(), gr.z.data(), L.zb);
+    std::memcpy(host + L.co(), gr.c.data(), L.cb);
+}
+
+static void copy_tables(char* dev, const Grouped& gr) {
     auto& C = ctx();
-    const size_t gb = gr.groups.size() * sizeof(GroupDev);
-    const size_t zb = gr.z.size() * sizeof(uint64_t);
-    const size_t cb = gr.c.size() * sizeof(double2);
-    char* base = static_cast<char*>(C.ops.ensure(gb + zb + cb + 64));
-    QHIP_CHECK(hipMemcpyAsync(base, gr.groups.data(), gb, hipMemcpyHostToDevice, C.stream));
-    QHIP_CHECK(hipMemcpyAsync(base + gb, gr.z.data(), zb, hipMemcpyHostToDevice, C.stream));
-    QHIP_CHECK(hipMemcpyAsync(base + gb + zb, gr.c.data(), cb, hipMemcpyHostToDevice, C.stream));
-    return {reinterpret_cast<const GroupDev*>(base), reinterpret_cast<const uint64_t*>(base + gb),
-            reinterpret_cast<const double2*>(base + gb + zb), static_cast<int>(gr.groups.size())};
+    const TableLayout L = layout_of(gr);
+    QHIP_CHECK(hipMemcpyAsync(dev, gr.groups.data(), L.gb, hipMemcpyHostToDevice, C.stream));
+    QHIP_CHECK(hipMemcpyAsync(dev + L.so(), gr.subs.data(), L.sb, hipMemcpyHostToDevice, C.stream));
+    QHIP_CHECK(hipMemcpyAsync(dev + L.zo(), gr.z.data(), L.zb, hipMemcpyHostToDevice, C.stream));
+    QHIP_CHECK(hipMemcpyAsync(dev + L.co(), gr.c.data(), L.cb, hipMemcpyHostToDevice, C.stream));
+}
+
+static DeviceGroups upload_groups(const Grouped& gr) {
+    char* base = static_cast<char*>(ctx().ops.ensure(layout_of(gr).total() + 64));
+    copy_tables(base, gr);
+    return view_groups(base, gr);
+}
+
+template <bool HERM>
+static void launch_transition(const qhip_state* bra, const qhip_state* ket, const DeviceGroups& dg, int64_t blocks,
+                              double2* part, uint64_t chunk0) {
+    auto& C = ctx();
+    const int same = bra->d == ket->d ? 1 : 0;
+    const dim3 grid(static_cast<uint32_t>(blocks));
+    auto go = [&](auto full_tag, const auto* b, const auto* k) {
+        constexpr bool FULL = decltype(full_tag)::value;
+        using A = std::remove_cv_t<std::remove_pointer_t<decltype(b)>>;
+        hipLaunchKernelGGL((transition_kernel<A, HERM, FULL>), grid, dim3(kT), 0, C.stream, b, k, ket->dim(), dg.g,
+                           dg.n_groups, dg.s, dg.z, dg.c, part, chunk0, same);
+    };
+    const bool full = ket->dim() >= kChunk;
+    if (ket->precision == QHIP_C128) {
+        const auto* b = static_cast<const double2*>(bra->d);
+        const auto* k = static_cast<const double2*>(ket->d);
+        if (full) go(std::true_type{}, b, k);
+        else go(std::false_type{}, b, k);
+    } else {
+        const auto* b = static_cast<const float2*>(bra->d);
+        const auto* k = static_cast<const float2*>(ket->d);
+        if (full) go(std::true_type{}, b, k);
+        else go(std::false_type{}, b, k);
+    }
+}
+
+static void launch_observable(const qhip_state* in, qhip_state* out, const DeviceGroups& dg) {
+    auto& C = ctx();
+    const uint64_t dim = in->dim();
+    const int blocks = static_cast<int>(std::min<uint64_t>((dim + kT - 1) / kT,
+                                                           static_cast<uint64_t>(std::max(1, C.n_cu)) * 64));
+    if (in->precision == QHIP_C128)
+        hipLaunchKernelGGL(observable_kernel<double2>, dim3(blocks), dim3(kT), 0, C.stream,
+                           static_cast<const double2*>(in->d), static_cast<double2*>(out->d), dim, dg.g,
+                           dg.n_groups, dg.s, dg.z, dg.c);
+    else
+        hipLaunchKernelGGL(observable_kernel<float2>, dim3(blocks), dim3(kT), 0, C.stream,
+                           static_cast<const float2*>(in->d), static_cast<float2*>(out->d), dim, dg.g,
+                           dg.n_groups, dg.s, dg.z, dg.c);
+    QHIP_CHECK(hipGetLastError());
 }
 
 static cplx finish(double2* part, int64_t n_part) {
@@ -259,49 +459,40 @@ cplx transition(const qhip_state* bra, const qhip_state* ket, const qhip_observa
     const DeviceGroups dg = upload_groups(gr);
     const uint64_t dim = ket->dim();
     const int64_t blocks = static_cast<int64_t>((dim + kChunk - 1) / kChunk);
-    double2* part = static_cast<double2*>(C.red.ensure(sizeof(double2) * (blocks + 1)));
+    const int64_t slices = (blocks + kSliceChunks - 1) / kSliceChunks;
+    double2* part = static_cast<double2*>(C.red.ensure(sizeof(double2) * (blocks + slices + 1)));
     bool herm = bra->d == ket->d;
     for (int64_t t = 0; herm && t < obs->n_terms; ++t) herm = obs->terms[t].coeff_im == 0.0;
-    auto go = [&](auto herm_tag) {
-        constexpr bool H = decltype(herm_tag)::value;
-        if (ket->precision == QHIP_C128)
-            hipLaunchKernelGGL((transition_kernel<double2, H>), dim3(static_cast<uint32_t>(blocks)), dim3(kT), 0,
-                               C.stream, static_cast<const double2*>(bra->d), static_cast<const double2*>(ket->d), dim,
-                               dg.g, dg.n_groups, dg.z, dg.c, part, uint64_t{0});
-        else
-            hipLaunchKernelGGL((transition_kernel<float2, H>), dim3(static_cast<uint32_t>(blocks)), dim3(kT), 0,
-                               C.stream, static_cast<const float2*>(bra->d), static_cast<const float2*>(ket->d), dim,
-                               dg.g, dg.n_groups, dg.z, dg.c, part, uint64_t{0});
-    };
-    if (herm) go(std::true_type{});
-    else go(std::false_type{});
+    if (herm) launch_transition<true>(bra, ket, dg, blocks, part, 0);
+    else launch_transition<false>(bra, ket, dg, blocks, part, 0);
     QHIP_CHECK(hipGetLastError());
-    const cplx r = finish(part, blocks);
+    hipLaunchKernelGGL(reduce_slices, dim3(static_cast<uint32_t>(slices)), dim3(kT), 0, C.stream, part, blocks,
+                       part + blocks);
+    QHIP_CHECK(hipGetLastError());
+    const cplx r = finish(part + blocks, slices);
     return herm ? cplx(r.real(), 0.0) : r;
 }
 
 // ── <bra|H|ket> with no device allocation ───────────────────────────────────
 // For callers that cannot allocate (graph capture, serving loops): the group
-// table and the reduction partials live in a fixed module-scope device array.
-// The ket runs in slices of kSliceChunks chunks; after each slice
-// accumulate_partials adds its chunk partials into 256 running sums, thread t
-// taking chunks t, t + 256, ... in increasing order — the additions
-// sum_partials performs over all chunks at once — so the result is bit-equal
+// tables, the chunk partials of one slice and the slice sums live in a fixed
+// module-scope device array.  The ket runs one slice of kSliceChunks chunks
+// per launch, each slice reduced by reduce_slices into its own sum, and the
+// slice sums are added as transition() adds them, so the result is bit-equal
 // to transition().  Every call in the process shares the array (the library
 // lock serialises them); callers that bypass that lock on one GPU race.
 
-constexpr int64_t kSliceChunks = 16384;              // multiple of kT
 constexpr int kNoAllocMaxGroups = 4096;
-constexpr int kNoAllocMaxTerms = 16384;
-constexpr size_t kNaGroupsOff = 0;
-constexpr size_t kNaZOff = kNaGroupsOff + sizeof(GroupDev) * kNoAllocMaxGroups;
-constexpr size_t kNaCOff = kNaZOff + sizeof(uint64_t) * kNoAllocMaxTerms;
-constexpr size_t kNaPartOff = kNaCOff + sizeof(double2) * kNoAllocMaxTerms;
-constexpr size_t kNaAccOff = kNaPartOff + sizeof(double2) * kSliceChunks;
-constexpr size_t kNaOutOff = kNaAccOff + sizeof(double2) * kT;
-constexpr size_t kNoAllocBytes = kNaOutOff + 256;    // 725,248 bytes
-static_assert(kSliceChunks % kT == 0, "slices must keep chunk residues mod kT");
-static_assert(kNaPartOff % 256 == 0 && kNaAccOff % 256 == 0 && kNaOutOff % 256 == 0, "aligned sections");
+constexpr int kNoAllocMaxTerms = 16384;              // patterns never outnumber terms
+constexpr int64_t kNoAllocMaxSlices = 4096;          // 2^38 amplitudes
+constexpr size_t kNaTablesBytes = sizeof(GroupDev) * kNoAllocMaxGroups + sizeof(SubDev) * kNoAllocMaxTerms +
+                                  sizeof(uint64_t) * kNoAllocMaxTerms + sizeof(double2) * kNoAllocMaxTerms + 256;
+constexpr size_t kNaPartOff = kNaTablesBytes;
+constexpr size_t kNaSliceOff = kNaPartOff + sizeof(double2) * kSliceChunks;
+constexpr size_t kNaOutOff = kNaSliceOff + sizeof(double2) * kNoAllocMaxSlices;
+constexpr size_t kNoAllocBytes = kNaOutOff + 256;    // 1,081,856 bytes
+static_assert(kNoAllocBytes == 1081856, "documented size");
+static_assert(kNaPartOff % 256 == 0 && kNaSliceOff % 256 == 0 && kNaOutOff % 256 == 0, "aligned sections");
 
 static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
@@ -326,17 +517,6 @@ static unsigned char* noalloc_scratch() {
     return static_cast<unsigned char*>(g_scratch_ptr[dev]);
 }
 
-__global__ __launch_bounds__(kT) void accumulate_partials(const double2* __restrict__ in, int64_t n,
-                                                          double2* __restrict__ acc, int first) {
-    double ax = 0.0, ay = 0.0;
-    if (!first) {
-        ax = acc[threadIdx.x].x;
-        ay = acc[threadIdx.x].y;
-    }
-    for (int64_t i = threadIdx.x; i < n; i += kT) { ax += in[i].x; ay += in[i].y; }
-    acc[threadIdx.x] = make_double2(ax, ay);
-}
-
 cplx transition_noalloc(const qhip_state* bra, const qhip_state* ket, const qhip_observable* obs) {
     const int n = ket->n;
     const Grouped gr = group_terms(obs, n, "pauli_transition");
@@ -346,51 +526,31 @@ cplx transition_noalloc(const qhip_state* bra, const qhip_state* ket, const qhip
                                " flip-mask groups and " + std::to_string(gr.z.size()) +
                                " terms; the fixed scratch holds at most " + std::to_string(kNoAllocMaxGroups) +
                                " groups and " + std::to_string(kNoAllocMaxTerms) + " terms");
+    if (layout_of(gr).total() > kNaTablesBytes) fail(QHIP_RUNTIME, "qhip_transition_noalloc: table layout overflow");
     auto& C = ctx();
     unsigned char* base = noalloc_scratch();
-    auto* d_groups = reinterpret_cast<GroupDev*>(base + kNaGroupsOff);
-    auto* d_z = reinterpret_cast<uint64_t*>(base + kNaZOff);
-    auto* d_c = reinterpret_cast<double2*>(base + kNaCOff);
     auto* d_part = reinterpret_cast<double2*>(base + kNaPartOff);
-    auto* d_acc = reinterpret_cast<double2*>(base + kNaAccOff);
+    auto* d_slices = reinterpret_cast<double2*>(base + kNaSliceOff);
     auto* d_out = reinterpret_cast<double2*>(base + kNaOutOff);
-    QHIP_CHECK(hipMemcpyAsync(d_groups, gr.groups.data(), gr.groups.size() * sizeof(GroupDev),
-                              hipMemcpyHostToDevice, C.stream));
-    QHIP_CHECK(hipMemcpyAsync(d_z, gr.z.data(), gr.z.size() * sizeof(uint64_t), hipMemcpyHostToDevice, C.stream));
-    QHIP_CHECK(hipMemcpyAsync(d_c, gr.c.data(), gr.c.size() * sizeof(double2), hipMemcpyHostToDevice, C.stream));
     const uint64_t dim = ket->dim();
     const int64_t chunks = static_cast<int64_t>((dim + kChunk - 1) / kChunk);
+    const int64_t slices = (chunks + kSliceChunks - 1) / kSliceChunks;
+    if (slices > kNoAllocMaxSlices)
+        fail(QHIP_INVALID, "qhip_transition_noalloc: " + std::to_string(ket->n) +
+                               " qubits exceed the fixed scratch's 2^38 amplitudes");
+    copy_tables(reinterpret_cast<char*>(base), gr);
+    const DeviceGroups dg = view_groups(reinterpret_cast<char*>(base), gr);
     bool herm = bra->d == ket->d;
     for (int64_t t = 0; herm && t < obs->n_terms; ++t) herm = obs->terms[t].coeff_im == 0.0;
-    const int ng = static_cast<int>(gr.groups.size());
-    for (int64_t c0 = 0; c0 < chunks; c0 += kSliceChunks) {
+    for (int64_t sl = 0; sl < slices; ++sl) {
+        const int64_t c0 = sl * kSliceChunks;
         const int64_t nb = std::min<int64_t>(kSliceChunks, chunks - c0);
-        const dim3 grid(static_cast<uint32_t>(nb));
-        const uint64_t off = static_cast<uint64_t>(c0);
-        if (ket->precision == QHIP_C128) {
-            const auto* b = static_cast<const double2*>(bra->d);
-            const auto* k = static_cast<const double2*>(ket->d);
-            if (herm)
-                hipLaunchKernelGGL((transition_kernel<double2, true>), grid, dim3(kT), 0, C.stream, b, k, dim, d_groups,
-                                   ng, d_z, d_c, d_part, off);
-            else
-                hipLaunchKernelGGL((transition_kernel<double2, false>), grid, dim3(kT), 0, C.stream, b, k, dim,
-                                   d_groups, ng, d_z, d_c, d_part, off);
-        } else {
-            const auto* b = static_cast<const float2*>(bra->d);
-            const auto* k = static_cast<const float2*>(ket->d);
-            if (herm)
-                hipLaunchKernelGGL((transition_kernel<float2, true>), grid, dim3(kT), 0, C.stream, b, k, dim, d_groups,
-                                   ng, d_z, d_c, d_part, off);
-            else
-                hipLaunchKernelGGL((transition_kernel<float2, false>), grid, dim3(kT), 0, C.stream, b, k, dim,
-                                   d_groups, ng, d_z, d_c, d_part, off);
-        }
-        hipLaunchKernelGGL(accumulate_partials, dim3(1), dim3(kT), 0, C.stream, d_part, nb, d_acc,
-                           c0 == 0 ? 1 : 0);
+        if (herm) launch_transition<true>(bra, ket, dg, nb, d_part, static_cast<uint64_t>(c0));
+        else launch_transition<false>(bra, ket, dg, nb, d_part, static_cast<uint64_t>(c0));
+        hipLaunchKernelGGL(reduce_slices, dim3(1), dim3(kT), 0, C.stream, d_part, nb, d_slices + sl);
         QHIP_CHECK(hipGetLastError());
     }
-    hipLaunchKernelGGL(sum_partials, dim3(1), dim3(kT), 0, C.stream, d_acc, static_cast<int64_t>(kT), d_out);
+    hipLaunchKernelGGL(sum_partials, dim3(1), dim3(kT), 0, C.stream, d_slices, slices, d_out);
     QHIP_CHECK(hipGetLastError());
     double2 r;
     QHIP_CHECK(hipMemcpyAsync(&r, d_out, sizeof(double2), hipMemcpyDeviceToHost, C.stream));
@@ -407,19 +567,7 @@ void apply_observable(const qhip_state* in, const qhip_observable* obs, qhip_sta
         QHIP_CHECK(hipStreamSynchronize(C.stream));
         return;
     }
-    const DeviceGroups dg = upload_groups(gr);
-    const uint64_t dim = in->dim();
-    const int blocks = static_cast<int>(std::min<uint64_t>((dim + kT - 1) / kT,
-                                                           static_cast<uint64_t>(std::max(1, C.n_cu)) * 64));
-    if (in->precision == QHIP_C128)
-        hipLaunchKernelGGL(observable_kernel<double2>, dim3(blocks), dim3(kT), 0, C.stream,
-                           static_cast<const double2*>(in->d), static_cast<double2*>(out->d), dim, dg.g,
-                           dg.n_groups, dg.z, dg.c);
-    else
-        hipLaunchKernelGGL(observable_kernel<float2>, dim3(blocks), dim3(kT), 0, C.stream,
-                           static_cast<const float2*>(in->d), static_cast<float2*>(out->d), dim, dg.g,
-                           dg.n_groups, dg.z, dg.c);
-    QHIP_CHECK(hipGetLastError());
+    launch_observable(in, out, upload_groups(gr));
     QHIP_CHECK(hipStreamSynchronize(C.stream));
 }
 
@@ -439,32 +587,22 @@ struct ObsArena {
 ObsArena* obs_arena_create(int n_obs, const int32_t* n_qubits, const qhip_observable* obs) {
     std::vector<Grouped> gs;
     gs.reserve(n_obs);
-    size_t gb = 0, zb = 0, cb = 0;
+    std::vector<size_t> off(static_cast<size_t>(n_obs) + 1, 0);
     for (int i = 0; i < n_obs; ++i) {
         gs.push_back(group_terms(&obs[i], n_qubits[i], "run_gradient"));
-        gb += gs.back().groups.size() * sizeof(GroupDev);
-        zb += gs.back().z.size() * sizeof(uint64_t);
-        cb += gs.back().c.size() * sizeof(double2);
+        off[i + 1] = off[i] + ((layout_of(gs.back()).total() + 255) & ~size_t{255});
     }
     auto a = std::make_unique<ObsArena>();
-    std::vector<char> host(gb + zb + cb + 64);
+    std::vector<char> host(off[n_obs] + 64);
     if (hipMalloc(&a->d, host.size()) != hipSuccess) {
         (void)hipGetLastError();
         a->d = nullptr;
         fail(QHIP_NOMEM, "batch: observable table allocation failed");
     }
     char* dev = static_cast<char*>(a->d);
-    size_t og = 0, oz = gb, oc = gb + zb;
-    for (const Grouped& g : gs) {
-        DeviceGroups dg{reinterpret_cast<const GroupDev*>(dev + og), reinterpret_cast<const uint64_t*>(dev + oz),
-                        reinterpret_cast<const double2*>(dev + oc), static_cast<int>(g.groups.size())};
-        std::memcpy(host.data() + og, g.groups.data(), g.groups.size() * sizeof(GroupDev));
-        std::memcpy(host.data() + oz, g.z.data(), g.z.size() * sizeof(uint64_t));
-        std::memcpy(host.data() + oc, g.c.data(), g.c.size() * sizeof(double2));
-        og += g.groups.size() * sizeof(GroupDev);
-        oz += g.z.size() * sizeof(uint64_t);
-        oc += g.c.size() * sizeof(double2);
-        a->per.push_back(dg);
+    for (int i = 0; i < n_obs; ++i) {
+        pack_tables(host.data() + off[i], gs[i]);
+        a->per.push_back(view_groups(dev + off[i], gs[i]));
     }
     QHIP_CHECK(hipMemcpy(a->d, host.data(), host.size(), hipMemcpyHostToDevice));
     return a.release();
@@ -474,20 +612,7 @@ void obs_arena_destroy(ObsArena* a) { delete a; }
 
 // out = H_i in, stream-ordered (no host synchronisation).
 void obs_apply_async(const ObsArena* a, int i, const qhip_state* in, qhip_state* out) {
-    auto& C = ctx();
-    const DeviceGroups& dg = a->per[i];
-    const uint64_t dim = in->dim();
-    const int blocks = static_cast<int>(std::min<uint64_t>((dim + kT - 1) / kT,
-                                                           static_cast<uint64_t>(std::max(1, C.n_cu)) * 64));
-    if (in->precision == QHIP_C128)
-        hipLaunchKernelGGL(observable_kernel<double2>, dim3(blocks), dim3(kT), 0, C.stream,
-                           static_cast<const double2*>(in->d), static_cast<double2*>(out->d), dim, dg.g,
-                           dg.n_groups, dg.z, dg.c);
-    else
-        hipLaunchKernelGGL(observable_kernel<float2>, dim3(blocks), dim3(kT), 0, C.stream,
-                           static_cast<const float2*>(in->d), static_cast<float2*>(out->d), dim, dg.g,
-                           dg.n_groups, dg.z, dg.c);
-    QHIP_CHECK(hipGetLastError());
+    launch_observable(in, out, a->per[i]);
 }
 
 // *d_out = <a|b> in FP64, stream-ordered; scratch holds inner_blocks(a) + 1 entries.
