@@ -248,9 +248,11 @@ int qhip_state_set_basis(qhip_state* s, const uint64_t* labels, int64_t n_labels
  * amplitudes, the indices whose magnitude hypot(re, im) exceeds `tolerance`
  * and the amplitudes (complex128, (re, im)), in ascending index order.  With
  * keys or values NULL only *n_out = the count is returned; otherwise up to
- * `capacity` entries are written (QHIP_INVALID if more are retained). */
+ * `capacity` entries are written (QHIP_INVALID if more are retained).
+ * *max_pruned (when not NULL) receives the largest magnitude not retained,
+ * 0 if none: the loop's call_max_pruned_amplitude. */
 int qhip_compact(const qhip_state* s, uint64_t n_amps, double tolerance, int64_t* keys, double* values,
-                 int64_t capacity, int64_t* n_out);
+                 int64_t capacity, int64_t* n_out, double* max_pruned);
 
 /* ── shot-batched trajectories ───────────────────────────────────────────────
  * The per-shot loop of QarpSimulator::run (src/qarpx/src/simulator/

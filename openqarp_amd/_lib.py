@@ -67,7 +67,7 @@ def _declare(lib):
         "qhip_apply_terminal": [vp, vp, vp, vp],
         "qhip_state_permute": [vp, vp, i32, vp],
         "qhip_state_set_basis": [vp, vp, i64, u64],
-        "qhip_compact": [vp, u64, C.c_double, vp, vp, i64, C.POINTER(i64)],
+        "qhip_compact": [vp, u64, C.c_double, vp, vp, i64, C.POINTER(i64), C.POINTER(C.c_double)],
         "qhip_shots_broadcast": [vp, vp],
         "qhip_shots_restore": [vp, vp, vp, u64],
         "qhip_shots_prob0": [vp, i32, u64, vp],

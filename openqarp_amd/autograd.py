@@ -1,13 +1,13 @@
 """torch.autograd boundary of the adjoint gradient.
 
 ``expectation(theta, sim, commands, n_qubits, observable, symbols)`` returns
-⟨ψ(θ)|H|ψ(θ)⟩ as a 0-d tensor whose backward is the device adjoint sweep:
-one qhip_run_gradient call re-applies the circuit from the root state and
-un-applies it gate by gate on ψ and λ = Hψ while every generator inner
+⟨ψ(θ)|H|ψ(θ)⟩ as a 0-d tensor whose backward is the device adjoint sweep.
+The forward only applies the circuit and evaluates ⟨H⟩; the backward makes
+one qhip_run_gradient call, which re-applies the circuit from the root state
+and un-applies it gate by gate on ψ and λ = Hψ while every generator inner
 product accumulates in FP64 — states are recomputed, no per-layer
-checkpoints are kept.  When ``theta`` requires a gradient the forward runs
-that sweep directly (it yields the energy and the gradient together);
-otherwise it only applies the circuit and evaluates ⟨H⟩.
+checkpoints are kept, and a forward that is never differentiated costs no
+gradient work.
 
 Tensors cross the boundary on the CPU: the device work happens in
 libqarp_hip.so on its own HIP stream, and only θ (the parameters) and the
@@ -31,18 +31,17 @@ class AdjointExpectation(torch.autograd.Function):
             raise ValueError(f"theta must be a vector of {len(symbols)} parameters, got shape {tuple(theta.shape)}")
         values = theta.detach().to("cpu", torch.float64).numpy()
         params = {s: float(v) for s, v in zip(symbols, values)}
-        if ctx.needs_input_grad[0]:
-            grad, energy = sim.run_gradient_energy(commands, n_qubits, observable, params, list(symbols))
-            ctx.save_for_backward(torch.from_numpy(np.ascontiguousarray(grad)))
-        else:
-            concrete = [c.substitute(params) if c.is_parametric() else c for c in commands]
-            energy = sim.expectation(concrete, n_qubits, observable)
+        concrete = [c.substitute(params) if c.is_parametric() else c for c in commands]
+        energy = sim.expectation(concrete, n_qubits, observable)
+        ctx.adjoint = (sim, commands, n_qubits, observable, list(symbols), params)
         return torch.tensor(energy, dtype=theta.dtype, device=theta.device)
 
     @staticmethod
     def backward(ctx, g: torch.Tensor):
-        (grad,) = ctx.saved_tensors
-        return (g * grad.to(g.device, g.dtype)), None, None, None, None, None
+        sim, commands, n_qubits, observable, symbols, params = ctx.adjoint
+        grad, _ = sim.run_gradient_energy(commands, n_qubits, observable, params, symbols)
+        grad = torch.from_numpy(np.ascontiguousarray(grad)).to(g.device, g.dtype)
+        return g * grad, None, None, None, None, None
 
 
 def expectation(theta: torch.Tensor, sim: QarpSimulator, commands, n_qubits: int, observable,

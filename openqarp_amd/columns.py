@@ -182,32 +182,47 @@ class TransitionColumnProvider:
 
     def _compact_on_device(self, src: np.ndarray, tol: float):
         """Full columns compacted on the device (qhip_compact): only the kept
-        entries cross to the host, straight into output arrays sized for
-        every entry (only the used prefix is touched).  Per batch: one
-        qhip_compact call, row offsets from the ascending keys by binary
-        search, labels masked in place.  Non-finite entries: the batch's
-        FP64 norm is finite whenever every amplitude is (short of overflow);
-        otherwise the batch is downloaded and checked entry by entry, pruned
-        entries included, as the reference's loop does
-        (transition_columns.cu:266-276)."""
+        entries cross to the host.  The output arrays hold the entries kept so
+        far plus room for one whole batch: after the first batch they are
+        sized from its kept fraction for all remaining batches, and grow by
+        half when a later batch needs more, so host memory follows the kept
+        entries, not the dense columns.  Per batch: one qhip_compact call
+        (which also returns the largest pruned magnitude), row offsets from
+        the ascending keys by binary search, labels masked in place.
+        Non-finite entries: the batch's FP64 norm is finite whenever every
+        amplitude is (short of overflow); otherwise the batch is downloaded
+        and checked entry by entry, pruned entries included, as the
+        reference's loop does (transition_columns.cu:266-276)."""
         lib = _lib.load()
         d = self._diag
-        keys = np.empty(src.size * self._dim, dtype=np.int64)
-        vals = np.empty(src.size * self._dim, dtype=np.complex128)
+        dense = src.size * self._dim
+        cap = min(dense, self._batch * self._dim)
+        keys = np.empty(cap, dtype=np.int64)
+        vals = np.empty(cap, dtype=np.complex128)
         offsets = np.zeros(src.size + 1, dtype=np.uint64)
         pos = 0
+        max_pruned = C.c_double()
         for start in range(0, src.size, self._batch):
             part = np.ascontiguousarray(src[start:start + self._batch])
+            n_amps = part.size * self._dim
+            if pos + n_amps > cap:
+                # Sized from the kept fraction so far (one resize for columns
+                # of even density), never below one more batch nor above dense.
+                est = int(pos / (start * self._dim) * dense * 1.05) + n_amps if start else 0
+                new_cap = min(dense, max(pos + n_amps, est, cap + cap // 2))
+                keys = np.concatenate([keys[:pos], np.empty(new_cap - pos, np.int64)])
+                vals = np.concatenate([vals[:pos], np.empty(new_cap - pos, np.complex128)])
+                cap = new_cap
             self._evaluate(part)
             t = time.perf_counter()
             if not math.isfinite(self._phi.norm2()):
-                rows = self._phi.download()[:part.size * self._dim]
+                rows = self._phi.download()[:n_amps]
                 if not np.isfinite(np.hypot(rows.real, rows.imag)).all():
                     raise RuntimeError("Exact column contains a nonfinite coefficient")
-            n_amps = part.size * self._dim
             cnt = C.c_int64()
             _lib.check(lib.qhip_compact(self._phi.handle, n_amps, tol, keys[pos:].ctypes.data,
-                                        vals[pos:].ctypes.data, n_amps, C.byref(cnt)))
+                                        vals[pos:].ctypes.data, cap - pos, C.byref(cnt), C.byref(max_pruned)))
+            d["max_pruned_amplitude"] = max(d["max_pruned_amplitude"], max_pruned.value)
             k = keys[pos:pos + cnt.value]
             ends = np.arange(1, part.size + 1, dtype=np.int64) << self._n
             offsets[start + 1:start + part.size + 1] = pos + np.searchsorted(k, ends)

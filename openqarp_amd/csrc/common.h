@@ -339,7 +339,7 @@ uint64_t walk_draw(const qhip_state* s, double u);
 // Entries of the first n_amps amplitudes with hypot(re, im) > tol, ascending
 // (kernels_sample.hip); returns the count, fills keys / values when given.
 int64_t compact_values(const qhip_state* s, uint64_t n_amps, double tol, int64_t* keys, double* values,
-                       int64_t capacity);
+                       int64_t capacity, double* max_pruned);
 // generate_canonical<double, 53> over std::mt19937(seed), which is also
 // uniform_real_distribution<double>(0, 1) (kernels_sample.hip).
 void mt_uniforms(uint32_t seed, int64_t n, double* out);

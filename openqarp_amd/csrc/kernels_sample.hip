@@ -166,6 +166,69 @@ __global__ void probs_kernel(const A* __restrict__ s, uint64_t dim, double* __re
 // ballots in index order.
 constexpr uint64_t kXChunk = kST * 16;
 
+// Two magnitudes, each the one its reference path computes.
+//
+// hypot_libm: std::hypot of the C library, which the column compaction of
+// transition_columns.cu:271-276 calls (and numpy's hypot).  It follows the
+// published algorithm that library uses (Borges, "An improved algorithm for
+// hypot(a, b)", 2019, the corrected form without fused multiply-add):
+// h = sqrt(x^2 + y^2) rounded, then one Newton step h -= (h^2 - x^2 - y^2)/(2h)
+// with the residual rearranged about h - y or h - x so that nothing cancels.
+// Arguments below 2^-450 are scaled by 2^600 (exact, and every step commutes
+// with it) so that no square underflows.  This file compiles with
+// contraction off, so none of these products is fused.
+__device__ __forceinline__ double hypot_libm(double x, double y) {
+    double ax = fabs(x), ay = fabs(y);
+    if (ax < ay) {
+        const double t = ax;
+        ax = ay;
+        ay = t;
+    }
+    if (ay == 0.0) return ax;
+    double scale = 1.0;
+    if (ax < 0x1p-450) {
+        ax *= 0x1p600;
+        ay *= 0x1p600;
+        scale = 0x1p-600;
+    }
+    double h = sqrt(ax * ax + ay * ay);
+    double t1, t2;
+    if (h <= 2.0 * ay) {
+        const double d = h - ay;
+        t1 = ax * (2.0 * d - ax);
+        t2 = (d - 2.0 * (ax - ay)) * d;
+    } else {
+        const double d = h - ax;
+        t1 = 2.0 * d * (ax - 2.0 * ay);
+        t2 = (4.0 * d - ay) * ay + d * d;
+    }
+    h -= (t1 + t2) / (2.0 * h);
+    return h * scale;
+}
+
+// abs_numpy: numpy's absolute value of a complex128 array, which the EXACT
+// readout squares (probs = np.abs(sv) ** 2, src/qarp/engines/_engine.py:
+// 100-102): with L = max(|x|, |y|) and S = min, L * sqrt(fma(S/L, S/L, 1))
+// (numpy's vectorised complex absolute value; 0 when L is 0).
+__device__ __forceinline__ double abs_numpy(double x, double y) {
+    const double ax = fabs(x), ay = fabs(y);
+    const double l = ax < ay ? ay : ax, sm = ax < ay ? ax : ay;
+    if (l == 0.0) return 0.0;
+    const double r = sm / l;
+    return l * sqrt(fma(r, r, 1.0));
+}
+
+__device__ __forceinline__ double born_prob(const double2* s, uint64_t i) {
+    const double2 a = s[i];
+    const double m = abs_numpy(a.x, a.y);
+    return m * m;
+}
+__device__ __forceinline__ double born_prob(const float2* s, uint64_t i) {
+    const float2 a = s[i];
+    const double m = abs_numpy(static_cast<double>(a.x), static_cast<double>(a.y));
+    return m * m;
+}
+
 template <typename A>
 __global__ __launch_bounds__(kST) void exact_count(const A* __restrict__ s, uint64_t dim, double thr,
                                                    uint64_t* __restrict__ counts) {
@@ -174,7 +237,7 @@ __global__ __launch_bounds__(kST) void exact_count(const A* __restrict__ s, uint
     uint32_t n = 0;
     for (int r = 0; r < 16; ++r) {
         const uint64_t i = base + static_cast<uint64_t>(r) * kST + threadIdx.x;
-        if (i < dim && prob(s, i) > thr) ++n;
+        if (i < dim && born_prob(s, i) > thr) ++n;
     }
     for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off);
     if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = n;
@@ -224,7 +287,7 @@ __global__ __launch_bounds__(kST) void exact_scatter(const A* __restrict__ s, ui
     uint64_t pos = offsets[blockIdx.x];
     for (int r = 0; r < 16; ++r) {
         const uint64_t i = base + static_cast<uint64_t>(r) * kST + threadIdx.x;
-        const double p = i < dim ? prob(s, i) : 0.0;
+        const double p = i < dim ? born_prob(s, i) : 0.0;
         const bool keep = i < dim && p > thr;
         const uint64_t ballot = __ballot(keep);
         const uint32_t rank = __popcll(ballot & ((1ull << lane) - 1ull));
@@ -247,24 +310,56 @@ __global__ __launch_bounds__(kST) void exact_scatter(const A* __restrict__ s, ui
 // Value compaction (transition columns): indices whose amplitude magnitude
 // hypot(re, im) exceeds a tolerance, with the amplitudes, in ascending index
 // order — the same count / scan / ballot-ranked scatter as the EXACT readout.
+// The count pass also keeps, per block, the largest magnitude it prunes (the
+// provider's max_pruned_amplitude diagnostic).
 template <typename A>
 __global__ __launch_bounds__(kST) void compact_count(const A* __restrict__ s, uint64_t dim, double tol,
-                                                     uint64_t* __restrict__ counts) {
+                                                     uint64_t* __restrict__ counts, double* __restrict__ pmax) {
     __shared__ uint32_t wsum[kST / 64];
+    __shared__ double wmax[kST / 64];
     const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kXChunk;
     uint32_t n = 0;
+    double mx = 0.0;
     for (int r = 0; r < 16; ++r) {
         const uint64_t i = base + static_cast<uint64_t>(r) * kST + threadIdx.x;
-        if (i < dim && hypot(static_cast<double>(s[i].x), static_cast<double>(s[i].y)) > tol) ++n;
+        if (i >= dim) continue;
+        const double m = hypot_libm(static_cast<double>(s[i].x), static_cast<double>(s[i].y));
+        if (m > tol) ++n;
+        else mx = fmax(mx, m);
     }
-    for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off);
-    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = n;
+    for (int off = 32; off > 0; off >>= 1) {
+        n += __shfl_xor(n, off);
+        mx = fmax(mx, __shfl_xor(mx, off));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        wsum[threadIdx.x >> 6] = n;
+        wmax[threadIdx.x >> 6] = mx;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         uint64_t t = 0;
-        for (int w = 0; w < kST / 64; ++w) t += wsum[w];
+        double m = 0.0;
+        for (int w = 0; w < kST / 64; ++w) {
+            t += wsum[w];
+            m = fmax(m, wmax[w]);
+        }
         counts[blockIdx.x] = t;
+        pmax[blockIdx.x] = m;
     }
+}
+
+__global__ __launch_bounds__(kST) void max_reduce(const double* __restrict__ in, uint64_t n,
+                                                  double* __restrict__ out) {
+    __shared__ double red[kST];
+    double m = 0.0;
+    for (uint64_t i = threadIdx.x; i < n; i += kST) m = fmax(m, in[i]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = kST / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
 }
 
 template <typename A>
@@ -279,7 +374,7 @@ __global__ __launch_bounds__(kST) void compact_scatter(const A* __restrict__ s, 
         const uint64_t i = base + static_cast<uint64_t>(r) * kST + threadIdx.x;
         double2 v = {0.0, 0.0};
         if (i < dim) v = {static_cast<double>(s[i].x), static_cast<double>(s[i].y)};
-        const bool keep = i < dim && hypot(v.x, v.y) > tol;
+        const bool keep = i < dim && hypot_libm(v.x, v.y) > tol;
         const uint64_t ballot = __ballot(keep);
         const uint32_t rank = __popcll(ballot & ((1ull << lane) - 1ull));
         if (lane == 0) wcount[w] = __popcll(ballot);
@@ -541,26 +636,33 @@ int64_t exact(const qhip_state* s, double threshold, int64_t* keys, double* prob
 }
 
 int64_t compact_values(const qhip_state* s, uint64_t n_amps, double tol, int64_t* keys, double* values,
-                       int64_t capacity) {
+                       int64_t capacity, double* max_pruned) {
     auto& C = ctx();
     if (n_amps > s->dim()) fail(QHIP_INVALID, "qhip_compact: more amplitudes than the state holds");
     if (!(tol >= 0.0) || tol == HUGE_VAL) fail(QHIP_INVALID, "qhip_compact: tolerance must be finite and >= 0");
     const uint64_t dim = n_amps;
+    if (max_pruned) *max_pruned = 0.0;
     if (dim == 0) return 0;
     const uint64_t nb = (dim + kXChunk - 1) / kXChunk;
-    uint64_t* counts = static_cast<uint64_t*>(C.red.ensure(sizeof(uint64_t) * (2 * nb + 1)));
+    // counts[nb] | offsets[nb + 1] | block maxima[nb] | overall maximum
+    uint64_t* counts = static_cast<uint64_t*>(C.red.ensure(sizeof(uint64_t) * (3 * nb + 2)));
     uint64_t* offsets = counts + nb;
+    double* pmax = reinterpret_cast<double*>(offsets + nb + 1);
     if (s->precision == QHIP_C128)
         hipLaunchKernelGGL(compact_count<double2>, dim3(static_cast<uint32_t>(nb)), dim3(kST), 0, C.stream,
-                           static_cast<const double2*>(s->d), dim, tol, counts);
+                           static_cast<const double2*>(s->d), dim, tol, counts, pmax);
     else
         hipLaunchKernelGGL(compact_count<float2>, dim3(static_cast<uint32_t>(nb)), dim3(kST), 0, C.stream,
-                           static_cast<const float2*>(s->d), dim, tol, counts);
+                           static_cast<const float2*>(s->d), dim, tol, counts, pmax);
     hipLaunchKernelGGL(exclusive_scan_u64, dim3(1), dim3(kST), 0, C.stream, counts, nb, offsets);
+    hipLaunchKernelGGL(max_reduce, dim3(1), dim3(kST), 0, C.stream, pmax, nb, pmax + nb);
     QHIP_CHECK(hipGetLastError());
     uint64_t total = 0;
     QHIP_CHECK(hipMemcpyAsync(&total, offsets + nb, sizeof(uint64_t), hipMemcpyDeviceToHost, C.stream));
+    double mp = 0.0;
+    QHIP_CHECK(hipMemcpyAsync(&mp, pmax + nb, sizeof(double), hipMemcpyDeviceToHost, C.stream));
     QHIP_CHECK(hipStreamSynchronize(C.stream));
+    if (max_pruned) *max_pruned = mp;
     if (!keys || !values) return static_cast<int64_t>(total);
     if (static_cast<uint64_t>(capacity) < total)
         fail(QHIP_INVALID, "qhip_compact: capacity " + std::to_string(capacity) + " is below the " +

@@ -669,10 +669,10 @@ int qhip_state_set_basis(qhip_state* s, const uint64_t* labels, int64_t n_labels
 }
 
 int qhip_compact(const qhip_state* s, uint64_t n_amps, double tolerance, int64_t* keys, double* values,
-                 int64_t capacity, int64_t* n_out) {
+                 int64_t capacity, int64_t* n_out, double* max_pruned) {
     return guarded([&] {
         need(s && n_out, "qhip_compact: NULL argument");
-        *n_out = qhip::compact_values(s, n_amps, tolerance, keys, values, capacity);
+        *n_out = qhip::compact_values(s, n_amps, tolerance, keys, values, capacity, max_pruned);
     });
 }
 

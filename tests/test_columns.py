@@ -149,6 +149,14 @@ def test_compact_columns_on_device(tol):
         lo, hi = int(offsets[j]), int(offsets[j + 1])
         assert np.array_equal(labels[lo:hi], np.nonzero(keep)[0].astype(np.uint64))
         assert np.abs(values[lo:hi] - want[j][keep]).max() < 1e-12
+    pruned = np.abs(want)[np.abs(want) <= tol]
+    assert prov.diagnostics["max_pruned_amplitude"] == pytest.approx(pruned.max() if pruned.size else 0.0,
+                                                                          abs=1e-12)
+    # Host buffers follow the kept entries: at most one batch of slack.
+    owner = labels
+    while owner.base is not None:
+        owner = owner.base
+    assert owner.size <= len(labels) + 2 * (1 << n) + len(labels) // 20 + 1
 
 
 @pytest.mark.gpu

@@ -71,11 +71,13 @@ def test_exact_compaction_at_22_qubits():
     circ = W.brickwork_haar(n, 2, 4)
     st = sim.new_state(n)
     sim.apply(circ, st)
-    probs = sim.probabilities_state(st)
+    probs = np.abs(st.download()) ** 2          # _exact_result's formula (_engine.py:100-102)
     for thr in (1e-12, 2e-7, 1e-6):
         keys, vals = sim.exact_state(st, thr)
         (idx,) = np.nonzero(probs > thr)
         assert np.array_equal(keys, idx) and np.array_equal(vals, probs[idx])
+    # The sampler's probabilities are std::norm's x*x + y*y, within an ulp of these.
+    assert np.abs(sim.probabilities_state(st) - probs).max() <= 2 * np.spacing(probs.max())
 
 
 def test_counts_sampler_is_bit_exact_with_the_oracle():
@@ -193,3 +195,41 @@ def test_autograd_function_backward_is_the_adjoint_gradient():
     ok = torch.autograd.gradcheck(lambda t: expectation(t, sim, w.commands, n, obs, w.symbols),
                                   (theta.detach().clone().requires_grad_(True),), eps=1e-6, atol=1e-7)
     assert ok
+
+
+def test_autograd_forward_does_no_gradient_work(monkeypatch):
+    """The forward of the autograd function evaluates <H> only; the adjoint
+    sweep runs when (and each time) backward is called."""
+    torch = pytest.importorskip("torch")
+    from openqarp_amd.autograd import expectation
+    n = 5
+    w = W.hea(n, 2, 4, symbolic=True)
+    obs = W.tfim_chain(n, 0.9)
+    sim = QarpSimulator()
+    calls = []
+    real = sim.run_gradient_energy
+    monkeypatch.setattr(sim, "run_gradient_energy", lambda *a, **k: calls.append(1) or real(*a, **k))
+    theta = torch.tensor([w.values[s] for s in w.symbols], dtype=torch.float64, requires_grad=True)
+    e = expectation(theta, sim, w.commands, n, obs, w.symbols)
+    assert calls == []
+    (2.0 * e).backward()
+    assert calls == [1]
+    want = 2.0 * np.asarray(sim.run_gradient(w.commands, n, obs, dict(w.values), w.symbols))
+    assert np.abs(theta.grad.numpy() - want).max() < 1e-12
+
+
+def test_batch_expectation_against_oracle():
+    """batch_expectation (qarp_simulator.cpp:971-984): one energy per
+    parameter set, each equal to the oracle's statevector + transition."""
+    n = 9
+    w = W.c3_gradient(n, 2, 13, 25)
+    sim = QarpSimulator()
+    rng = np.random.default_rng(6)
+    sets = [w.values] + [{s: float(rng.uniform(0, 6.3)) for s in w.symbols} for _ in range(4)]
+    got = sim.batch_expectation(w.commands, n, w.observable, sets)
+    po = pack_observable(w.observable)
+    assert len(got) == len(sets)
+    for e, vals in zip(got, sets):
+        sv = O.statevector(pack_stream([c.substitute(vals) for c in w.commands]), n, 3, 12)
+        assert abs(e - O.transition(sv, sv, n, po).real) < 1e-12
+    assert sim.batch_expectation(w.commands, n, w.observable, []) == []
