@@ -166,7 +166,7 @@ def _with_ket(prim, commands):
         return StateVector(commands, prim.n_qubits, prim.observable, bra=prim.bra, target=prim.target,
                            initial_state=prim.initial_state)
     if isinstance(prim, PauliAveraging):
-        return PauliAveraging(commands, prim.n_qubits, prim.observable, prim.n_shots)
+        return PauliAveraging(commands, prim.n_qubits, prim.observable, prim.n_shots, prim.grouping)
     raise CapabilityError(f"{type(prim).__name__} has no stencil gradient")
 
 

@@ -34,6 +34,7 @@ from typing import Mapping, Optional, Sequence, Union
 import numpy as np
 
 from .errors import CapabilityError
+from .grouping import FullyCommuting, GroupingStrategy, diagonalise_group
 from .ir import Command, GateType, as_commands
 from .results import (EXACT, EXACT_PRUNE_TOL, ExactResult, SamplingDistribution, Shots,
                       distribution_from_result, outcome_arrays)
@@ -180,29 +181,32 @@ class Sampler:
         return distribution_from_result(self.raw(sim, params, seed), self.measured)
 
 
-def qubitwise_groups(observable: Observable) -> tuple:
-    """(constant, groups): groups of (term index, {qubit: letter}) whose
-    letters agree on every shared qubit, first fit in term order."""
+def decompose_observable(observable: Observable) -> tuple:
+    """(constant, terms, coeffs) of a Hermitian observable: identity terms
+    summed into the constant, the others as {qubit: letter} dicts with their
+    real coefficients (PauliAveraging._decompose_operator,
+    pauli_averaging.py:100-118, which keeps the real part)."""
     constant = 0.0
-    groups: list = []   # [(letters dict, [term indices])]
-    terms = []
-    for t, (term, coeff) in enumerate(observable):
-        letters = {int(q): p for q, p in term if p != "I"}
-        c = complex(coeff)
-        if abs(c.imag) > 1e-12 * max(1.0, abs(c)):
-            raise ValueError("PauliAveraging: coefficients of a Hermitian operator must be real")
-        terms.append((letters, c.real))
+    terms, coeffs = [], []
+    for term, coeff in observable:
+        letters: dict = {}
+        seen = set()
+        for q, p in term:
+            q, p = int(q), str(p)
+            if len(p) != 1 or p not in "IXYZ":
+                raise ValueError(f"PauliAveraging: invalid Pauli '{p}'")
+            if q in seen:
+                raise ValueError(f"PauliAveraging: qubit {q} repeated within one Pauli string")
+            seen.add(q)
+            if p != "I":
+                letters[q] = p
+        c = complex(coeff).real
         if not letters:
-            constant += c.real
+            constant += c
             continue
-        for g in groups:
-            if all(g[0].get(q, p) == p for q, p in letters.items()):
-                g[0].update(letters)
-                g[1].append(t)
-                break
-        else:
-            groups.append((dict(letters), [t]))
-    return constant, groups, terms
+        terms.append(letters)
+        coeffs.append(c)
+    return constant, terms, np.asarray(coeffs, dtype=np.float64)
 
 
 def _parity(x: np.ndarray) -> np.ndarray:
@@ -216,40 +220,51 @@ def _parity(x: np.ndarray) -> np.ndarray:
 
 
 class PauliAveraging:
-    """⟨ψ|H|ψ⟩ estimated from measurement counts of one circuit per group."""
+    """⟨ψ|H|ψ⟩ estimated from the measurement counts of one circuit per group
+    of commuting terms: ket → the group's diagonalising Clifford → measure
+    every qubit (pauli_averaging.py:119-205).  ``grouping`` defaults to
+    FullyCommuting, as the reference's does."""
 
     gradient_kind = "expectation"
 
     def __init__(self, ket: Sequence[Command], n_qubits: int, observable: Observable,
-                 n_shots: Union[int, Shots] = 1024):
+                 n_shots: Union[int, Shots] = 1024, grouping: Optional[GroupingStrategy] = None):
         self.ket = as_commands(list(ket))
         self.n_qubits = int(n_qubits)
         self.observable = list(observable)
         self.n_shots = n_shots
-        self.constant, groups, self._terms = qubitwise_groups(self.observable)
-        self.groups = [(letters, idx) for letters, idx in groups]
+        self.grouping = grouping if grouping is not None else FullyCommuting()
+        self.constant, self._terms, self._coeffs = decompose_observable(self.observable)
+        for t in self._terms:
+            if max(t) >= self.n_qubits:
+                raise ValueError(f"PauliAveraging: observable qubit {max(t)} out of range for "
+                                 f"{self.n_qubits} qubits")
+        self.groups = []   # (term indices, Clifford commands, z masks, signs)
+        for grp in (self.grouping.group(self._terms, self.n_qubits) if self._terms else []):
+            clifford, masks, signs = diagonalise_group([self._terms[i] for i in grp], self.n_qubits)
+            self.groups.append((list(grp), clifford, masks, signs))
 
     @property
     def n_groups(self) -> int:
         return len(self.groups)
 
+    @property
+    def n_terms(self) -> int:
+        return len(self._terms)
+
     def symbols(self) -> list:
         return free_symbols(self.ket)
 
-    def group_circuit(self, letters: Mapping[int, str]) -> list:
-        basis = []
-        for q in sorted(letters):
-            if letters[q] == "X":
-                basis.append(Command(GateType.H, q))
-            elif letters[q] == "Y":
-                basis += [Command(GateType.Sdg, q), Command(GateType.H, q)]
-        return self.ket + basis
+    def group_circuit(self, g: int) -> list:
+        """The reference's circuit of group g without its measure layer:
+        ket, then the group's Clifford."""
+        return self.ket + list(self.groups[g][1])
 
     def run(self, sim: QarpSimulator, params=None, seeds: Optional[Sequence[Optional[int]]] = None) -> float:
         """The ket runs once into a device state; each group copies it,
-        applies only its basis change and samples the copy (the reference runs
-        ket + basis change per group, pauli_averaging.py:119-205: the same
-        state up to rounding, with the same per-group seeds)."""
+        applies only its Clifford and samples the copy (the reference runs
+        ket + Clifford per group: the same state up to rounding, with the same
+        per-group seeds)."""
         energy = self.constant
         if not self.groups:
             return float(energy)
@@ -261,32 +276,37 @@ class PauliAveraging:
             base = sim.new_state(self.n_qubits)
             sim.apply([c for c in ket if c.gate not in (GateType.Measure, GateType.Barrier)], base)
             work = sim.new_state(self.n_qubits)
-        for g, (letters, idx) in enumerate(self.groups):
+        for g, (idx, clifford, masks, signs) in enumerate(self.groups):
             seed = None if seeds is None else seeds[g]
             if not shared:   # trajectory commands: the per-group circuit raises as run() does
-                res = Sampler(self.group_circuit(letters), self.n_qubits, self.n_shots).raw(sim, params, seed)
+                res = Sampler(self.group_circuit(g), self.n_qubits, self.n_shots).raw(sim, params, seed)
             else:
                 work.copy_from(base)
-                basis = self.group_circuit(letters)[len(self.ket):]
-                if basis:
-                    sim.apply(basis, work)
+                if clifford:
+                    sim.apply(clifford, work)
                 if self.n_shots is EXACT:
                     keys, probs = sim.exact_state(work, EXACT_PRUNE_TOL)
                     res = ExactResult(self.n_qubits, keys, probs)
                 else:
                     res = sim._terminal_result(work, [], self.n_qubits, int(self.n_shots), seed)
-            energy += self.group_estimate(res, idx)
+            energy += self.group_estimate(res, g)
         return float(energy)
 
-    def group_estimate(self, res, idx: Sequence[int]) -> float:
-        """Σ_t c_t ⟨P_t⟩ of one group from its counts (or exact distribution):
-        each term's parity over its qubits, weighted by the outcome counts."""
+    def group_estimate(self, res, g: int) -> float:
+        """Σ_t c_t ⟨P_t⟩ of group g from its counts (or exact distribution):
+        after the Clifford, term t is s_t Z^{mask_t}, so its expectation is
+        s_t Σ_outcome w (-1)^popcount(outcome & mask_t) / n_shots."""
+        idx, _, masks, signs = self.groups[g]
         outcomes, weights = outcome_arrays(res)
-        masks = np.array([sum(1 << q for q in self._terms[t][0]) for t in idx], dtype=np.int64)
-        coeffs = np.array([self._terms[t][1] for t in idx])
-        parity = _parity(outcomes[:, None] & masks[None, :])
-        e_terms = (weights @ (1.0 - 2.0 * parity)) / res.n_shots
-        return float(coeffs @ e_terms)
+        m = np.asarray(masks, dtype=np.int64)
+        sg = np.where(signs, -1.0, 1.0)
+        parity = _parity(outcomes[:, None] & m[None, :])
+        e_terms = sg * (weights @ (1.0 - 2.0 * parity)) / res.n_shots
+        return float(self._coeffs[idx] @ e_terms)
+
+    def __repr__(self) -> str:
+        return (f"PauliAveraging(n_terms={self.n_terms}, n_groups={self.n_groups}, n_shots={self.n_shots}, "
+                f"grouping={self.grouping!r})")
 
 
 def require_capability(prim, method: str) -> None:

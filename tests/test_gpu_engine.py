@@ -10,8 +10,8 @@ from openqarp_amd._abi import pack_observable, pack_stream
 from openqarp_amd.engine import HipEngine
 from openqarp_amd.errors import CapabilityError
 from openqarp_amd.ir import Command, GateType, Param
-from openqarp_amd.primitives import (PauliAveraging, Sampler, StateVector, Target, diagonal_pauli_vector,
-                                     qubitwise_groups)
+from openqarp_amd.grouping import FullyCommuting, NoGrouping, QubitWiseCommuting
+from openqarp_amd.primitives import PauliAveraging, Sampler, StateVector, Target, diagonal_pauli_vector
 from openqarp_amd.results import EXACT
 from openqarp_amd.simulator import QarpSimulator
 from test_host_api import _gradient_circuit
@@ -99,9 +99,10 @@ def test_pauli_averaging_exact_equals_expectation_and_sampling_is_close():
     obs = W.random_pauli_sum(n, 25, 3) + [([], 0.4)]
     k = O.statevector(pack_stream(ket), n, 3, 12)
     want = O.transition(k, k, n, pack_observable(obs)).real
-    pa = PauliAveraging(ket, n, obs, EXACT)
-    assert pa.n_groups == len(qubitwise_groups(obs)[1])
-    assert abs(HipEngine().run(pa) - want) < 1e-10
+    for strategy in (FullyCommuting(), QubitWiseCommuting(), NoGrouping()):
+        pa = PauliAveraging(ket, n, obs, EXACT, strategy)
+        assert pa.n_groups == len(strategy.group(pa._terms, n))
+        assert abs(HipEngine().run(pa) - want) < 1e-10
     est = HipEngine(seed=11).run(PauliAveraging(ket, n, obs, 20000))
     scale = sum(abs(complex(c)) for t, c in obs if t)
     assert abs(est - want) < 0.05 * scale
@@ -119,9 +120,9 @@ def test_pauli_averaging_shares_the_ket_state():
     pa = PauliAveraging(ket, n, obs, 4000)
     seeds = list(range(100, 100 + pa.n_groups))
     want = pa.constant
-    for g, (letters, idx) in enumerate(pa.groups):
-        want += pa.group_estimate(Sampler(pa.group_circuit(letters), n, 4000).raw(sim, None, seeds[g]), idx)
-    assert abs(pa.run(sim, None, seeds) - want) < 1e-12
+    for g in range(pa.n_groups):
+        want += pa.group_estimate(Sampler(pa.group_circuit(g), n, 4000).raw(sim, None, seeds[g]), g)
+    assert pa.run(sim, None, seeds) == want
 
 
 def test_gradient_methods_agree():
@@ -233,3 +234,21 @@ def test_batch_expectation_against_oracle():
         sv = O.statevector(pack_stream([c.substitute(vals) for c in w.commands]), n, 3, 12)
         assert abs(e - O.transition(sv, sv, n, po).real) < 1e-12
     assert sim.batch_expectation(w.commands, n, w.observable, []) == []
+
+
+def test_pauli_averaging_bell_state_fully_commuting_single_group():
+    """XX + YY + ZZ on the Bell state (H, CX): one FullyCommuting group whose
+    Clifford reads all three terms at once (the reference's
+    test_pauli_averaging.py:92-111): exactly 1 in the EXACT readout, and the
+    sampled estimate equals the per-group circuit's bit for bit."""
+    n = 2
+    ket = [Command(GateType.H, 0), Command(GateType.CX, 0, 1)]
+    obs = [([(0, "X"), (1, "X")], 1.0), ([(0, "Y"), (1, "Y")], 1.0), ([(0, "Z"), (1, "Z")], 1.0)]
+    sim = QarpSimulator()
+    pa = PauliAveraging(ket, n, obs, EXACT)
+    assert pa.n_groups == 1
+    assert abs(pa.run(sim) - 1.0) < 1e-12
+    sampled = PauliAveraging(ket, n, obs, 3000)
+    got = sampled.run(sim, None, [17])
+    want = sampled.group_estimate(Sampler(sampled.group_circuit(0), n, 3000).raw(sim, None, 17), 0)
+    assert got == want and abs(got - 1.0) < 1e-12   # the Bell state is an eigenstate: every shot agrees

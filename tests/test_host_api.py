@@ -14,7 +14,7 @@ from openqarp_amd import gradients as G
 from openqarp_amd._abi import pack_observable, pack_stream
 from openqarp_amd.errors import CapabilityError
 from openqarp_amd.ir import Command, GateType, Param, SamplingResult
-from openqarp_amd.primitives import diagonal_pauli_vector, qubitwise_groups
+from openqarp_amd.primitives import decompose_observable, diagonal_pauli_vector
 from openqarp_amd.results import EXACT, ExactResult, distribution_from_result, pack_bits
 
 
@@ -31,17 +31,17 @@ def test_pack_bits_and_marginal_distribution():
     assert EXACT.value == "exact"
 
 
-def test_qubitwise_groups_share_letters():
-    obs = [([(0, "X"), (1, "Z")], 1.0), ([(1, "Z")], 0.5), ([(0, "Y")], 2.0), ([], 0.25),
-           ([(0, "X"), (2, "Y")], -1.0)]
-    const, groups, terms = qubitwise_groups(obs)
-    assert const == 0.25
-    assert [idx for _, idx in groups] == [[0, 1, 4], [2]]
-    for letters, idx in groups:
-        for t in idx:
-            assert all(letters[q] == p for q, p in terms[t][0].items())
-    with pytest.raises(ValueError):
-        qubitwise_groups([([(0, "Z")], 1j)])
+def test_decompose_observable_folds_identity_and_keeps_real_parts():
+    obs = [([(0, "X"), (1, "Z")], 1.0), ([(1, "Z")], 0.5), ([(0, "Y")], 2.0 + 0.5j), ([], 0.25),
+           ([(2, "I")], 0.1), ([(0, "X"), (2, "Y")], -1.0)]
+    const, terms, coeffs = decompose_observable(obs)
+    assert const == pytest.approx(0.35)
+    assert terms == [{0: "X", 1: "Z"}, {1: "Z"}, {0: "Y"}, {0: "X", 2: "Y"}]
+    assert coeffs.tolist() == [1.0, 0.5, 2.0, -1.0]
+    with pytest.raises(ValueError, match="repeated"):
+        decompose_observable([([(0, "Z"), (0, "X")], 1.0)])
+    with pytest.raises(ValueError, match="invalid Pauli"):
+        decompose_observable([([(0, "Q")], 1.0)])
 
 
 def test_diagonal_pauli_vector_is_the_operator_diagonal():
