@@ -274,6 +274,17 @@ int qhip_shots_prob0(const qhip_state* s, int32_t qubit, uint64_t stride, double
  * and scale; 2 move |1> to |0> and scale (Reset after outcome 1).  P0_gate /
  * P1_gate, normalize(p) = scale by sqrt(1/p), X (qarp_simulator.cpp:636-685). */
 int qhip_shots_project(qhip_state* s, int32_t qubit, uint64_t stride, const int8_t* action, const double* scale);
+/* Noise channels per shot (apply_channel, src/qarpx/src/device/
+ * noise_model.cpp:86-186, after each noisy gate, qarp_simulator.cpp:697-722):
+ * blocks j with active[j] != 0 get the 2x2 complex matrix mats[8j .. 8j + 8)
+ * (row-major (re, im)) on `qubit` — the sampled Pauli (X_gate / Y_gate /
+ * Z_gate) or the sampled Kraus operator K_i / sqrt(p_i)
+ * (single_qubit_dense_matrix_gate). */
+int qhip_shots_apply1q(qhip_state* s, int32_t qubit, uint64_t stride, const double* mats, const uint8_t* active);
+/* The reduced density matrix of `qubit` per block (reduce_density_matrix_1q,
+ * noise_model.cpp:39-58): out[4j .. 4j + 4) = (rho00, rho11, Re rho01,
+ * Im rho01), rho01 = sum a conj(b) over the pairs (a with the bit clear). */
+int qhip_shots_rho1(const qhip_state* s, int32_t qubit, uint64_t stride, double* out);
 /* Each shot's final draw (run()'s inline CDF walk, qarp_simulator.cpp:1151-1163):
  * for j < n_shots, out[j] = the first index of block j whose running sum of
  * |a|^2 (re*re + im*im, index order) reaches u[j] times the block's total,

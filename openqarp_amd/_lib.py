@@ -26,7 +26,7 @@ EXPORTS = [
     "qhip_plan_update", "qhip_plan_destroy",
     "qhip_apply", "qhip_apply_terminal", "qhip_state_permute", "qhip_state_controlled_powers",
     "qhip_state_set_basis", "qhip_compact", "qhip_shots_broadcast", "qhip_shots_restore", "qhip_shots_prob0",
-    "qhip_shots_project", "qhip_shots_sample", "qhip_mt_uniforms",
+    "qhip_shots_project", "qhip_shots_apply1q", "qhip_shots_rho1", "qhip_shots_sample", "qhip_mt_uniforms",
     "qhip_transition", "qhip_transition_noalloc", "qhip_apply_observable",
     "qhip_sample", "qhip_sample_uniforms", "qhip_probabilities", "qhip_exact",
     "qhip_run_gradient", "qhip_run_gradient_phi", "qhip_batch_energy_grad",
@@ -73,6 +73,8 @@ def _declare(lib):
         "qhip_shots_prob0": [vp, i32, u64, vp],
         "qhip_shots_project": [vp, i32, u64, vp, vp],
         "qhip_shots_sample": [vp, u64, vp, i64, vp],
+        "qhip_shots_apply1q": [vp, i32, u64, vp, vp],
+        "qhip_shots_rho1": [vp, i32, u64, vp],
         "qhip_mt_uniforms": [u32, i64, vp],
         "qhip_state_controlled_powers": [vp, vp, vp, vp, i32, vp, i32, vp],
         "qhip_transition": [vp, vp, vp, vp],

@@ -331,6 +331,8 @@ void set_basis_states(qhip_state* s, const uint64_t* labels, int64_t n_labels, u
 void shots_broadcast(qhip_state* dst, const qhip_state* src);
 void shots_restore(qhip_state* dst, const qhip_state* saved, const uint8_t* active, uint64_t stride);
 void shots_prob0(const qhip_state* s, int q, uint64_t stride, double* p0);
+void shots_apply1q(qhip_state* s, int q, uint64_t stride, const double* mats, const uint8_t* active);
+void shots_rho1(const qhip_state* s, int q, uint64_t stride, double* out);
 void shots_project(qhip_state* s, int q, uint64_t stride, const int8_t* action, const double* scale);
 void shots_sample(const qhip_state* s, uint64_t stride, const double* u, int64_t n_shots, uint64_t* out);
 // run()'s inline CDF walk over a whole state with uniform u, exact in

@@ -152,6 +152,64 @@ __global__ __launch_bounds__(64) void sample_blocks_kernel(const A* __restrict__
     out[j] = pick;
 }
 
+// Per block j with active[j]: the 2x2 complex matrix mats[8j .. 8j + 8)
+// (row-major (re, im)) on `q` — the noise channels' Pauli and Kraus
+// operators, chosen per shot.
+template <typename A>
+__global__ __launch_bounds__(kBlk) void apply1q_kernel(A* __restrict__ s, int q, int log_stride, uint64_t pairs_total,
+                                                       const double* __restrict__ mats,
+                                                       const uint8_t* __restrict__ active) {
+    const uint64_t step = static_cast<uint64_t>(gridDim.x) * kBlk;
+    const uint64_t low = (uint64_t{1} << q) - 1, bit = uint64_t{1} << q;
+    for (uint64_t p = static_cast<uint64_t>(blockIdx.x) * kBlk + threadIdx.x; p < pairs_total; p += step) {
+        const uint64_t i0 = ((p & ~low) << 1) | (p & low), i1 = i0 | bit;
+        const uint64_t j = i0 >> log_stride;
+        if (!active[j]) continue;
+        const double* m = mats + 8 * j;
+        const A a0 = s[i0], a1 = s[i1];
+        const double x0 = a0.x, y0 = a0.y, x1 = a1.x, y1 = a1.y;
+        A o0, o1;
+        o0.x = (m[0] * x0 - m[1] * y0) + (m[2] * x1 - m[3] * y1);
+        o0.y = (m[0] * y0 + m[1] * x0) + (m[2] * y1 + m[3] * x1);
+        o1.x = (m[4] * x0 - m[5] * y0) + (m[6] * x1 - m[7] * y1);
+        o1.y = (m[4] * y0 + m[5] * x0) + (m[6] * y1 + m[7] * x1);
+        s[i0] = o0;
+        s[i1] = o1;
+    }
+}
+
+// partial[(j * chunks + c) * 4 + {0..3}] = sums over pairs c*kChunk.. of shot
+// j of |a|^2, |b|^2, Re(a conj b), Im(a conj b), with a the amplitude with
+// the qubit's bit clear and b its partner (fixed-order reductions).
+template <typename A>
+__global__ __launch_bounds__(kBlk) void rho1_kernel(const A* __restrict__ s, int q, int log_stride, uint64_t pairs,
+                                                    uint64_t chunks, double* __restrict__ partial) {
+    __shared__ double red[4][kBlk];
+    const uint64_t j = blockIdx.x / chunks, c = blockIdx.x % chunks;
+    const uint64_t lo = c * kChunk, hi = std::min(pairs, lo + kChunk);
+    const uint64_t low = (uint64_t{1} << q) - 1, bit = uint64_t{1} << q;
+    double r00 = 0.0, r11 = 0.0, rx = 0.0, ry = 0.0;
+    for (uint64_t p = lo + threadIdx.x; p < hi; p += kBlk) {
+        const uint64_t i0 = (j << log_stride) | ((p & ~low) << 1) | (p & low);
+        const double ax = s[i0].x, ay = s[i0].y, bx = s[i0 | bit].x, by = s[i0 | bit].y;
+        r00 += ax * ax + ay * ay;
+        r11 += bx * bx + by * by;
+        rx += ax * bx + ay * by;
+        ry += ay * bx - ax * by;
+    }
+    red[0][threadIdx.x] = r00;
+    red[1][threadIdx.x] = r11;
+    red[2][threadIdx.x] = rx;
+    red[3][threadIdx.x] = ry;
+    __syncthreads();
+    for (int w = kBlk / 2; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w)
+            for (int k = 0; k < 4; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) partial[blockIdx.x * 4 + threadIdx.x] = red[threadIdx.x][0];
+}
+
 void check_blocks(const qhip_state* s, uint64_t stride, const char* what) {
     if (stride == 0 || stride > s->dim() || (stride & (stride - 1)) != 0)
         fail(QHIP_INVALID, std::string(what) + ": shot stride must be a power of two <= 2^n");
@@ -252,6 +310,55 @@ void shots_sample(const qhip_state* s, uint64_t stride, const double* u, int64_t
     QHIP_CHECK(hipGetLastError());
     QHIP_CHECK(hipMemcpyAsync(out, d_out, sizeof(uint64_t) * n_shots, hipMemcpyDeviceToHost, C.stream));
     QHIP_CHECK(hipStreamSynchronize(C.stream));
+}
+
+void shots_apply1q(qhip_state* s, int q, uint64_t stride, const double* mats, const uint8_t* active) {
+    check_blocks(s, stride, "qhip_shots_apply1q");
+    if (q < 0 || (uint64_t{1} << q) >= stride) fail(QHIP_INVALID, "qhip_shots_apply1q: qubit outside a shot");
+    auto& C = ctx();
+    const uint64_t nb = s->dim() / stride;
+    char* base = static_cast<char*>(C.misc2.ensure(nb * (8 * sizeof(double) + 1) + 64));
+    auto* d_mats = reinterpret_cast<double*>(base);
+    auto* d_act = reinterpret_cast<uint8_t*>(base + nb * 8 * sizeof(double));
+    QHIP_CHECK(hipMemcpyAsync(d_mats, mats, 8 * sizeof(double) * nb, hipMemcpyHostToDevice, C.stream));
+    QHIP_CHECK(hipMemcpyAsync(d_act, active, nb, hipMemcpyHostToDevice, C.stream));
+    const int ls = std::countr_zero(stride);
+    const uint64_t pairs = s->dim() / 2;
+    if (s->precision == QHIP_C128)
+        hipLaunchKernelGGL(apply1q_kernel<double2>, dim3(blocks_for(pairs)), dim3(kBlk), 0, C.stream,
+                           static_cast<double2*>(s->d), q, ls, pairs, d_mats, d_act);
+    else
+        hipLaunchKernelGGL(apply1q_kernel<float2>, dim3(blocks_for(pairs)), dim3(kBlk), 0, C.stream,
+                           static_cast<float2*>(s->d), q, ls, pairs, d_mats, d_act);
+    QHIP_CHECK(hipGetLastError());
+    QHIP_CHECK(hipStreamSynchronize(C.stream));
+}
+
+void shots_rho1(const qhip_state* s, int q, uint64_t stride, double* out) {
+    check_blocks(s, stride, "qhip_shots_rho1");
+    if (q < 0 || (uint64_t{1} << q) >= stride) fail(QHIP_INVALID, "qhip_shots_rho1: qubit outside a shot");
+    auto& C = ctx();
+    const uint64_t nb = s->dim() / stride, pairs = stride / 2;
+    const uint64_t chunks = (pairs + kChunk - 1) / kChunk;
+    auto* d_part = static_cast<double*>(C.misc2.ensure(4 * sizeof(double) * nb * chunks));
+    const int ls = std::countr_zero(stride);
+    const dim3 grid(static_cast<uint32_t>(nb * chunks));
+    if (s->precision == QHIP_C128)
+        hipLaunchKernelGGL(rho1_kernel<double2>, grid, dim3(kBlk), 0, C.stream, static_cast<const double2*>(s->d), q,
+                           ls, pairs, chunks, d_part);
+    else
+        hipLaunchKernelGGL(rho1_kernel<float2>, grid, dim3(kBlk), 0, C.stream, static_cast<const float2*>(s->d), q,
+                           ls, pairs, chunks, d_part);
+    QHIP_CHECK(hipGetLastError());
+    std::vector<double> part(4 * nb * chunks);
+    QHIP_CHECK(hipMemcpyAsync(part.data(), d_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, C.stream));
+    QHIP_CHECK(hipStreamSynchronize(C.stream));
+    for (uint64_t j = 0; j < nb; ++j)
+        for (int k = 0; k < 4; ++k) {
+            double t = 0.0;
+            for (uint64_t c = 0; c < chunks; ++c) t += part[(j * chunks + c) * 4 + k];
+            out[4 * j + k] = t;
+        }
 }
 
 void shots_project(qhip_state* s, int q, uint64_t stride, const int8_t* action, const double* scale) {

@@ -676,6 +676,20 @@ int qhip_compact(const qhip_state* s, uint64_t n_amps, double tolerance, int64_t
     });
 }
 
+int qhip_shots_apply1q(qhip_state* s, int32_t qubit, uint64_t stride, const double* mats, const uint8_t* active) {
+    return guarded([&] {
+        need(s && mats && active, "qhip_shots_apply1q: NULL argument");
+        qhip::shots_apply1q(s, qubit, stride, mats, active);
+    });
+}
+
+int qhip_shots_rho1(const qhip_state* s, int32_t qubit, uint64_t stride, double* out) {
+    return guarded([&] {
+        need(s && out, "qhip_shots_rho1: NULL argument");
+        qhip::shots_rho1(s, qubit, stride, out);
+    });
+}
+
 int qhip_shots_broadcast(qhip_state* dst, const qhip_state* src) {
     return guarded([&] {
         need(dst && src, "qhip_shots_broadcast: NULL argument");

@@ -135,11 +135,12 @@ class QarpSimulator:
     DEFAULT_FUSION_QUBITS = 3
     DEFAULT_FUSION_MIN_QUBITS = 12
 
-    def __init__(self, *, precision: str = "complex128", tile_qubits: int = 0, low_qubits: int = 0,
-                 max_ops_per_pass: int = 0, use_mfma: int = -1, relabel_qubits: int = -1):
+    def __init__(self, noise_model=None, *, precision: str = "complex128", tile_qubits: int = 0,
+                 low_qubits: int = 0, max_ops_per_pass: int = 0, use_mfma: int = -1, relabel_qubits: int = -1):
         if precision not in ("complex128", "complex64"):
             raise ValueError(f"precision must be complex128 or complex64, got {precision!r}")
         _lib.load()
+        self.noise_model = noise_model   # QarpSimulator(NoiseModel), qarp_simulator.cpp:72-77
         self.precision = precision
         self._fusion_max = _env_fusion_default()
         self._fusion_min = self.DEFAULT_FUSION_MIN_QUBITS
@@ -237,9 +238,21 @@ class QarpSimulator:
                                       n.value, C.byref(n)))
         return keys, probs
 
+    # ── noise (the trajectory path only) ──
+    def noise_active(self) -> bool:
+        """noise_active (qarp_simulator.h:266-268): enabled and any channel."""
+        return self.noise_model is not None and self.noise_model.active()
+
+    def _refuse_noise(self, where: str, what: str) -> None:
+        if self.noise_active():
+            raise RuntimeError(f"QarpSimulator::{where}: noise model is active — {what}")
+
     # ── reference surface ──
     def statevector(self, commands: Sequence[Command], n_qubits: int, *, initial_state=None) -> np.ndarray:
         commands = as_commands(commands)
+        self._refuse_noise("statevector", "the noisy ensemble is not representable as a single statevector. "
+                                          "Either disable the noise model (noise_model.enabled=False) or use "
+                                          "run() for trajectory simulation.")
         st = self.new_state(n_qubits, initial_state, "statevector")
         self.apply(commands, st)
         return st.download()
@@ -247,6 +260,7 @@ class QarpSimulator:
     def expectation(self, commands: Sequence[Command], n_qubits: int, observable: Observable, *,
                     initial_state=None) -> float:
         commands = as_commands(commands)
+        self._refuse_noise("statevector", "the noisy ensemble is not representable as a single statevector.")
         st = self.new_state(n_qubits, initial_state, "expectation")
         self.apply(commands, st)
         return self.expectation_state(st, observable)
@@ -286,9 +300,10 @@ class QarpSimulator:
         with mid-circuit measurement, Reset, conditions or branches take the
         shot-batched trajectory path (trajectory.py, :1038-1211)."""
         commands = as_commands(commands)
-        if not measurements_terminal(commands):
+        if self.noise_active() or not measurements_terminal(commands):
             from .trajectory import run_trajectories
-            return run_trajectories(self, commands, n_qubits, n_shots, seed, initial_state)
+            return run_trajectories(self, commands, n_qubits, n_shots, seed, initial_state,
+                                    self.noise_model if self.noise_active() else None)
         st = self.new_state(n_qubits, initial_state, "run")
         self.apply(commands, st, terminal=True)
         return self._terminal_result(st, commands, n_qubits, n_shots, seed)
@@ -347,6 +362,8 @@ class QarpSimulator:
     def program_statevector(self, program: Program, n_qubits: int, *, initial_state=None) -> np.ndarray:
         """statevector() for a structured program (qarp_simulator.cpp:822-849)."""
         self._check_program_width(program, n_qubits, "program_statevector")
+        self._refuse_noise("program_statevector", "the noisy ensemble is not representable as a single "
+                                                  "statevector.")
         for k in program.kernels():
             if k[0] == "gates" and any(needs_trajectory(c) for c in k[1]):
                 raise RuntimeError("QarpSimulator::program_statevector: program contains mid-circuit "
@@ -361,6 +378,7 @@ class QarpSimulator:
         """run()'s terminal sample-once path for a structured program
         (qarp_simulator.cpp:851-888)."""
         self._check_program_width(program, n_qubits, "program_run")
+        self._refuse_noise("program_run", "structured programs run the noiseless sample-once path only.")
         kernels = program.kernels()
         last: Sequence[Command] = []
         for i, k in enumerate(kernels):

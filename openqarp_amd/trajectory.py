@@ -21,10 +21,18 @@ the amplitude block [j·2^n, (j+1)·2^n):
   their running sum in index order, threshold = u · total, first index whose
   running sum reaches it.
 
+With an active noise model (noise.py) the per-shot loop starts at the first
+command, as the reference's does (split 0, qarp_simulator.cpp:1045), and every
+applied gate whose type carries a channel is followed by that channel on the
+shots that applied it (apply_post_gate_noise, :697-722): each shot draws from
+its own stream, and the sampled Pauli or Kraus operator is applied per shot
+(qhip_shots_apply1q, with qhip_shots_rho1 for the Kraus probabilities).
+
 Random numbers are consumed per shot exactly as the reference does (one per
-executed Measure or Reset, one for the final draw), so outcomes agree with
-the reference's except where a comparison lands within rounding of a
-probability."""
+executed Measure or Reset, one per noisy gate — plus a uniform integer when a
+many-qubit Pauli channel fires — and one for the final draw), so outcomes
+agree with the reference's except where a comparison lands within rounding of
+a probability."""
 from __future__ import annotations
 
 import ctypes as C
@@ -40,9 +48,10 @@ from .program import needs_trajectory
 MAX_BATCH_AMPS = 1 << 24        # amplitudes of one batched state
 
 
-def _parse(cmds: Sequence[Command], i: int = 0, stop=()) -> tuple:
+def _parse(cmds: Sequence[Command], i: int = 0, stop=(), noise=None) -> tuple:
     """Items of cmds[i:] up to a stop marker: ("gates", [..]) runs,
-    ("op", cmd) for Measure / Reset / conditional commands, and
+    ("op", cmd, channel) for Measure / Reset / conditional commands and for
+    gates followed by a noise channel (channel None otherwise), and
     ("branch", begin, then_items, else_items)."""
     items = []
     run: list = []
@@ -53,12 +62,13 @@ def _parse(cmds: Sequence[Command], i: int = 0, stop=()) -> tuple:
             run.clear()
     while i < len(cmds):
         c = cmds[i]
+        channel = noise.channel_for(c) if noise is not None else None
         if c.gate == GateType.BranchBegin:
             flush()
-            then, i = _parse(cmds, i + 1, (GateType.BranchElse, GateType.BranchEnd))
+            then, i = _parse(cmds, i + 1, (GateType.BranchElse, GateType.BranchEnd), noise)
             els = []
             if cmds[i].gate == GateType.BranchElse:
-                els, i = _parse(cmds, i + 1, (GateType.BranchEnd,))
+                els, i = _parse(cmds, i + 1, (GateType.BranchEnd,), noise)
             items.append(("branch", c, then, els))
             i += 1
         elif c.gate in (GateType.BranchElse, GateType.BranchEnd):
@@ -66,9 +76,9 @@ def _parse(cmds: Sequence[Command], i: int = 0, stop=()) -> tuple:
                 flush()
                 return items, i
             raise RuntimeError(f"branch table: {c.gate.name} without matching BranchBegin")
-        elif c.gate in (GateType.Measure, GateType.Reset) or c.condition_bits:
+        elif c.gate in (GateType.Measure, GateType.Reset) or c.condition_bits or channel is not None:
             flush()
-            items.append(("op", c))
+            items.append(("op", c, channel))
             i += 1
         elif c.gate == GateType.Barrier:
             i += 1
@@ -82,6 +92,7 @@ def _parse(cmds: Sequence[Command], i: int = 0, stop=()) -> tuple:
 
 
 def _draws(items) -> int:
+    """Uniforms a shot can draw along every branch (noiseless items)."""
     n = 0
     for it in items:
         if it[0] == "op" and it[1].gate in (GateType.Measure, GateType.Reset):
@@ -89,6 +100,26 @@ def _draws(items) -> int:
         elif it[0] == "branch":
             n += _draws(it[2]) + _draws(it[3])
     return n
+
+
+class _Precomputed:
+    """The uniforms of noiseless shots, drawn ahead on the library's host side
+    (qhip_mt_uniforms), read per shot in order."""
+
+    def __init__(self, u: np.ndarray):
+        self.u = u
+        self.ptr = np.zeros(u.shape[0], dtype=np.int64)
+
+    def uniform(self, idx: np.ndarray) -> np.ndarray:
+        out = self.u[idx, self.ptr[idx]]
+        self.ptr[idx] += 1
+        return out
+
+
+_X = np.array([0, 0, 1, 0, 1, 0, 0, 0], dtype=np.float64)
+_Y = np.array([0, 0, 0, -1, 0, 1, 0, 0], dtype=np.float64)
+_Z = np.array([1, 0, 0, 0, 0, 0, -1, 0], dtype=np.float64)
+_PAULI_MATS = {1: _X, 2: _Y, 3: _Z}
 
 
 def _holds(bits, values, regs: np.ndarray) -> np.ndarray:
@@ -99,7 +130,7 @@ def _holds(bits, values, regs: np.ndarray) -> np.ndarray:
 
 
 class _Batch:
-    def __init__(self, sim, n: int, b: int, n_cbits: int, uniforms: np.ndarray, plans: dict):
+    def __init__(self, sim, n: int, b: int, n_cbits: int, streams, plans: dict):
         from .simulator import DeviceState
         self.sim = sim
         self.plans = plans   # per call: id(run) -> (run, CompiledCircuit on n + b qubits), bare conditionals
@@ -107,8 +138,7 @@ class _Batch:
         self.state = DeviceState(n + b, sim.precision)
         self.saved: Optional[DeviceState] = None
         self.regs = np.zeros((self.B, n_cbits), dtype=bool)
-        self.u = uniforms                       # (B, draws + 1)
-        self.ptr = np.zeros(self.B, dtype=np.int64)
+        self.streams = streams                  # per-shot uniforms: _Precomputed or ShotStreams
         self.lib = _lib.load()
 
     def apply(self, cmds) -> None:
@@ -149,8 +179,7 @@ class _Batch:
         p0 = np.zeros(self.B)
         _lib.check(self.lib.qhip_shots_prob0(self.state.handle, q, self.stride, p0.ctypes.data))
         idx = np.nonzero(active)[0]
-        u = self.u[idx, self.ptr[idx]]
-        self.ptr[idx] += 1
+        u = self.streams.uniform(idx)
         one = u >= p0[idx]
         with np.errstate(divide="ignore"):
             p = np.where(one, np.maximum(1.0 - p0[idx], 0.0), np.maximum(p0[idx], 0.0))
@@ -163,6 +192,97 @@ class _Batch:
                                                scale.ctypes.data))
         if cbits and not reset:
             self.regs[idx, cbits[0]] = one
+
+    def paulis(self, q: int, codes: np.ndarray) -> None:
+        """Per shot the Pauli codes[j] (0 I, 1 X, 2 Y, 3 Z) on qubit q."""
+        act = np.ascontiguousarray(codes != 0, dtype=np.uint8)
+        if not act.any():
+            return
+        mats = np.zeros((self.B, 8))
+        for c, m in _PAULI_MATS.items():
+            mats[codes == c] = m
+        _lib.check(self.lib.qhip_shots_apply1q(self.state.handle, int(q), self.stride, mats.ctypes.data,
+                                               act.ctypes.data))
+
+    def channel(self, cmd, ch, active: np.ndarray) -> None:
+        """apply_channel (device/noise_model.cpp:86-186) on the shots in
+        active, each drawing from its own stream."""
+        from .noise import KrausChannel, PauliChannel, PauliChannel2q, PauliChannelKq
+        idx = np.nonzero(active)[0]
+        if idx.size == 0 or ch is None:
+            return
+        qs = list(cmd.qubits)
+        if isinstance(ch, PauliChannel):
+            if len(qs) != 1:
+                raise RuntimeError("apply_channel: PauliChannel requires exactly 1 target qubit")
+            r = self.streams.uniform(idx)
+            a1 = ch.p_x
+            a2 = a1 + ch.p_y
+            a3 = a2 + ch.p_z
+            codes = np.zeros(self.B, dtype=np.int64)
+            codes[idx] = np.where(r < a1, 1, np.where(r < a2, 2, np.where(r < a3, 3, 0)))
+            self.paulis(qs[0], codes)
+        elif isinstance(ch, PauliChannel2q):
+            if len(qs) != 2:
+                raise RuntimeError("apply_channel: PauliChannel2q requires exactly 2 target qubits")
+            r = self.streams.uniform(idx)
+            acc = np.empty(15)
+            t = 0.0
+            for k in range(15):
+                t += ch.p[k]
+                acc[k] = t
+            hit = r[:, None] < acc[None, :]
+            first = np.where(hit.any(axis=1), hit.argmax(axis=1), -1)
+            packed = np.zeros(self.B, dtype=np.int64)
+            packed[idx] = np.where(first >= 0, first + 1, 0)
+            self.paulis(qs[0], packed & 3)
+            self.paulis(qs[1], (packed >> 2) & 3)
+        elif isinstance(ch, PauliChannelKq):
+            k = len(qs)
+            if k == 0 or k > 31:
+                return
+            r = self.streams.uniform(idx)
+            codes = np.zeros((self.B, k), dtype=np.int64)
+            for j in idx[r < ch.p_total]:
+                code = self.streams.uniform_int(int(j), 1, (1 << (2 * k)) - 1)
+                codes[j] = [(code >> (2 * i)) & 3 for i in range(k)]
+            for i in range(k):
+                self.paulis(qs[i], codes[:, i])
+        elif isinstance(ch, KrausChannel):
+            if len(qs) != 1:
+                raise RuntimeError("apply_channel: KrausChannel requires exactly 1 target qubit")
+            if not ch.K:
+                return
+            rho4 = np.zeros((self.B, 4))
+            _lib.check(self.lib.qhip_shots_rho1(self.state.handle, qs[0], self.stride, rho4.ctypes.data))
+            mats = np.zeros((self.B, 8))
+            act = np.zeros(self.B, dtype=np.uint8)
+            for j in idx:
+                rho = np.array([[rho4[j, 0], complex(rho4[j, 2], rho4[j, 3])],
+                                [complex(rho4[j, 2], -rho4[j, 3]), rho4[j, 1]]], dtype=np.complex128)
+                probs = [max(0.0, float(np.trace(K @ rho @ K.conj().T).real)) for K in ch.K]
+                total = 0.0
+                for p in probs:
+                    total += p
+                if total <= 0.0:
+                    continue                    # no draw, as the reference returns before its uniform
+                probs = [p / total for p in probs]
+                u = float(self.streams.uniform(np.array([j]))[0])
+                acc = 0.0
+                for i, p in enumerate(probs):
+                    acc += p
+                    if u < acc:
+                        p_unnorm = p * total
+                        if p_unnorm > 0.0:
+                            m = np.asarray(ch.K[i], dtype=np.complex128) * (1.0 / np.sqrt(p_unnorm))
+                            mats[j] = m.view(np.float64).ravel()
+                            act[j] = 1
+                        break
+            if act.any():
+                _lib.check(self.lib.qhip_shots_apply1q(self.state.handle, qs[0], self.stride, mats.ctypes.data,
+                                                       act.ctypes.data))
+        else:
+            raise RuntimeError(f"apply_channel: unknown channel {ch!r}")
 
     def run(self, items, active: np.ndarray) -> None:
         for it in items:
@@ -183,11 +303,13 @@ class _Batch:
                     self.collapse(c.qubits[0], list(c.cbits), c.gate == GateType.Reset, act)
                 else:
                     self.gates(self.bare(it), act)
+                    if it[2] is not None:
+                        self.channel(c, it[2], act)
 
     def sample(self, count: int) -> np.ndarray:
         """Each shot's final draw on the device (qhip_shots_sample), with the
         shot's next uniform."""
-        u = np.ascontiguousarray(self.u[np.arange(count), self.ptr[:count]])
+        u = np.ascontiguousarray(self.streams.uniform(np.arange(count)))
         out = np.zeros(count, dtype=np.uint64)
         _lib.check(self.lib.qhip_shots_sample(self.state.handle, self.stride, u.ctypes.data, count,
                                               out.ctypes.data))
@@ -195,16 +317,21 @@ class _Batch:
 
 
 def run_trajectories(sim, commands: Sequence[Command], n_qubits: int, n_shots: int,
-                     seed: Optional[int] = None, initial_state=None) -> SamplingResult:
-    """QarpSimulator::run's trajectory path on the device (shots in batches)."""
+                     seed: Optional[int] = None, initial_state=None, noise=None) -> SamplingResult:
+    """QarpSimulator::run's trajectory path on the device (shots in batches);
+    ``noise`` an active NoiseModel or None."""
+    from .noise import ShotStreams
     n = int(n_qubits)
     n_shots = int(n_shots)
     if n_shots < 0:
         raise ValueError("QarpSimulator::run: n_shots must be non-negative")
     base_s = (secrets.randbits(32) if seed is None else int(seed)) & 0xFFFFFFFF
     commands = list(commands)
-    split = next((i for i, c in enumerate(commands) if needs_trajectory(c)), len(commands))
-    items, _ = _parse(commands[split:])
+    if noise is not None and not noise.active():
+        noise = None
+    split = 0 if noise is not None else next((i for i, c in enumerate(commands) if needs_trajectory(c)),
+                                             len(commands))
+    items, _ = _parse(commands[split:], noise=noise)
     n_draws = _draws(items) + 1
     n_cbits = cbit_register_width(commands)
     base = sim.new_state(n, initial_state, "run")
@@ -219,11 +346,15 @@ def run_trajectories(sim, commands: Sequence[Command], n_qubits: int, n_shots: i
     b = min(max(n_shots - 1, 0).bit_length(), max(0, MAX_BATCH_AMPS.bit_length() - 1 - n))
     for shot0 in range(0, n_shots, 1 << b):
         count = min(1 << b, n_shots - shot0)
-        u = np.zeros((1 << b, n_draws))
-        for s in range(count):
-            _lib.check(lib.qhip_mt_uniforms(C.c_uint32((base_s + shot0 + s) & 0xFFFFFFFF), n_draws,
-                                            u[s].ctypes.data))
-        batch = _Batch(sim, n, b, n_cbits, u, plans)
+        if noise is not None:
+            streams = ShotStreams(base_s, shot0, 1 << b, 2 * n_draws + 64)
+        else:
+            u = np.zeros((1 << b, n_draws))
+            for s in range(count):
+                _lib.check(lib.qhip_mt_uniforms(C.c_uint32((base_s + shot0 + s) & 0xFFFFFFFF), n_draws,
+                                                u[s].ctypes.data))
+            streams = _Precomputed(u)
+        batch = _Batch(sim, n, b, n_cbits, streams, plans)
         _lib.check(lib.qhip_shots_broadcast(batch.state.handle, base.handle))
         active = np.zeros(1 << b, dtype=bool)
         active[:count] = True

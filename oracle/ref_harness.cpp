@@ -13,6 +13,7 @@
 
 #include "qarpx/core/command.h"
 #include "qarpx/core/errors.h"
+#include "qarpx/device/noise_model.h"
 #include "qarpx/simulator/fusion.h"
 #include "qarpx/simulator/pauli_expectation.h"
 #include "qarpx/simulator/program.h"
@@ -297,6 +298,68 @@ int ref_run_trajectory(const qhip_stream* s, const int32_t* cond_count, const ui
         return 0;
     } catch (const std::invalid_argument& e) { return catch_all(e, QHIP_INVALID);
     } catch (const std::exception& e) { return catch_all(e, QHIP_RUNTIME); }
+}
+
+// One builder call of a noise model, as the tests describe it.
+struct ref_noise_entry {
+    int32_t        kind;        // 0 make_depolarizing(p0), 1 make_pauli(p0, p1, p2),
+                                // 2 make_bit_flip(p0), 3 make_amplitude_damping(p0)
+    int32_t        n_gates;
+    double         p[3];
+    const int32_t* gates;
+};
+
+// QarpSimulator(NoiseModel)::run with the model composed (operator+) from the
+// reference's own builders, and `enabled`; otherwise as ref_run_trajectory.
+int ref_run_noisy(const qhip_stream* s, const int32_t* cond_count, const uint32_t* cond_bits,
+                  const uint8_t* cond_values, const ref_noise_entry* entries, int n_entries, int enabled, int n,
+                  int n_shots, uint32_t seed, int64_t cap, int64_t* n_keys, uint64_t* keys, int32_t* counts,
+                  int32_t* n_cbits, uint8_t* hist, int64_t hist_cap) {
+    try {
+        NoiseModel model;
+        for (int e = 0; e < n_entries; ++e) {
+            std::vector<GateType> gates;
+            for (int g = 0; g < entries[e].n_gates; ++g) gates.push_back(static_cast<GateType>(entries[e].gates[g]));
+            const double* p = entries[e].p;
+            switch (entries[e].kind) {
+                case 0: model += make_depolarizing(p[0], gates); break;
+                case 1: model += make_pauli(p[0], p[1], p[2], gates); break;
+                case 2: model += make_bit_flip(p[0], gates); break;
+                default: model += make_amplitude_damping(p[0], gates);
+            }
+        }
+        model.enabled = enabled != 0;
+        QarpSimulator sim(model);
+        auto cmds = to_commands(s);
+        int64_t k0 = 0;
+        for (size_t i = 0; i < cmds.size(); ++i)
+            for (int j = 0; j < cond_count[i]; ++j, ++k0) {
+                cmds[i].condition_bits.push_back(cond_bits[k0]);
+                cmds[i].condition_values.push_back(cond_values[k0] != 0);
+            }
+        auto res = sim.run(cmds, n, n_shots, seed);
+        *n_keys = static_cast<int64_t>(res.counts.size());
+        std::vector<std::pair<uint64_t, int>> kv(res.counts.begin(), res.counts.end());
+        std::sort(kv.begin(), kv.end());
+        for (int64_t i = 0; i < static_cast<int64_t>(kv.size()) && i < cap; ++i) {
+            keys[i] = kv[i].first;
+            counts[i] = kv[i].second;
+        }
+        *n_cbits = res.n_cbits;
+        int64_t k = 0;
+        for (const auto& reg : res.cbit_history)
+            for (bool bit : reg)
+                if (k < hist_cap) hist[k++] = bit ? 1 : 0;
+        return 0;
+    } catch (const std::invalid_argument& e) { return catch_all(e, QHIP_INVALID);
+    } catch (const std::exception& e) { return catch_all(e, QHIP_RUNTIME); }
+}
+
+// std::uniform_int_distribution<uint64_t>(lo, hi) over std::mt19937(seed), n draws.
+void ref_uniform_int(uint32_t seed, uint64_t lo, uint64_t hi, int64_t n, uint64_t* out) {
+    std::mt19937 g(seed);
+    std::uniform_int_distribution<uint64_t> d(lo, hi);
+    for (int64_t i = 0; i < n; ++i) out[i] = d(g);
 }
 
 // The toolchain's std::mt19937 words and generate_canonical<double,53> values.

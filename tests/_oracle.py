@@ -78,6 +78,9 @@ def reference():
         lib.ref_program_statevector.argtypes = [vp, i32, i32, i32, i32, vp, vp]
         lib.ref_run_trajectory.argtypes = [vp, vp, vp, vp, i32, i32, C.c_uint32, i64, vp, vp, vp, vp, vp, i64]
         lib.ref_statevector_fingerprint.argtypes = [vp, i32, i32, i32, vp, i64, vp, vp, vp]
+        lib.ref_run_noisy.argtypes = [vp, vp, vp, vp, vp, i32, i32, i32, i32, C.c_uint32, i64, vp, vp, vp, vp, vp,
+                                      i64]
+        lib.ref_uniform_int.argtypes = [C.c_uint32, C.c_uint64, C.c_uint64, i64, vp]
         _ref = lib
     return _ref
 
@@ -359,7 +362,174 @@ def program_statevector(kernels, n, init=None, fusion_max=3, fusion_min=12, lib=
 
 # ── trajectories (QarpSimulator::run with mid-circuit measurement) ───────────
 
-def run_trajectory(commands, n, n_shots, seed, lib="orc"):
+# ── noise (the trajectory path) ─────────────────────────────────────────────
+# A model is described as builder calls [(kind, (p0, p1, p2), [gate ids])]:
+# kind 0 make_depolarizing(p0), 1 make_pauli(p0, p1, p2), 2 make_bit_flip(p0),
+# 3 make_amplitude_damping(p0), composed left to right (the right one wins
+# per gate), as the reference's operator+ does.
+
+class _RefNoiseEntry(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("n_gates", C.c_int32), ("p", C.c_double * 3), ("gates", C.c_void_p)]
+
+
+def _two_qubit(g):
+    from openqarp_amd.ir import GateType as G
+    return G(g) in {G.CX, G.CY, G.CZ, G.SWAP, G.ECR, G.iSWAP, G.iSWAPdg, G.CH, G.CS, G.CSdg, G.CSX, G.CSXdg,
+                    G.CRx, G.CRy, G.CRz, G.CP, G.RZZ, G.RXX, G.RYY, G.CU}
+
+
+def noise_channels(entries):
+    """{gate id: channel} of a described model, restating the builders
+    (device/noise_model.cpp:221-299): ("pauli", px, py, pz), ("pauli2", [15]),
+    ("kq", p), ("kraus", [K0, K1])."""
+    from openqarp_amd.ir import GateType as G
+    out = {}
+    for kind, p, gates in entries:
+        for g in gates:
+            g = int(g)
+            if kind == 0:
+                if g == int(G.MCZ):
+                    out[g] = ("kq", p[0])
+                elif _two_qubit(g):
+                    out[g] = ("pauli2", [p[0] / 15.0] * 15)
+                else:
+                    out[g] = ("pauli", p[0] / 3.0, p[0] / 3.0, p[0] / 3.0)
+            elif kind in (1, 2):
+                out[g] = ("pauli", p[0], p[1] if kind == 1 else 0.0, p[2] if kind == 1 else 0.0)
+            else:
+                k0 = np.array([[1.0, 0.0], [0.0, np.sqrt(1.0 - p[0])]], dtype=np.complex128)
+                k1 = np.array([[0.0, np.sqrt(p[0])], [0.0, 0.0]], dtype=np.complex128)
+                out[g] = ("kraus", [k0, k1])
+    return out
+
+
+class _Words:
+    """One shot's std::mt19937 stream (oracle words), read as uniforms
+    (generate_canonical from two words) and as uniform_int_distribution's
+    integers (Lemire's multiply-and-reject on 32-bit words)."""
+
+    def __init__(self, seed, n):
+        self.w = mt19937(seed, n).astype(np.uint64)
+        self.i = 0
+
+    def word(self):
+        v = int(self.w[self.i])
+        self.i += 1
+        return v
+
+    def uniform(self):
+        w0, w1 = self.word(), self.word()
+        u = (float(w0) + float(w1) * 4294967296.0) / 18446744073709551616.0
+        return u if u < 1.0 else float(np.nextafter(1.0, 0.0))
+
+    def below(self, r):
+        if r < 0xFFFFFFFF:
+            m = r + 1
+            prod = self.word() * m
+            if (prod & 0xFFFFFFFF) < m:
+                t = ((1 << 32) - m) % m
+                while (prod & 0xFFFFFFFF) < t:
+                    prod = self.word() * m
+            return prod >> 32
+        if r == 0xFFFFFFFF:
+            return self.word()
+        while True:
+            tmp = (1 << 32) * self.below(r >> 32)
+            ret = tmp + self.word()
+            if tmp <= ret <= r:
+                return ret
+
+
+def uniform_ints(seed, lo, hi, count, lib="orc"):
+    """count draws of std::uniform_int_distribution<uint64_t>(lo, hi) over
+    std::mt19937(seed)."""
+    if lib == "ref":
+        out = np.zeros(count, np.uint64)
+        reference().ref_uniform_int(seed, lo, hi, count, out.ctypes.data)
+        return out
+    w = _Words(seed, 4 * count + 64)
+    return np.array([lo + w.below(hi - lo) for _ in range(count)], dtype=np.uint64)
+
+
+def _pauli_1q(psi, q, letter):
+    idx = np.arange(psi.size)
+    bit = ((idx >> q) & 1).astype(bool)
+    if letter == 1:
+        return psi[idx ^ (1 << q)]
+    if letter == 2:
+        src = psi[idx ^ (1 << q)]
+        return np.where(bit, 1j * src, -1j * src)
+    if letter == 3:
+        return np.where(bit, -psi, psi)
+    return psi
+
+
+def _apply_channel(ch, qs, psi, words):
+    """apply_channel restated (device/noise_model.cpp:86-186)."""
+    if ch[0] == "pauli":
+        r = words.uniform()
+        acc = ch[1]
+        if r < acc:
+            return _pauli_1q(psi, qs[0], 1)
+        acc += ch[2]
+        if r < acc:
+            return _pauli_1q(psi, qs[0], 2)
+        acc += ch[3]
+        if r < acc:
+            return _pauli_1q(psi, qs[0], 3)
+        return psi
+    if ch[0] == "pauli2":
+        r = words.uniform()
+        acc = 0.0
+        for i, p in enumerate(ch[1]):
+            acc += p
+            if r < acc:
+                packed = i + 1
+                psi = _pauli_1q(psi, qs[0], packed & 3)
+                return _pauli_1q(psi, qs[1], (packed >> 2) & 3)
+        return psi
+    if ch[0] == "kq":
+        k = len(qs)
+        if k == 0 or k > 31:
+            return psi
+        if words.uniform() >= ch[1]:
+            return psi
+        code = 1 + words.below((1 << (2 * k)) - 2)
+        for i in range(k):
+            psi = _pauli_1q(psi, qs[i], (code >> (2 * i)) & 3)
+        return psi
+    q = qs[0]
+    idx = np.arange(psi.size)
+    zero = ((idx >> q) & 1) == 0
+    a, b = psi[zero], psi[idx[zero] | (1 << q)]
+    r00 = np.add.accumulate((np.conj(a) * a).real)[-1]
+    r11 = np.add.accumulate((np.conj(b) * b).real)[-1]
+    r01 = np.add.accumulate(a * np.conj(b))[-1]
+    rho = np.array([[r00, r01], [np.conj(r01), r11]], dtype=np.complex128)
+    probs = [max(0.0, float(np.trace(K @ rho @ K.conj().T).real)) for K in ch[1]]
+    total = 0.0
+    for p in probs:
+        total += p
+    if total <= 0.0:
+        return psi
+    probs = [p / total for p in probs]
+    r = words.uniform()
+    acc = 0.0
+    for i, p in enumerate(probs):
+        acc += p
+        if r < acc:
+            pu = p * total
+            if pu <= 0.0:
+                return psi
+            m = ch[1][i] * (1.0 / np.sqrt(pu))
+            out = psi.copy()
+            out[zero] = m[0, 0] * a + m[0, 1] * b
+            out[idx[zero] | (1 << q)] = m[1, 0] * a + m[1, 1] * b
+            return out
+    return psi
+
+
+def run_trajectory(commands, n, n_shots, seed, lib="orc", noise=None, enabled=True):
     """(counts dict, cbit history list) of QarpSimulator::run.  lib="ref":
     the reference build.  lib="orc": a per-shot restatement of the
     reference's trajectory loop (qarp_simulator.cpp:1038-1211,
@@ -384,14 +554,28 @@ def run_trajectory(commands, n, n_shots, seed, lib="orc"):
         ccount = np.array([len(c.condition_bits) for c in commands], dtype=np.int32)
         cbits = np.array([b for c in commands for b in c.condition_bits] or [0], dtype=np.uint32)
         cvals = np.array([bool(v) for c in commands for v in c.condition_values] or [0], dtype=np.uint8)
-        _check(L, L.ref_run_trajectory(ps.ref, ccount.ctypes.data, cbits.ctypes.data, cvals.ctypes.data, n,
-                                       n_shots, seed, cap, C.byref(nk), keys.ctypes.data, cnt.ctypes.data,
-                                       C.byref(nc), hist.ctypes.data, hist.size), "ref")
+        if noise is None:
+            _check(L, L.ref_run_trajectory(ps.ref, ccount.ctypes.data, cbits.ctypes.data, cvals.ctypes.data, n,
+                                           n_shots, seed, cap, C.byref(nk), keys.ctypes.data, cnt.ctypes.data,
+                                           C.byref(nc), hist.ctypes.data, hist.size), "ref")
+        else:
+            gate_arrays = [np.ascontiguousarray(g, dtype=np.int32) for _, _, g in noise]
+            ents = (_RefNoiseEntry * max(1, len(noise)))()
+            for e, ((kind, p, _), ga) in enumerate(zip(noise, gate_arrays)):
+                ents[e].kind, ents[e].n_gates = kind, ga.size
+                for k in range(3):
+                    ents[e].p[k] = float(p[k]) if k < len(p) else 0.0
+                ents[e].gates = ga.ctypes.data
+            _check(L, L.ref_run_noisy(ps.ref, ccount.ctypes.data, cbits.ctypes.data, cvals.ctypes.data, ents,
+                                      len(noise), 1 if enabled else 0, n, n_shots, seed, cap, C.byref(nk),
+                                      keys.ctypes.data, cnt.ctypes.data, C.byref(nc), hist.ctypes.data,
+                                      hist.size), "ref")
         counts = {int(keys[i]): int(cnt[i]) for i in range(nk.value)}
         h = hist[: n_shots * nc.value].reshape(n_shots, nc.value).astype(bool)
         return counts, [list(map(bool, r)) for r in h]
     cmds = list(commands)
-    split = next((i for i, c in enumerate(cmds) if needs_trajectory(c)), len(cmds))
+    channels = noise_channels(noise) if (noise and enabled) else {}
+    split = 0 if channels else next((i for i, c in enumerate(cmds) if needs_trajectory(c)), len(cmds))
     base = statevector(pack_stream(cmds[:split]), n, 3, 12)
     suffix = cmds[split:]
     width = cbit_register_width(cmds)
@@ -407,16 +591,14 @@ def run_trajectory(commands, n, n_shots, seed, lib="orc"):
             b, e = stack.pop()
             ends[b] = (e, i)
     counts, hist = {}, []
+    noiseless = {GateType.Measure, GateType.Reset, GateType.Barrier, GateType.GPhase}
     for shot in range(n_shots):
-        u = uniforms((seed + shot) & 0xFFFFFFFF, 4 * len(suffix) + 2)
-        k = [0]
+        words = _Words((seed + shot) & 0xFFFFFFFF, 8 * len(suffix) + 256)
         reg = [False] * width
         psi = base.copy()
 
         def draw():
-            v = u[k[0]]
-            k[0] += 1
-            return v
+            return words.uniform()
 
         def holds(c):
             return all(reg[b] == bool(v) for b, v in zip(c.condition_bits, c.condition_values))
@@ -454,6 +636,8 @@ def run_trajectory(commands, n, n_shots, seed, lib="orc"):
                 elif c.gate != GateType.Barrier:
                     bare = Command(c.gate, qubits=c.qubits, params=c.params, unitary=c.unitary)
                     psi = statevector(pack_stream([bare]), n, 0, 0, init=psi)
+                if c.gate not in noiseless and int(c.gate) in channels:
+                    psi = _apply_channel(channels[int(c.gate)], list(c.qubits), psi, words)
         run(0, len(suffix))
         probs = psi.real * psi.real + psi.imag * psi.imag
         run = np.add.accumulate(probs)             # the sequential running sum from 0.0

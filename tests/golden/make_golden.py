@@ -144,6 +144,29 @@ def trajectory_fixture(name, n, seed, shots, shot_seed):
                         history=np.array(hist, dtype=bool))
 
 
+def uniform_int_fixture(name):
+    """std::uniform_int_distribution<uint64_t>(lo, hi) draws of the reference
+    build's toolchain, including a range of 2^32 and more (upscaling)."""
+    cases = [(77, 1, 15), (5, 1, 63), (1234, 1, 4**10 - 1), (9, 1, 4**15 - 1), (31, 0, 2**40), (2, 1, 4**20 - 1),
+             (8, 1, 3)]
+    draws = np.stack([O.uniform_ints(seed, lo, hi, 500, lib="ref") for seed, lo, hi in cases])
+    np.savez_compressed(HERE / name, cases=np.array(cases, dtype=np.uint64), draws=draws)
+
+
+def noisy_trajectory_fixture(name, n, seed, shots, shot_seed):
+    """QarpSimulator(NoiseModel)::run of the reference build for every model
+    of tests/test_noise.py on its noisy circuit: counts and registers."""
+    from test_noise import models, noisy_circuit
+    out = dict(n=n, seed=seed, shots=shots, shot_seed=shot_seed)
+    for m, entries in models().items():
+        counts, hist = O.run_trajectory(noisy_circuit(n, seed), n, shots, shot_seed, lib="ref", noise=entries)
+        keys = np.array(sorted(counts), dtype=np.int64)
+        out[f"{m}_keys"] = keys
+        out[f"{m}_counts"] = np.array([counts[k] for k in keys.tolist()], dtype=np.int64)
+        out[f"{m}_history"] = np.array(hist, dtype=bool)
+    np.savez_compressed(HERE / name, **out)
+
+
 # ── the workloads at their stated sizes ──────────────────────────────────
 # States of 2^20..2^30 amplitudes are too large to commit, so these fixtures
 # hold fingerprints: the amplitudes at 4096 seeded indices, the norm, <Z_q>
@@ -247,6 +270,10 @@ def main_full(which):
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--full":
         return main_full(sys.argv[2:])
+    if len(sys.argv) > 1 and sys.argv[1] == "--noise":
+        uniform_int_fixture("uniform_int.npz")
+        noisy_trajectory_fixture("noisy_trajectory5.npz", 5, 4, 300, 4321)
+        return
     if O.reference() is None:
         raise SystemExit("oracle/_ref/libqarp_ref.so is not built (make -C oracle)")
     hea_fixture("hea12.npz", 12, 4, 7, 4000, 1234)
@@ -260,6 +287,8 @@ def main():
     sample_fixture("sample10.npz", 10, 21, 5000, 77)
     program_fixture("program10.npz", 10, 8)
     trajectory_fixture("trajectory6.npz", 6, 2, 400, 555)
+    uniform_int_fixture("uniform_int.npz")
+    noisy_trajectory_fixture("noisy_trajectory5.npz", 5, 4, 300, 4321)
     print("golden fixtures written to", HERE)
 
 
