@@ -67,14 +67,16 @@ def fused_apply(args, sim):
     log(f"brickwork n={n} layers={args.layers}: {len(cmds)} gates; autotuning")
     widths = (args.fusion,) if args.fusion else (2, 3, 4, 5)
     tuned = tune(sim, cmds, state, widths=widths, tiles=(args.tile,) if args.tile else None,
-                 lows=(args.low,) if args.low else None)
+                 lows=(args.low,) if args.low else None, try_stream=not args.no_stream)
     best = tuned["best"]
     for t in tuned["trials"]:
-        log(f"  trial k={t['fusion_max_qubits']} T={t['tile_qubits']} low={t['low_qubits']}: "
+        log(f"  trial k={t['fusion_max_qubits']} T={t['tile_qubits']} low={t['low_qubits']} "
+            f"stream={t['stream_tiles']}: "
             f"{t['ms']:.1f} ms, {t['passes']} passes, {t['fused_ops']} fused ops")
     log(f"autotune best {best}")
     cc = CompiledCircuit(sim, cmds, n, fusion_max_qubits=best["fusion_max_qubits"],
-                         tile_qubits=best["tile_qubits"], low_qubits=best["low_qubits"])
+                         tile_qubits=best["tile_qubits"], low_qubits=best["low_qubits"],
+                         stream_tiles=best["stream_tiles"])
     state.reset()
     for _ in range(args.warmup):
         cc.apply(state)
@@ -88,19 +90,24 @@ def fused_apply(args, sim):
     _lib.check(_lib.load().qhip_synchronize())
     wall = (time.perf_counter() - t0) / args.steps
     passes = int(cc.stats["passes"])
+    stream_passes = int(cc.stats["stream_passes"])
+    kernel = ("tile_stream_kernel" if stream_passes == passes else "tile_pass_kernel" if stream_passes == 0
+              else f"tile_stream_kernel ({stream_passes} passes) + tile_pass_kernel")
     bytes_per_pass = 2 * 16 * (1 << n)
     value = passes * bytes_per_pass / wall / 1e9
     per_pass = np.concatenate(pass_ms)
     kernel_ms = float(per_pass.mean())
     achieved = bytes_per_pass / (kernel_ms * 1e-3) / 1e9
     norm = state.norm2()
-    log(f"apply: {wall * 1e3:.2f} ms/step, {passes} passes, {value:.1f} GB/s, norm2 {norm:.15f}")
+    log(f"apply: {wall * 1e3:.2f} ms/step, {passes} passes ({stream_passes} streaming), {value:.1f} GB/s, "
+        f"kernel mean {kernel_ms:.3f} ms, norm2 {norm:.15f}")
     # Streaming floor: one tile pass carrying a single 1-qubit gate (same tile
     # shape), and the driver's device-to-device copy of the state, as the
     # read + write references for the per-pass bandwidth.
     if args.no_floor:
         return {
             "state": state, "n": n, "wall_ms": wall * 1e3, "value": value, "passes": passes,
+            "stream_passes": stream_passes, "kernel": kernel,
             "fused_ops": int(cc.stats["fused_ops"]), "gates": len(cmds), "best": best,
             "trials": tuned["trials"], "kernel_ms": kernel_ms, "achieved": achieved,
             "gate_equiv_gbs": len(cmds) * bytes_per_pass / wall / 1e9, "norm2": norm,
@@ -108,7 +115,8 @@ def fused_apply(args, sim):
         }
     from openqarp_amd.ir import Command, GateType
     one = CompiledCircuit(sim, [Command(GateType.H, 0)], n, fusion_max_qubits=best["fusion_max_qubits"],
-                          tile_qubits=best["tile_qubits"], low_qubits=best["low_qubits"])
+                          tile_qubits=best["tile_qubits"], low_qubits=best["low_qubits"],
+                          stream_tiles=best["stream_tiles"])
     one.apply(state)
     floor_ms = float(np.concatenate([one.apply_timed(state) for _ in range(5)]).mean())
     other = sim.new_state(n)
@@ -124,6 +132,7 @@ def fused_apply(args, sim):
     log(f"one-gate tile pass {floor_ms:.3f} ms = {floor_gbs:.1f} GB/s; device copy {copy_ms:.3f} ms = {copy_gbs:.1f} GB/s")
     return {
         "state": state, "n": n, "wall_ms": wall * 1e3, "value": value, "passes": passes,
+        "stream_passes": stream_passes, "kernel": kernel,
         "fused_ops": int(cc.stats["fused_ops"]), "gates": len(cmds), "best": best,
         "trials": tuned["trials"], "kernel_ms": kernel_ms, "achieved": achieved,
         "gate_equiv_gbs": len(cmds) * bytes_per_pass / wall / 1e9, "norm2": norm,
@@ -453,6 +462,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-structured", action="store_true",
                     help="skip the structured-program kernels, transition columns and trajectories")
     ap.add_argument("--no-floor", action="store_true", help="skip the one-gate pass and device-copy references")
+    ap.add_argument("--no-stream", action="store_true",
+                    help="run every pass one tile per workgroup (no persistent prefetching tile kernel)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the backend is single-GPU; "
@@ -460,7 +471,7 @@ def main(argv=None) -> int:
         return 2
 
     from openqarp_amd.simulator import QarpSimulator
-    sim = QarpSimulator()
+    sim = QarpSimulator(stream_tiles=0 if args.no_stream else -1)
     fa = fused_apply(args, sim)
     ex = None if args.no_expectation else expectation(sim, fa["state"], fa["n"])
     st = None if args.no_structured else structured(sim, fa["state"], fa["n"])
@@ -478,13 +489,15 @@ def main(argv=None) -> int:
         "config": {"workload": f"brickwork{fa['n']}-haar2q-c128", "n_qubits": fa["n"], "layers": args.layers,
                    "gates": fa["gates"], "fusion_max_qubits": fa["best"]["fusion_max_qubits"],
                    "tile_qubits": fa["best"]["tile_qubits"], "low_qubits": fa["best"]["low_qubits"],
-                   "fused_ops": fa["fused_ops"], "passes": fa["passes"], "parallelism": "single-gpu"},
+                   "stream_tiles": fa["best"]["stream_tiles"],
+                   "fused_ops": fa["fused_ops"], "passes": fa["passes"], "stream_passes": fa["stream_passes"],
+                   "parallelism": "single-gpu"},
         "pct_of_peak": round(100.0 * fa["value"] / HBM_PEAK_GBS, 2),
         "gate_equivalent_GBps": round(fa["gate_equiv_gbs"], 1),
         "fused_gates_per_pass": round(fa["fused_ops"] / max(fa["passes"], 1), 2),
         "roofline": {"bound": "hbm", "achieved": round(fa["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(fa["achieved"] / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
-                     "kernel": "tile_pass_kernel", "mean_launch_ms": round(fa["kernel_ms"], 4),
+                     "kernel": fa["kernel"], "mean_launch_ms": round(fa["kernel_ms"], 4),
                      "bytes_per_launch": 2 * 16 * (1 << fa["n"])},
         "pass_floor": fa["pass_floor"], "device_copy": fa["device_copy"],
         "autotune": fa["trials"],

@@ -119,7 +119,7 @@ typedef struct qhip_observable {
 /* Simulator knobs.  fusion_max_qubits / fusion_min_qubits have the meaning
  * of QarpSimulator::fusion_max_qubits / fusion_min_qubits
  * (simulator/qarp_simulator.h:41-61).  The rest steer the device schedule:
- * 0 (or -1 for use_mfma and relabel_qubits) selects the tuned default. */
+ * 0 (or -1 for use_mfma, relabel_qubits and stream_tiles) selects the default. */
 typedef struct qhip_config {
     int32_t fusion_max_qubits;   /* 0..8                                       */
     int32_t fusion_min_qubits;   /* registers below this get 1q fusion only    */
@@ -130,6 +130,10 @@ typedef struct qhip_config {
     int32_t relabel_qubits;      /* -1 auto, 0 never: a pass's store may move qubits into the contiguous low
                                     tile positions for the next pass (the layout is restored before a
                                     schedule ends) */
+    int32_t stream_tiles;        /* persistent workgroups that load their next tile while the current
+                                    tile's ops run: 1 for single-state passes with matrix-core ops and
+                                    for the widest tile, -1 for the widest tile only (which no other
+                                    kernel runs), 0 never (the widest tile is then refused) */
 } qhip_config;
 
 /* What one apply did (filled when the caller passes a non-NULL pointer). */
@@ -139,6 +143,7 @@ typedef struct qhip_apply_stats {
     int64_t diagonal_ops;    /* ops applied as index-phase (no tile qubits)    */
     int64_t mfma_ops;        /* dense ops applied on the matrix cores          */
     double  flops;           /* real FP64/FP32 flops issued for matvecs        */
+    int64_t stream_passes;   /* passes run on the streaming (prefetching) tile kernel */
 } qhip_apply_stats;
 
 typedef struct qhip_state qhip_state;   /* opaque device statevector */

@@ -50,12 +50,12 @@ class QhipConfig(C.Structure):
     _fields_ = [("fusion_max_qubits", C.c_int32), ("fusion_min_qubits", C.c_int32),
                 ("tile_qubits", C.c_int32), ("low_qubits", C.c_int32),
                 ("max_ops_per_pass", C.c_int32), ("use_mfma", C.c_int32),
-                ("relabel_qubits", C.c_int32)]
+                ("relabel_qubits", C.c_int32), ("stream_tiles", C.c_int32)]
 
 
 class QhipApplyStats(C.Structure):
     _fields_ = [("fused_ops", C.c_int64), ("passes", C.c_int64), ("diagonal_ops", C.c_int64),
-                ("mfma_ops", C.c_int64), ("flops", C.c_double)]
+                ("mfma_ops", C.c_int64), ("flops", C.c_double), ("stream_passes", C.c_int64)]
 
 
 def _ptr(a: Optional[np.ndarray]) -> Optional[int]:
@@ -231,6 +231,6 @@ def pack_observable(observable: Observable) -> PackedObservable:
 
 def config_struct(fusion_max_qubits: int, fusion_min_qubits: int, tile_qubits: int = 0,
                   low_qubits: int = 0, max_ops_per_pass: int = 0, use_mfma: int = -1,
-                  relabel_qubits: int = -1) -> QhipConfig:
+                  relabel_qubits: int = -1, stream_tiles: int = -1) -> QhipConfig:
     return QhipConfig(fusion_max_qubits, fusion_min_qubits, tile_qubits, low_qubits,
-                      max_ops_per_pass, use_mfma, relabel_qubits)
+                      max_ops_per_pass, use_mfma, relabel_qubits, stream_tiles)

@@ -3,7 +3,9 @@ schedules on the device.
 
 Each candidate (fusion width k, tile qubits T, contiguous low qubits) is
 compiled once and applied once on the caller's state with HIP-event timing
-of every pass; the fastest total wins.  Results are cached per (register
+of every pass; the fastest total wins.  The winner is then timed once more
+with its matrix-core passes on the streaming tile kernel (stream_tiles = 1),
+which is kept when faster.  Results are cached per (register
 width, precision, circuit fingerprint).  Applying a candidate changes the
 state's amplitudes, not the work done, so the caller resets the state
 afterwards when its contents matter.
@@ -51,23 +53,30 @@ def candidates(n: int, precision: str, widths: Iterable[int] = (2, 3, 4, 5),
 
 def tune(sim: QarpSimulator, commands: Sequence[Command], state: DeviceState,
          widths: Iterable[int] = (2, 3, 4, 5), repeats: int = 1, tiles: Optional[Iterable[int]] = None,
-         lows: Optional[Iterable[int]] = None) -> dict:
+         lows: Optional[Iterable[int]] = None, try_stream: bool = True) -> dict:
     widths = tuple(widths)
     lows = tuple(lows) if lows else default_lows(sim.precision)
     tiles = tuple(tiles) if tiles else None
-    key = (state.n_qubits, sim.precision, fingerprint(commands), widths, tiles, lows)
+    key = (state.n_qubits, sim.precision, fingerprint(commands), widths, tiles, lows, try_stream)
     if key in _cache:
         return _cache[key]
     results = []
-    for k, T, low in candidates(state.n_qubits, sim.precision, widths, tiles, lows):
+
+    def trial(k, T, low, stream):
         cc = CompiledCircuit(sim, commands, state.n_qubits, fusion_max_qubits=k, tile_qubits=T,
-                             low_qubits=low)
+                             low_qubits=low, stream_tiles=stream)
         cc.apply(state)                         # warm (first launch of each kernel)
         ms = min(float(cc.apply_timed(state).sum()) for _ in range(repeats))
-        results.append({"fusion_max_qubits": k, "tile_qubits": T, "low_qubits": low, "ms": ms,
-                        "passes": cc.stats["passes"], "fused_ops": cc.stats["fused_ops"],
-                        "mfma_ops": cc.stats["mfma_ops"]})
+        results.append({"fusion_max_qubits": k, "tile_qubits": T, "low_qubits": low, "stream_tiles": stream,
+                        "ms": ms, "passes": cc.stats["passes"], "fused_ops": cc.stats["fused_ops"],
+                        "mfma_ops": cc.stats["mfma_ops"], "stream_passes": cc.stats["stream_passes"]})
+
+    for k, T, low in candidates(state.n_qubits, sim.precision, widths, tiles, lows):
+        trial(k, T, low, sim.stream_tiles)
     best = min(results, key=lambda r: r["ms"])
+    if try_stream and sim.stream_tiles != 0 and best["mfma_ops"] > 0:
+        trial(best["fusion_max_qubits"], best["tile_qubits"], best["low_qubits"], 1)
+        best = min(results, key=lambda r: r["ms"])
     out = {"best": best, "trials": results}
     _cache[key] = out
     return out

@@ -191,8 +191,28 @@ static_assert(sizeof(DevOp) == 64, "DevOp layout");
 constexpr int kMaxIpPerPass = 64;
 
 // Widest tile: 2^T amplitudes of each state must fit the LDS of one workgroup
-// (160 KiB per CU; two states share it in adjoint passes).
-inline int max_tile_qubits(int precision) { return precision == QHIP_C128 ? 12 : 13; }
+// (160 KiB per CU; two states share it in adjoint passes).  The widest
+// single-state tile (128 KiB: one workgroup per CU) runs only on the
+// streaming kernel, whose prefetch overlaps its HBM traffic with its ops.
+inline int max_tile_qubits(int precision, bool two_states = false) {
+    return (precision == QHIP_C128 ? 12 : 13) + (two_states ? 0 : 1);
+}
+
+// Elements of one matrix-core op's fragments as the streaming tile kernel
+// stages them in LDS: k = 3 as stored; k = 4, 5 in complex128 without the
+// (Ur + Ui) pairs, which the kernel forms with the same double additions the
+// host makes, and with them in complex64 (the host rounds the double sum to
+// float once, which a float addition on the device would not reproduce).
+__host__ __device__ constexpr int staged_fragment_elems(int k, bool f32) {
+    return k == 3 ? 128 : k == 4 ? (f32 ? 384 : 256) : k == 5 ? (f32 ? 1536 : 1024) : 0;
+}
+// LDS the streaming tile kernel reserves for the fragments of one pass (ops
+// beyond it read their fragments from global memory).
+constexpr int kStreamFragBytes = 16384;
+// Smallest tile array (2^TB0 amplitudes per state) of each amplitude type;
+// the tile kernels are instantiated for TB0, TB0 + 1 and TB0 + 2.
+constexpr int kTileArrayQubitsC128 = 10;
+constexpr int kTileArrayQubitsC64 = 11;
 
 struct PassDesc {
     int32_t n;           // register width
@@ -200,7 +220,8 @@ struct PassDesc {
     int32_t op_begin;
     int32_t op_end;
     int32_t kmax;        // widest VALU dense / IP op in the pass (matrix-core ops excluded)
-    int32_t reserved;
+    int32_t frag;        // elements of the pass's matrix-core fragments as the streaming kernel
+                         // stages them in LDS; -1: the pass runs one tile per workgroup
     uint64_t tmask;      // global bits that are tile-local
     uint64_t hmask;      // global bits of local positions >= 8
     int32_t qpos[40];    // qpos[l] = global qubit of local position l
@@ -222,11 +243,22 @@ struct BatchItem {
     int64_t n_tiles;
 };
 
+// Whether a single-state pass runs on the streaming tile kernel: its VALU
+// dense ops are at most two-qubit, and its tile is the widest one (which
+// only that kernel runs) or fills one of the other arrays and the pass has
+// matrix-core ops (passes of VALU ops alone stream no faster than one tile
+// per workgroup).
+inline bool streaming_pass(const PassDesc& P, int precision) {
+    const int tb0 = precision == QHIP_C128 ? kTileArrayQubitsC128 : kTileArrayQubitsC64;
+    if (P.frag < 0 || P.kmax > 2) return false;
+    return P.T == tb0 + 3 || (P.T >= tb0 && P.T <= tb0 + 2 && P.frag > 0);
+}
+
 struct Plan {
     std::vector<PassDesc> passes;
     std::vector<DevOp> ops;
     std::vector<cplx> coefs;
-    int64_t dense_ops = 0, diag_ops = 0, big_ops = 0, mfma_ops = 0;
+    int64_t dense_ops = 0, diag_ops = 0, big_ops = 0, mfma_ops = 0, stream_passes = 0;
     double flops = 0.0;
 };
 
@@ -239,6 +271,8 @@ struct ScheduleOptions {
     bool register_blocks = true;  // group k <= 2 dense ops into OP_BLOCKs
     int mfma = -1;           // dense k >= 3 on the matrix cores: -1 auto, 0 never, 1 where possible
     bool relabel = true;     // passes may move the next pass's qubits into the contiguous low positions
+    int stream = -1;         // streaming (prefetching) tile kernel: -1 only for the widest tile,
+                             // 0 never, 1 also for passes with matrix-core ops
     int precision = QHIP_C128;
 };
 Plan schedule(const std::vector<HostOp>& ops, int n, int precision, const ScheduleOptions& opt);

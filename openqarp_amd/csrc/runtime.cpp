@@ -122,6 +122,7 @@ static ScheduleOptions options(const qhip_config* cfg, bool two) {
         o.max_ops = cfg->max_ops_per_pass;
         o.mfma = cfg->use_mfma;
         o.relabel = cfg->relabel_qubits != 0;
+        o.stream = cfg->stream_tiles < 0 ? -1 : cfg->stream_tiles > 0 ? 1 : 0;
     }
     return o;
 }
@@ -158,6 +159,7 @@ static void fill_stats(qhip_apply_stats* stats, const Plan& plan, int64_t fused)
     stats->diagonal_ops = plan.diag_ops;
     stats->mfma_ops = plan.mfma_ops;
     stats->flops = plan.flops;
+    stats->stream_passes = plan.stream_passes;
 }
 
 void apply_stream(const qhip_config* cfg, const qhip_stream* stream, qhip_state* s, qhip_apply_stats* stats,

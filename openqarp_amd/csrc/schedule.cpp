@@ -617,6 +617,15 @@ static void schedule_segment(Plan& plan, const std::vector<HostOp>& ops, const s
             }
         }
         P.op_end = static_cast<int32_t>(plan.ops.size());
+        P.frag = -1;
+        const bool widest = T == max_tile_qubits(opt.precision, false);
+        if (!opt.two_states && (opt.stream > 0 || (opt.stream < 0 && widest))) {
+            P.frag = 0;
+            for (int o = P.op_begin; o < P.op_end; ++o)
+                if (plan.ops[o].kind == OP_MFMA)
+                    P.frag += staged_fragment_elems(plan.ops[o].k, opt.precision != QHIP_C128);
+            plan.stream_passes += streaming_pass(P, opt.precision) ? 1 : 0;
+        }
         plan.passes.push_back(P);
         cur = next;
     };
@@ -740,9 +749,13 @@ Plan schedule(const std::vector<HostOp>& ops, int n, int precision, const Schedu
     ScheduleOptions opt = opt_in;
     opt.precision = precision;
     int T = opt.tile_qubits > 0 ? opt.tile_qubits : default_tile_qubits(precision, opt.two_states);
-    if (T > max_tile_qubits(precision))
+    if (T > max_tile_qubits(precision, opt.two_states))
         fail(QHIP_INVALID, "tile_qubits " + std::to_string(T) + " exceeds the LDS limit of " +
-                               std::to_string(max_tile_qubits(precision)) + " for this precision");
+                               std::to_string(max_tile_qubits(precision, opt.two_states)) + " for this precision" +
+                               (opt.two_states ? " in two-state passes" : ""));
+    if (T == max_tile_qubits(precision, false) && (opt.mfma == 0 || opt.stream == 0))
+        fail(QHIP_INVALID, "tile_qubits " + std::to_string(T) +
+                               " runs only on the streaming tile kernel (stream_tiles and use_mfma not 0)");
     T = std::max(T, std::min(kMaxTileK, n));
     T = std::min(T, n);
     int kdense = 0;
@@ -761,6 +774,7 @@ Plan schedule(const std::vector<HostOp>& ops, int n, int precision, const Schedu
             P.n = n;
             P.T = 0;   // wide pass
             P.kmax = op.k;
+            P.frag = -1;
             for (int i = 0; i < 16; ++i) P.sperm[i] = static_cast<int8_t>(i);
             int local_of[64];
             for (int q = 0; q < 64; ++q) local_of[q] = q;   // positions are global qubits

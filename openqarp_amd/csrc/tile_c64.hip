@@ -6,12 +6,12 @@ namespace qhip {
 
 void launch_tile_pass(const PassDesc& P, float2* psi, float2* phi, double2* ip_part, const DevOp* d_ops,
                       const float2* d_coefs, int64_t n_tiles, hipStream_t stream) {
-    tile::launch<float2, 11>(P, psi, phi, ip_part, d_ops, d_coefs, n_tiles, stream);
+    tile::launch<float2, kTileArrayQubitsC64>(P, psi, phi, ip_part, d_ops, d_coefs, n_tiles, stream);
 }
 
 void launch_tile_batch_c64(bool two, const BatchItem* items, const int64_t* first, int n_items,
                           int64_t total, int tmax, int kmax, hipStream_t stream) {
-    tile::launch_batch<float2, 11>(two, items, first, n_items, total, tmax, kmax, stream);
+    tile::launch_batch<float2, kTileArrayQubitsC64>(two, items, first, n_items, total, tmax, kmax, stream);
 }
 
 }  // namespace qhip

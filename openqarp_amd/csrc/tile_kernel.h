@@ -359,8 +359,10 @@ __device__ __forceinline__ void dense_mfma(A* s, uint32_t zoff, int G, const OpR
 // 4s + c, so lane group c feeds amplitudes c, c + 4, ...): D/16 tiles x D/4
 // steps x 3 products = 3/4 of the MFMAs of the realified form.  Fragments:
 // per tile and pair of steps, (Ur, Ui, Ur + Ui) as three pairs per lane at
-// [((mt * D/8 + t) * 3 + p) * 64 + lane].
-template <typename A, int K>
+// [((mt * D/8 + t) * 3 + p) * 64 + lane]; with PAIRS2 (complex128 fragments
+// staged in LDS) only (Ur, Ui) at [((mt * D/8 + t) * 2 + p) * 64 + lane], the
+// sum formed here with the additions the host makes.
+template <typename A, int K, bool PAIRS2>
 __device__ __forceinline__ void dense_mfma3(A* s, uint32_t zoff, int G, const OpRegs& op, const A* __restrict__ F) {
     using M = Mfma<A>;
     constexpr int sh = Amp<A>::kShift;
@@ -387,8 +389,19 @@ __device__ __forceinline__ void dense_mfma3(A* s, uint32_t zoff, int G, const Op
             typename M::v4 p1 = {0, 0, 0, 0}, p2 = p1, p3 = p1;
 #pragma unroll
             for (int t = 0; t < HP; ++t) {
-                const A* f = F + ((mt * HP + t) * 3) * 64 + lane;
-                const A fr = f[0], fi = f[64], fs = f[128];
+                A fr, fi, fs;
+                if constexpr (PAIRS2) {
+                    const A* f = F + ((mt * HP + t) * 2) * 64 + lane;
+                    fr = f[0];
+                    fi = f[64];
+                    fs.x = fr.x + fi.x;
+                    fs.y = fr.y + fi.y;
+                } else {
+                    const A* f = F + ((mt * HP + t) * 3) * 64 + lane;
+                    fr = f[0];
+                    fi = f[64];
+                    fs = f[128];
+                }
                 const A x0 = x[2 * t], x1 = x[2 * t + 1];
                 p1 = M::step(fr.x, x0.x, p1);
                 p2 = M::step(fi.x, x0.y, p2);
@@ -409,12 +422,213 @@ __device__ __forceinline__ void dense_mfma3(A* s, uint32_t zoff, int G, const Op
     }
 }
 
+// Chunk walk of a matrix-core op (chunks of 16 groups, lane % 16 = group-
+// number bits 0-3): wave w takes the chunks whose group-number bits 4 and 5
+// spell w (with G < 6 the waves beyond the chunk count sit out) and visits
+// the deposits of the remaining bits (local positions in m) as
+// hi -> (hi - m) & m from 0, which meets every subset of m once.  base is
+// this lane's byte offset at hi = 0; n the wave's chunk count.
+struct ChunkWalk {
+    uint32_t base, m, n;
+};
 template <typename A>
+__device__ __forceinline__ ChunkWalk chunk_walk(uint32_t zoff, int G, const OpRegs& op) {
+    constexpr int sh = Amp<A>::kShift;
+    const uint32_t j = threadIdx.x & 15u;
+    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int cb = G - 4, wb = cb < 2 ? cb : 2;
+    ChunkWalk cw;
+    cw.n = w < (1u << wb) ? 1u << (cb - wb) : 0u;
+    uint32_t lp = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) lp |= ((j >> b) & 1u) << op.g(b);
+    for (int b = 0; b < wb; ++b) lp |= ((w >> b) & 1u) << op.g(4 + b);
+    cw.base = (swz(lp) << sh) | zoff;
+    uint32_t m = 0;
+    for (int b = 4 + wb; b < G; ++b) m |= 1u << op.g(b);
+    cw.m = m;
+    return cw;
+}
+
+// k = 3 (realified) and k = 4 (three products) software-pipelined over a
+// wave's chunks: the products of chunk i are issued before the results of
+// chunk i - 1 are combined and written, so the matrix cores work while the
+// VALU and LDS finish the previous chunk, and the reads of chunk i overlap
+// the products of chunk i - 1.  Chunk i's reads come before chunk i - 1's
+// writes; the chunks hold disjoint groups.  Same products, same order of
+// accumulation per output as dense_mfma / dense_mfma3.
+template <typename A>
+__device__ __forceinline__ void dense_mfma_k3p(A* s, uint32_t zoff, int G, const OpRegs& op, const A* __restrict__ F) {
+    using M = Mfma<A>;
+    using V = typename M::v4;
+    constexpr int sh = Amp<A>::kShift;
+    constexpr int K = 3, H = 2;
+    const uint32_t lane = threadIdx.x & 63u, c = lane >> 4;
+    const ChunkWalk cw = chunk_walk<A>(zoff, G, op);
+    if (cw.n == 0) return;
+    uint32_t so_in[H];
+#pragma unroll
+    for (int t = 0; t < H; ++t) so_in[t] = swz(local_offset<K>(c + 4u * t, op)) << sh;
+    const uint32_t so0 = swz(local_offset<K>(2u * c, op)) << sh, so1 = swz(local_offset<K>(2u * c + 1u, op)) << sh;
+    A f[H];
+#pragma unroll
+    for (int t = 0; t < H; ++t) f[t] = F[t * 64 + lane];
+    auto issue = [&](const A* x) {
+        V acc = {0, 0, 0, 0};
+#pragma unroll
+        for (int t = 0; t < H; ++t) {
+            acc = M::step(f[t].x, x[t].x, acc);
+            acc = M::step(f[t].y, x[t].y, acc);
+        }
+        return acc;
+    };
+    auto finish = [&](uint32_t base, const V& acc) {
+        A y0, y1;
+        y0.x = acc[0];
+        y0.y = acc[1];
+        y1.x = acc[2];
+        y1.y = acc[3];
+        at(s, base ^ so0) = y0;
+        at(s, base ^ so1) = y1;
+    };
+    // Two accumulator sets in turn (a, b): no copy of a result that is
+    // still in flight.
+    uint32_t hi = 0, ba = cw.base, bb = 0;
+    A x[H];
+    auto load = [&](uint32_t base) {
+#pragma unroll
+        for (int t = 0; t < H; ++t) x[t] = at(s, base ^ so_in[t]);
+    };
+    auto next = [&]() {
+        hi = (hi - cw.m) & cw.m;
+        return cw.base ^ (swz(hi) << sh);
+    };
+    load(ba);
+    V a = issue(x), b;
+    uint32_t i = 1;
+    for (; i + 1 < cw.n; i += 2) {
+        bb = next();
+        load(bb);
+        b = issue(x);
+        finish(ba, a);
+        ba = next();
+        load(ba);
+        a = issue(x);
+        finish(bb, b);
+    }
+    if (i < cw.n) {
+        bb = next();
+        load(bb);
+        b = issue(x);
+        finish(ba, a);
+        finish(bb, b);
+    } else {
+        finish(ba, a);
+    }
+}
+
+template <typename A, bool PAIRS2>
+__device__ __forceinline__ void dense_mfma_k4p(A* s, uint32_t zoff, int G, const OpRegs& op, const A* __restrict__ F) {
+    using M = Mfma<A>;
+    using V = typename M::v4;
+    constexpr int sh = Amp<A>::kShift;
+    constexpr int K = 4, H = 4, HP = 2;
+    const uint32_t lane = threadIdx.x & 63u, c = lane >> 4;
+    const ChunkWalk cw = chunk_walk<A>(zoff, G, op);
+    if (cw.n == 0) return;
+    uint32_t so_in[H], so_out[4];
+#pragma unroll
+    for (int t = 0; t < H; ++t) so_in[t] = swz(local_offset<K>(c + 4u * t, op)) << sh;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+        so_out[r] = swz(local_offset<K>(M::kGroupRows ? 4u * c + r : 4u * r + c, op)) << sh;
+    A fr[HP], fi[HP], fs[HP];
+#pragma unroll
+    for (int t = 0; t < HP; ++t) {
+        if constexpr (PAIRS2) {
+            const A* f = F + (t * 2) * 64 + lane;
+            fr[t] = f[0];
+            fi[t] = f[64];
+            fs[t].x = fr[t].x + fi[t].x;
+            fs[t].y = fr[t].y + fi[t].y;
+        } else {
+            const A* f = F + (t * 3) * 64 + lane;
+            fr[t] = f[0];
+            fi[t] = f[64];
+            fs[t] = f[128];
+        }
+    }
+    struct P3 {
+        V p1, p2, p3;
+    };
+    auto issue = [&](const A* x) {
+        P3 a;
+        a.p1 = V{0, 0, 0, 0};
+        a.p2 = a.p1;
+        a.p3 = a.p1;
+#pragma unroll
+        for (int t = 0; t < HP; ++t) {
+            const A x0 = x[2 * t], x1 = x[2 * t + 1];
+            a.p1 = M::step(fr[t].x, x0.x, a.p1);
+            a.p2 = M::step(fi[t].x, x0.y, a.p2);
+            a.p3 = M::step(fs[t].x, x0.x + x0.y, a.p3);
+            a.p1 = M::step(fr[t].y, x1.x, a.p1);
+            a.p2 = M::step(fi[t].y, x1.y, a.p2);
+            a.p3 = M::step(fs[t].y, x1.x + x1.y, a.p3);
+        }
+        return a;
+    };
+    auto finish = [&](uint32_t base, const P3& a) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            A y;
+            y.x = a.p1[r] - a.p2[r];
+            y.y = a.p3[r] - a.p1[r] - a.p2[r];
+            at(s, base ^ so_out[r]) = y;
+        }
+    };
+    uint32_t hi = 0, ba = cw.base, bb = 0;
+    A x[H];
+    auto load = [&](uint32_t base) {
+#pragma unroll
+        for (int t = 0; t < H; ++t) x[t] = at(s, base ^ so_in[t]);
+    };
+    auto next = [&]() {
+        hi = (hi - cw.m) & cw.m;
+        return cw.base ^ (swz(hi) << sh);
+    };
+    load(ba);
+    P3 a = issue(x), b;
+    uint32_t i = 1;
+    for (; i + 1 < cw.n; i += 2) {
+        bb = next();
+        load(bb);
+        b = issue(x);
+        finish(ba, a);
+        ba = next();
+        load(ba);
+        a = issue(x);
+        finish(bb, b);
+    }
+    if (i < cw.n) {
+        bb = next();
+        load(bb);
+        b = issue(x);
+        finish(ba, a);
+        finish(bb, b);
+    } else {
+        finish(ba, a);
+    }
+}
+
+// STAGED: the fragments come from the streaming kernel's LDS copy.
+template <typename A, bool STAGED>
 __device__ __forceinline__ void mfma_op(A* s, uint32_t zoff, int G, const OpRegs& op, const A* __restrict__ F) {
+    constexpr bool kPairs2 = STAGED && std::is_same<A, double2>::value;
     switch (op.k) {
-        case 3: dense_mfma<A, 3>(s, zoff, G, op, F); break;
-        case 4: dense_mfma3<A, 4>(s, zoff, G, op, F); break;
-        case 5: dense_mfma3<A, 5>(s, zoff, G, op, F); break;
+        case 3: dense_mfma_k3p<A>(s, zoff, G, op, F); break;
+        case 4: dense_mfma_k4p<A, kPairs2>(s, zoff, G, op, F); break;
+        case 5: dense_mfma3<A, 5, kPairs2>(s, zoff, G, op, F); break;
         default: break;
     }
 }
@@ -783,17 +997,21 @@ __device__ __forceinline__ double2 ip_dense(A* s, uint32_t tp0, uint32_t tp1, in
 // The ops of pass P over the tile in LDS (psi at element 0; phi at element
 // 2^TB in two-state mode), a barrier after each.  te: this thread's byte
 // offset for amplitudes l = threadIdx.x + 256 r.  Returns the number of
-// inner-product ops, whose per-wave partials are left in wave_ip.
-template <typename A, bool TWO, int KMAX, int TB>
+// inner-product ops, whose per-wave partials are left in wave_ip.  STAGED
+// (single state): the matrix-core ops read their fragments from frag, the
+// LDS copy the streaming kernel made, in op order.
+template <typename A, bool TWO, int KMAX, int TB, bool STAGED>
 __device__ __forceinline__ int run_ops(A* s, const PassDesc& P, uint64_t gbase, uint32_t te,
                                        const DevOp* __restrict__ ops, const A* __restrict__ coefs,
-                                       double2* wave_ip) {
+                                       double2* wave_ip, const A* frag) {
+    static_assert(!(STAGED && TWO), "staged fragments are a single-state form");
     constexpr int sh = Amp<A>::kShift;
     constexpr uint32_t kPhi = (1u << TB) << sh;   // byte offset of phi's tile
     const uint32_t L = 1u << P.T;
     const uint32_t R = L >= kThreads ? L / kThreads : 1u;
     const bool active = threadIdx.x < L;
     int n_ip = 0;
+    uint32_t foff = 0;   // staged fragments consumed so far
     for (int o = P.op_begin; o < P.op_end; ++o) {
         const OpRegs op = load_op(ops + o);
         const int kind = op.kind, k = op.k;
@@ -808,8 +1026,20 @@ __device__ __forceinline__ int run_ops(A* s, const PassDesc& P, uint64_t gbase, 
             }
             case OP_MFMA: {
                 const int G = P.T - k;
-                mfma_op<A>(s, 0u, G, op, U);
-                if (TWO) mfma_op<A>(s, kPhi, G, op, U);
+                if constexpr (STAGED) {
+                    // The staging rule of tile_stream_kernel: an op's fragments
+                    // are in LDS when they fitted after the earlier ops'.
+                    const uint32_t nf = staged_fragment_elems(k, std::is_same<A, float2>::value);
+                    if (foff + nf <= kStreamFragBytes / sizeof(A)) {
+                        mfma_op<A, true>(s, 0u, G, op, frag + foff);
+                        foff += nf;
+                    } else {
+                        mfma_op<A, false>(s, 0u, G, op, U);
+                    }
+                } else {
+                    mfma_op<A, false>(s, 0u, G, op, U);
+                    if (TWO) mfma_op<A, false>(s, kPhi, G, op, U);
+                }
                 break;
             }
             case OP_BLOCK: {
@@ -1003,7 +1233,7 @@ __device__ __forceinline__ void tile_body(A* s, A* __restrict__ psi, A* __restri
     }
     __syncthreads();
 
-    const int n_ip = run_ops<A, TWO, KMAX, TB>(s, P, gbase, te, ops, coefs, wave_ip);
+    const int n_ip = run_ops<A, TWO, KMAX, TB, false>(s, P, gbase, te, ops, coefs, wave_ip, nullptr);
 
     if (TWO && static_cast<int>(threadIdx.x) < n_ip) {
         // Sum the per-wave values in fixed order: partial k of this tile.
@@ -1107,6 +1337,131 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(tile_w
                                 it.n_tiles, static_cast<uint64_t>(b - first[lo]));
 }
 
+// Streaming single-state tile pass.  Persistent workgroups walk the tiles
+// t = blockIdx.x, blockIdx.x + gridDim.x, ... of a pass whose tile fills the
+// LDS array (T = TB, so every thread holds R = 2^(TB-8) rows).  The loads of
+// a workgroup's next tile are issued into registers before the ops of the
+// current tile run, so the HBM reads overlap the pass's compute; stores leave
+// asynchronously as in the one-tile kernel.  The matrix-core fragments of the
+// pass are copied into LDS once per workgroup: a vector load from global
+// memory inside the op loop would wait for the prefetch in flight, since
+// vector memory loads complete in issue order.  Same arithmetic, same order
+// of operations per amplitude as tile_pass_kernel.
+template <typename A, int TB>
+struct StreamLds {
+    static constexpr uint32_t kTile = 1u << TB;
+    static constexpr uint32_t kFrag = kStreamFragBytes / sizeof(A);
+    static constexpr uint32_t kBytes = (kTile + kFrag) * sizeof(A);
+};
+// Waves per SIMD the LDS allows (160 KiB per CU, four SIMDs, one wave of
+// each workgroup per SIMD), at most four.
+template <typename A, int TB>
+constexpr int stream_waves() {
+    constexpr uint32_t w = 163840u / StreamLds<A, TB>::kBytes;
+    return w < 1 ? 1 : w > 4 ? 4 : static_cast<int>(w);
+}
+
+template <typename A, int TB>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(stream_waves<A, TB>()))) void tile_stream_kernel(
+    A* __restrict__ psi, const PassDesc P, const DevOp* __restrict__ ops, const A* __restrict__ coefs,
+    int64_t n_tiles) {
+    using L = StreamLds<A, TB>;
+    constexpr int sh = Amp<A>::kShift;
+    constexpr uint32_t R = L::kTile / kThreads;
+    constexpr uint32_t RB = R < kBatch ? R : kBatch;
+    constexpr bool f32 = std::is_same<A, float2>::value;
+    constexpr int kRowBits = TB - 8;
+    static_assert(TB >= 9 && TB <= 14, "rows of 256 amplitudes, at most six permuted row bits");
+    __shared__ __align__(16) A s[L::kTile + L::kFrag];
+    A* frag = s + L::kTile;
+
+    // Fragments of the pass's matrix-core ops, in op order, while they fit
+    // (run_ops applies the same rule).
+    uint32_t off = 0;
+    for (int o = P.op_begin; o < P.op_end; ++o) {
+        const OpRegs op = load_op(ops + o);
+        if (op.kind != OP_MFMA) continue;
+        const uint32_t n = static_cast<uint32_t>(staged_fragment_elems(op.k, f32));
+        if (off + n > L::kFrag) continue;
+        const A* src = coefs + op.coef;
+        for (uint32_t e = threadIdx.x; e < n; e += kThreads) {
+            const uint32_t j = e >> 6, lane = e & 63u;
+            frag[off + e] = src[(op.k >= 4 && !f32) ? ((j >> 1) * 3 + (j & 1)) * 64 + lane : e];
+        }
+        off += n;
+    }
+
+    const uint64_t full = P.n >= 64 ? ~0ull : ((1ull << P.n) - 1ull);
+    const uint64_t outer = full & ~P.tmask;
+    const uint32_t te = swz(threadIdx.x) << sh;
+    uint64_t dj = 0;   // the thread's eight low local bits, deposited
+#pragma unroll
+    for (int b = 0; b < 8; ++b) dj |= static_cast<uint64_t>((threadIdx.x >> b) & 1u) << P.qpos[b];
+    // Store permutation (as in tile_body): output row r of this thread reads
+    // LDS offset to ^ row_perm(r).
+    uint32_t tperm = 0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) tperm |= ((threadIdx.x >> b) & 1u) << P.sperm[b];
+    const uint32_t to = swz(tperm) << sh;
+    uint32_t rbit[kRowBits];
+#pragma unroll
+    for (int b = 0; b < kRowBits; ++b) rbit[b] = swz(1u << P.sperm[8 + b]) << sh;
+
+    A pre[R];
+    auto load_tile = [&](int64_t tt) {
+        const uint64_t g0 = tile_base(static_cast<uint64_t>(tt), outer) | dj;
+        uint64_t dr = 0;
+#pragma unroll
+        for (uint32_t r = 0; r < R; ++r) {
+            pre[r] = stream_load(psi + (g0 | dr));
+            dr = (dr - P.hmask) & P.hmask;
+        }
+    };
+    auto put_tile = [&]() {
+#pragma unroll
+        for (uint32_t r = 0; r < R; ++r) at(s, te ^ (swz(r << 8) << sh)) = pre[r];
+    };
+    int64_t t = blockIdx.x;
+    if (t >= n_tiles) return;   // the host sizes the grid to at most n_tiles
+    load_tile(t);
+    put_tile();
+    __syncthreads();   // the first tile and the fragments in LDS
+    for (;;) {
+        const int64_t tn = t + gridDim.x;
+        // The next tile's loads go out before this tile's ops run.  They are
+        // unconditional so that the wait before put_tile counts only this
+        // iteration's loads and stores: past the end the last tile is read
+        // again and never used.
+        load_tile(tn < n_tiles ? tn : n_tiles - 1);
+        const uint64_t gbase = tile_base(static_cast<uint64_t>(t), outer);
+        run_ops<A, false, 2, TB, true>(s, P, gbase, te, ops, coefs, nullptr, frag);
+        const uint64_t g0 = gbase | dj;
+        uint64_t dr = 0;
+#pragma unroll
+        for (uint32_t r0 = 0; r0 < R; r0 += RB) {
+            A v[RB];
+#pragma unroll
+            for (uint32_t i = 0; i < RB; ++i) {
+                const uint32_t r = r0 + i;
+                uint32_t pr = 0;
+#pragma unroll
+                for (int b = 0; b < kRowBits; ++b) pr ^= ((r >> b) & 1u) ? rbit[b] : 0u;
+                v[i] = at(s, to ^ pr);
+            }
+#pragma unroll
+            for (uint32_t i = 0; i < RB; ++i) {
+                stream_store(psi + (g0 | dr), v[i]);
+                dr = (dr - P.hmask) & P.hmask;
+            }
+        }
+        if (tn >= n_tiles) break;
+        __syncthreads();   // every wave has read this tile out of LDS
+        put_tile();
+        __syncthreads();
+        t = tn;
+    }
+}
+
 // ── launch ──────────────────────────────────────────────────────────────────
 // TB0 .. TB0 + 2 are the instantiated tile-array sizes of one amplitude type:
 // a pass runs with the smallest array that holds its 2^T amplitudes.
@@ -1140,11 +1495,52 @@ static void launch_tb(const PassDesc& P, A* psi, A* phi, double2* ip, const DevO
     }
 }
 
+inline int cu_count() {
+    static int cus[64] = {};
+    int dev = 0;
+    QHIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) fail(QHIP_RUNTIME, "device index out of range");
+    if (!cus[dev]) QHIP_CHECK(hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev));
+    return cus[dev];
+}
+
+// Persistent grid: as many workgroups as fit on the device at once (LDS
+// bound), never more than there are tiles.
+template <typename A, int TB>
+static void launch_stream(const PassDesc& P, A* psi, const DevOp* ops, const A* coefs, int64_t n_tiles,
+                          hipStream_t st) {
+    static int per_cu = 0;
+    if (!per_cu) {
+        int nb = 0;
+        QHIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, tile_stream_kernel<A, TB>, kThreads, 0));
+        per_cu = nb > 0 ? nb : 1;
+    }
+    const int64_t grid = std::min<int64_t>(n_tiles, static_cast<int64_t>(per_cu) * cu_count());
+    hipLaunchKernelGGL((tile_stream_kernel<A, TB>), dim3(static_cast<uint32_t>(grid)), dim3(kThreads), 0, st, psi,
+                       P, ops, coefs, n_tiles);
+}
+
+template <typename A, int TB0>
+static bool streams(const PassDesc& P, const A* phi) {
+    constexpr bool c128 = std::is_same<A, double2>::value;
+    static_assert(TB0 == (c128 ? kTileArrayQubitsC128 : kTileArrayQubitsC64), "tile arrays of streaming_pass");
+    return !phi && streaming_pass(P, c128 ? QHIP_C128 : QHIP_C64);
+}
+
 template <typename A, int TB0>
 static void launch(const PassDesc& P, A* psi, A* phi, double2* ip, const DevOp* ops, const A* coefs, int64_t n_tiles,
                    hipStream_t st) {
     if (P.T < 1 || P.T > P.n) fail(QHIP_RUNTIME, "tile pass: bad tile width");
-    if (phi) launch_tb<A, true, TB0>(P, psi, phi, ip, ops, coefs, n_tiles, st);
+    if (streams<A, TB0>(P, phi)) {
+        switch (P.T - TB0) {
+            case 0: launch_stream<A, TB0>(P, psi, ops, coefs, n_tiles, st); break;
+            case 1: launch_stream<A, TB0 + 1>(P, psi, ops, coefs, n_tiles, st); break;
+            case 2: launch_stream<A, TB0 + 2>(P, psi, ops, coefs, n_tiles, st); break;
+            default: launch_stream<A, TB0 + 3>(P, psi, ops, coefs, n_tiles, st); break;
+        }
+    } else if (P.T > TB0 + 2) {
+        fail(QHIP_INVALID, "tile pass: " + std::to_string(P.T) + " tile qubits run only on the streaming kernel");
+    } else if (phi) launch_tb<A, true, TB0>(P, psi, phi, ip, ops, coefs, n_tiles, st);
     else launch_tb<A, false, TB0>(P, psi, phi, ip, ops, coefs, n_tiles, st);
     QHIP_CHECK(hipGetLastError());
 }

@@ -136,7 +136,8 @@ class QarpSimulator:
     DEFAULT_FUSION_MIN_QUBITS = 12
 
     def __init__(self, noise_model=None, *, precision: str = "complex128", tile_qubits: int = 0,
-                 low_qubits: int = 0, max_ops_per_pass: int = 0, use_mfma: int = -1, relabel_qubits: int = -1):
+                 low_qubits: int = 0, max_ops_per_pass: int = 0, use_mfma: int = -1, relabel_qubits: int = -1,
+                 stream_tiles: int = -1):
         if precision not in ("complex128", "complex64"):
             raise ValueError(f"precision must be complex128 or complex64, got {precision!r}")
         _lib.load()
@@ -149,6 +150,9 @@ class QarpSimulator:
         self.max_ops_per_pass = int(max_ops_per_pass)
         self.use_mfma = int(use_mfma)   # dense k = 3..5 on the matrix cores: -1 auto, 0 never
         self.relabel_qubits = int(relabel_qubits)   # passes move qubits into the low tile positions: -1 auto, 0 never
+        # Persistent prefetching tile workgroups: 1 for passes with matrix-core
+        # ops, -1 only for the widest tile (2^13 complex128 / 2^14 complex64), 0 never.
+        self.stream_tiles = int(stream_tiles)
         self.last_stats: Optional[dict] = None
         self._pool: dict = {}
 
@@ -175,7 +179,7 @@ class QarpSimulator:
 
     def _config(self):
         return config_struct(self._fusion_max, self._fusion_min, self.tile_qubits, self.low_qubits,
-                             self.max_ops_per_pass, self.use_mfma, self.relabel_qubits)
+                             self.max_ops_per_pass, self.use_mfma, self.relabel_qubits, self.stream_tiles)
 
     # ── device-resident primitives ──
     def new_state(self, n_qubits: int, initial_state=None, where: str = "statevector") -> DeviceState:
@@ -195,7 +199,7 @@ class QarpSimulator:
         _lib.check(fn(C.byref(cfg), ps.ref, state.handle, C.byref(stats)))
         self.last_stats = {"fused_ops": stats.fused_ops, "passes": stats.passes,
                            "diagonal_ops": stats.diagonal_ops, "mfma_ops": stats.mfma_ops,
-                           "flops": stats.flops}
+                           "stream_passes": stats.stream_passes, "flops": stats.flops}
         return self.last_stats
 
     def transition_state(self, bra: DeviceState, ket: DeviceState, observable: Observable) -> complex:
@@ -580,7 +584,8 @@ class CompiledCircuit:
     def __init__(self, sim: "QarpSimulator", commands: Sequence[Command], n_qubits: int, *,
                  fusion_max_qubits: Optional[int] = None, tile_qubits: Optional[int] = None,
                  low_qubits: Optional[int] = None, use_mfma: Optional[int] = None,
-                 relabel_qubits: Optional[int] = None, values: Optional[Mapping[str, float]] = None):
+                 relabel_qubits: Optional[int] = None, values: Optional[Mapping[str, float]] = None,
+                 stream_tiles: Optional[int] = None):
         self.n_qubits = int(n_qubits)
         self.precision = sim.precision
         cfg = config_struct(sim.fusion_max_qubits if fusion_max_qubits is None else int(fusion_max_qubits),
@@ -588,10 +593,11 @@ class CompiledCircuit:
                             sim.tile_qubits if tile_qubits is None else int(tile_qubits),
                             sim.low_qubits if low_qubits is None else int(low_qubits),
                             sim.max_ops_per_pass, sim.use_mfma if use_mfma is None else int(use_mfma),
-                            sim.relabel_qubits if relabel_qubits is None else int(relabel_qubits))
+                            sim.relabel_qubits if relabel_qubits is None else int(relabel_qubits),
+                            sim.stream_tiles if stream_tiles is None else int(stream_tiles))
         self.config = {"fusion_max_qubits": cfg.fusion_max_qubits, "tile_qubits": cfg.tile_qubits,
                        "low_qubits": cfg.low_qubits, "use_mfma": cfg.use_mfma,
-                       "relabel_qubits": cfg.relabel_qubits}
+                       "relabel_qubits": cfg.relabel_qubits, "stream_tiles": cfg.stream_tiles}
         self._cfg = cfg
         self._commands = as_commands(list(commands))
         ps = pack_stream(self._commands, values=values)
@@ -607,14 +613,16 @@ class CompiledCircuit:
         stats = QhipApplyStats()
         _lib.check(_lib.load().qhip_plan_apply(self._h, state.handle, C.byref(stats)))
         self.stats = {"fused_ops": stats.fused_ops, "passes": stats.passes,
-                      "diagonal_ops": stats.diagonal_ops, "mfma_ops": stats.mfma_ops, "flops": stats.flops}
+                      "diagonal_ops": stats.diagonal_ops, "mfma_ops": stats.mfma_ops,
+                      "stream_passes": stats.stream_passes, "flops": stats.flops}
         return self.stats
 
     def apply_graph(self, state: DeviceState) -> dict:
         stats = QhipApplyStats()
         _lib.check(_lib.load().qhip_plan_apply_graph(self._h, state.handle, C.byref(stats)))
         self.stats = {"fused_ops": stats.fused_ops, "passes": stats.passes,
-                      "diagonal_ops": stats.diagonal_ops, "mfma_ops": stats.mfma_ops, "flops": stats.flops}
+                      "diagonal_ops": stats.diagonal_ops, "mfma_ops": stats.mfma_ops,
+                      "stream_passes": stats.stream_passes, "flops": stats.flops}
         return self.stats
 
     def update(self, values: Mapping[str, float]) -> bool:

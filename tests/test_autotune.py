@@ -41,8 +41,18 @@ def test_tune_picks_the_fastest_trial_and_keeps_parity():
     st = sim.new_state(n)
     out = tune(sim, cmds, st, widths=(2, 3, 4), tiles=(10, 11), lows=(3, 4))
     trials = out["trials"]
-    assert [(t["fusion_max_qubits"], t["tile_qubits"], t["low_qubits"]) for t in trials] == \
-        candidates(n, "complex128", (2, 3, 4), (10, 11), (3, 4))
+    grid = candidates(n, "complex128", (2, 3, 4), (10, 11), (3, 4))
+    assert [(t["fusion_max_qubits"], t["tile_qubits"], t["low_qubits"]) for t in trials[:len(grid)]] == grid
+    assert all(t["stream_tiles"] == -1 for t in trials[:len(grid)])
+    # One more trial: the grid's winner with its matrix-core passes streamed.
+    grid_best = min(trials[:len(grid)], key=lambda t: t["ms"])
+    if grid_best["mfma_ops"] > 0:
+        extra = trials[len(grid):]
+        assert len(extra) == 1 and extra[0]["stream_tiles"] == 1 and extra[0]["stream_passes"] > 0
+        assert [extra[0][key] for key in ("fusion_max_qubits", "tile_qubits", "low_qubits")] == \
+            [grid_best[key] for key in ("fusion_max_qubits", "tile_qubits", "low_qubits")]
+    else:
+        assert len(trials) == len(grid)
     best = out["best"]
     fastest = min(t["ms"] for t in trials)
     assert best["ms"] == fastest
@@ -51,7 +61,8 @@ def test_tune_picks_the_fastest_trial_and_keeps_parity():
     assert tune(sim, cmds, st, widths=(2, 3, 4), tiles=(10, 11), lows=(3, 4)) is out   # cached
     assert len(_cache) >= 1
     cc = CompiledCircuit(sim, cmds, n, fusion_max_qubits=best["fusion_max_qubits"],
-                         tile_qubits=best["tile_qubits"], low_qubits=best["low_qubits"])
+                         tile_qubits=best["tile_qubits"], low_qubits=best["low_qubits"],
+                         stream_tiles=best["stream_tiles"])
     st.reset()
     cc.apply(st)
     ref = O.statevector(pack_stream(cmds), n, best["fusion_max_qubits"], 0)
