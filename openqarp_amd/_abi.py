@@ -209,20 +209,28 @@ def pack_observable(observable: Observable) -> PackedObservable:
     terms = np.zeros(len(observable), dtype=TERM_DTYPE)
     qs: list[int] = []
     ls: list[int] = []
-    for t, (pauli, coeff) in enumerate(observable):
-        c = complex(coeff)
-        terms[t]["offset"] = len(qs)
-        terms[t]["n_letters"] = len(pauli)
-        terms[t]["coeff_re"] = c.real
-        terms[t]["coeff_im"] = c.imag
+    offs: list[int] = []
+    lens: list[int] = []
+    cs: list[complex] = []
+    for pauli, coeff in observable:
+        cs.append(complex(coeff))
+        offs.append(len(qs))
+        lens.append(len(pauli))
         for q, letter in pauli:
-            if int(q) < 0:
+            q = int(q)
+            if q < 0:
                 raise ValueError(f"pauli_transition: observable qubit {q} out of range")
-            qs.append(int(q))
+            qs.append(q)
             s = str(letter)
             if len(s) != 1:
                 raise ValueError(f"pauli_transition: invalid Pauli '{s}'")
             ls.append(ord(s))
+    if cs:   # whole columns at once (element-wise structured writes are slow)
+        c = np.asarray(cs, dtype=np.complex128)
+        terms["offset"] = offs
+        terms["n_letters"] = lens
+        terms["coeff_re"] = c.real
+        terms["coeff_im"] = c.imag
     q = np.asarray(qs, dtype=np.uint32)
     lt = np.asarray(ls, dtype=np.uint8)
     st = QhipObservable(_ptr(terms), len(observable), _ptr(q), _ptr(lt))

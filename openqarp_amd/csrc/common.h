@@ -112,6 +112,13 @@ inline bool is_ip_kind(int32_t kind) {
 
 // Inner-product members per register block (per-thread accumulators).
 constexpr int kMaxBlockIp = 4;
+// Register-block member patterns (DevOp::k of OP_SUB / OP_IPSUB), beyond
+// 0-2 (2x2 on block bit B), 3-5 (4x4 on a bit pair), 6-8 / 9-11 (diagonal
+// on one bit / a bit pair, sub-tables for operands outside the tile), 12 (no
+// block operand): kReal2 + B a real 2x2 matrix, kXGen / kYGen / kZGen + B the
+// Pauli generator on bit B (inner products only), kNegPair + p the sign flip
+// where both bits of pair p (01, 02, 12) are set.
+constexpr int kReal2 = 13, kXGen = 16, kYGen = 19, kZGen = 22, kNegPair = 25;
 
 struct HostOp {
     int32_t kind = OP_DENSE;

@@ -492,9 +492,12 @@ int qhip_schedule_inspect(const qhip_config* cfg, const qhip_stream* in, int32_t
                         break;
                     case qhip::OP_SUB:   // pos[] holds the member op's own operand positions
                     case qhip::OP_DENSE: {
-                        const bool sub = d.kind == qhip::OP_SUB, diag = sub && d.k >= 6;
+                        const bool sub = d.kind == qhip::OP_SUB;
+                        const bool real2 = sub && d.k >= qhip::kReal2 && d.k < qhip::kReal2 + 3;
+                        const bool neg = sub && d.k >= qhip::kNegPair && d.k < qhip::kNegPair + 3;
+                        const bool diag = sub && ((d.k >= 6 && d.k <= 12) || neg);
                         if (diag) {   // local operands (ascending), then the ones outside the tile
-                            const int kl = d.k == 12 ? 0 : d.k <= 8 ? 1 : 2, kg = d.spos[7];
+                            const int kl = neg ? 2 : d.k == 12 ? 0 : d.k <= 8 ? 1 : 2, kg = neg ? 0 : d.spos[7];
                             it.k = kl + kg;
                             for (int b = 0; b < kl; ++b) it.q[b] = global(d.pos[b]);
                             for (int j = 0; j < kg; ++j) it.q[kl + j] = static_cast<uint32_t>(d.spos[j]);
@@ -503,7 +506,7 @@ int qhip_schedule_inspect(const qhip_config* cfg, const qhip_stream* in, int32_t
                             for (size_t y = 0; y < D; ++y) it.m[y * D + y] = plan.coefs[d.coef + y];
                             break;
                         }
-                        it.k = sub ? (d.k <= 2 ? 1 : 2) : d.k;
+                        it.k = sub ? (d.k <= 2 || real2 ? 1 : 2) : d.k;
                         const size_t D = size_t{1} << it.k;
                         if (sub && it.k == 2 && d.pos[0] > d.pos[1]) {
                             // stored in block-bit order: report it on ascending positions

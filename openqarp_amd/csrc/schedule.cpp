@@ -17,6 +17,8 @@
 
 namespace qhip {
 
+static const cplx I0{0.0, 0.0}, I1{1.0, 0.0};
+
 int default_tile_qubits(int precision, bool two_states) {
     // 32 KiB of LDS per state (2^11 complex128 or 2^12 complex64
     // amplitudes): several workgroups share a CU's 160 KiB, so one tile's
@@ -250,10 +252,11 @@ static std::vector<Unit> form_units(const std::vector<HostOp>& ops, const std::v
         const HostOp& op = ops[idx];
         const uint64_t Q = op.qmask(), Ql = Q & tq;
         const bool ip = is_ip_host(op);
-        const bool diag = op.kind == OP_DIAG || (two && op.kind == OP_IP_DIAG);
+        // Mask phases (CZ, MCZ) join a block as diagonal members.
+        const bool diag = op.kind == OP_DIAG || (two && op.kind == OP_IP_DIAG) || op.kind == OP_MASKPHASE;
         const bool kind_ok = (op.kind == OP_DENSE && op.k <= 2) || (two && op.kind == OP_IP_DENSE && op.k <= 2) ||
                              (diag && std::popcount(Ql) <= 2 && std::popcount(Q & ~tq) <= 4);
-        const bool mergeable = enable && kind_ok && op.k >= 1;
+        const bool mergeable = enable && kind_ok && (op.k >= 1 || op.kind == OP_MASKPHASE);
         int last = -1;
         for (int u = static_cast<int>(units.size()) - 1; u >= 0; --u)
             if (op.kind == OP_SCALAR || (units[u].qall & Q)) { last = u; break; }
@@ -319,7 +322,16 @@ static void emit_block(Plan& plan, const std::vector<HostOp>& ops, const Unit& u
     int n_ip = 0;
     auto sw = [](int x) { return ((x & 1) << 1) | ((x >> 1) & 1); };
     for (size_t idx : u.ops) {
-        const HostOp& op = ops[idx];
+        HostOp as_diag;   // a mask phase as the diagonal over its operands
+        if (ops[idx].kind == OP_MASKPHASE) {
+            const HostOp& mp = ops[idx];
+            as_diag.kind = OP_DIAG;
+            for (int q = 0; q < 64; ++q)
+                if ((mp.mask >> q) & 1) as_diag.q[as_diag.k++] = static_cast<uint32_t>(q);
+            as_diag.m.assign(size_t{1} << as_diag.k, I1);
+            as_diag.m.back() = mp.phase;
+        }
+        const HostOp& op = ops[idx].kind == OP_MASKPHASE ? as_diag : ops[idx];
         const bool diag = op.kind == OP_DIAG || op.kind == OP_IP_DIAG;
         DevOp d{};
         d.kind = is_ip_host(op) ? OP_IPSUB : OP_SUB;
@@ -333,6 +345,12 @@ static void emit_block(Plan& plan, const std::vector<HostOp>& ops, const Unit& u
             int bits[2] = {0, 0};
             for (int i = 0; i < kl; ++i) bits[i] = bit_of(op.q[lb[i]]);
             d.k = kl == 0 ? 12 : kl == 1 ? 6 + bits[0] : 9 + (bits[0] == 0 ? (bits[1] == 1 ? 0 : 1) : 2);
+            // Sign patterns with every operand in the block: Z as a generator,
+            // and -1 where both bits are set (CZ).
+            if (kg == 0 && kl == 1 && d.kind == OP_IPSUB && op.m[0] == I1 && op.m[1] == -I1) d.k = kZGen + bits[0];
+            if (kg == 0 && kl == 2 && d.kind == OP_SUB && op.m[0] == I1 && op.m[1] == I1 && op.m[2] == I1 &&
+                op.m[3] == -I1)
+                d.k = kNegPair + (d.k - 9);
             for (int i = 0; i < kl; ++i) d.pos[i] = static_cast<int8_t>(local_of[op.q[lb[i]]]);
             for (int j = 0; j < kg; ++j) d.spos[j] = static_cast<int8_t>(pos[op.q[gb[j]]]);
             d.spos[7] = static_cast<int8_t>(kg);
@@ -347,6 +365,14 @@ static void emit_block(Plan& plan, const std::vector<HostOp>& ops, const Unit& u
             const std::vector<cplx>& M = op.m;
             if (op.k == 1) {
                 d.k = bit_of(op.q[0]);
+                const cplx i1(0.0, 1.0);
+                if (d.kind == OP_SUB && M[0].imag() == 0.0 && M[1].imag() == 0.0 && M[2].imag() == 0.0 &&
+                    M[3].imag() == 0.0)
+                    d.k += kReal2;   // real rotation (Ry): real-times-complex products
+                else if (d.kind == OP_IPSUB && M[0] == I0 && M[1] == I1 && M[2] == I1 && M[3] == I0)
+                    d.k += kXGen;
+                else if (d.kind == OP_IPSUB && M[0] == I0 && M[1] == -i1 && M[2] == i1 && M[3] == I0)
+                    d.k += kYGen;
                 plan.coefs.insert(plan.coefs.end(), M.begin(), M.end());
             } else {
                 const int b0 = bit_of(op.q[0]), b1 = bit_of(op.q[1]);

@@ -680,9 +680,39 @@ __device__ __forceinline__ void sub_diag(A* v, const A* __restrict__ Dg) {
     }
 }
 
+template <typename A, int B>   // real 2x2 matrix (zero imaginary parts) on block bit B
+__device__ __forceinline__ void sub_real(A* v, const A* __restrict__ U) {
+    const auto u00 = U[0].x, u01 = U[1].x, u10 = U[2].x, u11 = U[3].x;
+#pragma unroll
+    for (int x = 0; x < 8; ++x)
+        if (!((x >> B) & 1)) {
+            const A a = v[x], b = v[x | (1 << B)];
+            v[x].x = fma(u01, b.x, u00 * a.x);
+            v[x].y = fma(u01, b.y, u00 * a.y);
+            v[x | (1 << B)].x = fma(u11, b.x, u10 * a.x);
+            v[x | (1 << B)].y = fma(u11, b.y, u10 * a.y);
+        }
+}
+
+template <typename A, int B0, int B1>   // -1 where block bits B0 and B1 are both set
+__device__ __forceinline__ void sub_negate(A* v) {
+#pragma unroll
+    for (int x = 0; x < 8; ++x)
+        if (((x >> B0) & 1) && ((x >> B1) & 1)) {
+            v[x].x = -v[x].x;
+            v[x].y = -v[x].y;
+        }
+}
+
 template <typename A>
 __device__ __forceinline__ void sub_apply(A* v, int pattern, const A* __restrict__ U) {
     switch (pattern) {
+        case kReal2 + 0: sub_real<A, 0>(v, U); break;
+        case kReal2 + 1: sub_real<A, 1>(v, U); break;
+        case kReal2 + 2: sub_real<A, 2>(v, U); break;
+        case kNegPair + 0: sub_negate<A, 0, 1>(v); break;
+        case kNegPair + 1: sub_negate<A, 0, 2>(v); break;
+        case kNegPair + 2: sub_negate<A, 1, 2>(v); break;
         case 0: sub_single<A, 0>(v, U); break;
         case 1: sub_single<A, 1>(v, U); break;
         case 2: sub_single<A, 2>(v, U); break;
@@ -732,7 +762,7 @@ __device__ __forceinline__ void block_apply(A* s, uint32_t tp, int G, const OpRe
         for (int j = 0; j < nsub; ++j) {
             const OpRegs sub = load_op(subs + j);
             const A* U = coefs + sub.coef;
-            if (sub.k >= 6) U += diag_sub_table(gbase, sub) << member_local_bits(sub.k);
+            if (sub.k >= 6 && sub.k <= 12) U += diag_sub_table(gbase, sub) << member_local_bits(sub.k);
             sub_apply(v, sub.k, U);
         }
 #pragma unroll
@@ -862,6 +892,48 @@ __device__ __forceinline__ double2 ip_diag(const A* v, const A* f, const A* __re
     return acc;
 }
 
+// <f| P |v> for a Pauli generator P on block bit B: X swaps the pair, Y
+// swaps it with -i (upper) / +i (lower), Z negates the upper amplitude.
+template <typename A, int B>
+__device__ __forceinline__ double2 ip_pauli_x(const A* v, const A* f) {
+    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int x = 0; x < 8; ++x) ip_acc(acc, f[x], v[x ^ (1 << B)]);
+    return acc;
+}
+template <typename A, int B>
+__device__ __forceinline__ double2 ip_pauli_y(const A* v, const A* f) {
+    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int x = 0; x < 8; ++x) {
+        const A w = v[x ^ (1 << B)];
+        A g;
+        if ((x >> B) & 1) {   // i w
+            g.x = -w.y;
+            g.y = w.x;
+        } else {              // -i w
+            g.x = w.y;
+            g.y = -w.x;
+        }
+        ip_acc(acc, f[x], g);
+    }
+    return acc;
+}
+template <typename A, int B>
+__device__ __forceinline__ double2 ip_pauli_z(const A* v, const A* f) {
+    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int x = 0; x < 8; ++x) {
+        A g = v[x];
+        if ((x >> B) & 1) {
+            g.x = -g.x;
+            g.y = -g.y;
+        }
+        ip_acc(acc, f[x], g);
+    }
+    return acc;
+}
+
 // One OP_BLOCK over both states of a two-state pass: a thread holds a
 // group's 8 amplitudes of psi (v) and phi (f); OP_SUB members act on both,
 // OP_IPSUB members add <phi| G |psi> to their accumulator.  The header's
@@ -897,10 +969,19 @@ __device__ __forceinline__ int block_apply2(A* s, uint32_t tp, uint32_t kPhi, in
             for (int j = 0; j < nsub; ++j) {
                 const OpRegs sub = load_op(subs + j);
                 const A* U = coefs + sub.coef;
-                if (sub.k >= 6) U += diag_sub_table(gbase, sub) << member_local_bits(sub.k);
+                if (sub.k >= 6 && sub.k <= 12) U += diag_sub_table(gbase, sub) << member_local_bits(sub.k);
                 if (sub.kind == OP_IPSUB) {
                     double2 r;
                     switch (sub.k) {
+                        case kXGen + 0: r = ip_pauli_x<A, 0>(v, f); break;
+                        case kXGen + 1: r = ip_pauli_x<A, 1>(v, f); break;
+                        case kXGen + 2: r = ip_pauli_x<A, 2>(v, f); break;
+                        case kYGen + 0: r = ip_pauli_y<A, 0>(v, f); break;
+                        case kYGen + 1: r = ip_pauli_y<A, 1>(v, f); break;
+                        case kYGen + 2: r = ip_pauli_y<A, 2>(v, f); break;
+                        case kZGen + 0: r = ip_pauli_z<A, 0>(v, f); break;
+                        case kZGen + 1: r = ip_pauli_z<A, 1>(v, f); break;
+                        case kZGen + 2: r = ip_pauli_z<A, 2>(v, f); break;
                         case 0: r = ip_single<A, 0>(v, f, U); break;
                         case 1: r = ip_single<A, 1>(v, f, U); break;
                         case 2: r = ip_single<A, 2>(v, f, U); break;
