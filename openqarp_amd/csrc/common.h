@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <memory>
 #include <complex>
 #include <cstdint>
 #include <cstring>
@@ -50,6 +51,8 @@ struct Context {
     DeviceBuffer red;       // reduction partials
     DeviceBuffer misc;      // sampling / adjoint scratch
     DeviceBuffer misc2;
+    DeviceBuffer obs;       // prepared observable tables (adjoint sweep)
+    DeviceBuffer phi;       // the adjoint's second state, up to 1 GiB
     DeviceBuffer perm;      // out-of-place permutation target (one state)
     DeviceBuffer pinned_dummy;
 };
@@ -111,7 +114,7 @@ inline bool is_ip_kind(int32_t kind) {
 }
 
 // Inner-product members per register block (per-thread accumulators).
-constexpr int kMaxBlockIp = 4;
+constexpr int kMaxBlockIp = 8;
 // Register-block member patterns (DevOp::k of OP_SUB / OP_IPSUB), beyond
 // 0-2 (2x2 on block bit B), 3-5 (4x4 on a bit pair), 6-8 / 9-11 (diagonal
 // on one bit / a bit pair, sub-tables for operands outside the tile), 12 (no
@@ -344,6 +347,12 @@ cplx transition(const qhip_state* bra, const qhip_state* ket, const qhip_observa
 // device array (no device allocation); bit-equal to transition().
 cplx transition_noalloc(const qhip_state* bra, const qhip_state* ket, const qhip_observable* obs);
 void apply_observable(const qhip_state* in, const qhip_observable* obs, qhip_state* out);
+// apply_observable in two steps: group and upload (into ctx().obs), then
+// launch without host work (the adjoint overlaps its reverse schedule with
+// the device's forward sweep and H|psi>).
+struct PreparedObservable;
+std::shared_ptr<PreparedObservable> prepare_observable(const qhip_observable* obs, int n);
+void launch_prepared_observable(const PreparedObservable& po, const qhip_state* in, qhip_state* out);
 cplx inner(const qhip_state* a, const qhip_state* b);   // <a|b> in FP64
 // Grouped observables of many instances in one device allocation, and
 // stream-ordered H|in> and <a|b> (results stay on the device).

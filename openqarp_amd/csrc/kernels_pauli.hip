@@ -558,6 +558,32 @@ cplx transition_noalloc(const qhip_state* bra, const qhip_state* ket, const qhip
     return herm ? cplx(r.x, 0.0) : cplx(r.x, r.y);
 }
 
+// An observable grouped on the host with its tables already on the device
+// (in ctx().obs), so that launching H|in> needs no host work or copy.
+struct PreparedObservable {
+    Grouped gr;         // host tables (alive while the copies may run)
+    DeviceGroups dg{};
+};
+
+std::shared_ptr<PreparedObservable> prepare_observable(const qhip_observable* obs, int n) {
+    auto po = std::make_shared<PreparedObservable>();
+    po->gr = group_terms(obs, n, "run_gradient");
+    if (!po->gr.groups.empty()) {
+        char* base = static_cast<char*>(ctx().obs.ensure(layout_of(po->gr).total() + 64));
+        copy_tables(base, po->gr);
+        po->dg = view_groups(base, po->gr);
+    }
+    return po;
+}
+
+void launch_prepared_observable(const PreparedObservable& po, const qhip_state* in, qhip_state* out) {
+    if (po.gr.groups.empty()) {
+        QHIP_CHECK(hipMemsetAsync(out->d, 0, out->dim() * out->amp_bytes(), ctx().stream));
+        return;
+    }
+    launch_observable(in, out, po.dg);
+}
+
 void apply_observable(const qhip_state* in, const qhip_observable* obs, qhip_state* out) {
     const int n = in->n;
     const Grouped gr = group_terms(obs, n, "run_gradient");

@@ -170,20 +170,31 @@ void apply_stream(const qhip_config* cfg, const qhip_stream* stream, qhip_state*
     fill_stats(stats, plan, fused);
 }
 
+// The adjoint's second state: up to 1 GiB it lives in a grow-only context
+// buffer reused across calls (no allocation per gradient); larger states
+// are allocated for the call and freed after it.
 struct StateHolder {
+    static constexpr size_t kKeepBytes = size_t{1} << 30;
     qhip_state st;
+    bool own = false;
     explicit StateHolder(const qhip_state* like) {
         st.n = like->n;
         st.precision = like->precision;
-        st.owned = true;
-        if (hipMalloc(&st.d, st.dim() * st.amp_bytes()) != hipSuccess) {
+        st.owned = false;   // never freed through a qhip_state path
+        const size_t bytes = st.dim() * st.amp_bytes();
+        if (bytes <= kKeepBytes) {
+            st.d = ctx().phi.ensure(bytes);
+            return;
+        }
+        own = true;
+        if (hipMalloc(&st.d, bytes) != hipSuccess) {
             (void)hipGetLastError();
             st.d = nullptr;
             fail(QHIP_NOMEM, "adjoint: allocation of the phi state failed");
         }
     }
     ~StateHolder() {
-        if (st.d) {
+        if (own && st.d) {
             (void)hipStreamSynchronize(ctx().stream);
             (void)hipFree(st.d);
         }
@@ -238,18 +249,23 @@ void run_gradient(const qhip_config* cfg, const qhip_stream* stream, int n_param
     std::vector<GradSlot> slots;
     std::vector<HostOp> back;
     build_backward(st, n, slots, back);
-    // Forward sweep: psi <- U psi (fused; the gradient sweep itself is unfused).
-    apply_stream(cfg, stream, psi, nullptr, false);
+    // Host work first while the device is idle (the forward schedule, the
+    // observable's groups and their upload), then the forward sweep psi <-
+    // U psi (fused; the gradient sweep itself is unfused) and phi = H psi are
+    // queued, and the reverse schedule is built while the device runs them.
+    int64_t fused = 0;
+    const Plan fwd = plan_stream(cfg, stream, n, psi->precision, false, &fused);
+    const auto prepared = obs ? prepare_observable(obs, n) : nullptr;
+    launch_plan(fwd, psi, nullptr, nullptr, 0);
     StateHolder phi(psi);
     if (obs) {
-        apply_observable(psi, obs, &phi.st);
-        if (energy) *energy = inner(psi, &phi.st).real();
+        launch_prepared_observable(*prepared, psi, &phi.st);
     } else {
         QHIP_CHECK(hipMemcpyAsync(phi.st.d, phi_in->d, psi->dim() * psi->amp_bytes(), hipMemcpyDeviceToDevice,
                                   ctx().stream));
-        if (energy) *energy = 0.0;
     }
     const Plan plan = schedule(back, n, psi->precision, options(cfg, true));
+    if (energy) *energy = obs ? inner(psi, &phi.st).real() : 0.0;
     auto& C = ctx();
     std::vector<cplx> ip(slots.size());
     if (!slots.empty()) {

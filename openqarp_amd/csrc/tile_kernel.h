@@ -759,8 +759,10 @@ __device__ __forceinline__ void block_apply(A* s, uint32_t tp, int G, const OpRe
             a[x] = tp ^ (h ^ so[x]);
             v[x] = at(s, a[x]);
         }
+        OpRegs nxt = load_op(subs);   // descriptors one ahead, as in block_apply2
         for (int j = 0; j < nsub; ++j) {
-            const OpRegs sub = load_op(subs + j);
+            const OpRegs sub = nxt;
+            if (j + 1 < nsub) nxt = load_op(subs + j + 1);
             const A* U = coefs + sub.coef;
             if (sub.k >= 6 && sub.k <= 12) U += diag_sub_table(gbase, sub) << member_local_bits(sub.k);
             sub_apply(v, sub.k, U);
@@ -944,9 +946,10 @@ __device__ __forceinline__ int block_apply2(A* s, uint32_t tp, uint32_t kPhi, in
                                             const DevOp* __restrict__ subs, int nsub, const A* __restrict__ coefs,
                                             double2* wave_ip, int n_ip, uint64_t gbase) {
     constexpr int sh = Amp<A>::kShift;
-    static_assert(kMaxBlockIp == 4, "four named accumulators");
     const int nipb = static_cast<int>((hdr.spos >> 56) & 0xffu);
-    double2 acc0 = make_double2(0.0, 0.0), acc1 = acc0, acc2 = acc0, acc3 = acc0;
+    double2 acc[kMaxBlockIp];
+#pragma unroll
+    for (int t = 0; t < kMaxBlockIp; ++t) acc[t] = make_double2(0.0, 0.0);
     if (!(G < 8 && threadIdx.x >= (1u << G))) {
         uint32_t so[8];
 #pragma unroll
@@ -966,8 +969,12 @@ __device__ __forceinline__ int block_apply2(A* s, uint32_t tp, uint32_t kPhi, in
                 f[x] = at(s, a[x] | kPhi);
             }
             int j_ip = 0;
+            // Member descriptors one ahead: the next one's scalar load is in
+            // flight with this one's coefficient loads (one round trip each).
+            OpRegs nxt = load_op(subs);
             for (int j = 0; j < nsub; ++j) {
-                const OpRegs sub = load_op(subs + j);
+                const OpRegs sub = nxt;
+                if (j + 1 < nsub) nxt = load_op(subs + j + 1);
                 const A* U = coefs + sub.coef;
                 if (sub.k >= 6 && sub.k <= 12) U += diag_sub_table(gbase, sub) << member_local_bits(sub.k);
                 if (sub.kind == OP_IPSUB) {
@@ -1002,9 +1009,12 @@ __device__ __forceinline__ int block_apply2(A* s, uint32_t tp, uint32_t kPhi, in
                             break;
                         }
                     }
-                    double2& dst = j_ip == 0 ? acc0 : j_ip == 1 ? acc1 : j_ip == 2 ? acc2 : acc3;
-                    dst.x += r.x;
-                    dst.y += r.y;
+#pragma unroll
+                    for (int t = 0; t < kMaxBlockIp; ++t)
+                        if (t == j_ip) {   // wave-uniform: a scalar branch, no register indexing
+                            acc[t].x += r.x;
+                            acc[t].y += r.y;
+                        }
                     ++j_ip;
                     continue;
                 }
@@ -1018,11 +1028,10 @@ __device__ __forceinline__ int block_apply2(A* s, uint32_t tp, uint32_t kPhi, in
             }
         }
     }
-    const double2 accs[4] = {acc0, acc1, acc2, acc3};
 #pragma unroll
     for (int t = 0; t < kMaxBlockIp; ++t)
         if (t < nipb) {
-            const double2 r = wave_sum(accs[t]);
+            const double2 r = wave_sum(acc[t]);
             if ((threadIdx.x & 63) == 0) wave_ip[(n_ip + t) * kWaves + (threadIdx.x >> 6)] = r;
         }
     return nipb;
