@@ -22,8 +22,11 @@ Also on the JSON line:
   qaoa          34-qubit complex64 QAOA MaxCut: <C>, 1e5 shots, the adjoint
                 gradient at 33 qubits and complex64 vs complex128 at 32;
   cpu_baseline  the reference C++ simulator (compiled from its sources into
-                oracle/_ref) on a bounded sample of the same workload family,
-                on this host's cores.
+                oracle/_ref) on bounded samples of the same workloads, on
+                this host's cores, medians of three runs.
+
+    python bench.py --cpu-protocol   # the reference build alone, every leg at
+                                     # its stated size (no GPU used)
 """
 from __future__ import annotations
 
@@ -225,6 +228,16 @@ def batched(n_instances=512):
     t_direct = time.perf_counter() - t0
     out.update({"session_update_s": t_upd, "session_sweep_graph_s": t_graph, "session_sweep_direct_s": t_direct})
     log(f"session sweep: update {t_upd:.3f} s, graph replay {t_graph:.3f} s, direct launches {t_direct:.3f} s")
+    # Roofline of the sweep: algorithmic bytes from the session's schedules
+    # (forward passes 2·B·2^n, H|psi> gathers, reverse two-state passes
+    # 4·B·2^n per instance) over the graph-replayed sweep time.
+    info = sess.stats()
+    total = info["fwd_bytes"] + info["obs_bytes"] + info["rev_bytes"]
+    gbs = total / t_graph / 1e9
+    out["sweep_roofline"] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes": total, **info}
+    log(f"session sweep: {total / 1e9:.1f} GB algorithmic ({info['fwd_launches']} forward + "
+        f"{info['rev_launches']} reverse launches) in {t_graph:.3f} s = {gbs:.0f} GB/s")
     del sess
     e128, g128 = res["complex128"]
     e64, g64 = res["complex64"]
@@ -397,40 +410,104 @@ def trajectories(n=16, shots=2048):
             "distinct_outcomes": len(res.counts)}
 
 
-def cpu_baseline(n=24, layers=30, fusion=3):
-    """The reference simulator (oracle/_ref) on brickwork n=24 — the same
-    workload family at 1/64 the state size — in the metric's unit."""
+def _median_time(fn, reps):
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), ts
+
+
+def cpu_baseline(full=False, reps=3):
+    """The reference simulator compiled from its sources (oracle/_ref) on the
+    host's cores, timed as medians of `reps` runs.  Default sizes are bounded
+    samples of the benchmark workloads (a few tens of seconds in total); with
+    full=True every leg runs at its stated size (C2 apply at 30 qubits, C3 at
+    24), which takes most of an hour on 8 cores.
+
+    Legs: apply (brickwork, GB/s in the headline's unit), c1 (20-qubit HEA:
+    statevector, <H>, 4000 shots), c2_expectation (<H> of the 2000-string sum
+    on a brickwork state), c3_energy (forward energy of the CZ-brickwork
+    ansatz with 1000 local strings; one parameter-shift point is two of
+    these), c3_adjoint (the reference's adjoint gradient)."""
     cores = host_cores()
     os.environ["QARP_NUM_THREADS"] = str(cores)
     os.environ["OMP_NUM_THREADS"] = str(cores)
     sys.path.insert(0, str(ROOT / "tests"))
     import _oracle as O
     from openqarp_amd import workloads as W
-    from openqarp_amd._abi import pack_stream
-    cmds = W.brickwork_haar(n, layers, seed=2)
-    ps = pack_stream(cmds)
-    if O.reference() is not None:
-        nf = len(O.ref_fuse(ps, n, fusion, 12))
-        t0 = time.perf_counter()
-        O.statevector(ps, n, fusion, 12, lib="ref")
-        dt = time.perf_counter() - t0
-        kind = "reference"
-        used = cores
-    else:
-        n = 20
-        cmds = W.brickwork_haar(n, layers, seed=2)
-        ps = pack_stream(cmds)
+    from openqarp_amd._abi import pack_observable, pack_stream
+    if O.reference() is None:
+        # Without the reference build: the C restatement, single-threaded.
+        n, fusion = 20, 3
+        ps = pack_stream(W.brickwork_haar(n, 30, seed=2))
         nf = len(O.fuse(ps, n, fusion, 12))
-        t0 = time.perf_counter()
-        O.statevector(ps, n, fusion, 12)
-        dt = time.perf_counter() - t0
-        kind = "port"
-        used = 1
+        dt, ts = _median_time(lambda: O.statevector(ps, n, fusion, 12), reps)
+        gbs = nf * 2 * 16 * (1 << n) / dt / 1e9
+        log(f"cpu baseline (port) n={n}: median {dt:.2f} s, {nf} fused passes, {gbs:.2f} GB/s")
+        return {"value": gbs, "unit": "GB/s", "cores": 1, "kind": "port",
+                "sample": f"brickwork n={n} layers=30 fusion k={fusion}, {nf} fused passes, median of {reps} "
+                          f"runs {dt:.2f} s; GB/s = passes x 2 x 16 B x 2^n / time"}
+    legs = {}
+    # Headline family: brickwork at fusion width 3 (the reference's default).
+    n = 30 if full else 24
+    ps = pack_stream(W.brickwork_haar(n, 30, seed=2))
+    nf = len(O.ref_fuse(ps, n, 3, 12))
+    dt, ts = _median_time(lambda: O.statevector(ps, n, 3, 12, lib="ref"), reps)
     gbs = nf * 2 * 16 * (1 << n) / dt / 1e9
-    log(f"cpu baseline ({kind}) n={n}: {dt:.2f} s, {nf} fused passes, {gbs:.2f} GB/s")
-    return {"value": gbs, "unit": "GB/s", "cores": used, "kind": kind,
-            "sample": f"brickwork n={n} layers={layers} fusion k={fusion} (the reference's default width), "
-                      f"{nf} fused passes in {dt:.2f} s; GB/s = passes x 2 x 16 B x 2^n / time"}
+    legs["apply"] = {"n_qubits": n, "passes": nf, "s": dt, "runs_s": ts, "GBps": gbs}
+    log(f"cpu baseline (reference) brickwork n={n}: median {dt:.2f} s, {nf} fused passes, {gbs:.2f} GB/s")
+    # C1 in full.
+    w = W.c1_hea(20, 10, seed=1)
+    ps1, po1 = pack_stream(w.commands), pack_observable(w.observable)
+
+    def c1():
+        sv = O.statevector(ps1, 20, 3, 12, lib="ref")
+        O.transition(sv, sv, 20, po1, lib="ref")
+        O.sample(sv, 20, 4000, 1234, lib="ref")
+    dt, ts = _median_time(c1, reps)
+    legs["c1"] = {"n_qubits": 20, "s": dt, "runs_s": ts, "what": "statevector + <H> + 4000 shots"}
+    log(f"cpu baseline (reference) C1 20q: median {dt:.3f} s")
+    # C2's <H> (2000 strings) on a brickwork state.
+    n2 = 30 if full else 22
+    w2 = W.c2_brickwork(n2, 30, seed=2)
+    sv2 = O.statevector(pack_stream(w2.commands), n2, 3, 12, lib="ref")
+    po2 = pack_observable(w2.observable)
+    # At 30 qubits one run takes about half an hour on 8 cores: timed once.
+    dt, ts = _median_time(lambda: O.transition(sv2, sv2, n2, po2, lib="ref"), 1 if full else reps)
+    legs["c2_expectation"] = {"n_qubits": n2, "terms": len(w2.observable), "s": dt, "runs_s": ts}
+    log(f"cpu baseline (reference) C2 <H> {n2}q: median {dt:.2f} s")
+    del sv2
+    # C3: forward energy (one parameter-shift point is two of them) and adjoint.
+    n3 = 24 if full else 22
+    g = W.c3_gradient(n3, 8, seed=3, n_terms=1000)
+    idx = {s: i for i, s in enumerate(g.symbols)}
+    vals = np.array([g.values[s] for s in g.symbols])
+    ps3, po3 = pack_stream(g.commands, g.values, idx), pack_observable(g.observable)
+
+    def energy():
+        sv = O.statevector(ps3, n3, 3, 12, lib="ref")
+        return O.transition(sv, sv, n3, po3, lib="ref")
+    dt, ts = _median_time(energy, reps)
+    legs["c3_energy"] = {"n_qubits": n3, "params": len(vals), "terms": 1000, "s": dt, "runs_s": ts,
+                         "param_shift_gradient_s_est": 2 * len(vals) * dt}
+    log(f"cpu baseline (reference) C3 energy {n3}q: median {dt:.2f} s (a parameter-shift gradient: "
+        f"{2 * len(vals)} of them)")
+    na = 24 if full else 20
+    ga = W.c3_gradient(na, 8, seed=3, n_terms=1000)
+    ida = {s: i for i, s in enumerate(ga.symbols)}
+    va = np.array([ga.values[s] for s in ga.symbols])
+    psa, poa = pack_stream(ga.commands, ga.values, ida), pack_observable(ga.observable)
+    ra = reps if full else 1
+    dt, ts = _median_time(lambda: O.run_gradient(psa, na, poa, len(va), values=va, lib="ref"), ra)
+    legs["c3_adjoint"] = {"n_qubits": na, "params": len(va), "terms": 1000, "s": dt, "runs_s": ts,
+                          "grads_per_s": 1.0 / dt}
+    log(f"cpu baseline (reference) C3 adjoint {na}q, {len(va)} params: {dt:.2f} s")
+    return {"value": legs["apply"]["GBps"], "unit": "GB/s", "cores": cores, "kind": "reference",
+            "sample": f"brickwork n={n} layers=30 fusion k=3 (the reference's default width), {nf} fused passes, "
+                      f"median of {reps} runs {legs['apply']['s']:.2f} s; GB/s = passes x 2 x 16 B x 2^n / time",
+            "legs": legs}
 
 
 def traffic_from_profiles():
@@ -441,6 +518,9 @@ def traffic_from_profiles():
         except (ValueError, OSError):
             return None
     return None
+
+
+LEGS = ("apply", "expectation", "structured", "adjoint", "batch", "qaoa", "columns", "trajectories", "cpu")
 
 
 def main(argv=None) -> int:
@@ -464,24 +544,48 @@ def main(argv=None) -> int:
     ap.add_argument("--no-floor", action="store_true", help="skip the one-gate pass and device-copy references")
     ap.add_argument("--no-stream", action="store_true",
                     help="run every pass one tile per workgroup (no persistent prefetching tile kernel)")
+    ap.add_argument("--legs", default="all",
+                    help="comma-separated subset of " + ",".join(LEGS) + " (default: all)")
+    ap.add_argument("--cpu-protocol", action="store_true",
+                    help="time only the reference build, every leg at its stated size (no GPU; most of an hour)")
     args = ap.parse_args(argv)
+    if args.cpu_protocol:
+        print(json.dumps({"cpu_protocol": cpu_baseline(full=True)}), flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the backend is single-GPU; "
               "the multi-GPU layout is a design note, not a build target)", file=sys.stderr)
         return 2
 
+    legs = set(LEGS if args.legs == "all" else args.legs.split(","))
+    unknown = legs - set(LEGS)
+    if unknown:
+        print(f"unknown legs {sorted(unknown)}; choose from {','.join(LEGS)}", file=sys.stderr)
+        return 2
+    skip = {"expectation": args.no_expectation, "structured": args.no_structured, "adjoint": args.no_adjoint,
+            "batch": args.no_batch, "qaoa": args.no_qaoa, "columns": args.no_structured,
+            "trajectories": args.no_structured, "cpu": args.no_cpu}
+    legs = {leg for leg in legs if not skip.get(leg, False)}
+
     from openqarp_amd.simulator import QarpSimulator
     sim = QarpSimulator(stream_tiles=0 if args.no_stream else -1)
-    fa = fused_apply(args, sim)
-    ex = None if args.no_expectation else expectation(sim, fa["state"], fa["n"])
-    st = None if args.no_structured else structured(sim, fa["state"], fa["n"])
-    fa["state"] = None
-    adj = None if args.no_adjoint else adjoint(sim, points=args.ps_points or None)
-    bat = None if args.no_batch else batched()
-    qa = None if args.no_qaoa else qaoa()
-    cols = None if args.no_structured else columns()
-    traj = None if args.no_structured else trajectories()
-    cpu = None if args.no_cpu else cpu_baseline()
+    need_state = bool(legs & {"apply", "expectation", "structured"})
+    fa = fused_apply(args, sim) if need_state else None
+    ex = expectation(sim, fa["state"], fa["n"]) if "expectation" in legs else None
+    st = structured(sim, fa["state"], fa["n"]) if "structured" in legs else None
+    if fa is not None:
+        fa["state"] = None
+    adj = adjoint(sim, points=args.ps_points or None) if "adjoint" in legs else None
+    bat = batched() if "batch" in legs else None
+    qa = qaoa() if "qaoa" in legs else None
+    cols = columns() if "columns" in legs else None
+    traj = trajectories() if "trajectories" in legs else None
+    cpu = cpu_baseline() if "cpu" in legs else None
+    if fa is None:
+        print(json.dumps({"metric": METRIC, "legs": sorted(legs), "expectation": ex, "structured_program": st,
+                          "transition_columns": cols, "trajectories": traj, "adjoint": adj,
+                          "batched_vqe": bat, "qaoa": qa, "cpu_baseline": cpu}), flush=True)
+        return 0
     line = {
         "metric": METRIC, "value": round(fa["value"], 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(fa["wall_ms"], 3), "higher_is_better": True,

@@ -424,6 +424,18 @@ int qhip_batch_run(qhip_batch* b, double* energies, double* grads, const int64_t
                    int32_t use_graph);
 int qhip_batch_update(qhip_batch* b, const qhip_stream* streams, int32_t* rebuilt);
 int qhip_batch_destroy(qhip_batch* b);
+/* Algorithmic HBM bytes of one sweep of a session, from the schedules it
+ * launches (no reference counterpart: measurement only).  B = bytes per
+ * amplitude; per batched instance of n qubits: fwd_bytes += forward passes x
+ * 2·B·2^n; obs_bytes += (x-groups + 1)·B·2^n for H|psi> (gathered reads and
+ * the write of |phi>) + 2·B·2^n for the energy <psi|phi>; rev_bytes +=
+ * reverse two-state passes x 4·B·2^n.  Launch counts are the tile-pass
+ * launches of the sweep. */
+typedef struct qhip_batch_info {
+    double  fwd_bytes, obs_bytes, rev_bytes;
+    int64_t fwd_launches, rev_launches, instances_batched, instances_single;
+} qhip_batch_info;
+int qhip_batch_stats(const qhip_batch* b, qhip_batch_info* out);
 
 #ifdef __cplusplus
 }

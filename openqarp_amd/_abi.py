@@ -58,6 +58,12 @@ class QhipApplyStats(C.Structure):
                 ("mfma_ops", C.c_int64), ("flops", C.c_double), ("stream_passes", C.c_int64)]
 
 
+class QhipBatchInfo(C.Structure):
+    _fields_ = [("fwd_bytes", C.c_double), ("obs_bytes", C.c_double), ("rev_bytes", C.c_double),
+                ("fwd_launches", C.c_int64), ("rev_launches", C.c_int64),
+                ("instances_batched", C.c_int64), ("instances_single", C.c_int64)]
+
+
 def _ptr(a: Optional[np.ndarray]) -> Optional[int]:
     return None if a is None or a.size == 0 else a.ctypes.data
 

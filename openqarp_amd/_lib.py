@@ -30,7 +30,7 @@ EXPORTS = [
     "qhip_transition", "qhip_transition_noalloc", "qhip_apply_observable",
     "qhip_sample", "qhip_sample_uniforms", "qhip_probabilities", "qhip_exact",
     "qhip_run_gradient", "qhip_run_gradient_phi", "qhip_batch_energy_grad",
-    "qhip_batch_create", "qhip_batch_run", "qhip_batch_update", "qhip_batch_destroy",
+    "qhip_batch_create", "qhip_batch_run", "qhip_batch_update", "qhip_batch_destroy", "qhip_batch_stats",
 ]
 
 
@@ -91,6 +91,7 @@ def _declare(lib):
         "qhip_batch_run": [vp, vp, vp, vp, i32],
         "qhip_batch_update": [vp, vp, C.POINTER(i32)],
         "qhip_batch_destroy": [vp],
+        "qhip_batch_stats": [vp, vp],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

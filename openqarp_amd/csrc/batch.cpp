@@ -22,6 +22,7 @@
 //     on the host as in run_gradient.
 // Instances whose schedules contain a pass the batched kernel does not run
 // (a dense op wider than five qubits) go through run_gradient one by one.
+#include <cmath>
 #include "common.h"
 
 #include <algorithm>
@@ -500,6 +501,24 @@ void batch_run(BatchSession* S, double* energies, double* grads, const int64_t* 
 bool batch_update(BatchSession* S, const qhip_stream* streams) { return S->n_inst ? S->update(streams) : false; }
 
 void batch_destroy(BatchSession* S) { delete S; }
+
+void batch_stats(const BatchSession* S, qhip_batch_info* out) {
+    *out = qhip_batch_info{};
+    out->fwd_launches = static_cast<int64_t>(S->fsteps.size());
+    out->rev_launches = static_cast<int64_t>(S->bsteps.size());
+    for (int i = 0; i < S->n_inst; ++i) {
+        const Inst& I = S->inst[static_cast<size_t>(i)];
+        if (!I.batched) {
+            ++out->instances_single;
+            continue;
+        }
+        ++out->instances_batched;
+        const double state = std::ldexp(static_cast<double>(S->amp), I.n);
+        out->fwd_bytes += 2.0 * state * static_cast<double>(I.fwd.passes.size());
+        out->obs_bytes += (static_cast<double>(obs_arena_groups(S->obs.get(), i)) + 1.0) * state + 2.0 * state;
+        if (S->want_grad) out->rev_bytes += 4.0 * state * static_cast<double>(I.back.passes.size());
+    }
+}
 
 void batch_energy_grad(const qhip_config* cfg, int n_instances, const int32_t* n_qubits,
                        const qhip_stream* streams, const qhip_observable* observables,

@@ -360,6 +360,7 @@ struct ObsArena;
 ObsArena* obs_arena_create(int n_obs, const int32_t* n_qubits, const qhip_observable* obs);
 void obs_arena_destroy(ObsArena* a);
 void obs_apply_async(const ObsArena* a, int i, const qhip_state* in, qhip_state* out);
+int obs_arena_groups(const ObsArena* a, int i);   // flip-mask groups of observable i
 int inner_blocks(const qhip_state* a);
 void inner_async(const qhip_state* a, const qhip_state* b, double2* scratch, double2* d_out);
 
@@ -420,6 +421,7 @@ BatchSession* batch_create(const qhip_config* cfg, int n_instances, const int32_
 void batch_run(BatchSession* s, double* energies, double* grads, const int64_t* grad_offsets, bool use_graph);
 bool batch_update(BatchSession* s, const qhip_stream* streams);
 void batch_destroy(BatchSession* s);
+void batch_stats(const BatchSession* s, qhip_batch_info* out);
 void batch_energy_grad(const qhip_config* cfg, int n_instances, const int32_t* n_qubits,
                        const qhip_stream* streams, const qhip_observable* observables,
                        const int32_t* n_params, int precision, double* energies, double* grads,

@@ -636,6 +636,8 @@ ObsArena* obs_arena_create(int n_obs, const int32_t* n_qubits, const qhip_observ
 
 void obs_arena_destroy(ObsArena* a) { delete a; }
 
+int obs_arena_groups(const ObsArena* a, int i) { return a->per[static_cast<size_t>(i)].n_groups; }
+
 // out = H_i in, stream-ordered (no host synchronisation).
 void obs_apply_async(const ObsArena* a, int i, const qhip_state* in, qhip_state* out) {
     launch_observable(in, out, a->per[i]);

@@ -953,6 +953,13 @@ int qhip_batch_update(qhip_batch* b, const qhip_stream* streams, int32_t* rebuil
     });
 }
 
+int qhip_batch_stats(const qhip_batch* b, qhip_batch_info* out) {
+    return guarded([&] {
+        need(b && out, "qhip_batch_stats: NULL argument");
+        qhip::batch_stats(reinterpret_cast<const qhip::BatchSession*>(b), out);
+    });
+}
+
 int qhip_batch_destroy(qhip_batch* b) {
     return guarded([&] { qhip::batch_destroy(reinterpret_cast<qhip::BatchSession*>(b)); });
 }

@@ -559,6 +559,15 @@ class BatchSession:
                                               self._b.offs.ctypes.data, int(bool(use_graph))))
         return energies, self._b.split(grads) if self.gradients else None
 
+    def stats(self) -> dict:
+        """Algorithmic HBM bytes of one sweep from the session's schedules
+        (qhip_batch_stats): forward passes, H|psi> and the energies, reverse
+        two-state passes; and the tile-pass launch counts."""
+        from ._abi import QhipBatchInfo
+        info = QhipBatchInfo()
+        _lib.check(_lib.load().qhip_batch_stats(self._h, C.byref(info)))
+        return {name: getattr(info, name) for name, _ in QhipBatchInfo._fields_}
+
     def update(self, values: Sequence[Mapping[str, float]]) -> bool:
         if len(values) != self._b.m:
             raise ValueError(f"update: expected {self._b.m} parameter dicts, got {len(values)}")

@@ -47,8 +47,8 @@ int main(void) {
   printf("%zu %zu %zu %zu %zu\n", sizeof(qhip_pauli_term), offsetof(qhip_pauli_term, offset),
     offsetof(qhip_pauli_term, n_letters), offsetof(qhip_pauli_term, coeff_re),
     offsetof(qhip_pauli_term, coeff_im));
-  printf("%zu %zu %zu %zu\n", sizeof(qhip_stream), sizeof(qhip_observable), sizeof(qhip_config),
-    sizeof(qhip_apply_stats));
+  printf("%zu %zu %zu %zu %zu\n", sizeof(qhip_stream), sizeof(qhip_observable), sizeof(qhip_config),
+    sizeof(qhip_apply_stats), sizeof(qhip_batch_info));
   return 0;
 }
 '''
@@ -73,7 +73,7 @@ def test_struct_layouts_match_the_header(tmp_path):
     assert term[1:] == [t.fields[n][1] for n in ["offset", "n_letters", "coeff_re", "coeff_im"]]
     sizes = list(map(int, lines[2].split()))
     assert sizes == [C.sizeof(_abi.QhipStream), C.sizeof(_abi.QhipObservable), C.sizeof(_abi.QhipConfig),
-                     C.sizeof(_abi.QhipApplyStats)]
+                     C.sizeof(_abi.QhipApplyStats), C.sizeof(_abi.QhipBatchInfo)]
 
 
 def test_missing_library_raises_backend_unavailable(monkeypatch, tmp_path):
