@@ -55,13 +55,25 @@ def compile_one(src: Path) -> Path:
     return obj
 
 
-def build(jobs: int = 0, verbose: bool = False) -> Path:
+def stale(src: Path) -> bool:
+    """An object is out of date when its source or any header is newer."""
+    obj = OBJ / (src.name + ".o")
+    if not obj.exists():
+        return True
+    t = obj.stat().st_mtime
+    headers = list(SRC.glob("*.h")) + list((PKG.parent / "include").glob("*.h"))
+    return any(p.stat().st_mtime > t for p in [src, *headers])
+
+
+def build(jobs: int = 0, verbose: bool = False, changed_only: bool = False) -> Path:
     OBJ.mkdir(exist_ok=True)
     LIB_DIR.mkdir(exist_ok=True)
-    srcs = sources()
+    all_srcs = sources()
+    srcs = [s for s in all_srcs if stale(s)] if changed_only else all_srcs
     jobs = jobs or min(len(srcs), os.cpu_count() or 4, 8)
-    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(compile_one, srcs))
+    with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        list(ex.map(compile_one, srcs))
+    objs = [OBJ / (src.name + ".o") for src in all_srcs]
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [hipcc(), "-shared", "-fPIC", "-pthread", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp)]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -76,8 +88,9 @@ def build(jobs: int = 0, verbose: bool = False) -> Path:
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=0)
+    ap.add_argument("--changed", action="store_true", help="recompile only sources newer than their objects")
     a = ap.parse_args(argv)
-    build(a.jobs, verbose=True)
+    build(a.jobs, verbose=True, changed_only=a.changed)
     return 0
 
 

@@ -45,8 +45,20 @@ struct GroupDev {
 struct SubDev {
     int32_t rpat;                        // z bits 8..11 shared by the terms
     int32_t t_begin, t_end;
-    int32_t unused;
+    // Its terms run real coefficients first (n_real of them), then purely
+    // imaginary ones (n_imag), then the rest; both counts are 0 when either
+    // would not fit in 16 bits (every term is then treated as general, which
+    // gives the same values).
+    uint16_t n_real, n_imag;
 };
+
+// Coefficient kinds of a term (imaginary part 0, real part 0, or neither):
+// in H|psi> a real or imaginary coefficient adds one multiply-add per
+// amplitude to its group's weight instead of two.
+enum TermKind { kTermReal = 0, kTermImag = 1, kTermGeneral = 2 };
+inline TermKind term_kind(const cplx& c) {
+    return c.imag() == 0.0 ? kTermReal : c.real() == 0.0 ? kTermImag : kTermGeneral;
+}
 
 struct Grouped {
     std::vector<GroupDev> groups;
@@ -99,19 +111,34 @@ static Grouped group_terms(const qhip_observable* obs, int n, const char* who) {
         GroupDev d;
         d.x = xs[g];
         d.s_begin = static_cast<int32_t>(out.subs.size());
-        // Terms by pattern, first appearance order kept within a pattern.
+        // Terms by pattern, then by coefficient kind; first appearance order
+        // kept within a (pattern, kind) run.
         auto& terms = per[g];
-        std::stable_sort(terms.begin(), terms.end(), [](const auto& a, const auto& b) {
-            return ((a.first >> kRShift) & 15) < ((b.first >> kRShift) & 15);
-        });
+        auto key = [](const std::pair<uint64_t, cplx>& t) {
+            return static_cast<int>((t.first >> kRShift) & 15) * 3 + static_cast<int>(term_kind(t.second));
+        };
+        std::stable_sort(terms.begin(), terms.end(), [&](const auto& a, const auto& b) { return key(a) < key(b); });
+        size_t n_kind[3] = {0, 0, 0};
+        auto close_sub = [&] {
+            SubDev& S = out.subs.back();
+            const bool fits = n_kind[kTermReal] <= 0xffff && n_kind[kTermImag] <= 0xffff;
+            S.n_real = fits ? static_cast<uint16_t>(n_kind[kTermReal]) : 0;
+            S.n_imag = fits ? static_cast<uint16_t>(n_kind[kTermImag]) : 0;
+            n_kind[0] = n_kind[1] = n_kind[2] = 0;
+        };
         for (size_t t = 0; t < terms.size(); ++t) {
             const int32_t pat = static_cast<int32_t>((terms[t].first >> kRShift) & 15);
-            if (t == 0 || pat != out.subs.back().rpat)
-                out.subs.push_back(SubDev{pat, static_cast<int32_t>(out.z.size()), 0, 0});
+            if (t == 0 || pat != out.subs.back().rpat) {
+                if (t > 0) close_sub();
+                out.subs.push_back(SubDev{pat, static_cast<int32_t>(out.z.size()), 0, 0, 0});
+            }
+            const cplx ct = terms[t].second;
+            ++n_kind[term_kind(ct)];
             out.z.push_back(terms[t].first);
-            out.c.push_back(make_double2(terms[t].second.real(), terms[t].second.imag()));
+            out.c.push_back(make_double2(ct.real(), ct.imag()));
             out.subs.back().t_end = static_cast<int32_t>(out.z.size());
         }
+        if (!terms.empty()) close_sub();
         d.s_end = static_cast<int32_t>(out.subs.size());
         d.t_begin = out.subs[d.s_begin].t_begin;
         d.t_end = out.subs.back().t_end;
@@ -258,37 +285,128 @@ __global__ __launch_bounds__(kT) void transition_kernel(const A* __restrict__ br
     if (threadIdx.x == 0) partials[blockIdx.x] = tot;
 }
 
-// out[j] = sum_g w_g(j ^ x_g) in[j ^ x_g], one output per thread and pass of
-// a grid-stride loop: the partner reads of every group are gathers that the
-// caches mostly serve, so this kernel keeps many waves in flight (few
-// registers) rather than many outputs per thread.  The group's terms are
-// contiguous across its patterns.
-template <typename A>
+// out[j] = sum_g w_g(j ^ x_g) in[j ^ x_g] with w_g(i) = sum_{t in g} c_t
+// (-1)^popcount(i & z_t).  The group and term tables are read through the
+// scalar unit (one per CU), so the cost that bounds this kernel is the scalar
+// work per group and term; each thread therefore carries kObsOut outputs
+// (j, j + Q, ..., Q = grid threads) through one walk of the tables, and issues
+// the partner loads of kObsAhead groups before it weighs the first.  A real or
+// imaginary term costs an AND, a popcount, a shift-OR and one multiply-add
+// per output.  I is uint32_t when the state has at most 2^32 amplitudes
+// (every index and mask fits), uint64_t otherwise.
+constexpr int kObsOut = 4;
+constexpr int kObsAhead = 2;
+
+// (-1)^popcount(src & z) as a double: the parity goes to the sign bit of 1.0.
+template <typename I>
+__device__ __forceinline__ double parity_sign(I src, uint64_t zt) {
+    const uint32_t p = sizeof(I) == 4 ? __popc(static_cast<uint32_t>(src) & static_cast<uint32_t>(zt))
+                                      : __popcll(static_cast<uint64_t>(src) & zt);
+    return __hiloint2double(static_cast<int>((p << 31) | 0x3ff00000u), 0);
+}
+
+// acc[u] += w v[u] with w = sum over the group's terms of c_t (-1)^popcount(
+// src[u] & z_t) (the weight summed first, as the reference's group sums are).
+// A real or imaginary coefficient adds one multiply-add to w; the product
+// with v is four multiply-adds (a zero part of w leaves acc exactly as is).
+template <typename I>
+__device__ __forceinline__ void add_group(double2 (&acc)[kObsOut], const I (&src)[kObsOut], const GroupDev& G,
+                                          const SubDev* __restrict__ subs, const uint64_t* __restrict__ z,
+                                          const double2* __restrict__ c, const double2 (&v)[kObsOut]) {
+    double wr[kObsOut], wi[kObsOut];
+#pragma unroll
+    for (int u = 0; u < kObsOut; ++u) wr[u] = wi[u] = 0.0;
+    for (int si = G.s_begin; si < G.s_end; ++si) {
+        const SubDev S = subs[si];
+        const int r_end = S.t_begin + S.n_real, i_end = r_end + S.n_imag;
+        for (int t = S.t_begin; t < r_end; ++t) {
+            const uint64_t zt = z[t];
+            const double cr = c[t].x;
+#pragma unroll
+            for (int u = 0; u < kObsOut; ++u) wr[u] = __fma_rn(parity_sign(src[u], zt), cr, wr[u]);
+        }
+        for (int t = r_end; t < i_end; ++t) {
+            const uint64_t zt = z[t];
+            const double ci = c[t].y;
+#pragma unroll
+            for (int u = 0; u < kObsOut; ++u) wi[u] = __fma_rn(parity_sign(src[u], zt), ci, wi[u]);
+        }
+        for (int t = i_end; t < S.t_end; ++t) {
+            const uint64_t zt = z[t];
+            const double2 ct = c[t];
+#pragma unroll
+            for (int u = 0; u < kObsOut; ++u) {
+                const double sg = parity_sign(src[u], zt);
+                wr[u] = __fma_rn(sg, ct.x, wr[u]);
+                wi[u] = __fma_rn(sg, ct.y, wi[u]);
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kObsOut; ++u) {
+        acc[u].x = __fma_rn(wr[u], v[u].x, acc[u].x);
+        acc[u].x = __fma_rn(-wi[u], v[u].y, acc[u].x);
+        acc[u].y = __fma_rn(wr[u], v[u].y, acc[u].y);
+        acc[u].y = __fma_rn(wi[u], v[u].x, acc[u].y);
+    }
+}
+
+template <typename A, typename I>
 __global__ __launch_bounds__(kT) void observable_kernel(const A* __restrict__ in, A* __restrict__ out,
                                                         uint64_t dim, const GroupDev* __restrict__ groups,
                                                         int n_groups, const SubDev* __restrict__ subs,
                                                         const uint64_t* __restrict__ z,
                                                         const double2* __restrict__ c) {
-    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kT;
-    for (uint64_t j = static_cast<uint64_t>(blockIdx.x) * kT + threadIdx.x; j < dim; j += stride) {
-        double2 acc = make_double2(0.0, 0.0);
-        for (int g = 0; g < n_groups; ++g) {
-            const GroupDev G = groups[g];
-            const uint64_t src = j ^ G.x;
-            const double2 v = load_amp(in, src);
-            double wx = 0.0, wy = 0.0;
-            for (int t = G.t_begin; t < G.t_end; ++t) {
-                const double sg = (__popcll(src & z[t]) & 1) ? -1.0 : 1.0;
-                wx += sg * c[t].x;
-                wy += sg * c[t].y;
-            }
-            acc.x += wx * v.x - wy * v.y;
-            acc.y += wx * v.y + wy * v.x;
+    const uint64_t q = static_cast<uint64_t>(gridDim.x) * kT;
+    const int full = n_groups - n_groups % kObsAhead;
+    for (uint64_t b = static_cast<uint64_t>(blockIdx.x) * kT + threadIdx.x; b < dim; b += q * kObsOut) {
+        // Outputs past the end read the first output's partners and are not stored.
+        I j[kObsOut];
+#pragma unroll
+        for (int u = 0; u < kObsOut; ++u) {
+            const uint64_t ju = b + static_cast<uint64_t>(u) * q;
+            j[u] = static_cast<I>(ju < dim ? ju : b);
         }
-        A o;
-        o.x = acc.x;
-        o.y = acc.y;
-        out[j] = o;
+        double2 acc[kObsOut];
+#pragma unroll
+        for (int u = 0; u < kObsOut; ++u) acc[u] = make_double2(0.0, 0.0);
+        for (int g0 = 0; g0 < full; g0 += kObsAhead) {
+            double2 v[kObsAhead][kObsOut];
+#pragma unroll
+            for (int a = 0; a < kObsAhead; ++a) {
+                const I x = static_cast<I>(groups[g0 + a].x);
+#pragma unroll
+                for (int u = 0; u < kObsOut; ++u) v[a][u] = load_amp(in, j[u] ^ x);
+            }
+#pragma unroll
+            for (int a = 0; a < kObsAhead; ++a) {
+                const GroupDev G = groups[g0 + a];
+                I src[kObsOut];
+#pragma unroll
+                for (int u = 0; u < kObsOut; ++u) src[u] = j[u] ^ static_cast<I>(G.x);
+                add_group(acc, src, G, subs, z, c, v[a]);
+            }
+        }
+        for (int g = full; g < n_groups; ++g) {
+            const GroupDev G = groups[g];
+            I src[kObsOut];
+            double2 v[kObsOut];
+#pragma unroll
+            for (int u = 0; u < kObsOut; ++u) {
+                src[u] = j[u] ^ static_cast<I>(G.x);
+                v[u] = load_amp(in, src[u]);
+            }
+            add_group(acc, src, G, subs, z, c, v);
+        }
+#pragma unroll
+        for (int u = 0; u < kObsOut; ++u) {
+            if (b + static_cast<uint64_t>(u) * q < dim) {
+                A o;
+                o.x = acc[u].x;
+                o.y = acc[u].y;
+                out[j[u]] = o;
+            }
+        }
     }
 }
 
@@ -428,16 +546,23 @@ static void launch_transition(const qhip_state* bra, const qhip_state* ket, cons
 static void launch_observable(const qhip_state* in, qhip_state* out, const DeviceGroups& dg) {
     auto& C = ctx();
     const uint64_t dim = in->dim();
-    const int blocks = static_cast<int>(std::min<uint64_t>((dim + kT - 1) / kT,
+    const int blocks = static_cast<int>(std::min<uint64_t>((dim + kT * kObsOut - 1) / (kT * kObsOut),
                                                            static_cast<uint64_t>(std::max(1, C.n_cu)) * 64));
-    if (in->precision == QHIP_C128)
-        hipLaunchKernelGGL(observable_kernel<double2>, dim3(blocks), dim3(kT), 0, C.stream,
-                           static_cast<const double2*>(in->d), static_cast<double2*>(out->d), dim, dg.g,
-                           dg.n_groups, dg.s, dg.z, dg.c);
-    else
-        hipLaunchKernelGGL(observable_kernel<float2>, dim3(blocks), dim3(kT), 0, C.stream,
-                           static_cast<const float2*>(in->d), static_cast<float2*>(out->d), dim, dg.g,
-                           dg.n_groups, dg.s, dg.z, dg.c);
+    auto go = [&](auto amp_tag, auto index_tag) {
+        using A = decltype(amp_tag);
+        using I = decltype(index_tag);
+        hipLaunchKernelGGL((observable_kernel<A, I>), dim3(blocks), dim3(kT), 0, C.stream,
+                           static_cast<const A*>(in->d), static_cast<A*>(out->d), dim, dg.g, dg.n_groups, dg.s, dg.z,
+                           dg.c);
+    };
+    const bool small = dim <= (uint64_t{1} << 32);
+    if (in->precision == QHIP_C128) {
+        if (small) go(double2{}, uint32_t{});
+        else go(double2{}, uint64_t{});
+    } else {
+        if (small) go(float2{}, uint32_t{});
+        else go(float2{}, uint64_t{});
+    }
     QHIP_CHECK(hipGetLastError());
 }
 
