@@ -361,6 +361,20 @@ def structured(sim, state, n):
     tf = 8.0 * 2.0 ** (n - 1) * 1024 / (ms * 1e-3) / 1e12
     out["controlled_power_1x10"] = {"ms": round(ms, 3), "tflops_f64": round(tf, 2)}
     log(f"controlled power (1 control, 10 targets): {ms:.2f} ms = {tf:.1f} TFLOP/s f64")
+    # Phase-estimation shape: four controls with exponents 1, 2, 4, 8 on the
+    # same ten targets: three squarings of the 1024 x 1024 matrix on the
+    # device, then four controlled GEMMs.
+    p = Program()
+    p.add_controlled_powers([n - 4, n - 3, n - 2, n - 1], [1, 2, 4, 8], list(range(10)), haar_unitary(1 << 10, rng))
+    sim.execute_program(p, state)
+    _lib.check(_lib.load().qhip_synchronize())
+    t0 = time.perf_counter()
+    sim.execute_program(p, state)
+    _lib.check(_lib.load().qhip_synchronize())
+    ms = (time.perf_counter() - t0) * 1e3
+    tf = (4 * 8.0 * 2.0 ** (n - 1) * 1024 + 3 * 8.0 * 1024.0 ** 3) / (ms * 1e-3) / 1e12
+    out["controlled_powers_4x10"] = {"ms": round(ms, 3), "tflops_f64": round(tf, 2)}
+    log(f"controlled powers (4 controls, exponents 1..8, 10 targets): {ms:.2f} ms = {tf:.1f} TFLOP/s f64")
     return out
 
 

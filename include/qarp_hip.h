@@ -232,11 +232,12 @@ int qhip_state_permute(qhip_state* s, const uint32_t* qubits, int32_t k, const u
 /* Stands in for apply_controlled_powers (program.cpp:648-712): matrix^exponents[j]
  * on `targets` wherever control j is |1> (the powers of one matrix commute).
  * matrix: 2^m x 2^m, row-major, complex128 (re, im) interleaved; the powers
- * are formed on the host along one squaring chain as the reference does.  For
- * m <= 7 each controlled power runs as a Custom op on (targets..., control)
- * through the fused apply; for 8 <= m <= 12 a controlled dense kernel applies
- * it to the groups whose control bit is set; m > 12 returns QHIP_CAPABILITY
- * (the reference accepts up to 26 targets). */
+ * are formed along one squaring chain as the reference does.  For m <= 7 the
+ * chain runs on the host and each controlled power is a Custom op on
+ * (targets..., control) of the fused apply; for 8 <= m <= 14 the chain runs
+ * on the device and each controlled power is a matrix-core GEMM over the
+ * groups whose control bit is set; m > 14 returns QHIP_CAPABILITY (the
+ * reference accepts up to 26 targets, a 2^26 x 2^26 host matrix). */
 int qhip_state_controlled_powers(const qhip_config* cfg, qhip_state* s, const uint32_t* controls,
                                  const uint64_t* exponents, int32_t n_controls, const uint32_t* targets,
                                  int32_t m, const double* matrix);

@@ -4,8 +4,8 @@
 //    place through a scratch state (apply_permutation, program.cpp:610-646);
 //  - controlled matrix powers matrix^e_j on the targets wherever control j
 //    is |1>, lowered to controlled dense ops of the ordinary tile pass on up
-//    to 7 targets, and run by a controlled dense kernel on 8..12
-//    (apply_controlled_powers, program.cpp:648-712).
+//    to 7 targets, and run as matrix-core GEMMs (squaring chain included) on
+//    8..14 (apply_controlled_powers, program.cpp:648-712).
 #include "common.h"
 
 #include <algorithm>
@@ -134,123 +134,292 @@ Mat matmul(const Mat& a, const Mat& b, size_t d) {
     return c;
 }
 
-// ── controlled powers on 8..12 targets ─────────────────────────────────────
-// Too wide for one tile-pass op (targets + control > 8 operands): per
-// control, a kernel applies P on the targets of every group whose control
-// bit is set (apply_controlled_powers, program.cpp:676-706).  A group is the
-// 2^m amplitudes base | deposit(l); the groups with the control set are
-// enumerated directly (zero bits inserted at the sorted targets and control,
-// then the control bit set), so no group is visited and skipped.
-constexpr int kMaxDenseTargets = 12;    // 2^12 amplitudes per group
-constexpr int kCtlBlock = 256;
-constexpr int kCtlAmps = 4096;          // LDS amplitudes per block: G groups x d rows
+// ── controlled powers on 8..14 targets: a matrix-core GEMM ─────────────────
+// Too wide for one tile-pass op (targets + control > 8 operands).  For a
+// control bit c, the 2^(n-1-m) groups of 2^m amplitudes whose control bit is
+// set are the columns of X (group g, amplitude k at base(g) | off(k): zero
+// bits inserted at the sorted targets and control, then the control bit set;
+// off(k) deposits k on the targets in operand order), and the kernel forms
+// Y = P X on the matrix cores (apply_controlled_powers, program.cpp:676-706,
+// one matrix-vector product per group).  A 256-thread workgroup computes a
+// 64-row x 64-group tile of Y: chunks of 16 k of P and X are staged in LDS
+// (the next chunk's loads are issued before the current one is multiplied),
+// each wave owns 32 x 32 outputs as 2 x 2 blocks of 16 x 16 with real and
+// imaginary accumulators (four real products per complex block), and Y is
+// written row-major (Y[i * groups + g]) into a scratch buffer, from which one
+// scatter pass puts it back into the state.  Tiles map to blocks so that the
+// row tiles of one group tile run on one XCD (block b runs on XCD b mod 8),
+// whose L2 then serves their shared X panel.  The same kernel squares
+// matrices on the device: a row-major d x d matrix read as a 2m-qubit
+// register (targets on the high m bits, no control) gives Y = P M row-major.
+constexpr int kMaxGemmTargets = 14;    // P and its squares: 4 GiB each at 14
+constexpr int kGemmT = 256;
+constexpr int kTileI = 64, kTileG = 64, kChunkK = 16;
 
-struct CtlGeom {
-    uint8_t tpos[kMaxDenseTargets];      // local bit b <-> global bit tpos[b]
-    uint8_t ins[kMaxDenseTargets + 1];   // targets and control, ascending
+struct GemmGeom {
+    uint8_t  tpos[kMaxTableQubits];      // target local bit b <-> register bit tpos[b]
+    uint8_t  ins[kMaxTableQubits + 1];   // bits removed from the group number, ascending
+    int32_t  n_ins;
+    int32_t  m;
+    uint64_t cbit;                       // control bit (0: none)
+    uint64_t groups;                     // columns of X
+    int32_t  row_tiles, group_tiles;
+    int32_t  kfast;                      // loaders walk k (targets start at bit 0) or groups
 };
 
-__device__ __forceinline__ double2 cmac(double2 acc, double2 u, double2 x) {
-    acc.x = __fma_rn(u.x, x.x, acc.x);
-    acc.x = __fma_rn(-u.y, x.y, acc.x);
-    acc.y = __fma_rn(u.x, x.y, acc.y);
-    acc.y = __fma_rn(u.y, x.x, acc.y);
-    return acc;
-}
-__device__ __forceinline__ float2 cmac(float2 acc, float2 u, float2 x) {
-    acc.x = __fmaf_rn(u.x, x.x, acc.x);
-    acc.x = __fmaf_rn(-u.y, x.y, acc.x);
-    acc.y = __fmaf_rn(u.x, x.y, acc.y);
-    acc.y = __fmaf_rn(u.y, x.x, acc.y);
-    return acc;
+// Matrix-core steps: A at lane (i = lane % 16, k = lane / 16), B at lane
+// (j = lane % 16, k = lane / 16); D row i of the 16 x 16 result is in
+// register r of lane group q = lane / 16 as i = 4r + q (f64) or 4q + r (f32).
+template <typename A> struct GemmMfma;
+template <> struct GemmMfma<double2> {
+    typedef double R;
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 step(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int r, int q) { return 4 * r + q; }
+};
+template <> struct GemmMfma<float2> {
+    typedef float R;
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 step(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int r, int q) { return 4 * q + r; }
+};
+
+// The geometry is read through a pointer (scalar loads with wave-uniform
+// indices); a by-value kernel argument indexed by run-time bit numbers would
+// be copied to scratch memory.
+__device__ __forceinline__ uint64_t gemm_deposit(uint64_t k, const GemmGeom* __restrict__ geo, int from, int to) {
+    uint64_t o = 0;
+    for (int b = from; b < to; ++b) o |= ((k >> b) & 1u) << geo->tpos[b];
+    return o;
 }
 
-// d = 256 R rows (m = 8 + log2 R); each thread owns rows threadIdx.x + 256 r
-// of G = 4096 / d groups staged in LDS.  PT is P transposed (PT[k d + row]):
-// for a fixed k the block reads one contiguous row of PT, and every group's
-// amplitude k is an LDS broadcast.  PT (d^2 entries, at most 256 MiB) stays
-// in L2 / HBM and is read once per G groups.
-template <typename A, int R>
-__global__ __launch_bounds__(kCtlBlock) void controlled_dense_kernel(A* __restrict__ s, const A* __restrict__ PT,
-                                                                     const CtlGeom geo, int m, uint64_t cbit,
-                                                                     uint64_t groups) {
-    constexpr int d = kCtlBlock * R;
-    constexpr int G = kCtlAmps / d;
-    __shared__ A col[kCtlAmps];
-    uint64_t off[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const uint32_t l = threadIdx.x + kCtlBlock * r;
-        uint64_t o = 0;
-        for (int b = 0; b < m; ++b) o |= static_cast<uint64_t>((l >> b) & 1u) << geo.tpos[b];
-        off[r] = o;
+__device__ __forceinline__ uint64_t gemm_base(uint64_t g, const GemmGeom* __restrict__ geo) {
+    uint64_t base = g;
+    const int n_ins = geo->n_ins;
+    for (int b = 0; b < n_ins; ++b) {
+        const uint64_t p = geo->ins[b];
+        base = ((base >> p) << (p + 1)) | (base & ((uint64_t{1} << p) - 1u));
     }
-    auto group_base = [&](uint64_t gi) {
-        uint64_t base = gi;
-        for (int b = 0; b <= m; ++b) {
-            const uint64_t p = geo.ins[b];
-            base = ((base >> p) << (p + 1)) | (base & ((uint64_t{1} << p) - 1u));
-        }
-        return base | cbit;
-    };
-    const uint64_t supers = (groups + G - 1) / G;
-    for (uint64_t sg = blockIdx.x; sg < supers; sg += gridDim.x) {
+    return base | geo->cbit;
+}
+
+template <typename A>
+__global__ __launch_bounds__(kGemmT) void controlled_gemm_kernel(const A* __restrict__ in, const A* __restrict__ P,
+                                                                 A* __restrict__ Y, const GemmGeom* __restrict__ gp) {
+    using MF = GemmMfma<A>;
+    using R = typename MF::R;
+    using v4 = typename MF::v4;
+    __shared__ A Ps[kChunkK][kTileI];
+    __shared__ A Xs[kChunkK][kTileG];
+    const uint32_t row_tiles = static_cast<uint32_t>(gp->row_tiles);
+    const uint32_t b = blockIdx.x, xcd = b & 7u, local = b >> 3;
+    const uint32_t gt = (local / row_tiles) * 8u + xcd;
+    const uint32_t rt = local % row_tiles;
+    if (gt >= static_cast<uint32_t>(gp->group_tiles)) return;   // the whole block
+    const int m = gp->m;
+    const uint64_t d = uint64_t{1} << m, groups = gp->groups;
+    const bool kfast = gp->kfast != 0;
+    const int t = threadIdx.x;
+
+    // Loaders: four P and four X elements per thread and chunk.  P is read
+    // along its rows (4 consecutive k per thread); X along k when the targets
+    // start at register bit 0, else along the groups.
+    const int pi_ = t >> 2, pk = (t & 3) * 4;
+    int xg[4], xk[4];
 #pragma unroll
-        for (int g = 0; g < G; ++g) {
-            const uint64_t gi = sg * G + g;
-            if (gi < groups) {
-                const uint64_t base = group_base(gi);
+    for (int s = 0; s < 4; ++s) {
+        if (kfast) { xg[s] = (t >> 4) + 16 * s; xk[s] = t & 15; }
+        else { xg[s] = t & 63; xk[s] = (t >> 6) + 4 * s; }
+    }
+    uint64_t xoff[4];
+    bool xok[4];
 #pragma unroll
-                for (int r = 0; r < R; ++r) col[g * d + threadIdx.x + kCtlBlock * r] = s[base | off[r]];
-            }
-        }
+    for (int s = 0; s < 4; ++s) {
+        const uint64_t g = static_cast<uint64_t>(gt) * kTileG + xg[s];
+        xok[s] = g < groups;
+        xoff[s] = (xok[s] ? gemm_base(g, gp) : 0) | gemm_deposit(static_cast<uint64_t>(xk[s]), gp, 0, 4);
+    }
+    const A* prow = P + (static_cast<uint64_t>(rt) * kTileI + pi_) * d + pk;
+    A pv0, pv1, pv2, pv3, xv0, xv1, xv2, xv3;
+#define QHIP_GEMM_LOAD(K0)                                                         \
+    {                                                                              \
+        const uint64_t hi_ = gemm_deposit((K0), gp, 4, m);                         \
+        pv0 = prow[(K0) + 0];                                                      \
+        pv1 = prow[(K0) + 1];                                                      \
+        pv2 = prow[(K0) + 2];                                                      \
+        pv3 = prow[(K0) + 3];                                                      \
+        xv0 = xok[0] ? in[xoff[0] | hi_] : A{};                                    \
+        xv1 = xok[1] ? in[xoff[1] | hi_] : A{};                                    \
+        xv2 = xok[2] ? in[xoff[2] | hi_] : A{};                                    \
+        xv3 = xok[3] ? in[xoff[3] | hi_] : A{};                                    \
+    }
+
+    const int w = t >> 6, lane = t & 63, li = lane & 15, lq = lane >> 4;
+    const int wi = (w & 1) * 32, wg = (w >> 1) * 32;
+    v4 accr[2][2], acci[2][2];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int gb = 0; gb < 2; ++gb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) { accr[rb][gb][r] = 0; acci[rb][gb][r] = 0; }
+
+    QHIP_GEMM_LOAD(uint64_t{0})
+    for (uint64_t k0 = 0; k0 < d; k0 += kChunkK) {
+        Ps[pk + 0][pi_] = pv0;
+        Ps[pk + 1][pi_] = pv1;
+        Ps[pk + 2][pi_] = pv2;
+        Ps[pk + 3][pi_] = pv3;
+        Xs[xk[0]][xg[0]] = xv0;
+        Xs[xk[1]][xg[1]] = xv1;
+        Xs[xk[2]][xg[2]] = xv2;
+        Xs[xk[3]][xg[3]] = xv3;
         __syncthreads();
-        A acc[R][G];
+        if (k0 + kChunkK < d) QHIP_GEMM_LOAD(k0 + kChunkK)
 #pragma unroll
-        for (int r = 0; r < R; ++r)
+        for (int ks = 0; ks < kChunkK / 4; ++ks) {
+            const int kq = ks * 4 + lq;
+            R ar[2], ai[2], an[2], br[2], bi[2];
 #pragma unroll
-            for (int g = 0; g < G; ++g) { acc[r][g].x = 0; acc[r][g].y = 0; }
-#pragma unroll 2
-        for (int k = 0; k < d; ++k) {
-            A p[R];
+            for (int rb = 0; rb < 2; ++rb) {
+                const A v = Ps[kq][wi + rb * 16 + li];
+                ar[rb] = v.x;
+                ai[rb] = v.y;
+                an[rb] = -v.y;
+            }
 #pragma unroll
-            for (int r = 0; r < R; ++r) p[r] = PT[static_cast<size_t>(k) * d + threadIdx.x + kCtlBlock * r];
+            for (int gb = 0; gb < 2; ++gb) {
+                const A v = Xs[kq][wg + gb * 16 + li];
+                br[gb] = v.x;
+                bi[gb] = v.y;
+            }
 #pragma unroll
-            for (int g = 0; g < G; ++g) {
-                const A c = col[g * d + k];
+            for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
-                for (int r = 0; r < R; ++r) acc[r][g] = cmac(acc[r][g], p[r], c);
+                for (int gb = 0; gb < 2; ++gb) {
+                    accr[rb][gb] = MF::step(ar[rb], br[gb], accr[rb][gb]);
+                    accr[rb][gb] = MF::step(an[rb], bi[gb], accr[rb][gb]);
+                    acci[rb][gb] = MF::step(ar[rb], bi[gb], acci[rb][gb]);
+                    acci[rb][gb] = MF::step(ai[rb], br[gb], acci[rb][gb]);
+                }
+        }
+        __syncthreads();   // Ps / Xs are rewritten by the next chunk
+    }
+#undef QHIP_GEMM_LOAD
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int gb = 0; gb < 2; ++gb) {
+            const uint64_t g = static_cast<uint64_t>(gt) * kTileG + wg + gb * 16 + li;
+            if (g >= groups) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const uint64_t i = static_cast<uint64_t>(rt) * kTileI + wi + rb * 16 + MF::row(r, lq);
+                A o;
+                o.x = accr[rb][gb][r];
+                o.y = acci[rb][gb][r];
+                Y[i * groups + g] = o;
             }
         }
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-            const uint64_t gi = sg * G + g;
-            if (gi < groups) {
-                const uint64_t base = group_base(gi);
-#pragma unroll
-                for (int r = 0; r < R; ++r) s[base | off[r]] = acc[r][g];
-            }
-        }
-        __syncthreads();   // col is rewritten by the next super-group
+}
+
+// state[base(g) | off(i)] = Y[i * groups + g] for every row i and group g.
+template <typename A>
+__global__ __launch_bounds__(kBlock) void gemm_scatter_kernel(A* __restrict__ s, const A* __restrict__ Y,
+                                                              const GemmGeom* __restrict__ gp) {
+    const int m = gp->m;
+    const uint64_t groups = gp->groups, total = groups << m;
+    const int gshift = __builtin_ctzll(groups);
+    const uint64_t step = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t e = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; e < total; e += step) {
+        const uint64_t i = e >> gshift, g = e & (groups - 1);
+        s[gemm_base(g, gp) | gemm_deposit(i, gp, 0, m)] = Y[e];
     }
 }
 
 template <typename A>
-void launch_controlled_dense(qhip_state* s, const void* PT, const CtlGeom& geo, int m, uint64_t cbit) {
-    auto& C = ctx();
-    const uint64_t groups = uint64_t{1} << (s->n - m - 1);
-    const uint64_t supers = (groups + (kCtlAmps >> m) - 1) / (kCtlAmps >> m);
-    const int blocks = static_cast<int>(std::min<uint64_t>(supers, static_cast<uint64_t>(C.n_cu) * 4));
-    auto* st = static_cast<A*>(s->d);
-    auto* pt = static_cast<const A*>(PT);
-    switch (m) {
-        case 8: hipLaunchKernelGGL((controlled_dense_kernel<A, 1>), dim3(blocks), dim3(kCtlBlock), 0, C.stream, st, pt, geo, m, cbit, groups); break;
-        case 9: hipLaunchKernelGGL((controlled_dense_kernel<A, 2>), dim3(blocks), dim3(kCtlBlock), 0, C.stream, st, pt, geo, m, cbit, groups); break;
-        case 10: hipLaunchKernelGGL((controlled_dense_kernel<A, 4>), dim3(blocks), dim3(kCtlBlock), 0, C.stream, st, pt, geo, m, cbit, groups); break;
-        case 11: hipLaunchKernelGGL((controlled_dense_kernel<A, 8>), dim3(blocks), dim3(kCtlBlock), 0, C.stream, st, pt, geo, m, cbit, groups); break;
-        default: hipLaunchKernelGGL((controlled_dense_kernel<A, 16>), dim3(blocks), dim3(kCtlBlock), 0, C.stream, st, pt, geo, m, cbit, groups); break;
+__global__ __launch_bounds__(kBlock) void narrow_kernel(const double2* __restrict__ in, A* __restrict__ out,
+                                                        uint64_t n) {
+    const uint64_t step = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t e = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; e < n; e += step) {
+        A o;
+        o.x = in[e].x;
+        o.y = in[e].y;
+        out[e] = o;
     }
+}
+
+GemmGeom gemm_geom(int m, const uint32_t* tpos, const std::vector<uint32_t>& removed, uint64_t cbit,
+                   uint64_t groups) {
+    GemmGeom geo{};
+    geo.m = m;
+    for (int b = 0; b < m; ++b) geo.tpos[b] = static_cast<uint8_t>(tpos[b]);
+    std::vector<uint32_t> srt(removed);
+    std::sort(srt.begin(), srt.end());
+    geo.n_ins = static_cast<int32_t>(srt.size());
+    for (size_t b = 0; b < srt.size(); ++b) geo.ins[b] = static_cast<uint8_t>(srt[b]);
+    geo.cbit = cbit;
+    geo.groups = groups;
+    geo.row_tiles = static_cast<int32_t>((uint64_t{1} << m) / kTileI);
+    geo.group_tiles = static_cast<int32_t>((groups + kTileG - 1) / kTileG);
+    geo.kfast = tpos[0] == 0 ? 1 : 0;
+    return geo;
+}
+
+// geo: the host copy (grid sizes); dgeo: the same record on the device.
+template <typename A>
+void launch_gemm(const A* in, const A* P, A* Y, const GemmGeom& geo, const GemmGeom* dgeo) {
+    auto& C = ctx();
+    const uint64_t gt8 = (static_cast<uint64_t>(geo.group_tiles) + 7) / 8 * 8;
+    const uint64_t blocks = gt8 * static_cast<uint64_t>(geo.row_tiles);
+    if (blocks > 0x7fffffffu) fail(QHIP_CAPABILITY, "controlled powers: GEMM grid too large");
+    hipLaunchKernelGGL(controlled_gemm_kernel<A>, dim3(static_cast<uint32_t>(blocks)), dim3(kGemmT), 0, C.stream,
+                       in, P, Y, dgeo);
     QHIP_CHECK(hipGetLastError());
+}
+
+template <typename A>
+void launch_scatter(A* s, const A* Y, const GemmGeom& geo, const GemmGeom* dgeo) {
+    auto& C = ctx();
+    const uint64_t total = geo.groups << geo.m;
+    const int blocks = static_cast<int>(std::min<uint64_t>((total + kBlock - 1) / kBlock,
+                                                           static_cast<uint64_t>(C.n_cu) * 32));
+    hipLaunchKernelGGL(gemm_scatter_kernel<A>, dim3(blocks), dim3(kBlock), 0, C.stream, s, Y, dgeo);
+    QHIP_CHECK(hipGetLastError());
+}
+
+// Device matrices (row-major d x d, complex128) for the squaring chain.
+struct DevMat {
+    void* p = nullptr;
+    explicit DevMat(size_t bytes) {
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            fail(QHIP_NOMEM, "controlled powers: device allocation of a " + std::to_string(bytes) +
+                                 "-byte matrix failed");
+        }
+    }
+    DevMat(const DevMat&) = delete;
+    DevMat& operator=(const DevMat&) = delete;
+    ~DevMat() {
+        if (p) {
+            (void)hipStreamSynchronize(ctx().stream);
+            (void)hipFree(p);
+        }
+    }
+    double2* d() const { return static_cast<double2*>(p); }
+};
+
+// Geometry of out = a b for row-major d x d matrices: b read as a 2m-qubit
+// register whose high m bits are the targets (no control).
+GemmGeom matmul_geom(int m) {
+    std::vector<uint32_t> tpos(static_cast<size_t>(m)), removed(static_cast<size_t>(m));
+    for (int q = 0; q < m; ++q) tpos[q] = removed[q] = static_cast<uint32_t>(m + q);
+    return gemm_geom(m, tpos.data(), removed, 0, uint64_t{1} << m);
 }
 
 void check_distinct(std::vector<uint32_t> q, const char* what) {
@@ -339,20 +508,93 @@ void controlled_powers(const qhip_config* cfg, qhip_state* s, const uint32_t* co
         if (q >= static_cast<uint32_t>(s->n))
             fail(QHIP_INVALID, "qhip_state_controlled_powers: qubit " + std::to_string(q) + " outside the " +
                                    std::to_string(s->n) + "-qubit register");
-    if (m > kMaxDenseTargets)
-        fail(QHIP_CAPABILITY, "qhip_state_controlled_powers: at most " + std::to_string(kMaxDenseTargets) +
-                                  " targets (one group of 2^m amplitudes is staged in LDS)");
+    if (m > kMaxGemmTargets)
+        fail(QHIP_CAPABILITY, "qhip_state_controlled_powers: at most " + std::to_string(kMaxGemmTargets) +
+                                  " targets (the 2^m x 2^m matrix and its squares are held on the device)");
     const size_t d = size_t{1} << m;
-    Mat M(d * d);
-    for (size_t i = 0; i < d * d; ++i) M[i] = cplx(matrix[2 * i], matrix[2 * i + 1]);
-
-    // squares[b] = M^(2^b) along one chain; an exponent with several set bits
-    // multiplies its squares from the lowest bit up (program.cpp:656-673).
     uint64_t any = 0;
     for (int j = 0; j < n_controls; ++j) any |= exponents[j];
-    std::vector<Mat> squares;
     const int n_sq = static_cast<int>(std::bit_width(any));
-    if (n_sq > 0) squares.push_back(M);
+    if (n_sq == 0) return;
+
+    if (m > kMaxPowerTargets) {
+        // squares[b] = M^(2^b) along one chain, and a product of squares from
+        // the lowest bit up for an exponent with several bits
+        // (program.cpp:656-673), all on the device; then per control one GEMM
+        // into the scratch buffer and one scatter back into the state.
+        auto& C = ctx();
+        const size_t mat_bytes = d * d * sizeof(double2);
+        const uint64_t groups = uint64_t{1} << (s->n - m - 1);
+        // Geometries: [0] the chain's products, [1 + j] control j.
+        std::vector<GemmGeom> geos(1 + static_cast<size_t>(n_controls));
+        geos[0] = matmul_geom(m);
+        for (int j = 0; j < n_controls; ++j) {
+            std::vector<uint32_t> removed(targets, targets + m);
+            removed.push_back(controls[j]);
+            geos[1 + j] = gemm_geom(m, targets, removed, uint64_t{1} << controls[j], groups);
+        }
+        DevMat dgeo(geos.size() * sizeof(GemmGeom));
+        QHIP_CHECK(hipMemcpyAsync(dgeo.p, geos.data(), geos.size() * sizeof(GemmGeom), hipMemcpyHostToDevice,
+                                  C.stream));
+        const GemmGeom* dg = static_cast<const GemmGeom*>(dgeo.p);
+        auto matmul = [&](const double2* a, const double2* b, double2* out) {
+            launch_gemm<double2>(b, a, out, geos[0], dg);
+        };
+        std::vector<std::unique_ptr<DevMat>> squares;
+        squares.push_back(std::make_unique<DevMat>(mat_bytes));
+        QHIP_CHECK(hipMemcpyAsync(squares[0]->p, matrix, mat_bytes, hipMemcpyHostToDevice, C.stream));
+        for (int b = 1; b < n_sq; ++b) {
+            squares.push_back(std::make_unique<DevMat>(mat_bytes));
+            matmul(squares[b - 1]->d(), squares[b - 1]->d(), squares[b]->d());
+        }
+        std::map<uint64_t, std::unique_ptr<DevMat>> products;
+        for (int j = 0; j < n_controls; ++j) {
+            const uint64_t e = exponents[j];
+            if (std::popcount(e) < 2 || products.count(e)) continue;
+            auto r = std::make_unique<DevMat>(mat_bytes);
+            auto tmp = std::make_unique<DevMat>(mat_bytes);
+            QHIP_CHECK(hipMemcpyAsync(r->p, squares[static_cast<size_t>(std::countr_zero(e))]->p, mat_bytes,
+                                      hipMemcpyDeviceToDevice, C.stream));
+            for (uint64_t x = e & (e - 1); x != 0; x &= x - 1) {
+                matmul(r->d(), squares[static_cast<size_t>(std::countr_zero(x))]->d(), tmp->d());
+                std::swap(r, tmp);
+            }
+            products.emplace(e, std::move(r));
+        }
+        const bool c128 = s->precision == QHIP_C128;
+        void* Y = C.perm.ensure(static_cast<size_t>(groups << m) * s->amp_bytes());
+        std::unique_ptr<DevMat> narrow;
+        if (!c128) narrow = std::make_unique<DevMat>(d * d * sizeof(float2));
+        for (int j = 0; j < n_controls; ++j) {
+            const uint64_t e = exponents[j];
+            if (e == 0) continue;
+            const double2* P = std::has_single_bit(e) ? squares[static_cast<size_t>(std::countr_zero(e))]->d()
+                                                      : products.at(e)->d();
+            const GemmGeom& geo = geos[1 + j];
+            if (c128) {
+                launch_gemm<double2>(static_cast<const double2*>(s->d), P, static_cast<double2*>(Y), geo, dg + 1 + j);
+                launch_scatter<double2>(static_cast<double2*>(s->d), static_cast<const double2*>(Y), geo, dg + 1 + j);
+            } else {
+                const int blocks = static_cast<int>(std::min<uint64_t>((d * d + kBlock - 1) / kBlock,
+                                                                       static_cast<uint64_t>(C.n_cu) * 32));
+                hipLaunchKernelGGL(narrow_kernel<float2>, dim3(blocks), dim3(kBlock), 0, C.stream, P,
+                                   static_cast<float2*>(narrow->p), static_cast<uint64_t>(d * d));
+                QHIP_CHECK(hipGetLastError());
+                launch_gemm<float2>(static_cast<const float2*>(s->d), static_cast<const float2*>(narrow->p),
+                                    static_cast<float2*>(Y), geo, dg + 1 + j);
+                launch_scatter<float2>(static_cast<float2*>(s->d), static_cast<const float2*>(Y), geo, dg + 1 + j);
+            }
+        }
+        QHIP_CHECK(hipStreamSynchronize(C.stream));
+        return;
+    }
+
+    Mat M(d * d);
+    for (size_t i = 0; i < d * d; ++i) M[i] = cplx(matrix[2 * i], matrix[2 * i + 1]);
+    // squares[b] = M^(2^b) along one chain; an exponent with several set bits
+    // multiplies its squares from the lowest bit up (program.cpp:656-673).
+    std::vector<Mat> squares;
+    squares.push_back(M);
     for (int b = 1; b < n_sq; ++b) squares.push_back(matmul(squares.back(), squares.back(), d));
     std::map<uint64_t, Mat> products;
     for (int j = 0; j < n_controls; ++j) {
@@ -362,43 +604,6 @@ void controlled_powers(const qhip_config* cfg, qhip_state* s, const uint32_t* co
         for (uint64_t x = e & (e - 1); x != 0; x &= x - 1)
             r = matmul(r, squares[static_cast<size_t>(std::countr_zero(x))], d);
         products.emplace(e, std::move(r));
-    }
-
-    if (m > kMaxPowerTargets) {
-        // Per control, the controlled dense kernel with P transposed in the
-        // state's precision; the stream is drained before the host buffer is
-        // refilled for the next control.
-        auto& C = ctx();
-        CtlGeom geo{};
-        std::vector<uint32_t> ins(targets, targets + m);
-        for (int b = 0; b < m; ++b) geo.tpos[b] = static_cast<uint8_t>(targets[b]);
-        const bool c128 = s->precision == QHIP_C128;
-        void* dPT = C.misc2.ensure(d * d * s->amp_bytes());
-        std::vector<double> h64(c128 ? 2 * d * d : 0);
-        std::vector<float> h32(c128 ? 0 : 2 * d * d);
-        for (int j = 0; j < n_controls; ++j) {
-            const uint64_t e = exponents[j];
-            if (e == 0) continue;
-            const Mat& P = std::has_single_bit(e) ? squares[static_cast<size_t>(std::countr_zero(e))] : products.at(e);
-            for (size_t row = 0; row < d; ++row)
-                for (size_t k = 0; k < d; ++k) {
-                    const cplx v = P[row * d + k];
-                    const size_t t = 2 * (k * d + row);
-                    if (c128) { h64[t] = v.real(); h64[t + 1] = v.imag(); }
-                    else { h32[t] = static_cast<float>(v.real()); h32[t + 1] = static_cast<float>(v.imag()); }
-                }
-            std::vector<uint32_t> srt(ins);
-            srt.push_back(controls[j]);
-            std::sort(srt.begin(), srt.end());
-            for (int b = 0; b <= m; ++b) geo.ins[b] = static_cast<uint8_t>(srt[b]);
-            QHIP_CHECK(hipMemcpyAsync(dPT, c128 ? static_cast<const void*>(h64.data()) : h32.data(),
-                                      d * d * s->amp_bytes(), hipMemcpyHostToDevice, C.stream));
-            const uint64_t cbit = uint64_t{1} << controls[j];
-            if (c128) launch_controlled_dense<double2>(s, dPT, geo, m, cbit);
-            else launch_controlled_dense<float2>(s, dPT, geo, m, cbit);
-            QHIP_CHECK(hipStreamSynchronize(C.stream));
-        }
-        return;
     }
 
     // One Custom op per control with a non-zero exponent: operands

@@ -1201,7 +1201,7 @@ static void mat_power(const cplx* m, uint64_t e, cplx* out, uint64_t d) {
  * basis state whose control bit is set. */
 int orc_controlled_powers(double* state, int n, const uint32_t* controls, const uint64_t* exponents,
                           int n_controls, const uint32_t* targets, int m, const double* matrix) {
-    if (m < 1 || m > 12) return fail(QHIP_INVALID, "orc_controlled_powers: 1..12 targets");
+    if (m < 1 || m > 26) return fail(QHIP_INVALID, "orc_controlled_powers: 1..26 targets");
     const uint64_t dim = (uint64_t)1 << n, d = (uint64_t)1 << m;
     uint64_t tmask = 0;
     for (int b = 0; b < m; ++b) tmask |= (uint64_t)1 << targets[b];

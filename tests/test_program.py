@@ -278,28 +278,34 @@ def test_program_errors():
     q.add_permutation([1], [1, 0])
     with pytest.raises(ValueError, match="outside the program's last gates kernel"):
         sim.program_run(q, 2, 10)
-    # 13 targets: rejected by the library before the matrix is read.
+    # 15 targets: rejected by the library before the matrix is read.
     import ctypes as C
     from openqarp_amd import _lib
     from openqarp_amd.simulator import DeviceState
-    st = DeviceState(14, "complex128")
-    cs, es, ts = np.array([13], np.uint32), np.array([1], np.uint64), np.arange(13, dtype=np.uint32)
+    st = DeviceState(16, "complex128")
+    cs, es, ts = np.array([15], np.uint32), np.array([1], np.uint64), np.arange(15, dtype=np.uint32)
     cfg = sim._config()
-    with pytest.raises(CapabilityError, match="at most 12 targets"):
+    with pytest.raises(CapabilityError, match="at most 14 targets"):
         _lib.check(_lib.load().qhip_state_controlled_powers(C.byref(cfg), st.handle, cs.ctypes.data, es.ctypes.data,
-                                                            1, ts.ctypes.data, 13, np.zeros(2).ctypes.data))
+                                                            1, ts.ctypes.data, 15, np.zeros(2).ctypes.data))
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("m,n,precision,exps", [
-    (8, 12, "complex128", [5, 0, 6, 1]), (9, 12, "complex64", [3, 1]), (10, 13, "complex128", [1]),
-    (11, 13, "complex64", [1]), (12, 13, "complex128", [1])])
-def test_wide_controlled_powers_against_oracle(m, n, precision, exps):
-    """Controlled powers on 8..12 targets (wider than one tile-pass op) run
-    in the controlled dense kernel: scattered targets and controls, zero
-    exponents skipped, products of squares (m = 8, 9), against the oracle."""
-    rng = np.random.default_rng(40 + m)
+@pytest.mark.parametrize("m,n,precision,exps,low", [
+    (8, 12, "complex128", [5, 0, 6, 1], False), (9, 12, "complex64", [3, 1], False),
+    (10, 13, "complex128", [1], False), (11, 13, "complex64", [1], False), (12, 13, "complex128", [1], False),
+    (8, 18, "complex128", [3, 1], True), (8, 18, "complex128", [6], False), (9, 17, "complex64", [2], True),
+    (13, 15, "complex128", [1], True)])
+def test_wide_controlled_powers_against_oracle(m, n, precision, exps, low):
+    """Controlled powers on 8..14 targets (wider than one tile-pass op) run
+    as matrix-core GEMMs over the groups whose control is set, with the
+    squaring chain on the device: scattered targets and controls (or targets
+    on the low qubits, low=True), zero exponents skipped, products of squares,
+    partial and full 64-group tiles, against the oracle."""
+    rng = np.random.default_rng(40 + m + n)
     qs = [int(q) for q in rng.permutation(n)[: m + len(exps)]]
+    if low:
+        qs = list(range(m)) + [int(q) for q in rng.permutation(np.arange(m, n))[: len(exps)]]
     d = 1 << m
     if m <= 10:
         M = haar(d, rng)
