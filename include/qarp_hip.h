@@ -242,6 +242,24 @@ int qhip_state_controlled_powers(const qhip_config* cfg, qhip_state* s, const ui
                                  const uint64_t* exponents, int32_t n_controls, const uint32_t* targets,
                                  int32_t m, const double* matrix);
 
+/* Stands in for permutation_table (program.cpp:714-724; Python binding
+ * qx._permutation_table, bindings.cpp:1302-1315): the basis permutation the
+ * stream applies to its k local qubits, exact including phase.  `local` must
+ * already be remapped onto qubits [0, k) and hold only concrete, unconditioned
+ * unitary commands (the caller rejects the rest, as to_local does,
+ * program.cpp:404-436).  *found = 1 and table[0 .. 2^k) written when the
+ * stream is a permutation the reference derives (classically gate by gate,
+ * or with at most max_rest simulated qubits within 2^max_work_log2 x
+ * |stream| work); *found = 0 otherwise.  Host computation, no device call. */
+int qhip_permutation_table(const qhip_stream* local, int32_t k, uint32_t max_rest, uint32_t max_work_log2,
+                           int32_t* found, uint64_t* table);
+
+/* FNV-1a over the 8 little-endian bytes of each word, from `seed`: the mixing
+ * step of commands_digest / local_commands_digest / parts_digest
+ * (program.cpp:739-829); the caller lays a stream out as words in the
+ * reference's order.  Host computation, no device call. */
+int qhip_fnv1a_words(const uint64_t* words, int64_t n, uint64_t seed, uint64_t* out);
+
 /* Batched basis states for transition columns: the state is read as blocks
  * of `stride` amplitudes (a power of two, at most 2^n); block b is set
  * to |labels[b]> for b < n_labels and to zero beyond.  Stands in for

@@ -31,6 +31,7 @@ EXPORTS = [
     "qhip_sample", "qhip_sample_uniforms", "qhip_probabilities", "qhip_exact",
     "qhip_run_gradient", "qhip_run_gradient_phi", "qhip_batch_energy_grad",
     "qhip_batch_create", "qhip_batch_run", "qhip_batch_update", "qhip_batch_destroy", "qhip_batch_stats",
+    "qhip_permutation_table", "qhip_fnv1a_words",
 ]
 
 
@@ -92,6 +93,8 @@ def _declare(lib):
         "qhip_batch_update": [vp, vp, C.POINTER(i32)],
         "qhip_batch_destroy": [vp],
         "qhip_batch_stats": [vp, vp],
+        "qhip_permutation_table": [vp, i32, u32, u32, vp, vp],
+        "qhip_fnv1a_words": [vp, i64, u64, vp],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

@@ -422,6 +422,10 @@ void batch_run(BatchSession* s, double* energies, double* grads, const int64_t* 
 bool batch_update(BatchSession* s, const qhip_stream* streams);
 void batch_destroy(BatchSession* s);
 void batch_stats(const BatchSession* s, qhip_batch_info* out);
+
+// Planner derivations (derive.cpp).
+bool permutation_table(const qhip_stream* local, int k, uint32_t max_rest, uint32_t max_work_log2, uint64_t* out);
+uint64_t fnv1a_words(const uint64_t* words, int64_t n, uint64_t h);
 void batch_energy_grad(const qhip_config* cfg, int n_instances, const int32_t* n_qubits,
                        const qhip_stream* streams, const qhip_observable* observables,
                        const int32_t* n_params, int precision, double* energies, double* grads,

@@ -953,6 +953,21 @@ int qhip_batch_update(qhip_batch* b, const qhip_stream* streams, int32_t* rebuil
     });
 }
 
+int qhip_permutation_table(const qhip_stream* local, int32_t k, uint32_t max_rest, uint32_t max_work_log2,
+                           int32_t* found, uint64_t* table) {
+    return guarded([&] {
+        need(local && found && table, "qhip_permutation_table: NULL argument");
+        *found = qhip::permutation_table(local, k, max_rest, max_work_log2, table) ? 1 : 0;
+    });
+}
+
+int qhip_fnv1a_words(const uint64_t* words, int64_t n, uint64_t seed, uint64_t* out) {
+    return guarded([&] {
+        need(out && n >= 0 && (words || n == 0), "qhip_fnv1a_words: NULL argument");
+        *out = qhip::fnv1a_words(words, n, seed);
+    });
+}
+
 int qhip_batch_stats(const qhip_batch* b, qhip_batch_info* out) {
     return guarded([&] {
         need(b && out, "qhip_batch_stats: NULL argument");

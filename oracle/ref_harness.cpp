@@ -463,4 +463,39 @@ int ref_program_statevector(const ref_kernel* ks, int nk, int n, int fusion_max,
     } catch (const std::exception& e) { return catch_all(e, QHIP_RUNTIME); }
 }
 
+// permutation_table (program.cpp:714-724): *found = 0 for nullopt, else the
+// 2^k table.
+int ref_permutation_table(const qhip_stream* s, const uint32_t* qubits, int k, uint32_t max_rest,
+                          uint32_t max_work_log2, int32_t* found, uint64_t* out) {
+    try {
+        auto t = permutation_table(to_commands(s), std::vector<uint32_t>(qubits, qubits + k), max_rest,
+                                   max_work_log2);
+        *found = t ? 1 : 0;
+        if (t) std::copy(t->begin(), t->end(), out);
+        return 0;
+    } catch (const std::invalid_argument& e) { return catch_all(e, QHIP_INVALID);
+    } catch (const std::exception& e) { return catch_all(e, QHIP_RUNTIME); }
+}
+
+// local_unitary_of (program.cpp:726-738), row-major (re, im).
+int ref_local_unitary_of(const qhip_stream* s, const uint32_t* qubits, int k, double* out) {
+    try {
+        const Eigen::MatrixXcd U = local_unitary_of(to_commands(s), std::vector<uint32_t>(qubits, qubits + k));
+        const Eigen::Index D = U.rows();
+        for (Eigen::Index r = 0; r < D; ++r)
+            for (Eigen::Index c = 0; c < D; ++c) {
+                out[2 * (r * D + c)] = U(r, c).real();
+                out[2 * (r * D + c) + 1] = U(r, c).imag();
+            }
+        return 0;
+    } catch (const std::invalid_argument& e) { return catch_all(e, QHIP_INVALID);
+    } catch (const std::exception& e) { return catch_all(e, QHIP_RUNTIME); }
+}
+
+// commands_digest / local_commands_digest (program.cpp:782-803).
+uint64_t ref_commands_digest(const qhip_stream* s) { return commands_digest(to_commands(s)); }
+uint64_t ref_local_commands_digest(const qhip_stream* s, const uint32_t* qubits, int k) {
+    return local_commands_digest(to_commands(s), std::vector<uint32_t>(qubits, qubits + k));
+}
+
 }  // extern "C"

@@ -81,6 +81,12 @@ def reference():
         lib.ref_run_noisy.argtypes = [vp, vp, vp, vp, vp, i32, i32, i32, i32, C.c_uint32, i64, vp, vp, vp, vp, vp,
                                       i64]
         lib.ref_uniform_int.argtypes = [C.c_uint32, C.c_uint64, C.c_uint64, i64, vp]
+        lib.ref_permutation_table.argtypes = [vp, vp, i32, C.c_uint32, C.c_uint32, vp, vp]
+        lib.ref_local_unitary_of.argtypes = [vp, vp, i32, vp]
+        lib.ref_commands_digest.argtypes = [vp]
+        lib.ref_commands_digest.restype = C.c_uint64
+        lib.ref_local_commands_digest.argtypes = [vp, vp, i32]
+        lib.ref_local_commands_digest.restype = C.c_uint64
         _ref = lib
     return _ref
 
@@ -646,3 +652,32 @@ def run_trajectory(commands, n, n_shots, seed, lib="orc", noise=None, enabled=Tr
         counts[out] = counts.get(out, 0) + 1
         hist.append(list(reg))
     return counts, hist
+
+
+def ref_permutation_table(packed, qubits, max_rest, max_work_log2):
+    """permutation_table of the reference build (int64 table or None)."""
+    qs = np.ascontiguousarray(qubits, dtype=np.uint32)
+    out = np.zeros(1 << qs.size, dtype=np.uint64)
+    found = C.c_int32(0)
+    L = reference()
+    _check(L, L.ref_permutation_table(packed.ref, qs.ctypes.data, qs.size, max_rest, max_work_log2,
+                                      C.byref(found), out.ctypes.data), "ref")
+    return out.astype(np.int64) if found.value else None
+
+
+def ref_local_unitary_of(packed, qubits):
+    qs = np.ascontiguousarray(qubits, dtype=np.uint32)
+    d = 1 << qs.size
+    out = np.zeros(d * d, dtype=np.complex128)
+    L = reference()
+    _check(L, L.ref_local_unitary_of(packed.ref, qs.ctypes.data, qs.size, out.ctypes.data), "ref")
+    return out.reshape(d, d)
+
+
+def ref_commands_digest(packed):
+    return int(reference().ref_commands_digest(packed.ref))
+
+
+def ref_local_commands_digest(packed, qubits):
+    qs = np.ascontiguousarray(qubits, dtype=np.uint32)
+    return int(reference().ref_local_commands_digest(packed.ref, qs.ctypes.data, qs.size))
