@@ -342,7 +342,44 @@ __global__ __launch_bounds__(kBlock) void gemm_scatter_kernel(A* __restrict__ s,
     const uint64_t step = static_cast<uint64_t>(gridDim.x) * kBlock;
     for (uint64_t e = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; e < total; e += step) {
         const uint64_t i = e >> gshift, g = e & (groups - 1);
-        s[gemm_base(g, gp) | gemm_deposit(i, gp, 0, m)] = Y[e];
+        const A v = Y[e];
+        A* dst = s + (gemm_base(g, gp) | gemm_deposit(i, gp, 0, m));
+        __builtin_nontemporal_store(v.x, &dst->x);   // plain stores wrote twice the bytes
+        __builtin_nontemporal_store(v.y, &dst->y);
+    }
+}
+
+// The same scatter when the targets' first six bits are register bits 0..5
+// (off(i) contiguous over 64 consecutive i): a 64-row x 32-group tile of Y
+// is read along its rows (contiguous groups), transposed through LDS, and
+// written 64 contiguous amplitudes per group.
+constexpr int kScRows = 64, kScGroups = 32;
+
+template <typename A>
+__global__ __launch_bounds__(kBlock) void gemm_scatter_rows_kernel(A* __restrict__ s, const A* __restrict__ Y,
+                                                                   const GemmGeom* __restrict__ gp) {
+    __shared__ A T[kScGroups][kScRows + 1];
+    const int m = gp->m;
+    const uint64_t groups = gp->groups;
+    const uint64_t gtiles = (groups + kScGroups - 1) / kScGroups;
+    const uint64_t i0 = (blockIdx.x / gtiles) * kScRows, g0 = (blockIdx.x % gtiles) * kScGroups;
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < kScRows * kScGroups / kBlock; ++q) {
+        const int r = (t >> 5) + (kBlock / kScGroups) * q, c = t & (kScGroups - 1);
+        if (g0 + c < groups) T[c][r] = Y[(i0 + r) * groups + g0 + c];
+    }
+    __syncthreads();
+    const uint64_t hi = gemm_deposit(i0, gp, 6, m);
+#pragma unroll
+    for (int q = 0; q < kScRows * kScGroups / kBlock; ++q) {
+        const int r = t & (kScRows - 1), c = (t >> 6) + (kBlock / kScRows) * q;
+        if (g0 + c < groups) {
+            const A v = T[c][r];
+            A* dst = s + (gemm_base(g0 + c, gp) | hi | static_cast<uint64_t>(r));
+            __builtin_nontemporal_store(v.x, &dst->x);
+            __builtin_nontemporal_store(v.y, &dst->y);
+        }
     }
 }
 
@@ -390,6 +427,15 @@ void launch_gemm(const A* in, const A* P, A* Y, const GemmGeom& geo, const GemmG
 template <typename A>
 void launch_scatter(A* s, const A* Y, const GemmGeom& geo, const GemmGeom* dgeo) {
     auto& C = ctx();
+    bool low6 = geo.m >= 6;
+    for (int b = 0; b < 6 && low6; ++b) low6 = geo.tpos[b] == b;
+    if (low6) {
+        const uint64_t blocks = (uint64_t{1} << geo.m) / kScRows * ((geo.groups + kScGroups - 1) / kScGroups);
+        hipLaunchKernelGGL(gemm_scatter_rows_kernel<A>, dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0,
+                           C.stream, s, Y, dgeo);
+        QHIP_CHECK(hipGetLastError());
+        return;
+    }
     const uint64_t total = geo.groups << geo.m;
     const int blocks = static_cast<int>(std::min<uint64_t>((total + kBlock - 1) / kBlock,
                                                            static_cast<uint64_t>(C.n_cu) * 32));
