@@ -22,6 +22,7 @@
 #include "common.h"
 
 #include <algorithm>
+#include <bit>
 #include <cstring>
 #include <memory>
 #include <type_traits>
@@ -34,6 +35,7 @@ constexpr int kT = 256;
 constexpr int kPer = 16;                 // ket amplitudes per thread
 constexpr uint64_t kChunk = kT * kPer;   // per workgroup
 constexpr int kRShift = 8;               // a thread's amplitudes differ in bits 8..11
+constexpr int kChunkBits = 12;           // log2(kChunk)
 static_assert(kT == 1 << kRShift && kPer == 16, "r occupies index bits 8..11");
 
 struct GroupDev {
@@ -43,7 +45,7 @@ struct GroupDev {
 };
 
 struct SubDev {
-    int32_t rpat;                        // z bits 8..11 shared by the terms
+    int32_t rpat;                        // z bits 8..11 shared by the terms; bits 16..31: sign mask
     int32_t t_begin, t_end;
     // Its terms run real coefficients first (n_real of them), then purely
     // imaginary ones (n_imag), then the rest; both counts are 0 when either
@@ -107,7 +109,13 @@ static Grouped group_terms(const qhip_observable* obs, int n, const char* who) {
         std::swap(xs[0], xs[it->second]);
         std::swap(per[0], per[it->second]);
     }
-    for (size_t g = 0; g < per.size(); ++g) {
+    // Then by the flip mask above bit kChunkBits (stable): H|psi> stages one
+    // partner chunk per run of equal high parts (x = 0 stays first).
+    std::vector<size_t> order(per.size());
+    for (size_t g = 0; g < order.size(); ++g) order[g] = g;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t a, size_t b) { return (xs[a] >> kChunkBits) < (xs[b] >> kChunkBits); });
+    for (size_t g : order) {
         GroupDev d;
         d.x = xs[g];
         d.s_begin = static_cast<int32_t>(out.subs.size());
@@ -139,6 +147,13 @@ static Grouped group_terms(const qhip_observable* obs, int n, const char* who) {
             out.subs.back().t_end = static_cast<int32_t>(out.z.size());
         }
         if (!terms.empty()) close_sub();
+        // Bits 16..31 of rpat: bit r = (-1)^popcount(r & pattern) is -1 (the
+        // sign of the pattern's terms at the r-th output of a thread).
+        for (int32_t si = d.s_begin; si < static_cast<int32_t>(out.subs.size()); ++si) {
+            int32_t mask = 0;
+            for (int r = 0; r < 16; ++r) mask |= (std::popcount(static_cast<unsigned>(r & out.subs[si].rpat)) & 1) << r;
+            out.subs[si].rpat |= mask << 16;
+        }
         d.s_end = static_cast<int32_t>(out.subs.size());
         d.t_begin = out.subs[d.s_begin].t_begin;
         d.t_end = out.subs.back().t_end;
@@ -410,6 +425,115 @@ __global__ __launch_bounds__(kT) void observable_kernel(const A* __restrict__ in
     }
 }
 
+// H|psi> for states of at least one chunk (2^12 amplitudes): workgroup c
+// owns the outputs of chunk c, thread t those at j_r = c 2^12 + t + 256 r
+// (r < 16, so the outputs differ in bits 8..11).  The partners of a group
+// with flip mask x are chunk c ^ (x >> 12) with the local index XOR-ed by
+// x's low bits, and the host orders groups by x >> 12: each run of equal
+// high parts stages its partner chunk in LDS once (coalesced loads, widened
+// to FP64), and every group of the run reads its partners there — the
+// gathered loads of observable_kernel, which the texture-address path bound,
+// become one chunk load per run.  Weights: per term one parity per thread at
+// j_0 ^ x; the output r's sign of a term with pattern p (its z bits 8..11) is
+// (-1)^popcount(r & p), read from the pattern's mask (SubDev::rpat bits
+// 16..31); per pattern A = sum of its terms' signed coefficients, then
+// acc_r += (+-A) psi[partner_r] — the group weight is summed per pattern
+// first.
+// acc[r] += (-1)^popcount(r & P) a psi[partner of output r] for a pattern P
+// known at compile time (so every sign is a modifier of the multiply-add);
+// vb: the byte offset in the staged chunk of output 0's partner, output r's
+// partner at vb ^ (r << 12).
+template <int P, bool REAL>
+__device__ __forceinline__ void accumulate_pattern(double2 (&acc)[kPer], const char* S, uint32_t vb, double2 a) {
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+        const double2 v = *reinterpret_cast<const double2*>(S + (vb ^ (static_cast<uint32_t>(r) << kChunkBits)));
+        const bool neg = (__builtin_popcount(static_cast<unsigned>(r & P)) & 1) != 0;
+        const double ax = neg ? -a.x : a.x;
+        acc[r].x = __fma_rn(ax, v.x, acc[r].x);
+        acc[r].y = __fma_rn(ax, v.y, acc[r].y);
+        if (!REAL) {
+            const double ay = neg ? -a.y : a.y;
+            acc[r].x = __fma_rn(-ay, v.y, acc[r].x);
+            acc[r].y = __fma_rn(ay, v.x, acc[r].y);
+        }
+    }
+}
+
+// Sum over a pattern's terms of c_t (-1)^popcount(src & z_t) (the pattern
+// sum of the expectation kernel, with I-wide indices and masks).
+template <typename I>
+__device__ __forceinline__ double2 pattern_weight(I src, int t0, int t1, const uint64_t* __restrict__ z,
+                                                  const double2* __restrict__ c) {
+    double ax = 0.0, ay = 0.0;
+    for (int t = t0; t < t1; ++t) {
+        const uint32_t p = sizeof(I) == 4 ? __popc(static_cast<uint32_t>(src) & static_cast<uint32_t>(z[t]))
+                                          : __popcll(static_cast<uint64_t>(src) & z[t]);
+        const double sg = __hiloint2double(static_cast<int>((p << 31) | 0x3ff00000u), 0);
+        ax = __fma_rn(sg, c[t].x, ax);
+        ay = __fma_rn(sg, c[t].y, ay);
+    }
+    return make_double2(ax, ay);
+}
+
+template <typename A, typename I>
+__global__ __launch_bounds__(kT) void observable_chunk_kernel(const A* __restrict__ in, A* __restrict__ out,
+                                                              const GroupDev* __restrict__ groups, int n_groups,
+                                                              const SubDev* __restrict__ subs,
+                                                              const uint64_t* __restrict__ z,
+                                                              const double2* __restrict__ c) {
+    __shared__ double2 S[kChunk];
+    const char* Sb = reinterpret_cast<const char*>(S);
+    const uint64_t chunk = blockIdx.x;
+    const uint32_t t = threadIdx.x;
+    const uint64_t j0 = (chunk << kChunkBits) | t;
+    double2 acc[kPer];
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) acc[r] = make_double2(0.0, 0.0);
+    uint64_t staged = ~uint64_t{0};
+    for (int g = 0; g < n_groups; ++g) {
+        const GroupDev G = groups[g];
+        const uint64_t xh = G.x >> kChunkBits;
+        if (xh != staged) {   // wave-uniform: every thread sees the same group table
+            __syncthreads();  // the previous run's reads are done
+            const A* src = in + ((chunk ^ xh) << kChunkBits);
+#pragma unroll
+            for (int r = 0; r < kPer; ++r) S[t + kT * r] = load_amp(src, t + kT * r);
+            __syncthreads();
+            staged = xh;
+        }
+        const uint32_t xlo = static_cast<uint32_t>(G.x) & (kChunk - 1);
+        const uint32_t vb = (t << 4) ^ (xlo << 4);   // output 0's partner: local index t ^ xlo
+        const I src = static_cast<I>(j0 ^ G.x);
+        for (int si = G.s_begin; si < G.s_end; ++si) {
+            // One 16-byte scalar load: rpat (pattern and sign mask), the
+            // term range and the kind counts.
+            const int4 w = reinterpret_cast<const int4*>(subs)[si];
+            const double2 a = pattern_weight<I>(src, w.y, w.z, z, c);
+            const bool real = w.y + (w.w & 0xffff) == w.z;
+            const int pat = w.x & (kPer - 1);
+#define QHIP_PAT(P)                                                                    \
+    case P:                                                                            \
+        if (real) accumulate_pattern<P, true>(acc, Sb, vb, a);                         \
+        else accumulate_pattern<P, false>(acc, Sb, vb, a);                             \
+        break;
+            switch (pat) {
+                QHIP_PAT(0) QHIP_PAT(1) QHIP_PAT(2) QHIP_PAT(3) QHIP_PAT(4) QHIP_PAT(5) QHIP_PAT(6) QHIP_PAT(7)
+                QHIP_PAT(8) QHIP_PAT(9) QHIP_PAT(10) QHIP_PAT(11) QHIP_PAT(12) QHIP_PAT(13) QHIP_PAT(14)
+                QHIP_PAT(15)
+            }
+#undef QHIP_PAT
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+        A o;
+        o.x = acc[r].x;
+        o.y = acc[r].y;
+        out[j0 + kT * r] = o;
+    }
+}
+
 template <typename A>
 __global__ __launch_bounds__(kT) void inner_kernel(const A* __restrict__ a, const A* __restrict__ b, uint64_t dim,
                                                    double2* __restrict__ partials) {
@@ -555,6 +679,27 @@ static void launch_observable(const qhip_state* in, qhip_state* out, const Devic
                            static_cast<const A*>(in->d), static_cast<A*>(out->d), dim, dg.g, dg.n_groups, dg.s, dg.z,
                            dg.c);
     };
+    if (dim >= kChunk) {
+        if (dim / kChunk > 0x7fffffffu) fail(QHIP_CAPABILITY, "apply_observable: state too large for one grid");
+        const dim3 grid(static_cast<uint32_t>(dim / kChunk));
+        auto chunked = [&](auto amp_tag, auto index_tag) {
+            using A = decltype(amp_tag);
+            using I = decltype(index_tag);
+            hipLaunchKernelGGL((observable_chunk_kernel<A, I>), grid, dim3(kT), 0, C.stream,
+                               static_cast<const A*>(in->d), static_cast<A*>(out->d), dg.g, dg.n_groups, dg.s, dg.z,
+                               dg.c);
+        };
+        const bool small32 = dim <= (uint64_t{1} << 32);
+        if (in->precision == QHIP_C128) {
+            if (small32) chunked(double2{}, uint32_t{});
+            else chunked(double2{}, uint64_t{});
+        } else {
+            if (small32) chunked(float2{}, uint32_t{});
+            else chunked(float2{}, uint64_t{});
+        }
+        QHIP_CHECK(hipGetLastError());
+        return;
+    }
     const bool small = dim <= (uint64_t{1} << 32);
     if (in->precision == QHIP_C128) {
         if (small) go(double2{}, uint32_t{});
