@@ -138,7 +138,7 @@ def pack_stream(commands: Sequence[Command], values: Optional[Mapping[str, float
     for i, cmd in enumerate(commands):
         g = cmd.gate
         qs = cmd.qubits
-        gate[i] = g.value
+        gate[i] = int(g)   # IntEnum: int() is cheaper than the .value descriptor
         nq[i] = len(qs)
         qoff[i] = len(qubits)
         qubits.extend(qs)
@@ -212,35 +212,37 @@ class PackedObservable:
 def pack_observable(observable: Observable) -> PackedObservable:
     """The observable ABI ``[([(qubit, 'X'|'Y'|'Z'|'I'), ...], coeff), ...]``
     (simulator/pauli_expectation.h:15-16)."""
-    terms = np.zeros(len(observable), dtype=TERM_DTYPE)
-    qs: list[int] = []
-    ls: list[int] = []
-    offs: list[int] = []
-    lens: list[int] = []
-    cs: list[complex] = []
-    for pauli, coeff in observable:
-        cs.append(complex(coeff))
-        offs.append(len(qs))
-        lens.append(len(pauli))
-        for q, letter in pauli:
-            q = int(q)
-            if q < 0:
-                raise ValueError(f"pauli_transition: observable qubit {q} out of range")
-            qs.append(q)
-            s = str(letter)
-            if len(s) != 1:
-                raise ValueError(f"pauli_transition: invalid Pauli '{s}'")
-            ls.append(ord(s))
-    if cs:   # whole columns at once (element-wise structured writes are slow)
-        c = np.asarray(cs, dtype=np.complex128)
+    n = len(observable)
+    terms = np.zeros(n, dtype=TERM_DTYPE)
+    # Flattened with comprehensions and checked with numpy: a per-letter
+    # Python loop costs about a millisecond per thousand strings.
+    lens = [len(pauli) for pauli, _ in observable]
+    qs = [q for pauli, _ in observable for q, _ in pauli]
+    letters = [str(letter) for pauli, _ in observable for _, letter in pauli]
+    q = np.asarray(qs, dtype=np.int64) if qs else np.zeros(0, np.int64)
+    if q.size and q.min() < 0:
+        raise ValueError(f"pauli_transition: observable qubit {int(q.min())} out of range")
+    joined = "".join(letters)
+    if len(joined) != len(letters):
+        bad = next(s for s in letters if len(s) != 1)
+        raise ValueError(f"pauli_transition: invalid Pauli '{bad}'")
+    try:
+        lt = np.frombuffer(joined.encode("ascii"), dtype=np.uint8).copy()
+    except UnicodeEncodeError:
+        bad = next(s for s in letters if not s.isascii())
+        raise ValueError(f"pauli_transition: invalid Pauli '{bad}'") from None
+    if n:   # whole columns at once (element-wise structured writes are slow)
+        c = np.asarray([complex(coeff) for _, coeff in observable], dtype=np.complex128)
+        offs = np.zeros(n, dtype=np.int64)
+        if n > 1:
+            np.cumsum(lens[:-1], out=offs[1:])
         terms["offset"] = offs
         terms["n_letters"] = lens
         terms["coeff_re"] = c.real
         terms["coeff_im"] = c.imag
-    q = np.asarray(qs, dtype=np.uint32)
-    lt = np.asarray(ls, dtype=np.uint8)
-    st = QhipObservable(_ptr(terms), len(observable), _ptr(q), _ptr(lt))
-    return PackedObservable(terms, q, lt, st)
+    q32 = q.astype(np.uint32)
+    st = QhipObservable(_ptr(terms), n, _ptr(q32), _ptr(lt))
+    return PackedObservable(terms, q32, lt, st)
 
 
 def config_struct(fusion_max_qubits: int, fusion_min_qubits: int, tile_qubits: int = 0,

@@ -113,13 +113,33 @@ __device__ __forceinline__ A cmul(A a, A b) {
     return cmad(z, a, b);
 }
 
-__device__ __forceinline__ double2 wave_sum(double2 v) {
-    for (int off = 32; off > 0; off >>= 1) {
-        v.x += __shfl_xor(v.x, off);
-        v.y += __shfl_xor(v.y, off);
-    }
-    return v;
+// A double moved across lanes by a DPP control (both halves).
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
 }
+
+__device__ __forceinline__ double wave_sum1(double v) {
+    v += dpp_move<0xb1>(v);    // quad_perm [1,0,3,2]: lane ^ 1
+    v += dpp_move<0x4e>(v);    // quad_perm [2,3,0,1]: lane ^ 2 (every lane: its quad's sum)
+    v += dpp_move<0x141>(v);   // row_half_mirror: the other quad of the half-row
+    v += dpp_move<0x140>(v);   // row_mirror: the other half of the row (every lane: its row's sum)
+    const double r1 = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 16),
+                                       __builtin_amdgcn_readlane(__double2loint(v), 16));
+    const double r2 = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 32),
+                                       __builtin_amdgcn_readlane(__double2loint(v), 32));
+    const double r3 = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 48),
+                                       __builtin_amdgcn_readlane(__double2loint(v), 48));
+    return ((v + r1) + r2) + r3;
+}
+
+// Sum over the 64 lanes of a wave, valid in lanes 0..15 (every lane must be
+// active).  Lane moves within rows of 16 by DPP and three lane reads across
+// rows: no LDS instructions (a __shfl_xor chain is six dependent
+// ds_bpermute round trips per component).
+__device__ __forceinline__ double2 wave_sum(double2 v) { return make_double2(wave_sum1(v.x), wave_sum1(v.y)); }
 
 // One DevOp as read through the scalar cache: eight 8-B loads from a
 // wave-uniform address, byte fields unpacked with shifts (sub-dword loads

@@ -188,6 +188,37 @@ class QarpSimulator:
             st.upload(_check_initial_state(initial_state, st.dim, where))
         return st
 
+    # States of at most _POOL_BYTES that a call uses and drops (energies,
+    # gradients) are kept for the next call of the same width instead of a
+    # device allocation and free per call; at most two per width.  A state is
+    # taken out of the pool while a call uses it, so concurrent calls never
+    # share one.
+    _POOL_BYTES = 1 << 30
+
+    def _borrow_state(self, n_qubits: int, initial_state, where: str) -> DeviceState:
+        pool = self._pool.get((int(n_qubits), self.precision))
+        st = None
+        if pool:
+            try:
+                st = pool.pop()
+            except IndexError:
+                st = None
+        if st is None:
+            return self.new_state(n_qubits, initial_state, where)
+        if initial_state is not None:
+            st.upload(_check_initial_state(initial_state, st.dim, where))
+        else:
+            st.reset()
+        return st
+
+    def _return_state(self, st: DeviceState) -> None:
+        amp = 16 if self.precision == "complex128" else 8
+        if st.dim * amp > self._POOL_BYTES:
+            return
+        pool = self._pool.setdefault((st.n_qubits, self.precision), [])
+        if len(pool) < 2:
+            pool.append(st)
+
     def apply(self, commands: Sequence[Command], state: DeviceState, *, terminal: bool = False,
               packed: Optional[PackedStream] = None) -> dict:
         """Run the command stream on a device state in place (the gate loop of
@@ -431,12 +462,16 @@ class QarpSimulator:
         index = {name: i for i, name in enumerate(param_order)}
         ps = pack_stream(commands, values=params, param_index=index)
         po = pack_observable(observable)
-        st = state if state is not None else self.new_state(n_qubits, initial_state, "run_gradient")
-        grad = np.zeros(len(param_order))
-        energy = C.c_double()
-        cfg = self._config()
-        _lib.check(_lib.load().qhip_run_gradient(C.byref(cfg), ps.ref, len(param_order), st.handle, po.ref,
-                                                 grad.ctypes.data if grad.size else None, C.byref(energy)))
+        st = state if state is not None else self._borrow_state(n_qubits, initial_state, "run_gradient")
+        try:
+            grad = np.zeros(len(param_order))
+            energy = C.c_double()
+            cfg = self._config()
+            _lib.check(_lib.load().qhip_run_gradient(C.byref(cfg), ps.ref, len(param_order), st.handle, po.ref,
+                                                     grad.ctypes.data if grad.size else None, C.byref(energy)))
+        finally:
+            if state is None:
+                self._return_state(st)
         return grad, energy.value
 
     def run_gradient_phi(self, commands: Sequence[Command], n_qubits: int, phi, params: Mapping[str, float],
