@@ -363,6 +363,7 @@ void obs_apply_async(const ObsArena* a, int i, const qhip_state* in, qhip_state*
 int obs_arena_groups(const ObsArena* a, int i);   // flip-mask groups of observable i
 int inner_blocks(const qhip_state* a);
 void inner_async(const qhip_state* a, const qhip_state* b, double2* scratch, double2* d_out);
+int inner_blocks(const qhip_state* a);   // partials inner_async needs in scratch
 
 void sample(const qhip_state* s, const double* uniforms, int64_t n, uint64_t* out);
 void probabilities(const qhip_state* s, double* out);

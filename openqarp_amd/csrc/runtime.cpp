@@ -265,18 +265,25 @@ void run_gradient(const qhip_config* cfg, const qhip_stream* stream, int n_param
                                   ctx().stream));
     }
     const Plan plan = schedule(back, n, psi->precision, options(cfg, true));
-    if (energy) *energy = obs ? inner(psi, &phi.st).real() : 0.0;
+    // One device buffer: the inner-product totals, the energy <psi|phi>, and
+    // the energy's partials.  The energy is reduced on the stream before the
+    // reverse sweep changes psi and phi, and read back with the totals, so
+    // the host does not wait between the two.
     auto& C = ctx();
-    std::vector<cplx> ip(slots.size());
-    if (!slots.empty()) {
-        double2* d_tot = static_cast<double2*>(C.misc2.ensure(sizeof(double2) * slots.size()));
-        launch_plan(plan, psi, &phi.st, d_tot, static_cast<int>(slots.size()));
-        QHIP_CHECK(hipMemcpyAsync(ip.data(), d_tot, sizeof(double2) * slots.size(), hipMemcpyDeviceToHost,
-                                  C.stream));
+    const size_t ns = slots.size();
+    const bool want_energy = energy && obs;
+    const int ib = want_energy ? inner_blocks(psi) : 0;
+    double2* d_tot = static_cast<double2*>(C.misc2.ensure(sizeof(double2) * (ns + 2 + static_cast<size_t>(ib))));
+    if (want_energy) inner_async(psi, &phi.st, d_tot + ns + 1, d_tot + ns);
+    launch_plan(plan, psi, &phi.st, ns ? d_tot : nullptr, static_cast<int>(ns));
+    std::vector<double2> h(ns + 1);
+    if (ns || want_energy) {
+        QHIP_CHECK(hipMemcpyAsync(h.data(), d_tot, sizeof(double2) * (ns + 1), hipMemcpyDeviceToHost, C.stream));
         QHIP_CHECK(hipStreamSynchronize(C.stream));
-    } else {
-        launch_plan(plan, psi, &phi.st, nullptr, 0);
     }
+    if (energy) *energy = want_energy ? h[ns].x : 0.0;
+    std::vector<cplx> ip(ns);
+    for (size_t k = 0; k < ns; ++k) ip[k] = cplx(h[k].x, h[k].y);
     accumulate_gradient(stream, slots, ip.data(), n_params, grad);
     // psi is back at the root state after the reverse sweep; leave U|root>
     // as documented: re-run the forward sweep only if the caller needs it —
