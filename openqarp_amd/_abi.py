@@ -12,7 +12,7 @@ from typing import Mapping, Optional, Sequence
 
 import numpy as np
 
-from .ir import Command, GateType, Param, affine_coefficients
+from .ir import Command, GateType, affine_coefficients
 from .errors import CapabilityError
 
 QHIP_OK, QHIP_INVALID, QHIP_RUNTIME, QHIP_CAPABILITY, QHIP_DEVICE, QHIP_NOMEM = range(6)
@@ -84,38 +84,6 @@ class PackedStream:
         return C.byref(self.struct)
 
 
-def _param_values(cmd: Command, values: Optional[Mapping[str, float]]) -> list[float]:
-    out = []
-    for p in cmd.params:
-        if p.is_symbolic():
-            if values is None:
-                raise RuntimeError(f"gate '{cmd.gate.name}' has unresolved symbolic parameter")
-            out.append(p.evaluate(values))
-        else:
-            out.append(p.value())
-    return out
-
-
-def _param_values_fast(cmd: Command, values: Optional[Mapping[str, float]]) -> list[float]:
-    """_param_values with the affine case inlined (same arithmetic as
-    Param.evaluate: offset + sum of coefficient x value)."""
-    out = []
-    for p in cmd.params:
-        if p._fn is not None:
-            return _param_values(cmd, values)
-        t = p._terms
-        if not t:
-            out.append(p._offset)
-            continue
-        if values is None:
-            raise RuntimeError(f"gate '{cmd.gate.name}' has unresolved symbolic parameter")
-        try:
-            out.append(p._offset + sum(c * float(values[s]) for s, c in t.items()))
-        except KeyError:
-            return _param_values(cmd, values)   # raises with the unbound symbols named
-    return out
-
-
 def pack_stream(commands: Sequence[Command], values: Optional[Mapping[str, float]] = None,
                 param_index: Optional[Mapping[str, int]] = None) -> PackedStream:
     """Flatten commands into qhip_op records.
@@ -135,6 +103,7 @@ def pack_stream(commands: Sequence[Command], values: Optional[Mapping[str, float
     dep_idx: list[int] = []
     dep_alpha: list[float] = []
     custom = GateType.Custom
+    slot = {s: int(j) for s, j in param_index.items()} if param_index is not None else None
     for i, cmd in enumerate(commands):
         g = cmd.gate
         qs = cmd.qubits
@@ -143,12 +112,56 @@ def pack_stream(commands: Sequence[Command], values: Optional[Mapping[str, float
         qoff[i] = len(qubits)
         qubits.extend(qs)
         ps = cmd.params
+        doff[i] = len(dep_idx)
         if ps:
-            vals = _param_values_fast(cmd, values)
-            if len(vals) > 4:
-                raise ValueError(f"gate '{g.name}' carries {len(vals)} parameters")
+            if len(ps) > 4:
+                raise ValueError(f"gate '{g.name}' carries {len(ps)} parameters")
+            vals = []
+            nd = 0
+            for p in ps:
+                t = p._terms
+                if p._fn is None:
+                    if not t:
+                        vals.append(p._offset)
+                        continue
+                    if values is None:
+                        raise RuntimeError(f"gate '{g.name}' has unresolved symbolic parameter")
+                    # Affine: Param.evaluate's arithmetic (the offset plus
+                    # the sum of coefficient x value, summed from 0 in term
+                    # order) in the same walk that reads the terms as the
+                    # gradient coefficients.
+                    acc = 0
+                    try:
+                        for s, c in t.items():
+                            acc += c * float(values[s])
+                            if slot is not None:
+                                j = slot.get(s)
+                                if j is not None:
+                                    dep_idx.append(j)
+                                    dep_alpha.append(float(c))
+                                    nd += 1
+                    except KeyError:
+                        p.evaluate(values)   # raises naming the unbound symbols
+                    vals.append(p._offset + acc)
+                    continue
+                # Compound: evaluated by its closure, coefficients probed.
+                if values is None:
+                    raise RuntimeError(f"gate '{g.name}' has unresolved symbolic parameter")
+                vals.append(p.evaluate(values))
+                if slot is not None:
+                    try:
+                        coeffs = affine_coefficients(p)
+                    except CapabilityError as e:
+                        raise CapabilityError(f"run_gradient: {e}") from None
+                    for name, alpha in coeffs.items():
+                        j = slot.get(name)
+                        if j is not None:
+                            dep_idx.append(j)
+                            dep_alpha.append(float(alpha))
+                            nd += 1
             npar[i] = len(vals)
             params[i] = tuple(vals) + (0.0,) * (4 - len(vals))
+            ndep[i] = nd
         if cmd.condition_bits:
             flags[i] = QHIP_OP_CONDITIONAL
         cbit[i] = cmd.cbits[0] if cmd.cbits else -1
@@ -158,23 +171,6 @@ def pack_stream(commands: Sequence[Command], values: Optional[Mapping[str, float
             u_off += cmd.unitary.size
         else:
             uoff[i] = -1
-        doff[i] = len(dep_idx)
-        if param_index is not None and ps:
-            nd = 0
-            for p in ps:
-                if p._fn is None:
-                    coeffs = p._terms   # affine: the coefficients are the terms
-                else:
-                    try:
-                        coeffs = affine_coefficients(p)
-                    except CapabilityError as e:
-                        raise CapabilityError(f"run_gradient: {e}") from None
-                for name, alpha in coeffs.items():
-                    if name in param_index:
-                        dep_idx.append(int(param_index[name]))
-                        dep_alpha.append(float(alpha))
-                        nd += 1
-            ndep[i] = nd
     if n:
         ops["gate"] = gate
         ops["n_qubits"] = nq
