@@ -250,18 +250,20 @@ def batched(n_instances=512):
     return out
 
 
-def qaoa(n=34, p=4, seed=5, shots=100_000, n_adj=33, n_cmp=32, fusion=2):
+def qaoa(n=34, p=4, seed=5, shots=100_000, n_adj=33, n_cmp=32, fusion=1, low=3):
     """QAOA MaxCut on a seeded random 3-regular graph: <C> and 1e5 shots on a
     34-qubit complex64 state (128 GiB), the adjoint gradient at 33 qubits in
-    complex64, and complex64 vs complex128 at 32 qubits.  The apply fuses up to
-    2 qubits (the fastest width for this circuit at 34 qubits: 17 passes, 1.9 s
-    against 3.9 s at width 4 and 5.3 s at width 3)."""
+    complex64, and complex64 vs complex128 at 32 qubits.  The apply fuses
+    single-qubit runs only, so each pass's share of a cost layer (RZZ gates,
+    consecutive) is one merged phase op, with 3 contiguous low qubits: 12
+    passes, 1.69 s, against 1.89 s at fusion width 2 (17 passes of dense
+    2-qubit ops), 3.9 s at width 4 and 5.3 s at width 3."""
     import gc
     from openqarp_amd import workloads as W
     from openqarp_amd.simulator import DeviceState, QarpSimulator
     from openqarp_amd import _lib
     from openqarp_amd.simulator import CompiledCircuit
-    sim = QarpSimulator(precision="complex64")
+    sim = QarpSimulator(precision="complex64", low_qubits=low)
     sim.fusion_max_qubits = fusion
     w = W.qaoa_maxcut(n, p, seed)
     st = DeviceState(n, "complex64")

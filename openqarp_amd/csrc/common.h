@@ -106,6 +106,17 @@ enum OpKind : int32_t {
     // a generator on block bits (k: pattern as for OP_SUB, coef: its matrix,
     // lmask: accumulator slot); two-state passes only.
     OP_IPSUB = 11,
+    // A run of diagonal phase ops merged into one (host and device): the
+    // amplitude at index j is multiplied by exp(i theta(j)) with
+    // theta(j) = c + sum_q h_q z_q + sum_{a<b} J_ab z_a z_b, z_q = 1 - 2 j_q
+    // (HostOp::angle0 = c, HostOp::terms = the h and J terms).  On the
+    // device coef points at exp(i c), then two records per term (exp(i w),
+    // {operand code, 0}): the terms that touch no tile position >= 8, then
+    // per row bit m = position 8 + m the terms with one operand there, then
+    // (when lmask bits 16..31 are 1) the product of the terms with both
+    // operands on row bits, one entry per row code; lmask / gmask hold the
+    // list lengths (16 bits each: none-row, pair table, rows 0..5).
+    OP_PHASE = 12,
 };
 
 // Device op kinds that accumulate an inner product.
@@ -120,8 +131,15 @@ constexpr int kMaxBlockIp = 8;
 // on one bit / a bit pair, sub-tables for operands outside the tile), 12 (no
 // block operand): kReal2 + B a real 2x2 matrix, kXGen / kYGen / kZGen + B the
 // Pauli generator on bit B (inner products only), kNegPair + p the sign flip
-// where both bits of pair p (01, 02, 12) are set.
-constexpr int kReal2 = 13, kXGen = 16, kYGen = 19, kZGen = 22, kNegPair = 25;
+// where both bits of pair p (01, 02, 12) are set, kImag2 + B a 2x2 matrix
+// with a real diagonal and an imaginary off-diagonal (Rx, Y).
+constexpr int kReal2 = 13, kXGen = 16, kYGen = 19, kZGen = 22, kNegPair = 25, kImag2 = 28;
+
+// One term of a merged phase op: weight w times z_a (b < 0) or z_a z_b.
+struct PhaseTerm {
+    int32_t a = -1, b = -1;
+    double w = 0.0;
+};
 
 struct HostOp {
     int32_t kind = OP_DENSE;
@@ -131,8 +149,10 @@ struct HostOp {
     uint64_t mask = 0;               // maskphase operands
     cplx phase{1.0, 0.0};            // maskphase / scalar
     int32_t slot = -1;               // IP ops: accumulator slot
+    double angle0 = 0.0;             // OP_PHASE: constant angle c
+    std::vector<PhaseTerm> terms;    // OP_PHASE: h (b < 0) and J terms
     uint64_t qmask() const {
-        if (kind == OP_MASKPHASE) return mask;
+        if (kind == OP_MASKPHASE || kind == OP_PHASE) return mask;
         uint64_t r = 0;
         for (int b = 0; b < k; ++b) r |= uint64_t{1} << q[b];
         return r;

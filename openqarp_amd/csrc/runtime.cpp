@@ -3,6 +3,7 @@
 #include "common.h"
 
 #include <algorithm>
+#include <bit>
 #include <cmath>
 #include <memory>
 #include <mutex>
@@ -516,7 +517,8 @@ int qhip_schedule_inspect(const qhip_config* cfg, const qhip_stream* in, int32_t
                     case qhip::OP_SUB:   // pos[] holds the member op's own operand positions
                     case qhip::OP_DENSE: {
                         const bool sub = d.kind == qhip::OP_SUB;
-                        const bool real2 = sub && d.k >= qhip::kReal2 && d.k < qhip::kReal2 + 3;
+                        const bool real2 = sub && ((d.k >= qhip::kReal2 && d.k < qhip::kReal2 + 3) ||
+                                                   (d.k >= qhip::kImag2 && d.k < qhip::kImag2 + 3));
                         const bool neg = sub && d.k >= qhip::kNegPair && d.k < qhip::kNegPair + 3;
                         const bool diag = sub && ((d.k >= 6 && d.k <= 12) || neg);
                         if (diag) {   // local operands (ascending), then the ones outside the tile
@@ -573,6 +575,59 @@ int qhip_schedule_inspect(const qhip_config* cfg, const qhip_stream* in, int32_t
                         it.gate = QHIP_GPHASE;
                         it.phase = plan.coefs[d.coef];
                         break;
+                    case qhip::OP_PHASE: {
+                        // One GPhase for the constant, then each term as the
+                        // diagonal exp(i w z_a z_b) (or exp(i w z_a)).
+                        auto push = [&](qhip::FusedItem&& x) {
+                            f->items.push_back(std::move(x));
+                            f->src_ops.push_back(qhip_op{});
+                            f->src_qubits.emplace_back();
+                            f->src_mats.emplace_back();
+                        };
+                        const qhip::cplx* rec = plan.coefs.data() + d.coef;
+                        it.gate = QHIP_GPHASE;
+                        it.phase = rec[0];
+                        push(std::move(it));
+                        auto key_qubit = [&](int kb) { return kb >= 56 ? static_cast<int>(global(kb - 56)) : kb == 55 ? -1 : kb; };
+                        // A term: factor e where its operand bits agree, conj(e) where they differ.
+                        auto term = [&](int qa, int qb, qhip::cplx e) {
+                            qhip::FusedItem t;
+                            t.pass = static_cast<int32_t>(pi);
+                            if (qa < 0) std::swap(qa, qb);
+                            t.k = qb < 0 ? 1 : 2;
+                            t.q[0] = static_cast<uint32_t>(qa);
+                            if (qb >= 0) t.q[1] = static_cast<uint32_t>(qb);
+                            const size_t D = size_t{1} << t.k;
+                            t.m.assign(D * D, qhip::cplx(0.0));
+                            for (size_t y = 0; y < D; ++y) t.m[y * D + y] = std::popcount(y) & 1 ? std::conj(e) : e;
+                            push(std::move(t));
+                        };
+                        const uint64_t lm = d.lmask, gm = d.gmask;
+                        const size_t n_none = lm & 0xffffu;
+                        const bool pair_table = ((lm >> 16) & 0xffffu) != 0;
+                        size_t o = 1;
+                        for (size_t i = 0; i < n_none; ++i, o += 2) {
+                            const int code = static_cast<int>(rec[o + 1].real());
+                            term(key_qubit(code & 0xff), key_qubit(code >> 8), rec[o]);
+                        }
+                        for (int m = 0; m < 6; ++m) {
+                            const size_t cnt = (m < 2 ? lm >> (32 + 16 * m) : gm >> (16 * (m - 2))) & 0xffffu;
+                            for (size_t i = 0; i < cnt; ++i, o += 2)
+                                term(static_cast<int>(global(8 + m)), key_qubit(static_cast<int>(rec[o + 1].real())),
+                                     rec[o]);
+                        }
+                        if (pair_table) {   // a diagonal over the row bits
+                            qhip::FusedItem t;
+                            t.pass = static_cast<int32_t>(pi);
+                            t.k = P.T - 8;
+                            for (int m = 0; m < t.k; ++m) t.q[m] = global(8 + m);
+                            const size_t D = size_t{1} << t.k;
+                            t.m.assign(D * D, qhip::cplx(0.0));
+                            for (size_t y = 0; y < D; ++y) t.m[y * D + y] = rec[o + y];
+                            push(std::move(t));
+                        }
+                        continue;
+                    }
                     default:
                         qhip::fail(QHIP_RUNTIME, "qhip_schedule_inspect: unexpected op kind");
                 }

@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <bit>
+#include <map>
 
 namespace qhip {
 
@@ -33,6 +34,7 @@ static double op_flops(const HostOp& op) {
         case OP_IP_DENSE: return 8.0 * static_cast<double>(1 << op.k) + 8.0;
         case OP_DIAG: case OP_MASKPHASE: case OP_SCALAR: return 6.0;
         case OP_IP_DIAG: return 14.0;
+        case OP_PHASE: return 36.0;
         default: return 0.0;
     }
 }
@@ -217,10 +219,173 @@ static void emit_op(Plan& plan, const HostOp& op, const int* local_of, const int
             plan.coefs.push_back(op.phase);
             ++plan.diag_ops;
             break;
+        case OP_PHASE: {
+            // Terms by where their operands sit: an operand at tile position
+            // p < 8 (thread bit p) reads bit 56 + p of the thread's key, one
+            // outside the tile its global bit (n <= 55), a missing one bit
+            // 55 (always clear); an operand at position 8 + m is row bit m.
+            std::vector<cplx> none_row, rows[6], pairs;
+            auto where = [&](int q, int& row) {
+                row = -1;
+                if (q < 0) return 55;
+                const int l = local_of[q];
+                if (l < 0) return pos[q];
+                if (l >= 8) { row = l - 8; return 0; }
+                return 56 + l;
+            };
+            // Each term: its factor exp(i w), then {operand code, 0}.
+            auto put = [](std::vector<cplx>& v, double w, int code) {
+                v.push_back(std::polar(1.0, w));
+                v.emplace_back(static_cast<double>(code), 0.0);
+            };
+            for (const PhaseTerm& t : op.terms) {
+                int ra, rb;
+                const int ka = where(t.a, ra), kb = where(t.b, rb);
+                if (ra < 0 && rb < 0) put(none_row, t.w, ka | kb << 8);
+                else if (ra >= 0 && rb >= 0) put(pairs, t.w, ra | rb << 8);
+                else if (ra >= 0) put(rows[ra], t.w, kb);
+                else put(rows[rb], t.w, ka);
+            }
+            plan.coefs.push_back(std::polar(1.0, op.angle0));
+            plan.coefs.insert(plan.coefs.end(), none_row.begin(), none_row.end());
+            for (const auto& r : rows) plan.coefs.insert(plan.coefs.end(), r.begin(), r.end());
+            // Terms with both operands on row bits depend on the row code
+            // alone: one table over the 2^(T-8) row codes.
+            const int n_rows = T > 8 ? T - 8 : 0;
+            if (!pairs.empty()) {
+                for (uint32_t rr = 0; rr < (1u << n_rows); ++rr) {
+                    cplx f(1.0, 0.0);
+                    for (size_t t = 0; t < pairs.size(); t += 2) {
+                        const int code = static_cast<int>(pairs[t + 1].real());
+                        const bool odd = ((rr >> (code & 0xff)) ^ (rr >> (code >> 8))) & 1u;
+                        f *= odd ? std::conj(pairs[t]) : pairs[t];
+                    }
+                    plan.coefs.push_back(f);
+                }
+            }
+            d.lmask = none_row.size() / 2 | uint64_t{pairs.empty() ? 0u : 1u} << 16 |
+                      uint64_t{rows[0].size() / 2} << 32 | uint64_t{rows[1].size() / 2} << 48;
+            d.gmask = rows[2].size() / 2 | rows[3].size() / 2 << 16 | uint64_t{rows[4].size() / 2} << 32 |
+                      uint64_t{rows[5].size() / 2} << 48;
+            d.k = 0;
+            ++plan.diag_ops;
+            break;
+        }
     }
     if (op.kind == OP_IP_DENSE || op.kind == OP_IP_DIAG) d.lmask = static_cast<uint64_t>(op.slot);
     plan.ops.push_back(d);
     plan.flops += op_flops(op) * static_cast<double>(uint64_t{1} << n);
+}
+
+// Merged phase ops.  A diagonal op on one or two qubits, a mask phase on
+// one or two, or a scalar, all with entries of modulus one, is
+// exp(i theta) with theta a polynomial in the z_q = 1 - 2 j_q of its
+// qubits: the Walsh coefficients of its entries' angles.  A run of at least
+// kMinPhaseRun such ops, consecutive in a pass's op order, becomes one
+// OP_PHASE (one LDS round trip and a few complex products per amplitude
+// instead of one round trip per op); its terms on equal qubit sets are
+// summed.
+constexpr int kMinPhaseRun = 4;
+constexpr size_t kMaxPhaseTerms = 4096;
+
+using PhaseAcc = std::map<std::pair<int, int>, double>;
+
+static bool unit_modulus(cplx v) { return std::abs(std::abs(v) - 1.0) <= 1e-14; }
+
+static bool phase_walsh(const HostOp& op, double& c, PhaseAcc* acc) {
+    auto add = [&](int a, int b, double w) {
+        if (acc && w != 0.0) (*acc)[{a, b}] += w;
+    };
+    switch (op.kind) {
+        case OP_SCALAR:
+            if (!unit_modulus(op.phase)) return false;
+            c += std::arg(op.phase);
+            return true;
+        case OP_MASKPHASE: {
+            const int nq = std::popcount(op.mask);
+            if (nq < 1 || nq > 2 || !unit_modulus(op.phase)) return false;
+            const double f = std::arg(op.phase);   // angle where every operand bit is set
+            const int a = std::countr_zero(op.mask);
+            if (nq == 1) {
+                c += f / 2;
+                add(a, -1, -f / 2);
+                return true;
+            }
+            const int b = 63 - std::countl_zero(op.mask);
+            c += f / 4;
+            add(a, -1, -f / 4);
+            add(b, -1, -f / 4);
+            add(a, b, f / 4);
+            return true;
+        }
+        case OP_DIAG: {
+            if (op.k < 1 || op.k > 2) return false;
+            for (const cplx& e : op.m)
+                if (!unit_modulus(e)) return false;
+            const int q0 = static_cast<int>(op.q[0]);
+            if (op.k == 1) {
+                const double f0 = std::arg(op.m[0]), f1 = std::arg(op.m[1]);
+                c += (f0 + f1) / 2;
+                add(q0, -1, (f0 - f1) / 2);
+                return true;
+            }
+            const int q1 = static_cast<int>(op.q[1]);
+            // entry index = bit of q0 | bit of q1 << 1
+            const double f0 = std::arg(op.m[0]), f1 = std::arg(op.m[1]), f2 = std::arg(op.m[2]),
+                         f3 = std::arg(op.m[3]);
+            c += (f0 + f1 + f2 + f3) / 4;
+            add(q0, -1, (f0 - f1 + f2 - f3) / 4);
+            add(q1, -1, (f0 + f1 - f2 - f3) / 4);
+            add(std::min(q0, q1), std::max(q0, q1), (f0 - f1 - f2 + f3) / 4);
+            return true;
+        }
+        default:
+            return false;
+    }
+}
+
+// The pass's ops (chosen, in order) with each run of mergeable phase ops
+// replaced by one OP_PHASE; false (out untouched) when no run qualifies.
+static bool merge_phase_runs(const std::vector<HostOp>& ops, const std::vector<size_t>& chosen,
+                             std::vector<HostOp>& out) {
+    std::vector<char> ok(chosen.size());
+    double scratch = 0.0;
+    bool any = false;
+    for (size_t i = 0, run = 0; i < chosen.size(); ++i) {
+        ok[i] = phase_walsh(ops[chosen[i]], scratch, nullptr);
+        run = ok[i] ? run + 1 : 0;
+        any = any || run >= static_cast<size_t>(kMinPhaseRun);
+    }
+    if (!any) return false;
+    out.clear();
+    size_t i = 0;
+    while (i < chosen.size()) {
+        size_t j = i;
+        while (j < chosen.size() && ok[j]) ++j;
+        if (j - i < static_cast<size_t>(kMinPhaseRun)) {
+            for (size_t e = i; e < std::max(j, i + 1); ++e) out.push_back(ops[chosen[e]]);
+            i = std::max(j, i + 1);
+            continue;
+        }
+        HostOp m;
+        PhaseAcc acc;
+        auto flush = [&] {
+            m.kind = OP_PHASE;
+            for (const auto& [key, w] : acc) m.terms.push_back(PhaseTerm{key.first, key.second, w});
+            out.push_back(std::move(m));
+            m = HostOp();
+            acc.clear();
+        };
+        for (size_t e = i; e < j; ++e) {
+            if (acc.size() + 3 > kMaxPhaseTerms) flush();
+            const HostOp& op = ops[chosen[e]];
+            phase_walsh(op, m.angle0, &acc);
+            m.mask |= op.qmask();
+        }
+        flush();
+        i = j;
+    }
+    return true;
 }
 
 // Register blocks.  Dense ops on at most two qubits are gathered into blocks
@@ -369,6 +534,9 @@ static void emit_block(Plan& plan, const std::vector<HostOp>& ops, const Unit& u
                 if (d.kind == OP_SUB && M[0].imag() == 0.0 && M[1].imag() == 0.0 && M[2].imag() == 0.0 &&
                     M[3].imag() == 0.0)
                     d.k += kReal2;   // real rotation (Ry): real-times-complex products
+                else if (d.kind == OP_SUB && M[0].imag() == 0.0 && M[3].imag() == 0.0 && M[1].real() == 0.0 &&
+                         M[2].real() == 0.0)
+                    d.k += kImag2;   // Rx, Y: real diagonal, imaginary off-diagonal
                 else if (d.kind == OP_IPSUB && M[0] == I0 && M[1] == I1 && M[2] == I1 && M[3] == I0)
                     d.k += kXGen;
                 else if (d.kind == OP_IPSUB && M[0] == I0 && M[1] == -i1 && M[2] == i1 && M[3] == I0)
@@ -632,14 +800,23 @@ static void schedule_segment(Plan& plan, const std::vector<HostOp>& ops, const s
         for (int b = 0; b < n; ++b)
             if ((tile >> b) & 1) Q |= uint64_t{1} << cur.at[b];
         const bool blocks = opt.register_blocks && T >= 3;
-        for (const Unit& u : form_units(ops, chosen, blocks, opt.two_states, Q)) {
+        // Single-state passes merge runs of phase ops (the thread key of
+        // OP_PHASE holds global bits below 55).
+        std::vector<HostOp> merged;
+        std::vector<size_t> merged_sel;
+        const bool use_merged = !opt.two_states && n <= 55 && merge_phase_runs(ops, chosen, merged);
+        if (use_merged)
+            for (size_t i = 0; i < merged.size(); ++i) merged_sel.push_back(i);
+        const std::vector<HostOp>& pops = use_merged ? merged : ops;
+        const std::vector<size_t>& psel = use_merged ? merged_sel : chosen;
+        for (const Unit& u : form_units(pops, psel, blocks, opt.two_states, Q)) {
             if (u.ops.size() >= 2) {
-                emit_block(plan, ops, u, local_of, cur.pos, n, T);   // runs in every kernel variant
+                emit_block(plan, pops, u, local_of, cur.pos, n, T);   // runs in every kernel variant
             } else {
-                emit_op(plan, ops[u.ops[0]], local_of, cur.pos, n, T, opt);
+                emit_op(plan, pops[u.ops[0]], local_of, cur.pos, n, T, opt);
                 // kmax sizes the VALU dense path; matrix-core ops run in every variant.
-                if (ops[u.ops[0]].needs_local() && plan.ops.back().kind != OP_MFMA)
-                    P.kmax = std::max(P.kmax, ops[u.ops[0]].k);
+                if (pops[u.ops[0]].needs_local() && plan.ops.back().kind != OP_MFMA)
+                    P.kmax = std::max(P.kmax, pops[u.ops[0]].k);
             }
         }
         P.op_end = static_cast<int32_t>(plan.ops.size());

@@ -714,6 +714,20 @@ __device__ __forceinline__ void sub_real(A* v, const A* __restrict__ U) {
         }
 }
 
+template <typename A, int B>   // [[a, i b], [i c, d]], a..d real, on block bit B
+__device__ __forceinline__ void sub_imag(A* v, const A* __restrict__ U) {
+    const auto a = U[0].x, b = U[1].y, c = U[2].y, d = U[3].x;
+#pragma unroll
+    for (int x = 0; x < 8; ++x)
+        if (!((x >> B) & 1)) {
+            const A p = v[x], q = v[x | (1 << B)];
+            v[x].x = fma(-b, q.y, a * p.x);
+            v[x].y = fma(b, q.x, a * p.y);
+            v[x | (1 << B)].x = fma(-c, p.y, d * q.x);
+            v[x | (1 << B)].y = fma(c, p.x, d * q.y);
+        }
+}
+
 template <typename A, int B0, int B1>   // -1 where block bits B0 and B1 are both set
 __device__ __forceinline__ void sub_negate(A* v) {
 #pragma unroll
@@ -730,6 +744,9 @@ __device__ __forceinline__ void sub_apply(A* v, int pattern, const A* __restrict
         case kReal2 + 0: sub_real<A, 0>(v, U); break;
         case kReal2 + 1: sub_real<A, 1>(v, U); break;
         case kReal2 + 2: sub_real<A, 2>(v, U); break;
+        case kImag2 + 0: sub_imag<A, 0>(v, U); break;
+        case kImag2 + 1: sub_imag<A, 1>(v, U); break;
+        case kImag2 + 2: sub_imag<A, 2>(v, U); break;
         case kNegPair + 0: sub_negate<A, 0, 1>(v); break;
         case kNegPair + 1: sub_negate<A, 0, 2>(v); break;
         case kNegPair + 2: sub_negate<A, 1, 2>(v); break;
@@ -1104,6 +1121,119 @@ __device__ __forceinline__ double2 ip_dense(A* s, uint32_t tp0, uint32_t tp1, in
     return acc;
 }
 
+// e or conj(e) (FP64), by a parity bit.
+__device__ __forceinline__ double2 unit_or_conj(double2 e, uint64_t odd) {
+    return make_double2(e.x, odd ? -e.y : e.y);
+}
+__device__ __forceinline__ double2 mul_unit(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// f times, over cnt terms of a merged phase op (two records each: exp(i w),
+// then {operand code, 0}), exp(i w) or its conjugate as odd(code) says.
+// Four terms' records are loaded before their products, so the loads are in
+// flight together.
+template <typename A, typename Odd>
+__device__ __forceinline__ double2 term_product(double2 f, const A*& r, uint32_t cnt, Odd odd) {
+    uint32_t i = 0;
+    for (; i + 4 <= cnt; i += 4, r += 8) {
+        const A* u = r;
+        A e[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) e[t] = u[t];
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            f = mul_unit(f, unit_or_conj(make_double2(static_cast<double>(e[2 * t].x), static_cast<double>(e[2 * t].y)),
+                                         odd(static_cast<uint32_t>(e[2 * t + 1].x))));
+    }
+    for (; i < cnt; ++i, r += 2) {
+        const A e0 = r[0], e1 = r[1];
+        f = mul_unit(f, unit_or_conj(make_double2(static_cast<double>(e0.x), static_cast<double>(e0.y)),
+                                     odd(static_cast<uint32_t>(e1.x))));
+    }
+    return f;
+}
+
+// A merged phase op (OP_PHASE) on this thread's amplitudes l = threadIdx.x +
+// 256 r, r < R.  Each term is stored as its factor exp(i w) and its operand
+// code; it contributes exp(i w) where its operands' bits agree and the
+// conjugate where they differ.  The thread's key holds the tile's global
+// bits and, at bits 56..63, the thread bits (tile positions 0..7); bit 55 is
+// clear.  The factor of amplitude r = f0 (the constant and the terms with no
+// operand on a row bit, per thread) x, per row bit m (tile position 8 + m),
+// g_m or its conjugate (the terms with one operand there) x the row-pair
+// table's entry for row code r.  Products in FP64 in both precisions.
+template <typename A, int MAXROWS>
+__device__ __forceinline__ void phase_op(A* s, int T, uint64_t gbase, uint32_t te, uint32_t R, bool active,
+                                         const OpRegs& op, const A* __restrict__ rec) {
+    constexpr int sh = Amp<A>::kShift;
+    const uint64_t key = gbase | (static_cast<uint64_t>(threadIdx.x) << 56);
+    const A* r = rec;
+    double2 f0 = make_double2(static_cast<double>(r[0].x), static_cast<double>(r[0].y));
+    ++r;
+    f0 = term_product(f0, r, static_cast<uint32_t>(op.lmask & 0xffffu),
+                      [&](uint32_t c) { return ((key >> (c & 0xffu)) ^ (key >> (c >> 8))) & 1u; });
+    const int rows = T > 8 ? T - 8 : 0;
+    double2 g[MAXROWS > 0 ? MAXROWS : 1];
+#pragma unroll
+    for (int m = 0; m < MAXROWS; ++m) {
+        g[m] = make_double2(1.0, 0.0);
+        const uint32_t cnt = static_cast<uint32_t>(
+            (m < 2 ? op.lmask >> (32 + 16 * m) : op.gmask >> (16 * (m - 2))) & 0xffffu);
+        if (m < rows) g[m] = term_product(g[m], r, cnt, [&](uint32_t c) { return (key >> c) & 1u; });
+    }
+    const bool pair_table = ((op.lmask >> 16) & 0xffffu) != 0;   // row-pair terms: a table over row codes
+    // Rows in Gray-code order: consecutive codes differ in one row bit m, so
+    // each factor is the previous one times conj(g_m)^2 (bit m set) or
+    // g_m^2 (bit m cleared).  Four amplitudes per step: their LDS reads are
+    // issued together, then the products, then the writes.
+    double2 sq[MAXROWS > 0 ? MAXROWS : 1];
+    double2 base = f0;
+#pragma unroll
+    for (int m = 0; m < MAXROWS; ++m)
+        if (m < rows) {
+            base = mul_unit(base, g[m]);
+            sq[m] = mul_unit(g[m], g[m]);
+        }
+    if (!active) return;
+    for (uint32_t i0 = 0; i0 < R; i0 += 4) {
+        A v[4];
+        uint32_t ad[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (i0 + u < R) {
+                const uint32_t i = i0 + u, rr = i ^ (i >> 1);
+                ad[u] = te ^ (swz(rr << 8) << sh);
+                v[u] = at(s, ad[u]);
+            }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (i0 + u < R) {
+                const uint32_t i = i0 + u, rr = i ^ (i >> 1);
+                if (i > 0) {
+                    const int mb = __builtin_ctz(i);   // wave-uniform
+                    double2 q = make_double2(1.0, 0.0);
+#pragma unroll
+                    for (int m = 0; m < MAXROWS; ++m)
+                        if (m == mb) q = sq[m];
+                    base = mul_unit(base, unit_or_conj(q, (rr >> mb) & 1u));
+                }
+                double2 f = base;
+                if (pair_table) {
+                    const A e = r[rr];
+                    f = mul_unit(f, make_double2(static_cast<double>(e.x), static_cast<double>(e.y)));
+                }
+                A fa;
+                fa.x = f.x;
+                fa.y = f.y;
+                v[u] = cmul(v[u], fa);
+            }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (i0 + u < R) at(s, ad[u]) = v[u];
+    }
+}
+
 // The ops of pass P over the tile in LDS (psi at element 0; phi at element
 // 2^TB in two-state mode), a barrier after each.  te: this thread's byte
 // offset for amplitudes l = threadIdx.x + 256 r.  Returns the number of
@@ -1226,6 +1356,9 @@ __device__ __forceinline__ int run_ops(A* s, const PassDesc& P, uint64_t gbase, 
                     }
                 break;
             }
+            case OP_PHASE:   // single-state passes only
+                if constexpr (!TWO) phase_op<A, (TB > 14 ? 6 : TB > 8 ? TB - 8 : 0)>(s, P.T, gbase, te, R, active, op, U);
+                break;
             case OP_IP_DENSE:
             case OP_IP_DIAG:
             case OP_IP_DIAGG:

@@ -129,3 +129,79 @@ def test_relabeling_saves_passes_on_the_brickwork():
     assert max(p for p, _ in on) + 1 <= 56
     assert max(p for p, _ in off) + 1 == 62
     assert _swaps(off) == 0
+
+
+def phase_run_circuit(n, layers, seed, run_len=None):
+    """Layers of H / RX / RY mixers around runs of diagonal gates of every
+    kind that merges (RZZ, CZ, CP, CRZ, CS, CSdg, 2-qubit MCZ, RZ, P, S, Sdg,
+    T, Tdg, Z, GPhase, a Custom diagonal unitary)."""
+    from openqarp_amd.ir import Command, GateType
+    rng = np.random.default_rng(seed)
+    two = [GateType.RZZ, GateType.CZ, GateType.CP, GateType.CRz, GateType.CS, GateType.CSdg, GateType.MCZ]
+    one = [GateType.Rz, GateType.P, GateType.S, GateType.Sdg, GateType.T, GateType.Tdg, GateType.Z]
+    cmds = [Command(GateType.H, q) for q in range(n)]
+    for layer in range(layers):
+        for _ in range(run_len if run_len is not None else int(rng.integers(4, 3 * n + 4))):
+            r = rng.random()
+            if r < 0.6 and n >= 2:
+                g = two[rng.integers(len(two))]
+                a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+                cmds.append(Command(g, qubits=[a, b], params=list(rng.uniform(-3, 3, 1 if g in
+                                    (GateType.RZZ, GateType.CP, GateType.CRz) else 0))))
+            elif r < 0.9:
+                g = one[rng.integers(len(one))]
+                cmds.append(Command(g, qubits=[int(rng.integers(n))],
+                                    params=list(rng.uniform(-3, 3, 1 if g in (GateType.Rz, GateType.P) else 0))))
+            elif r < 0.95:
+                cmds.append(Command(GateType.GPhase, params=[float(rng.uniform(-3, 3))]))
+            else:
+                k = 1 if n < 2 else int(rng.integers(1, 3))
+                qs = [int(x) for x in rng.choice(n, k, replace=False)]
+                cmds.append(Command(GateType.Custom, qubits=qs,
+                                    unitary=np.diag(np.exp(1j * rng.uniform(-3, 3, 1 << k)))))
+        mixer = [GateType.Rx, GateType.Ry][layer % 2]
+        for q in range(n):
+            cmds.append(Command(mixer, q, float(rng.uniform(-3, 3))))
+    return cmds
+
+
+@pytest.mark.parametrize("n", [3, 6, 10, 13, 16])
+@pytest.mark.parametrize("tile,low", [(0, 0), (6, 2), (9, 3), (12, 4)])
+@pytest.mark.parametrize("precision", ["complex128", "complex64"])
+def test_phase_runs_merge(n, tile, low, precision):
+    """Runs of four or more diagonal phase ops become one merged op per pass
+    (reported as a GPhase plus one diagonal per term): the replayed schedule
+    equals the circuit, with terms on tile positions below 8, on row
+    positions 8.. and outside the tile."""
+    cmds = phase_run_circuit(n, 4, 7 * n + tile)
+    ref = O.statevector(pack_stream(cmds), n, 0, 0)
+    got, sched = scheduled_state(cmds, n, fusion_max_qubits=1, fusion_min_qubits=0, tile_qubits=tile,
+                                 low_qubits=low, precision=precision)
+    assert np.abs(got - ref).max() < 1e-12
+    from openqarp_amd.ir import GateType
+    plain = [c for c in cmds if c.gate != GateType.GPhase]
+    sched = inspect_schedule(plain, n, fusion_max_qubits=1, fusion_min_qubits=0, tile_qubits=tile, low_qubits=low,
+                             precision=precision)
+    assert any(c.gate == GateType.GPhase for _, c in sched)   # the merged ops' constants
+
+
+def test_short_phase_runs_stay_as_they_are():
+    """Runs of three diagonal ops are not merged (no GPhase appears)."""
+    from openqarp_amd.ir import GateType
+    n = 8
+    cmds = [c for c in phase_run_circuit(n, 3, 11, run_len=3) if c.gate != GateType.GPhase]
+    got, sched = scheduled_state(cmds, n, fusion_max_qubits=1, fusion_min_qubits=0)
+    assert np.abs(got - O.statevector(pack_stream(cmds), n, 0, 0)).max() < 1e-12
+    assert not any(c.gate == GateType.GPhase for _, c in sched)
+
+
+def test_qaoa_cost_layers_merge():
+    """The QAOA MaxCut circuit at fusion 1: each cost layer's 51 RZZ gates
+    (one run) merge, so the schedule holds far fewer items than gates."""
+    from openqarp_amd.ir import GateType
+    w = W.qaoa_maxcut(16, 4, 5)
+    ref = O.statevector(pack_stream(w.commands), 16, 0, 0)
+    got, sched = scheduled_state(w.commands, 16, fusion_max_qubits=1, fusion_min_qubits=0, tile_qubits=12,
+                                 low_qubits=4)
+    assert np.abs(got - ref).max() < 1e-12
+    assert sum(c.gate == GateType.GPhase for _, c in sched) >= 4
