@@ -287,7 +287,7 @@ void BatchSession::build() {
         S.first.push_back(S.total);
         S.total += b.n_tiles;
         S.tmax = std::max(S.tmax, P.T);
-        S.kmax = std::max(S.kmax, P.kmax);
+        S.kmax = std::max(S.kmax & ~kPhaseKernel, P.kmax & ~kPhaseKernel) | ((S.kmax | P.kmax) & kPhaseKernel);
         S.items.push_back(b);
     };
     for (size_t k = 0; k < max_fwd; ++k) {

@@ -117,11 +117,20 @@ enum OpKind : int32_t {
     // operands on row bits, one entry per row code; lmask / gmask hold the
     // list lengths (16 bits each: none-row, pair table, rows 0..5).
     OP_PHASE = 12,
+    // Adjoint: the inner product <phi| D |psi> for a diagonal D(j) = k0 +
+    // sum_q k_q z_q + sum_{a<b} k_ab z_a z_b with complex k (a run's
+    // diagonal generators of one parameter, weighted by their prefactors and
+    // dependency coefficients).  HostOp::phase = k0, terms hold k (w, wi).
+    // On the device: records as OP_PHASE's (k0, then {k, {code, 0}} per term,
+    // then the row-pair table, additive), lmask the accumulator slot, gmask
+    // the lengths none-row | pair table << 16 | row 0 << 32 | row 1 << 48,
+    // gpos bytes 0..7 rows 2..5 (16 bits each).
+    OP_IP_PHASE = 13,
 };
 
 // Device op kinds that accumulate an inner product.
 inline bool is_ip_kind(int32_t kind) {
-    return kind == OP_IP_DENSE || kind == OP_IP_DIAG || kind == OP_IP_DIAGG || kind == OP_IPSUB;
+    return kind == OP_IP_DENSE || kind == OP_IP_DIAG || kind == OP_IP_DIAGG || kind == OP_IPSUB || kind == OP_IP_PHASE;
 }
 
 // Inner-product members per register block (per-thread accumulators).
@@ -135,10 +144,11 @@ constexpr int kMaxBlockIp = 8;
 // with a real diagonal and an imaginary off-diagonal (Rx, Y).
 constexpr int kReal2 = 13, kXGen = 16, kYGen = 19, kZGen = 22, kNegPair = 25, kImag2 = 28;
 
-// One term of a merged phase op: weight w times z_a (b < 0) or z_a z_b.
+// One term of a merged phase op: weight w times z_a (b < 0) or z_a z_b
+// (an inner-product op's weight is complex: w + i wi).
 struct PhaseTerm {
     int32_t a = -1, b = -1;
-    double w = 0.0;
+    double w = 0.0, wi = 0.0;
 };
 
 struct HostOp {
@@ -152,7 +162,7 @@ struct HostOp {
     double angle0 = 0.0;             // OP_PHASE: constant angle c
     std::vector<PhaseTerm> terms;    // OP_PHASE: h (b < 0) and J terms
     uint64_t qmask() const {
-        if (kind == OP_MASKPHASE || kind == OP_PHASE) return mask;
+        if (kind == OP_MASKPHASE || kind == OP_PHASE || kind == OP_IP_PHASE) return mask;
         uint64_t r = 0;
         for (int b = 0; b < k; ++b) r |= uint64_t{1} << q[b];
         return r;
@@ -244,12 +254,18 @@ constexpr int kStreamFragBytes = 16384;
 constexpr int kTileArrayQubitsC128 = 10;
 constexpr int kTileArrayQubitsC64 = 11;
 
+// PassDesc::kmax flag: a pass with OP_PHASE / OP_IP_PHASE ops runs the
+// kernel variant that has them (the other variants, the streaming kernel
+// among them, leave them out and keep their registers and code).
+constexpr int kPhaseKernel = 8;
+
 struct PassDesc {
     int32_t n;           // register width
     int32_t T;           // tile qubits; 0 marks a wide pass (one dense op, k > 5)
     int32_t op_begin;
     int32_t op_end;
-    int32_t kmax;        // widest VALU dense / IP op in the pass (matrix-core ops excluded)
+    int32_t kmax;        // widest VALU dense / IP op in the pass (matrix-core ops excluded), plus
+                         // kPhaseKernel in passes that carry merged phase ops
     int32_t frag;        // elements of the pass's matrix-core fragments as the streaming kernel
                          // stages them in LDS; -1: the pass runs one tile per workgroup
     uint64_t tmask;      // global bits that are tile-local
@@ -280,7 +296,7 @@ struct BatchItem {
 // per workgroup).
 inline bool streaming_pass(const PassDesc& P, int precision) {
     const int tb0 = precision == QHIP_C128 ? kTileArrayQubitsC128 : kTileArrayQubitsC64;
-    if (P.frag < 0 || P.kmax > 2) return false;
+    if (P.frag < 0 || (P.kmax & ~kPhaseKernel) > 2) return false;
     return P.T == tb0 + 3 || (P.T >= tb0 && P.T <= tb0 + 2 && P.frag > 0);
 }
 
@@ -427,8 +443,18 @@ Plan plan_stream(const qhip_config* cfg, const qhip_stream* stream, int n, int p
 struct GradSlot {
     cplx c;        // generator prefactor
     int64_t op;    // stream op index
+    // -1: the slot's inner product goes to the op's parameters through its
+    // dependency coefficients; p >= 0: it adds 2 Re(c ip) to parameter p
+    // directly (a merged run); kMergedSlot: folded into such a slot.
+    int32_t param = -1;
 };
+constexpr int32_t kMergedSlot = -2;
 void build_backward(const Stream& st, int n, std::vector<GradSlot>& slots, std::vector<HostOp>& back);
+// Runs of diagonal gates in the reverse sweep whose generators share few
+// parameters: their inner products are taken together before the run's
+// un-applies (all diagonal, so they commute), one OP_IP_PHASE per
+// parameter, and the un-applies merge into one OP_PHASE.
+void merge_adjoint_runs(const Stream& st, int n, std::vector<GradSlot>& slots, std::vector<HostOp>& back);
 void accumulate_gradient(const qhip_stream* stream, const std::vector<GradSlot>& slots, const cplx* ip,
                          int n_params, double* grad);
 ScheduleOptions schedule_options(const qhip_config* cfg, bool two_states);

@@ -225,6 +225,7 @@ void build_backward(const Stream& st, int n, std::vector<GradSlot>& slots, std::
         }
         lower_dagger(st, i, back);
     }
+    merge_adjoint_runs(st, n, slots, back);
 }
 
 void accumulate_gradient(const qhip_stream* stream, const std::vector<GradSlot>& slots, const cplx* ip,
@@ -232,8 +233,13 @@ void accumulate_gradient(const qhip_stream* stream, const std::vector<GradSlot>&
     const Stream st{stream};
     for (int p = 0; p < n_params; ++p) grad[p] = 0.0;
     for (size_t s = 0; s < slots.size(); ++s) {
-        const qhip_op& op = st.op(slots[s].op);
+        if (slots[s].param == kMergedSlot) continue;   // taken by a merged run's slot
         const double base = 2.0 * (slots[s].c * ip[s]).real();
+        if (slots[s].param >= 0) {
+            if (slots[s].param < n_params) grad[slots[s].param] += base;
+            continue;
+        }
+        const qhip_op& op = st.op(slots[s].op);
         for (int d = 0; d < op.n_deps; ++d) {
             const int p = stream->dep_index[op.dep_offset + d];
             if (p >= 0 && p < n_params) grad[p] += stream->dep_alpha[op.dep_offset + d] * base;

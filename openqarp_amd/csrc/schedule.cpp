@@ -35,6 +35,7 @@ static double op_flops(const HostOp& op) {
         case OP_DIAG: case OP_MASKPHASE: case OP_SCALAR: return 6.0;
         case OP_IP_DIAG: return 14.0;
         case OP_PHASE: return 36.0;
+        case OP_IP_PHASE: return 24.0;
         default: return 0.0;
     }
 }
@@ -153,6 +154,63 @@ std::vector<cplx> mfma_matrix(const cplx* f, int k, bool f32) {
     return U;
 }
 
+// The records of a merged phase op (additive = false: factors exp(i w),
+// constant exp(i c), row-pair table of products) or of an inner-product
+// phase op (additive: weights w + i wi, constant HostOp::phase, row-pair
+// table of signed sums).  Terms by where their operands sit: an operand at
+// tile position p < 8 (thread bit p) reads bit 56 + p of the thread's key,
+// one outside the tile its global bit (n <= 55), a missing one bit 55
+// (always clear); an operand at position 8 + m is row bit m.  Each term
+// takes two records, its weight then {operand code, 0}.  cnt: the none-row
+// term count, 1 when a row-pair table follows, the term counts of rows 0..5.
+static void emit_phase_terms(Plan& plan, const HostOp& op, const int* local_of, const int* pos, int T,
+                             bool additive, uint64_t cnt[8]) {
+    std::vector<cplx> none_row, rows[6], pairs;
+    auto where = [&](int q, int& row) {
+        row = -1;
+        if (q < 0) return 55;
+        const int l = local_of[q];
+        if (l < 0) return pos[q];
+        if (l >= 8) {
+            row = l - 8;
+            return 0;
+        }
+        return 56 + l;
+    };
+    auto put = [&](std::vector<cplx>& v, const PhaseTerm& t, int code) {
+        v.push_back(additive ? cplx(t.w, t.wi) : std::polar(1.0, t.w));
+        v.emplace_back(static_cast<double>(code), 0.0);
+    };
+    for (const PhaseTerm& t : op.terms) {
+        int ra, rb;
+        const int ka = where(t.a, ra), kb = where(t.b, rb);
+        if (ra < 0 && rb < 0) put(none_row, t, ka | kb << 8);
+        else if (ra >= 0 && rb >= 0) put(pairs, t, ra | rb << 8);
+        else if (ra >= 0) put(rows[ra], t, kb);
+        else put(rows[rb], t, ka);
+    }
+    plan.coefs.push_back(additive ? op.phase : std::polar(1.0, op.angle0));
+    plan.coefs.insert(plan.coefs.end(), none_row.begin(), none_row.end());
+    for (const auto& r : rows) plan.coefs.insert(plan.coefs.end(), r.begin(), r.end());
+    // Terms with both operands on row bits depend on the row code alone:
+    // one table over the 2^(T-8) row codes.
+    const int n_rows = T > 8 ? T - 8 : 0;
+    if (!pairs.empty())
+        for (uint32_t rr = 0; rr < (1u << n_rows); ++rr) {
+            cplx f = additive ? cplx(0.0, 0.0) : cplx(1.0, 0.0);
+            for (size_t t = 0; t < pairs.size(); t += 2) {
+                const int code = static_cast<int>(pairs[t + 1].real());
+                const bool odd = ((rr >> (code & 0xff)) ^ (rr >> (code >> 8))) & 1u;
+                if (additive) f += odd ? -pairs[t] : pairs[t];
+                else f *= odd ? std::conj(pairs[t]) : pairs[t];
+            }
+            plan.coefs.push_back(f);
+        }
+    cnt[0] = none_row.size() / 2;
+    cnt[1] = pairs.empty() ? 0 : 1;
+    for (int m = 0; m < 6; ++m) cnt[2 + m] = rows[m].size() / 2;
+}
+
 // local_of[q]: tile position of qubit q (-1 outside the tile); pos[q]: the
 // global bit that holds qubit q (the identity unless passes relabel).
 static void emit_op(Plan& plan, const HostOp& op, const int* local_of, const int* pos, int n, int T,
@@ -220,59 +278,27 @@ static void emit_op(Plan& plan, const HostOp& op, const int* local_of, const int
             ++plan.diag_ops;
             break;
         case OP_PHASE: {
-            // Terms by where their operands sit: an operand at tile position
-            // p < 8 (thread bit p) reads bit 56 + p of the thread's key, one
-            // outside the tile its global bit (n <= 55), a missing one bit
-            // 55 (always clear); an operand at position 8 + m is row bit m.
-            std::vector<cplx> none_row, rows[6], pairs;
-            auto where = [&](int q, int& row) {
-                row = -1;
-                if (q < 0) return 55;
-                const int l = local_of[q];
-                if (l < 0) return pos[q];
-                if (l >= 8) { row = l - 8; return 0; }
-                return 56 + l;
-            };
-            // Each term: its factor exp(i w), then {operand code, 0}.
-            auto put = [](std::vector<cplx>& v, double w, int code) {
-                v.push_back(std::polar(1.0, w));
-                v.emplace_back(static_cast<double>(code), 0.0);
-            };
-            for (const PhaseTerm& t : op.terms) {
-                int ra, rb;
-                const int ka = where(t.a, ra), kb = where(t.b, rb);
-                if (ra < 0 && rb < 0) put(none_row, t.w, ka | kb << 8);
-                else if (ra >= 0 && rb >= 0) put(pairs, t.w, ra | rb << 8);
-                else if (ra >= 0) put(rows[ra], t.w, kb);
-                else put(rows[rb], t.w, ka);
-            }
-            plan.coefs.push_back(std::polar(1.0, op.angle0));
-            plan.coefs.insert(plan.coefs.end(), none_row.begin(), none_row.end());
-            for (const auto& r : rows) plan.coefs.insert(plan.coefs.end(), r.begin(), r.end());
-            // Terms with both operands on row bits depend on the row code
-            // alone: one table over the 2^(T-8) row codes.
-            const int n_rows = T > 8 ? T - 8 : 0;
-            if (!pairs.empty()) {
-                for (uint32_t rr = 0; rr < (1u << n_rows); ++rr) {
-                    cplx f(1.0, 0.0);
-                    for (size_t t = 0; t < pairs.size(); t += 2) {
-                        const int code = static_cast<int>(pairs[t + 1].real());
-                        const bool odd = ((rr >> (code & 0xff)) ^ (rr >> (code >> 8))) & 1u;
-                        f *= odd ? std::conj(pairs[t]) : pairs[t];
-                    }
-                    plan.coefs.push_back(f);
-                }
-            }
-            d.lmask = none_row.size() / 2 | uint64_t{pairs.empty() ? 0u : 1u} << 16 |
-                      uint64_t{rows[0].size() / 2} << 32 | uint64_t{rows[1].size() / 2} << 48;
-            d.gmask = rows[2].size() / 2 | rows[3].size() / 2 << 16 | uint64_t{rows[4].size() / 2} << 32 |
-                      uint64_t{rows[5].size() / 2} << 48;
+            uint64_t cnt[8];
+            emit_phase_terms(plan, op, local_of, pos, T, false, cnt);
+            d.lmask = cnt[0] | cnt[1] << 16 | cnt[2] << 32 | cnt[3] << 48;
+            d.gmask = cnt[4] | cnt[5] << 16 | cnt[6] << 32 | cnt[7] << 48;
+            d.k = 0;
+            ++plan.diag_ops;
+            break;
+        }
+        case OP_IP_PHASE: {
+            uint64_t cnt[8];
+            emit_phase_terms(plan, op, local_of, pos, T, true, cnt);
+            d.gmask = cnt[0] | cnt[1] << 16 | cnt[2] << 32 | cnt[3] << 48;
+            const uint64_t g0 = cnt[4] | cnt[5] << 16 | cnt[6] << 32 | cnt[7] << 48;
+            std::memcpy(d.gpos, &g0, sizeof(g0));
             d.k = 0;
             ++plan.diag_ops;
             break;
         }
     }
-    if (op.kind == OP_IP_DENSE || op.kind == OP_IP_DIAG) d.lmask = static_cast<uint64_t>(op.slot);
+    if (op.kind == OP_IP_DENSE || op.kind == OP_IP_DIAG || op.kind == OP_IP_PHASE)
+        d.lmask = static_cast<uint64_t>(op.slot);
     plan.ops.push_back(d);
     plan.flops += op_flops(op) * static_cast<double>(uint64_t{1} << n);
 }
@@ -388,6 +414,120 @@ static bool merge_phase_runs(const std::vector<HostOp>& ops, const std::vector<s
     return true;
 }
 
+// Adjoint runs.  In the reverse sweep every parametric gate contributes
+// its generator's inner product, then its un-apply.  Where a run of such
+// entries is all diagonal (the un-applies are mergeable phase ops, the
+// generators real diagonals on at most two qubits: Z, ZZ, projectors,
+// |1><1| (x) Z, GPhase's identity), the matrices commute, so every inner
+// product of the run may be taken before any of its un-applies.  Per
+// parameter p the run's generators then add up to one diagonal D_p = sum of
+// alpha c G over the gates depending on p (alpha the dependency coefficient,
+// c the generator's prefactor), whose Walsh coefficients are complex: one
+// OP_IP_PHASE per parameter, its slot adding 2 Re <phi|D_p|psi> to p, then
+// one OP_PHASE for the un-applies.  Done where it pays: four or more inner
+// products per parameter (a QAOA cost layer: 51 RZZ gates, one parameter).
+using PhaseAccC = std::map<std::pair<int, int>, cplx>;
+constexpr size_t kMaxAdjointRun = 1024;   // entries per merged run (terms stay below 4096)
+
+static bool real_diag_ip(const HostOp& op) {
+    if (op.kind != OP_IP_DIAG || op.k > 2) return false;
+    for (const cplx& e : op.m)
+        if (e.imag() != 0.0) return false;
+    return true;
+}
+
+// Adds f times the Walsh coefficients of a real diagonal generator.
+static void ip_walsh(const HostOp& op, cplx f, cplx& k0, PhaseAccC& acc) {
+    auto add = [&](int a, int b, cplx w) {
+        if (w != cplx(0.0, 0.0)) acc[{a, b}] += w;
+    };
+    const std::vector<cplx>& e = op.m;
+    if (op.k == 0) {
+        k0 += f * e[0].real();
+        return;
+    }
+    const int q0 = static_cast<int>(op.q[0]);
+    if (op.k == 1) {
+        k0 += f * ((e[0].real() + e[1].real()) / 2);
+        add(q0, -1, f * ((e[0].real() - e[1].real()) / 2));
+        return;
+    }
+    const int q1 = static_cast<int>(op.q[1]);
+    const double e0 = e[0].real(), e1 = e[1].real(), e2 = e[2].real(), e3 = e[3].real();
+    k0 += f * ((e0 + e1 + e2 + e3) / 4);
+    add(q0, -1, f * ((e0 - e1 + e2 - e3) / 4));
+    add(q1, -1, f * ((e0 + e1 - e2 - e3) / 4));
+    add(std::min(q0, q1), std::max(q0, q1), f * ((e0 - e1 - e2 + e3) / 4));
+}
+
+void merge_adjoint_runs(const Stream& st, int n, std::vector<GradSlot>& slots, std::vector<HostOp>& back) {
+    if (n > 55) return;   // the thread key of the phase ops holds global bits below 55
+    std::vector<HostOp> out;
+    out.reserve(back.size());
+    double scratch = 0.0;
+    size_t i = 0;
+    while (i < back.size()) {
+        size_t j = i;
+        size_t n_ip = 0;
+        while (j < back.size() && j - i < kMaxAdjointRun) {
+            if (real_diag_ip(back[j])) ++n_ip;
+            else if (!phase_walsh(back[j], scratch, nullptr)) break;
+            ++j;
+        }
+        std::vector<int32_t> params;
+        for (size_t e = i; e < j && n_ip >= 4; ++e) {
+            if (back[e].kind != OP_IP_DIAG) continue;
+            const qhip_op& g = st.op(slots[back[e].slot].op);
+            for (int d = 0; d < g.n_deps; ++d) {
+                const int32_t p = st.s->dep_index[g.dep_offset + d];
+                if (p >= 0 && std::find(params.begin(), params.end(), p) == params.end()) params.push_back(p);
+            }
+        }
+        if (n_ip < 4 || params.empty() || params.size() * 4 > n_ip) {
+            const size_t stop = std::max(j, i + 1);
+            for (size_t e = i; e < stop; ++e) out.push_back(std::move(back[e]));
+            i = stop;
+            continue;
+        }
+        uint64_t mask = 0;
+        for (size_t e = i; e < j; ++e) mask |= back[e].qmask();
+        for (const int32_t p : params) {
+            HostOp ip;
+            ip.kind = OP_IP_PHASE;
+            ip.mask = mask;
+            ip.phase = cplx(0.0, 0.0);
+            PhaseAccC acc;
+            for (size_t e = i; e < j; ++e) {
+                if (back[e].kind != OP_IP_DIAG) continue;
+                const GradSlot& gs = slots[back[e].slot];
+                const qhip_op& g = st.op(gs.op);
+                for (int d = 0; d < g.n_deps; ++d)
+                    if (st.s->dep_index[g.dep_offset + d] == p)
+                        ip_walsh(back[e], gs.c * st.s->dep_alpha[g.dep_offset + d], ip.phase, acc);
+            }
+            for (const auto& [key, w] : acc) ip.terms.push_back(PhaseTerm{key.first, key.second, w.real(), w.imag()});
+            ip.slot = static_cast<int32_t>(slots.size());
+            slots.push_back(GradSlot{cplx(1.0, 0.0), -1, p});
+            out.push_back(std::move(ip));
+        }
+        HostOp ph;   // the un-applies
+        ph.kind = OP_PHASE;
+        PhaseAcc pacc;
+        for (size_t e = i; e < j; ++e) {
+            if (back[e].kind == OP_IP_DIAG) {
+                slots[back[e].slot].param = kMergedSlot;
+                continue;
+            }
+            phase_walsh(back[e], ph.angle0, &pacc);
+            ph.mask |= back[e].qmask();
+        }
+        for (const auto& [key, w] : pacc) ph.terms.push_back(PhaseTerm{key.first, key.second, w});
+        out.push_back(std::move(ph));
+        i = j;
+    }
+    back = std::move(out);   // every entry was moved into out
+}
+
 // Register blocks.  Dense ops on at most two qubits are gathered into blocks
 // whose qubit union has at most three qubits; the kernel loads a block's
 // 2^3-amplitude groups into registers once, applies every member op there
@@ -402,7 +542,9 @@ struct Unit {
     bool block_ok = false;
 };
 
-static bool is_ip_host(const HostOp& op) { return op.kind == OP_IP_DENSE || op.kind == OP_IP_DIAG; }
+static bool is_ip_host(const HostOp& op) {
+    return op.kind == OP_IP_DENSE || op.kind == OP_IP_DIAG || op.kind == OP_IP_PHASE;
+}
 
 // Members: dense ops on at most two qubits, and diagonal ops with at most
 // two operands on tile qubits (tile mask tq) and up to four outside the
@@ -627,11 +769,11 @@ PassPick pick_pass(const SegmentDag& dag, const std::vector<size_t>& seg, const 
         pk.chosen.push_back(r);
         pk.Q |= op.needs_local() ? op.qmask() : 0;
         flops += op_flops(op);
-        n_ip += op.kind == OP_IP_DENSE || op.kind == OP_IP_DIAG;
+        n_ip += is_ip_host(op);
         n_dense += op.needs_local();
     };
     auto admissible = [&](const HostOp& op) {
-        if ((op.kind == OP_IP_DENSE || op.kind == OP_IP_DIAG) && n_ip >= kMaxIpPerPass) return false;
+        if (is_ip_host(op) && n_ip >= kMaxIpPerPass) return false;
         if (opt.max_ops > 0 && n_dense >= opt.max_ops && op.needs_local()) return false;
         if (opt.max_flops_per_amp > 0 && !pk.chosen.empty() && flops + op_flops(op) > opt.max_flops_per_amp)
             return false;
@@ -820,6 +962,8 @@ static void schedule_segment(Plan& plan, const std::vector<HostOp>& ops, const s
             }
         }
         P.op_end = static_cast<int32_t>(plan.ops.size());
+        for (int o = P.op_begin; o < P.op_end; ++o)
+            if (plan.ops[o].kind == OP_PHASE || plan.ops[o].kind == OP_IP_PHASE) P.kmax |= kPhaseKernel;
         P.frag = -1;
         const bool widest = T == max_tile_qubits(opt.precision, false);
         if (!opt.two_states && (opt.stream > 0 || (opt.stream < 0 && widest))) {

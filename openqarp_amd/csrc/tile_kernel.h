@@ -1154,6 +1154,93 @@ __device__ __forceinline__ double2 term_product(double2 f, const A*& r, uint32_t
     return f;
 }
 
+// f plus, over cnt terms of an inner-product phase op (two records each:
+// the weight k, then {operand code, 0}), k or -k as odd(code) says.
+template <typename A, typename Odd>
+__device__ __forceinline__ double2 term_sum(double2 f, const A*& r, uint32_t cnt, Odd odd) {
+    uint32_t i = 0;
+    for (; i + 4 <= cnt; i += 4, r += 8) {
+        A e[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) e[t] = r[t];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const double sg = odd(static_cast<uint32_t>(e[2 * t + 1].x)) ? -1.0 : 1.0;
+            f.x = fma(sg, static_cast<double>(e[2 * t].x), f.x);
+            f.y = fma(sg, static_cast<double>(e[2 * t].y), f.y);
+        }
+    }
+    for (; i < cnt; ++i, r += 2) {
+        const A e0 = r[0], e1 = r[1];
+        const double sg = odd(static_cast<uint32_t>(e1.x)) ? -1.0 : 1.0;
+        f.x = fma(sg, static_cast<double>(e0.x), f.x);
+        f.y = fma(sg, static_cast<double>(e0.y), f.y);
+    }
+    return f;
+}
+
+// <phi| D |psi> over this thread's amplitudes for an inner-product phase op
+// (OP_IP_PHASE): D at amplitude r = k0 plus the terms with no row operand
+// (a0, per thread) plus, per row bit m, +-g_m (the terms with one operand
+// there) plus the row-pair table's entry; the thread key as for OP_PHASE.
+// Sums and products in FP64.
+template <typename A, int MAXROWS>
+__device__ __forceinline__ double2 ip_phase_op(A* s, int T, uint64_t gbase, uint32_t te, uint32_t R, bool active,
+                                               uint32_t phi_off, const OpRegs& op, const A* __restrict__ rec) {
+    constexpr int sh = Amp<A>::kShift;
+    const uint64_t key = gbase | (static_cast<uint64_t>(threadIdx.x) << 56);
+    const A* r = rec;
+    double2 a0 = make_double2(static_cast<double>(r[0].x), static_cast<double>(r[0].y));
+    ++r;
+    a0 = term_sum(a0, r, static_cast<uint32_t>(op.gmask & 0xffffu),
+                  [&](uint32_t c) { return ((key >> (c & 0xffu)) ^ (key >> (c >> 8))) & 1u; });
+    const int rows = T > 8 ? T - 8 : 0;
+    double2 g[MAXROWS > 0 ? MAXROWS : 1];
+#pragma unroll
+    for (int m = 0; m < MAXROWS; ++m) {
+        g[m] = make_double2(0.0, 0.0);
+        const uint32_t cnt = static_cast<uint32_t>(
+            (m < 2 ? op.gmask >> (32 + 16 * m) : op.gpos0 >> (16 * (m - 2))) & 0xffffu);
+        if (m < rows) g[m] = term_sum(g[m], r, cnt, [&](uint32_t c) { return (key >> c) & 1u; });
+    }
+    const bool pair_table = ((op.gmask >> 16) & 0xffffu) != 0;
+    double2 acc = make_double2(0.0, 0.0);
+    if (!active) return acc;
+    for (uint32_t i0 = 0; i0 < R; i0 += 4) {
+        A v[4], f[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (i0 + u < R) {
+                const uint32_t a = te ^ (swz((i0 + u) << 8) << sh);
+                v[u] = at(s, a);
+                f[u] = at(s, a | phi_off);
+            }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (i0 + u < R) {
+                const uint32_t rr = i0 + u;
+                double2 d = a0;
+#pragma unroll
+                for (int m = 0; m < MAXROWS; ++m)
+                    if (m < rows) {
+                        const double sg = ((rr >> m) & 1u) ? -1.0 : 1.0;
+                        d.x = fma(sg, g[m].x, d.x);
+                        d.y = fma(sg, g[m].y, d.y);
+                    }
+                if (pair_table) {
+                    const A e = r[rr];
+                    d.x += static_cast<double>(e.x);
+                    d.y += static_cast<double>(e.y);
+                }
+                const double xr = v[u].x, xi = v[u].y, fr = f[u].x, fi = f[u].y;
+                const double gr = d.x * xr - d.y * xi, gi = d.x * xi + d.y * xr;   // (D psi)_r
+                acc.x += fr * gr + fi * gi;                                          // conj(phi_r) (D psi)_r
+                acc.y += fr * gi - fi * gr;
+            }
+    }
+    return acc;
+}
+
 // A merged phase op (OP_PHASE) on this thread's amplitudes l = threadIdx.x +
 // 256 r, r < R.  Each term is stored as its factor exp(i w) and its operand
 // code; it contributes exp(i w) where its operands' bits agree and the
@@ -1163,9 +1250,9 @@ __device__ __forceinline__ double2 term_product(double2 f, const A*& r, uint32_t
 // operand on a row bit, per thread) x, per row bit m (tile position 8 + m),
 // g_m or its conjugate (the terms with one operand there) x the row-pair
 // table's entry for row code r.  Products in FP64 in both precisions.
-template <typename A, int MAXROWS>
+template <typename A, bool TWO, int MAXROWS>
 __device__ __forceinline__ void phase_op(A* s, int T, uint64_t gbase, uint32_t te, uint32_t R, bool active,
-                                         const OpRegs& op, const A* __restrict__ rec) {
+                                         uint32_t phi_off, const OpRegs& op, const A* __restrict__ rec) {
     constexpr int sh = Amp<A>::kShift;
     const uint64_t key = gbase | (static_cast<uint64_t>(threadIdx.x) << 56);
     const A* r = rec;
@@ -1197,7 +1284,7 @@ __device__ __forceinline__ void phase_op(A* s, int T, uint64_t gbase, uint32_t t
         }
     if (!active) return;
     for (uint32_t i0 = 0; i0 < R; i0 += 4) {
-        A v[4];
+        A v[4], w[4];
         uint32_t ad[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u)
@@ -1205,6 +1292,7 @@ __device__ __forceinline__ void phase_op(A* s, int T, uint64_t gbase, uint32_t t
                 const uint32_t i = i0 + u, rr = i ^ (i >> 1);
                 ad[u] = te ^ (swz(rr << 8) << sh);
                 v[u] = at(s, ad[u]);
+                if (TWO) w[u] = at(s, ad[u] | phi_off);
             }
 #pragma unroll
         for (int u = 0; u < 4; ++u)
@@ -1227,10 +1315,14 @@ __device__ __forceinline__ void phase_op(A* s, int T, uint64_t gbase, uint32_t t
                 fa.x = f.x;
                 fa.y = f.y;
                 v[u] = cmul(v[u], fa);
+                if (TWO) w[u] = cmul(w[u], fa);
             }
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            if (i0 + u < R) at(s, ad[u]) = v[u];
+            if (i0 + u < R) {
+                at(s, ad[u]) = v[u];
+                if (TWO) at(s, ad[u] | phi_off) = w[u];
+            }
     }
 }
 
@@ -1245,6 +1337,8 @@ __device__ __forceinline__ int run_ops(A* s, const PassDesc& P, uint64_t gbase, 
                                        const DevOp* __restrict__ ops, const A* __restrict__ coefs,
                                        double2* wave_ip, const A* frag) {
     static_assert(!(STAGED && TWO), "staged fragments are a single-state form");
+    constexpr int KD = KMAX & ~kPhaseKernel;                      // widest VALU dense op
+    constexpr bool PH = (KMAX & kPhaseKernel) != 0;              // merged phase ops compiled in
     constexpr int sh = Amp<A>::kShift;
     constexpr uint32_t kPhi = (1u << TB) << sh;   // byte offset of phi's tile
     const uint32_t L = 1u << P.T;
@@ -1260,8 +1354,8 @@ __device__ __forceinline__ int run_ops(A* s, const PassDesc& P, uint64_t gbase, 
             case OP_DENSE: {
                 const int G = P.T - k;
                 const uint32_t tp = thread_part<A>(op, G);
-                dense_op<A, KMAX>(s, tp, G, op, U);
-                if (TWO) dense_op<A, KMAX>(s, tp | kPhi, G, op, U);
+                dense_op<A, KD>(s, tp, G, op, U);
+                if (TWO) dense_op<A, KD>(s, tp | kPhi, G, op, U);
                 break;
             }
             case OP_MFMA: {
@@ -1356,8 +1450,18 @@ __device__ __forceinline__ int run_ops(A* s, const PassDesc& P, uint64_t gbase, 
                     }
                 break;
             }
-            case OP_PHASE:   // single-state passes only
-                if constexpr (!TWO) phase_op<A, (TB > 14 ? 6 : TB > 8 ? TB - 8 : 0)>(s, P.T, gbase, te, R, active, op, U);
+            case OP_PHASE:
+                if constexpr (PH)
+                    phase_op<A, TWO, (TB > 14 ? 6 : TB > 8 ? TB - 8 : 0)>(s, P.T, gbase, te, R, active, kPhi, op, U);
+                break;
+            case OP_IP_PHASE:   // two-state passes only
+                if constexpr (TWO && PH) {
+                    double2 acc =
+                        ip_phase_op<A, (TB > 14 ? 6 : TB > 8 ? TB - 8 : 0)>(s, P.T, gbase, te, R, active, kPhi, op, U);
+                    acc = wave_sum(acc);
+                    if ((threadIdx.x & 63) == 0) wave_ip[n_ip * kWaves + (threadIdx.x >> 6)] = acc;
+                    ++n_ip;
+                }
                 break;
             case OP_IP_DENSE:
             case OP_IP_DIAG:
@@ -1548,8 +1652,8 @@ constexpr int two_state_waves(bool two, int kmax) { return two && kmax == 2 ? 2 
 // (The batched kernel's instance lookup does not fit that budget.)
 template <typename A, bool TWO, int KMAX, int TB, bool BATCH>
 constexpr int tile_waves() {
-    if (TWO) return two_state_waves(TWO, KMAX);
-    return (!BATCH && KMAX == 2 && TB + Amp<A>::kShift == 15) ? 4 : 1;
+    if (TWO) return two_state_waves(TWO, KMAX & ~kPhaseKernel);
+    return (!BATCH && (KMAX & ~kPhaseKernel) == 2 && TB + Amp<A>::kShift == 15) ? 4 : 1;
 }
 
 template <typename A, bool TWO, int KMAX, int TB>
@@ -1677,7 +1781,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(stream
         // again and never used.
         load_tile(tn < n_tiles ? tn : n_tiles - 1);
         const uint64_t gbase = tile_base(static_cast<uint64_t>(t), outer);
-        run_ops<A, false, 2, TB, true>(s, P, gbase, te, ops, coefs, nullptr, frag);
+        run_ops<A, false, 2 | kPhaseKernel, TB, true>(s, P, gbase, te, ops, coefs, nullptr, frag);
         const uint64_t g0 = gbase | dj;
         uint64_t dr = 0;
 #pragma unroll
@@ -1720,15 +1824,32 @@ static void launch_k(int km, const PassDesc& P, A* psi, A* phi, double2* ip, con
                                    n_tiles); break;
         case 4: hipLaunchKernelGGL((tile_pass_kernel<A, TWO, 4, TB>), grid, block, 0, st, psi, phi, P, ops, coefs, ip,
                                    n_tiles); break;
+        case 2 | kPhaseKernel:
+            hipLaunchKernelGGL((tile_pass_kernel<A, TWO, 2 | kPhaseKernel, TB>), grid, block, 0, st, psi, phi, P, ops,
+                               coefs, ip, n_tiles);
+            break;
+        case 5 | kPhaseKernel:
+            hipLaunchKernelGGL((tile_pass_kernel<A, TWO, 5 | kPhaseKernel, TB>), grid, block, 0, st, psi, phi, P, ops,
+                               coefs, ip, n_tiles);
+            break;
         default: hipLaunchKernelGGL((tile_pass_kernel<A, TWO, 5, TB>), grid, block, 0, st, psi, phi, P, ops, coefs,
                                     ip, n_tiles); break;
     }
 }
 
+// Kernel variant of a pass: its VALU dense width (2..5), and for passes
+// with merged phase ops the phase variant (width 2 or 5).
+inline int kernel_variant(int kmax) {
+    const int w = kmax & ~kPhaseKernel;
+    const int km = w <= 2 ? 2 : std::min(w, 5);
+    if (kmax & kPhaseKernel) return (km == 2 ? 2 : 5) | kPhaseKernel;
+    return km;
+}
+
 template <typename A, bool TWO, int TB0>
 static void launch_tb(const PassDesc& P, A* psi, A* phi, double2* ip, const DevOp* ops, const A* coefs,
                       int64_t n_tiles, hipStream_t st) {
-    const int km = P.kmax <= 2 ? 2 : std::min(P.kmax, 5);
+    const int km = kernel_variant(P.kmax);
     const int tb = std::max(P.T, TB0);
     switch (tb - TB0) {
         case 0: launch_k<A, TWO, TB0>(km, P, psi, phi, ip, ops, coefs, n_tiles, st); break;
@@ -1796,6 +1917,14 @@ static void launch_batch_k(int km, const BatchItem* items, const int64_t* first,
         case 2: hipLaunchKernelGGL((tile_batch_kernel<A, TWO, 2, TB>), grid, block, 0, st, items, first, n_items); break;
         case 3: hipLaunchKernelGGL((tile_batch_kernel<A, TWO, 3, TB>), grid, block, 0, st, items, first, n_items); break;
         case 4: hipLaunchKernelGGL((tile_batch_kernel<A, TWO, 4, TB>), grid, block, 0, st, items, first, n_items); break;
+        case 2 | kPhaseKernel:
+            hipLaunchKernelGGL((tile_batch_kernel<A, TWO, 2 | kPhaseKernel, TB>), grid, block, 0, st, items, first,
+                               n_items);
+            break;
+        case 5 | kPhaseKernel:
+            hipLaunchKernelGGL((tile_batch_kernel<A, TWO, 5 | kPhaseKernel, TB>), grid, block, 0, st, items, first,
+                               n_items);
+            break;
         default: hipLaunchKernelGGL((tile_batch_kernel<A, TWO, 5, TB>), grid, block, 0, st, items, first, n_items); break;
     }
 }
@@ -1804,11 +1933,11 @@ template <typename A, int TB0>
 static void launch_batch(bool two, const BatchItem* items, const int64_t* first, int n_items, int64_t total, int tmax,
                          int kmax, hipStream_t st) {
     if (n_items <= 0 || total <= 0) return;
-    const int km = kmax <= 2 ? 2 : std::min(kmax, 5);
     const int tb = std::max(tmax, TB0) - TB0;
     if (tb > 2) fail(QHIP_INVALID, "batched pass: " + std::to_string(tmax) + " tile qubits exceed the LDS tile");
     auto go = [&](auto two_tag) {
         constexpr bool TWO = decltype(two_tag)::value;
+        const int km = kernel_variant(kmax);
         switch (tb) {
             case 0: launch_batch_k<A, TWO, TB0>(km, items, first, n_items, total, st); break;
             case 1: launch_batch_k<A, TWO, TB0 + 1>(km, items, first, n_items, total, st); break;

@@ -106,3 +106,75 @@ def test_batched_instances_with_phase_runs():
         ps = pack_stream(cmds, values=params, param_index=index)
         sv = O.statevector(ps, n, 0, 0)
         assert abs(e - O.transition(sv, sv, n, pack_observable(obs)).real) < 1e-12
+
+
+def diagonal_gradient_circuit(n, layers, seed):
+    """Mixer layers around runs of parametric diagonal gates sharing a few
+    symbols (affine angles, some with two symbols), with unparametrised CZ,
+    S and T gates in the runs: the reverse sweep merges such runs."""
+    from openqarp_amd.ir import Command, GateType, Param
+    rng = np.random.default_rng(seed)
+    syms = [Param.symbol(f"t{i}") for i in range(2 * layers + 1)]
+    values = {f"t{i}": float(v) for i, v in enumerate(rng.uniform(-2, 2, len(syms)))}
+    cmds = [Command(GateType.H, q) for q in range(n)]
+    for layer in range(layers):
+        a, b = syms[2 * layer], syms[2 * layer + 1]
+        for _ in range(3 * n):
+            r = rng.random()
+            q0, q1 = (int(x) for x in rng.choice(n, 2, replace=False))
+            w = float(rng.uniform(0.3, 1.7))
+            if r < 0.35:
+                cmds.append(Command(GateType.RZZ, q0, q1, a * w))
+            elif r < 0.5:
+                cmds.append(Command(GateType.CRz, q0, q1, a * w + b * 0.5))
+            elif r < 0.6:
+                cmds.append(Command(GateType.CP, q0, q1, b * w))
+            elif r < 0.75:
+                cmds.append(Command(GateType.Rz, q0, a * w - 0.2))
+            elif r < 0.85:
+                cmds.append(Command(GateType.P, q0, b * w))
+            elif r < 0.9:
+                cmds.append(Command(GateType.GPhase, params=[a * 0.7]))
+            elif r < 0.95:
+                cmds.append(Command(GateType.CZ, q0, q1))
+            else:
+                cmds.append(Command([GateType.S, GateType.T][int(rng.integers(2))], q0))
+        for q in range(n):
+            cmds.append(Command(GateType.Rx, q, syms[-1] * float(rng.uniform(0.5, 1.5))))
+    names = [f"t{i}" for i in range(len(syms))]
+    obs = W.random_pauli_sum(n, 12, seed + 1)
+    return cmds, values, names, obs
+
+
+@pytest.mark.parametrize("precision", ["complex128", "complex64"])
+@pytest.mark.parametrize("n,tile", [(6, 0), (12, 0), (17, 0), (17, 12)])
+def test_adjoint_merges_diagonal_runs(n, tile, precision):
+    """Runs of parametric diagonal gates in the reverse sweep: one inner
+    product per symbol before the run's merged un-applies; the gradient
+    equals the oracle's (1e-10 in complex128)."""
+    cmds, values, names, obs = diagonal_gradient_circuit(n, 3, 40 + n)
+    print(f"adjoint diagonal runs n={n} tile={tile} {precision}", flush=True)
+    sim = QarpSimulator(precision=precision, tile_qubits=tile)
+    g, e = sim.run_gradient_energy(cmds, n, obs, values, names)
+    index = {s: i for i, s in enumerate(names)}
+    ps = pack_stream(cmds, values=values, param_index=index)
+    g_ref = O.run_gradient(ps, n, pack_observable(obs), len(names))
+    tol = 1e-10 if precision == "complex128" else 1e-3
+    assert np.abs(g - g_ref).max() < tol
+    sv = O.statevector(ps, n, 0, 0)
+    assert abs(e - O.transition(sv, sv, n, pack_observable(obs)).real) < tol
+
+
+def test_adjoint_phi_with_merged_runs():
+    """run_gradient_phi shares the reverse sweep: a caller-supplied phi
+    against the oracle's run_gradient_phi."""
+    n = 11
+    cmds, values, names, _ = diagonal_gradient_circuit(n, 2, 9)
+    rng = np.random.default_rng(3)
+    phi = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    sim = QarpSimulator()
+    g = np.array(sim.run_gradient_phi(cmds, n, phi, values, names))
+    index = {s: i for i, s in enumerate(names)}
+    ps = pack_stream(cmds, values=values, param_index=index)
+    g_ref = O.run_gradient_phi(ps, n, phi, len(names))
+    assert np.abs(g - g_ref).max() < 1e-10 * max(1.0, np.abs(g_ref).max())
