@@ -15,6 +15,7 @@ Also on the JSON line:
                 over its mean launch time from HIP events on the library's
                 stream, against the 8 TB/s HBM3E peak;
   expectation   <H> of a 2000-string random Pauli sum on the 30-qubit state;
+  c1            20-qubit hardware-efficient ansatz: state, <H> (TFIM), 4000 shots;
   adjoint       24-qubit, 384-parameter VQE energy gradient: adjoint sweep vs
                 a parameter-shift sweep (768 shifted energies) on the GPU;
   batched_vqe   512 independent instances, energies + adjoint gradients in
@@ -403,6 +404,30 @@ def columns(n=20, capacity=32, n_sources=128, n_terms=400):
             "u_dagger_s": round(d["u_dagger_wall_seconds"], 4), "transfer_s": round(d["transfer_wall_seconds"], 4)}
 
 
+def c1(reps=5):
+    """The 20-qubit hardware-efficient ansatz (10 layers of RY, RZ and a CX
+    ladder) in complex128 from |0>: the state (downloaded to the host, as
+    the reference returns it), <H> for the open-chain TFIM and 4000 shots,
+    median of `reps` runs after one warm-up."""
+    from openqarp_amd import workloads as W
+    from openqarp_amd.simulator import QarpSimulator
+    w = W.c1_hea(20, 10, seed=1)
+    sim = QarpSimulator()
+
+    def run():
+        st = sim.new_state(20)
+        sim.apply(w.commands, st)
+        e = sim.expectation_state(st, w.observable)
+        shots = sim.sample_state(st, 4000, 1234)
+        return st.download(), e, shots
+    run()
+    dt, ts = _median_time(run, reps)
+    sv, e, _ = run()
+    log(f"C1 20q HEA: state + <H> + 4000 shots in {dt * 1e3:.2f} ms (median of {reps}), <H> = {e:.12f}")
+    return {"n_qubits": 20, "layers": 10, "s": dt, "runs_s": ts, "energy": e, "norm2": float(np.vdot(sv, sv).real),
+            "what": "statevector (downloaded) + <H> + 4000 shots, complex128"}
+
+
 def trajectories(n=16, shots=2048):
     """Trajectory path: a 16-qubit brickwork, mid-circuit measurements with
     a conditional correction and a reset, more brickwork, 2048 shots in
@@ -536,7 +561,7 @@ def traffic_from_profiles():
     return None
 
 
-LEGS = ("apply", "expectation", "structured", "adjoint", "batch", "qaoa", "columns", "trajectories", "cpu")
+LEGS = ("apply", "expectation", "structured", "c1", "adjoint", "batch", "qaoa", "columns", "trajectories", "cpu")
 
 
 def main(argv=None) -> int:
@@ -596,10 +621,11 @@ def main(argv=None) -> int:
     qa = qaoa() if "qaoa" in legs else None
     cols = columns() if "columns" in legs else None
     traj = trajectories() if "trajectories" in legs else None
+    hea = c1() if "c1" in legs else None
     cpu = cpu_baseline() if "cpu" in legs else None
     if fa is None:
-        print(json.dumps({"metric": METRIC, "legs": sorted(legs), "expectation": ex, "structured_program": st,
-                          "transition_columns": cols, "trajectories": traj, "adjoint": adj,
+        print(json.dumps({"metric": METRIC, "legs": sorted(legs), "c1": hea, "expectation": ex,
+                          "structured_program": st, "transition_columns": cols, "trajectories": traj, "adjoint": adj,
                           "batched_vqe": bat, "qaoa": qa, "cpu_baseline": cpu}), flush=True)
         return 0
     line = {
@@ -621,7 +647,7 @@ def main(argv=None) -> int:
                      "bytes_per_launch": 2 * 16 * (1 << fa["n"])},
         "pass_floor": fa["pass_floor"], "device_copy": fa["device_copy"],
         "autotune": fa["trials"],
-        "expectation": ex, "structured_program": st, "transition_columns": cols, "trajectories": traj,
+        "c1": hea, "expectation": ex, "structured_program": st, "transition_columns": cols, "trajectories": traj,
         "adjoint": adj,
         "batched_vqe": bat, "qaoa": qa,
         "cpu_baseline": cpu,
