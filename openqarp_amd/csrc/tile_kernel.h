@@ -1132,32 +1132,35 @@ __device__ __forceinline__ double2 mul_unit(double2 a, double2 b) {
 // f times, over cnt terms of a merged phase op (two records each: exp(i w),
 // then {operand code, 0}), exp(i w) or its conjugate as odd(code) says.
 // Four terms' records are loaded before their products, so the loads are in
-// flight together.
+// flight together, and the products run in four independent chains joined
+// at the end (a single chain of dependent FP64 products is latency-bound).
 template <typename A, typename Odd>
 __device__ __forceinline__ double2 term_product(double2 f, const A*& r, uint32_t cnt, Odd odd) {
+    double2 p[4] = {f, make_double2(1.0, 0.0), make_double2(1.0, 0.0), make_double2(1.0, 0.0)};
     uint32_t i = 0;
     for (; i + 4 <= cnt; i += 4, r += 8) {
-        const A* u = r;
         A e[8];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) e[t] = u[t];
+        for (int t = 0; t < 8; ++t) e[t] = r[t];
 #pragma unroll
         for (int t = 0; t < 4; ++t)
-            f = mul_unit(f, unit_or_conj(make_double2(static_cast<double>(e[2 * t].x), static_cast<double>(e[2 * t].y)),
-                                         odd(static_cast<uint32_t>(e[2 * t + 1].x))));
+            p[t] = mul_unit(p[t], unit_or_conj(make_double2(static_cast<double>(e[2 * t].x),
+                                                            static_cast<double>(e[2 * t].y)),
+                                               odd(static_cast<uint32_t>(e[2 * t + 1].x))));
     }
     for (; i < cnt; ++i, r += 2) {
         const A e0 = r[0], e1 = r[1];
-        f = mul_unit(f, unit_or_conj(make_double2(static_cast<double>(e0.x), static_cast<double>(e0.y)),
-                                     odd(static_cast<uint32_t>(e1.x))));
+        p[0] = mul_unit(p[0], unit_or_conj(make_double2(static_cast<double>(e0.x), static_cast<double>(e0.y)),
+                                           odd(static_cast<uint32_t>(e1.x))));
     }
-    return f;
+    return mul_unit(mul_unit(p[0], p[1]), mul_unit(p[2], p[3]));
 }
 
 // f plus, over cnt terms of an inner-product phase op (two records each:
 // the weight k, then {operand code, 0}), k or -k as odd(code) says.
 template <typename A, typename Odd>
 __device__ __forceinline__ double2 term_sum(double2 f, const A*& r, uint32_t cnt, Odd odd) {
+    double2 p[4] = {f, make_double2(0.0, 0.0), make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
     uint32_t i = 0;
     for (; i + 4 <= cnt; i += 4, r += 8) {
         A e[8];
@@ -1166,17 +1169,17 @@ __device__ __forceinline__ double2 term_sum(double2 f, const A*& r, uint32_t cnt
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const double sg = odd(static_cast<uint32_t>(e[2 * t + 1].x)) ? -1.0 : 1.0;
-            f.x = fma(sg, static_cast<double>(e[2 * t].x), f.x);
-            f.y = fma(sg, static_cast<double>(e[2 * t].y), f.y);
+            p[t].x = fma(sg, static_cast<double>(e[2 * t].x), p[t].x);
+            p[t].y = fma(sg, static_cast<double>(e[2 * t].y), p[t].y);
         }
     }
     for (; i < cnt; ++i, r += 2) {
         const A e0 = r[0], e1 = r[1];
         const double sg = odd(static_cast<uint32_t>(e1.x)) ? -1.0 : 1.0;
-        f.x = fma(sg, static_cast<double>(e0.x), f.x);
-        f.y = fma(sg, static_cast<double>(e0.y), f.y);
+        p[0].x = fma(sg, static_cast<double>(e0.x), p[0].x);
+        p[0].y = fma(sg, static_cast<double>(e0.y), p[0].y);
     }
-    return f;
+    return make_double2((p[0].x + p[1].x) + (p[2].x + p[3].x), (p[0].y + p[1].y) + (p[2].y + p[3].y));
 }
 
 // <phi| D |psi> over this thread's amplitudes for an inner-product phase op
