@@ -534,7 +534,10 @@ void merge_adjoint_runs(const Stream& st, int n, std::vector<GradSlot>& slots, s
 // and stores once, so LDS traffic and addressing are paid per block, not per
 // op.  An op joins the latest block that touches its qubits (it must follow
 // that block, and no later unit touches them, so moving it up commutes it
-// past disjoint ops only); an op touching no unit may join any block.
+// past disjoint ops only), or else a block after the latest unit touching
+// its qubits (a merged phase op touches many qubits and is no block; the
+// ops after it still share blocks); an op touching no unit may join any
+// block.
 struct Unit {
     std::vector<size_t> ops;
     uint64_t qset = 0;   // tile qubits (a block's bits)
@@ -572,10 +575,13 @@ static std::vector<Unit> form_units(const std::vector<HostOp>& ops, const std::v
             return units[u].block_ok && std::popcount(units[u].qset | Ql) <= 3 && (!ip || n_ip[u] < kMaxBlockIp);
         };
         if (mergeable) {
-            if (last >= 0) {
-                if (fits(last)) target = last;
+            // The latest unit touching the op's qubits first (it keeps the
+            // op with them), else the latest block after it (no unit after
+            // `last` touches the op's qubits, so the op commutes there).
+            if (last >= 0 && fits(last)) {
+                target = last;
             } else {
-                for (int u = static_cast<int>(units.size()) - 1; u >= 0; --u)
+                for (int u = static_cast<int>(units.size()) - 1; u > last; --u)
                     if (fits(u)) { target = u; break; }
             }
         }
