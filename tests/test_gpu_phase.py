@@ -178,3 +178,26 @@ def test_adjoint_phi_with_merged_runs():
     ps = pack_stream(cmds, values=values, param_index=index)
     g_ref = O.run_gradient_phi(ps, n, phi, len(names))
     assert np.abs(g - g_ref).max() < 1e-10 * max(1.0, np.abs(g_ref).max())
+
+
+@pytest.mark.parametrize("precision", ["complex128", "complex64"])
+@pytest.mark.parametrize("fusion,tile,low", [(0, 6, 2), (1, 9, 4), (2, 11, 3), (3, 12, 4), (4, 12, 3), (5, 10, 1)])
+def test_phase_runs_inside_random_circuits(fusion, tile, low, precision):
+    """Phase runs between random gates of every kind (dense, Custom, MCZ,
+    markers) at several fusion widths and tile shapes, with qubit relabeling
+    on: the merged ops address tile positions, row bits and relabeled global
+    bits as the pass's layout says."""
+    from test_gpu_parity import random_circuit
+    n = 17
+    rng = np.random.default_rng(fusion * 31 + tile)
+    cmds = []
+    for part in range(4):
+        cmds += random_circuit(n, 25, int(rng.integers(1 << 30)))
+        cmds += [c for c in phase_run_circuit(n, 1, int(rng.integers(1 << 30)))[n:]
+                 if c.gate.name not in ("Rx", "Ry")]
+    print(f"random + phase runs fusion={fusion} tile={tile} low={low} {precision}", flush=True)
+    sim = QarpSimulator(precision=precision, tile_qubits=tile, low_qubits=low)
+    sim.fusion_max_qubits, sim.fusion_min_qubits = fusion, 0
+    got = sim.statevector(cmds, n)
+    ref = O.statevector(pack_stream(cmds), n, fusion, 0)
+    assert np.abs(got - ref).max() < TOL[precision]
