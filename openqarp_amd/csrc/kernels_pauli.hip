@@ -490,39 +490,49 @@ __global__ __launch_bounds__(kT) void observable_chunk_kernel(const A* __restric
     double2 acc[kPer];
 #pragma unroll
     for (int r = 0; r < kPer; ++r) acc[r] = make_double2(0.0, 0.0);
-    uint64_t staged = ~uint64_t{0};
-    for (int g = 0; g < n_groups; ++g) {
-        const GroupDev G = groups[g];
-        const uint64_t xh = G.x >> kChunkBits;
-        if (xh != staged) {   // wave-uniform: every thread sees the same group table
-            __syncthreads();  // the previous run's reads are done
-            const A* src = in + ((chunk ^ xh) << kChunkBits);
+    // The next run's partner chunk is loaded into registers while the
+    // current run computes (the runs are wave-uniform: every thread reads
+    // the same group table), so a run's staging waits only for LDS.
+    A pre[kPer];
+    auto fetch = [&](uint64_t xh) {
+        const A* src = in + ((chunk ^ xh) << kChunkBits);
 #pragma unroll
-            for (int r = 0; r < kPer; ++r) S[t + kT * r] = load_amp(src, t + kT * r);
-            __syncthreads();
-            staged = xh;
-        }
-        const uint32_t xlo = static_cast<uint32_t>(G.x) & (kChunk - 1);
-        const uint32_t vb = (t << 4) ^ (xlo << 4);   // output 0's partner: local index t ^ xlo
-        const I src = static_cast<I>(j0 ^ G.x);
-        for (int si = G.s_begin; si < G.s_end; ++si) {
-            // One 16-byte scalar load: rpat (pattern and sign mask), the
-            // term range and the kind counts.
-            const int4 w = reinterpret_cast<const int4*>(subs)[si];
-            const double2 a = pattern_weight<I>(src, w.y, w.z, z, c);
-            const bool real = w.y + (w.w & 0xffff) == w.z;
-            const int pat = w.x & (kPer - 1);
+        for (int r = 0; r < kPer; ++r) pre[r] = src[t + kT * r];
+    };
+    if (n_groups > 0) fetch(groups[0].x >> kChunkBits);
+    for (int g = 0; g < n_groups;) {
+        const uint64_t xh = groups[g].x >> kChunkBits;
+        int g_end = g + 1;
+        while (g_end < n_groups && (groups[g_end].x >> kChunkBits) == xh) ++g_end;
+        __syncthreads();  // the previous run's reads are done
+#pragma unroll
+        for (int r = 0; r < kPer; ++r) S[t + kT * r] = make_double2(pre[r].x, pre[r].y);
+        __syncthreads();
+        if (g_end < n_groups) fetch(groups[g_end].x >> kChunkBits);
+        for (; g < g_end; ++g) {
+            const GroupDev G = groups[g];
+            const uint32_t xlo = static_cast<uint32_t>(G.x) & (kChunk - 1);
+            const uint32_t vb = (t << 4) ^ (xlo << 4);   // output 0's partner: local index t ^ xlo
+            const I src = static_cast<I>(j0 ^ G.x);
+            for (int si = G.s_begin; si < G.s_end; ++si) {
+                // One 16-byte scalar load: rpat (pattern and sign mask), the
+                // term range and the kind counts.
+                const int4 w = reinterpret_cast<const int4*>(subs)[si];
+                const double2 a = pattern_weight<I>(src, w.y, w.z, z, c);
+                const bool real = w.y + (w.w & 0xffff) == w.z;
+                const int pat = w.x & (kPer - 1);
 #define QHIP_PAT(P)                                                                    \
     case P:                                                                            \
         if (real) accumulate_pattern<P, true>(acc, Sb, vb, a);                         \
         else accumulate_pattern<P, false>(acc, Sb, vb, a);                             \
         break;
-            switch (pat) {
-                QHIP_PAT(0) QHIP_PAT(1) QHIP_PAT(2) QHIP_PAT(3) QHIP_PAT(4) QHIP_PAT(5) QHIP_PAT(6) QHIP_PAT(7)
-                QHIP_PAT(8) QHIP_PAT(9) QHIP_PAT(10) QHIP_PAT(11) QHIP_PAT(12) QHIP_PAT(13) QHIP_PAT(14)
-                QHIP_PAT(15)
-            }
+                switch (pat) {
+                    QHIP_PAT(0) QHIP_PAT(1) QHIP_PAT(2) QHIP_PAT(3) QHIP_PAT(4) QHIP_PAT(5) QHIP_PAT(6) QHIP_PAT(7)
+                    QHIP_PAT(8) QHIP_PAT(9) QHIP_PAT(10) QHIP_PAT(11) QHIP_PAT(12) QHIP_PAT(13) QHIP_PAT(14)
+                    QHIP_PAT(15)
+                }
 #undef QHIP_PAT
+            }
         }
     }
 #pragma unroll
@@ -856,7 +866,7 @@ void launch_prepared_observable(const PreparedObservable& po, const qhip_state* 
 
 void apply_observable(const qhip_state* in, const qhip_observable* obs, qhip_state* out) {
     const int n = in->n;
-    const Grouped gr = group_terms(obs, n, "run_gradient");
+    const Grouped gr = group_terms(obs, n, "apply_observable");
     auto& C = ctx();
     if (gr.groups.empty()) {
         QHIP_CHECK(hipMemsetAsync(out->d, 0, out->dim() * out->amp_bytes(), C.stream));

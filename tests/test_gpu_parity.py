@@ -426,6 +426,39 @@ def test_apply_observable_matches_oracle():
     assert np.abs(b.download() - O.apply_observable(f["ket"], n, po)).max() < ATOL
 
 
+@pytest.mark.parametrize("precision", ["complex128", "complex64"])
+@pytest.mark.parametrize("n,n_terms,probs", [(12, 40, (0.4, 0.2, 0.2, 0.2)), (13, 60, (0.5, 0.2, 0.2, 0.1)),
+                                             (15, 200, (0.4, 0.2, 0.2, 0.2)), (17, 150, (0.6, 0.15, 0.15, 0.1))])
+def test_apply_observable_chunked_matches_oracle(precision, n, n_terms, probs):
+    """H|psi> on states of 2^12 amplitudes or more: one workgroup per 2^12
+    chunk, one staged partner chunk per run of flip masks with equal bits 12
+    and up (the next run's chunk loaded while the current one computes).
+    Random sums with X/Y on every qubit give runs of one group and of many,
+    a diagonal run and complex coefficients; the oracle works from the
+    amplitudes the device holds (rounded once for complex64)."""
+    from openqarp_amd import _lib
+    rng = np.random.default_rng(n * 7 + n_terms)
+    dim = 1 << n
+    psi = rng.standard_normal(dim) + 1j * rng.standard_normal(dim)
+    psi /= np.linalg.norm(psi)
+    obs = W.random_pauli_sum(n, n_terms, n + 3, probs=probs)
+    obs = [(p, c * (1 + 0.5j)) if i % 3 == 0 else (p, c) for i, (p, c) in enumerate(obs)]
+    obs += [([(n - 1, "X")], 0.75), ([(0, "Z"), (n - 1, "Z")], -0.5)]
+    po = pack_observable(obs)
+    a = DeviceState(n, precision).upload(psi)
+    b = DeviceState(n, precision)
+    _lib.check(_lib.load().qhip_apply_observable(a.handle, po.ref, b.handle))
+    ref = O.apply_observable(a.download(), n, po)
+    got = b.download()
+    scale = max(1.0, float(np.abs(ref).max()))
+    tol = ATOL if precision == "complex128" else 1e-6   # complex64: the output's one rounding
+    assert np.abs(got - ref).max() < tol * scale
+    one = pack_observable(obs[:1])                      # a single group: one run
+    b2 = DeviceState(n, precision)
+    _lib.check(_lib.load().qhip_apply_observable(a.handle, one.ref, b2.handle))
+    assert np.abs(b2.download() - O.apply_observable(a.download(), n, one)).max() < tol * scale
+
+
 # ── sampling ───────────────────────────────────────────────────────────────
 
 @pytest.mark.parametrize("name", ["sample10.npz", "hea12.npz", "hea16.npz"])
