@@ -771,6 +771,45 @@ __device__ __forceinline__ void sub_apply(A* v, int pattern, const A* __restrict
     }
 }
 
+// One member on both states of a two-state block (psi in v, phi in f) under
+// one dispatch: the member's coefficients are read once, and the two states'
+// products are independent chains the compiler can interleave.
+template <typename A>
+__device__ __forceinline__ void sub_apply2(A* v, A* f, int pattern, const A* __restrict__ U) {
+    switch (pattern) {
+        case kReal2 + 0: sub_real<A, 0>(v, U); sub_real<A, 0>(f, U); break;
+        case kReal2 + 1: sub_real<A, 1>(v, U); sub_real<A, 1>(f, U); break;
+        case kReal2 + 2: sub_real<A, 2>(v, U); sub_real<A, 2>(f, U); break;
+        case kImag2 + 0: sub_imag<A, 0>(v, U); sub_imag<A, 0>(f, U); break;
+        case kImag2 + 1: sub_imag<A, 1>(v, U); sub_imag<A, 1>(f, U); break;
+        case kImag2 + 2: sub_imag<A, 2>(v, U); sub_imag<A, 2>(f, U); break;
+        case kNegPair + 0: sub_negate<A, 0, 1>(v); sub_negate<A, 0, 1>(f); break;
+        case kNegPair + 1: sub_negate<A, 0, 2>(v); sub_negate<A, 0, 2>(f); break;
+        case kNegPair + 2: sub_negate<A, 1, 2>(v); sub_negate<A, 1, 2>(f); break;
+        case 0: sub_single<A, 0>(v, U); sub_single<A, 0>(f, U); break;
+        case 1: sub_single<A, 1>(v, U); sub_single<A, 1>(f, U); break;
+        case 2: sub_single<A, 2>(v, U); sub_single<A, 2>(f, U); break;
+        case 3: sub_pair<A, 0, 1>(v, U); sub_pair<A, 0, 1>(f, U); break;
+        case 4: sub_pair<A, 0, 2>(v, U); sub_pair<A, 0, 2>(f, U); break;
+        case 5: sub_pair<A, 1, 2>(v, U); sub_pair<A, 1, 2>(f, U); break;
+        case 6: sub_diag<A, 0, -1>(v, U); sub_diag<A, 0, -1>(f, U); break;
+        case 7: sub_diag<A, 1, -1>(v, U); sub_diag<A, 1, -1>(f, U); break;
+        case 8: sub_diag<A, 2, -1>(v, U); sub_diag<A, 2, -1>(f, U); break;
+        case 9: sub_diag<A, 0, 1>(v, U); sub_diag<A, 0, 1>(f, U); break;
+        case 10: sub_diag<A, 0, 2>(v, U); sub_diag<A, 0, 2>(f, U); break;
+        case 11: sub_diag<A, 1, 2>(v, U); sub_diag<A, 1, 2>(f, U); break;
+        default: {   // 12: no operand on the block
+            const A d = U[0];
+#pragma unroll
+            for (int x = 0; x < 8; ++x) {
+                v[x] = cmul(v[x], d);
+                f[x] = cmul(f[x], d);
+            }
+            break;
+        }
+    }
+}
+
 
 // One OP_BLOCK over one state's tile: load each group's 8 amplitudes, run
 // the nsub OP_SUB entries that follow the header, store.
@@ -1055,8 +1094,7 @@ __device__ __forceinline__ int block_apply2(A* s, uint32_t tp, uint32_t kPhi, in
                     ++j_ip;
                     continue;
                 }
-                sub_apply(v, sub.k, U);
-                sub_apply(f, sub.k, U);
+                sub_apply2(v, f, sub.k, U);
             }
 #pragma unroll
             for (int x = 0; x < 8; ++x) {
