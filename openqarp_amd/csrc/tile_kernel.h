@@ -812,8 +812,9 @@ __device__ __forceinline__ void sub_apply2(A* v, A* f, int pattern, const A* __r
 
 
 // One OP_BLOCK over one state's tile: load each group's 8 amplitudes, run
-// the nsub OP_SUB entries that follow the header, store.
-template <typename A>
+// the nsub OP_SUB entries that follow the header, store.  PAIRS: a thread
+// with two or more groups takes them two at a time.
+template <typename A, bool PAIRS>
 __device__ __forceinline__ void block_apply(A* s, uint32_t tp, int G, const OpRegs& hdr,
                                             const DevOp* __restrict__ subs, int nsub, const A* __restrict__ coefs,
                                             uint64_t gbase) {
@@ -825,6 +826,39 @@ __device__ __forceinline__ void block_apply(A* s, uint32_t tp, int G, const OpRe
     const uint32_t hm = static_cast<uint32_t>(hdr.gmask);
     const uint32_t iters = G > 8 ? 1u << (G - 8) : 1u;
     uint32_t hi = 0;
+    if (PAIRS && iters >= 2) {
+        // Two groups at a time: each member is dispatched once for both, and
+        // the two groups' products are independent chains.
+        for (uint32_t i = 0; i < iters; i += 2) {
+            const uint32_t h0 = swz(hi) << sh;
+            hi = (hi - hm) & hm;
+            const uint32_t h1 = swz(hi) << sh;
+            hi = (hi - hm) & hm;
+            uint32_t a[8], b[8];
+            A v[8], w[8];
+#pragma unroll
+            for (int x = 0; x < 8; ++x) {
+                a[x] = tp ^ (h0 ^ so[x]);
+                b[x] = tp ^ (h1 ^ so[x]);
+                v[x] = at(s, a[x]);
+                w[x] = at(s, b[x]);
+            }
+            OpRegs nxt = load_op(subs);
+            for (int j = 0; j < nsub; ++j) {
+                const OpRegs sub = nxt;
+                if (j + 1 < nsub) nxt = load_op(subs + j + 1);
+                const A* U = coefs + sub.coef;
+                if (sub.k >= 6 && sub.k <= 12) U += diag_sub_table(gbase, sub) << member_local_bits(sub.k);
+                sub_apply2(v, w, sub.k, U);
+            }
+#pragma unroll
+            for (int x = 0; x < 8; ++x) {
+                at(s, a[x]) = v[x];
+                at(s, b[x]) = w[x];
+            }
+        }
+        return;
+    }
     for (uint32_t i = 0; i < iters; ++i) {
         const uint32_t h = swz(hi) << sh;
         hi = (hi - hm) & hm;
@@ -847,6 +881,12 @@ __device__ __forceinline__ void block_apply(A* s, uint32_t tp, int G, const OpRe
         for (int x = 0; x < 8; ++x) at(s, a[x]) = v[x];
     }
 }
+
+// Whether a one-circuit kernel's single-state blocks take their groups two
+// at a time: only tiles of 2^12 amplitudes or more give a thread two groups,
+// and the batched and streaming kernels have no registers to spare for the
+// second group's eight amplitudes.
+constexpr bool block_pairs(int tb) { return tb >= 12; }
 
 // Grouped diagonal over one state's tile: each thread scales its groups'
 // 2^K amplitudes by the entries D[x] (wave-uniform, through the scalar cache).
@@ -1373,7 +1413,7 @@ __device__ __forceinline__ void phase_op(A* s, int T, uint64_t gbase, uint32_t t
 // inner-product ops, whose per-wave partials are left in wave_ip.  STAGED
 // (single state): the matrix-core ops read their fragments from frag, the
 // LDS copy the streaming kernel made, in op order.
-template <typename A, bool TWO, int KMAX, int TB, bool STAGED>
+template <typename A, bool TWO, int KMAX, int TB, bool STAGED, bool PAIRS>
 __device__ __forceinline__ int run_ops(A* s, const PassDesc& P, uint64_t gbase, uint32_t te,
                                        const DevOp* __restrict__ ops, const A* __restrict__ coefs,
                                        double2* wave_ip, const A* frag) {
@@ -1422,14 +1462,11 @@ __device__ __forceinline__ int run_ops(A* s, const PassDesc& P, uint64_t gbase, 
                 const int G = P.T - 3;
                 const uint32_t tp = thread_part<A>(op, G);
                 if constexpr (TWO) {
-                    if ((op.spos >> 56) & 0xffu) {   // inner-product members: both states in registers
-                        n_ip += block_apply2<A>(s, tp, kPhi, G, op, ops + o + 1, nsub, coefs, wave_ip, n_ip, gbase);
-                    } else {
-                        block_apply<A>(s, tp, G, op, ops + o + 1, nsub, coefs, gbase);
-                        block_apply<A>(s, tp | kPhi, G, op, ops + o + 1, nsub, coefs, gbase);
-                    }
+                    // Both states in registers, each member dispatched once
+                    // for the two (and inner-product members, if any).
+                    n_ip += block_apply2<A>(s, tp, kPhi, G, op, ops + o + 1, nsub, coefs, wave_ip, n_ip, gbase);
                 } else {
-                    block_apply<A>(s, tp, G, op, ops + o + 1, nsub, coefs, gbase);
+                    block_apply<A, PAIRS>(s, tp, G, op, ops + o + 1, nsub, coefs, gbase);
                 }
                 o += nsub;   // the member entries are consumed here
                 break;
@@ -1564,8 +1601,8 @@ struct TileLds {
 };
 
 // One tile t of pass P in the LDS array s (TileLds<A, TWO, TB>::kElems
-// elements).
-template <typename A, bool TWO, int KMAX, int TB>
+// elements).  PAIRS: single-state blocks take a thread's groups two at a time.
+template <typename A, bool TWO, int KMAX, int TB, bool PAIRS>
 __device__ __forceinline__ void tile_body(A* s, A* __restrict__ psi, A* __restrict__ phi, const PassDesc& P,
                                           const DevOp* __restrict__ ops, const A* __restrict__ coefs,
                                           double2* __restrict__ ip_out, int64_t n_tiles, uint64_t t) {
@@ -1621,7 +1658,7 @@ __device__ __forceinline__ void tile_body(A* s, A* __restrict__ psi, A* __restri
     }
     __syncthreads();
 
-    const int n_ip = run_ops<A, TWO, KMAX, TB, false>(s, P, gbase, te, ops, coefs, wave_ip, nullptr);
+    const int n_ip = run_ops<A, TWO, KMAX, TB, false, PAIRS>(s, P, gbase, te, ops, coefs, wave_ip, nullptr);
 
     if (TWO && static_cast<int>(threadIdx.x) < n_ip) {
         // Sum the per-wave values in fixed order: partial k of this tile.
@@ -1702,7 +1739,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(tile_w
     A* __restrict__ psi, A* __restrict__ phi, const PassDesc P, const DevOp* __restrict__ ops,
     const A* __restrict__ coefs, double2* __restrict__ ip_out, int64_t n_tiles) {
     __shared__ __align__(16) A s[TileLds<A, TWO, TB>::kElems];
-    tile_body<A, TWO, KMAX, TB>(s, psi, phi, P, ops, coefs, ip_out, n_tiles, blockIdx.x);
+    tile_body<A, TWO, KMAX, TB, block_pairs(TB)>(s, psi, phi, P, ops, coefs, ip_out, n_tiles, blockIdx.x);
 }
 
 // Many instances' passes in one launch: workgroup b runs tile b - first[j]
@@ -1720,7 +1757,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(tile_w
         else hi = mid - 1;
     }
     const BatchItem& it = items[lo];
-    tile_body<A, TWO, KMAX, TB>(s, static_cast<A*>(it.psi), static_cast<A*>(it.phi), it.P,
+    tile_body<A, TWO, KMAX, TB, false>(s, static_cast<A*>(it.psi), static_cast<A*>(it.phi), it.P,
                                 static_cast<const DevOp*>(it.ops), static_cast<const A*>(it.coefs), it.ip_out,
                                 it.n_tiles, static_cast<uint64_t>(b - first[lo]));
 }
@@ -1822,7 +1859,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(stream
         // again and never used.
         load_tile(tn < n_tiles ? tn : n_tiles - 1);
         const uint64_t gbase = tile_base(static_cast<uint64_t>(t), outer);
-        run_ops<A, false, 2 | kPhaseKernel, TB, true>(s, P, gbase, te, ops, coefs, nullptr, frag);
+        run_ops<A, false, 2 | kPhaseKernel, TB, true, false>(s, P, gbase, te, ops, coefs, nullptr, frag);
         const uint64_t g0 = gbase | dj;
         uint64_t dr = 0;
 #pragma unroll
