@@ -257,7 +257,7 @@ def qaoa(n=34, p=4, seed=5, shots=100_000, n_adj=33, n_cmp=32, fusion=1, low=3):
     complex64, and complex64 vs complex128 at 32 qubits.  The apply fuses
     single-qubit runs only, so each pass's share of a cost layer (RZZ gates,
     consecutive) is one merged phase op, with 3 contiguous low qubits: 12
-    passes, 1.69 s, against 1.89 s at fusion width 2 (17 passes of dense
+    passes, 1.41 s, against 1.89 s at fusion width 2 (17 passes of dense
     2-qubit ops), 3.9 s at width 4 and 5.3 s at width 3."""
     import gc
     from openqarp_amd import workloads as W
