@@ -1057,7 +1057,7 @@ __device__ __forceinline__ double2 ip_pauli_z(const A* v, const A* f) {
 // OP_IPSUB members add <phi| G |psi> to their accumulator.  The header's
 // spos[7] counts the OP_IPSUB members; their per-wave sums go to wave_ip
 // slots n_ip, n_ip + 1, ... in member order.  Returns that count.
-template <typename A>
+template <typename A, bool JUMP>
 __device__ __forceinline__ int block_apply2(A* s, uint32_t tp, uint32_t kPhi, int G, const OpRegs& hdr,
                                             const DevOp* __restrict__ subs, int nsub, const A* __restrict__ coefs,
                                             double2* wave_ip, int n_ip, uint64_t gbase) {
@@ -1125,12 +1125,30 @@ __device__ __forceinline__ int block_apply2(A* s, uint32_t tp, uint32_t kPhi, in
                             break;
                         }
                     }
+                    // j_ip is wave-uniform: a scalar branch, no register
+                    // indexing.  JUMP: one switch (a three-level branch
+                    // tree); otherwise a compare per accumulator, which the
+                    // batched kernel, short of registers, keeps (the switch
+                    // cost it 9 % there while saving 4 % in the one-circuit
+                    // kernel).
+                    static_assert(kMaxBlockIp == 8, "one case per accumulator");
+                    if constexpr (!JUMP) {
 #pragma unroll
-                    for (int t = 0; t < kMaxBlockIp; ++t)
-                        if (t == j_ip) {   // wave-uniform: a scalar branch, no register indexing
-                            acc[t].x += r.x;
-                            acc[t].y += r.y;
-                        }
+                        for (int t = 0; t < kMaxBlockIp; ++t)
+                            if (t == j_ip) {
+                                acc[t].x += r.x;
+                                acc[t].y += r.y;
+                            }
+                    } else switch (j_ip) {
+#define QHIP_ACC(T)        \
+    case T:                \
+        acc[T].x += r.x;   \
+        acc[T].y += r.y;   \
+        break;
+                        QHIP_ACC(0) QHIP_ACC(1) QHIP_ACC(2) QHIP_ACC(3) QHIP_ACC(4) QHIP_ACC(5) QHIP_ACC(6) QHIP_ACC(7)
+#undef QHIP_ACC
+                        default: break;
+                    }
                     ++j_ip;
                     continue;
                 }
@@ -1413,7 +1431,7 @@ __device__ __forceinline__ void phase_op(A* s, int T, uint64_t gbase, uint32_t t
 // inner-product ops, whose per-wave partials are left in wave_ip.  STAGED
 // (single state): the matrix-core ops read their fragments from frag, the
 // LDS copy the streaming kernel made, in op order.
-template <typename A, bool TWO, int KMAX, int TB, bool STAGED, bool PAIRS>
+template <typename A, bool TWO, int KMAX, int TB, bool STAGED, bool PAIRS, bool BATCH>
 __device__ __forceinline__ int run_ops(A* s, const PassDesc& P, uint64_t gbase, uint32_t te,
                                        const DevOp* __restrict__ ops, const A* __restrict__ coefs,
                                        double2* wave_ip, const A* frag) {
@@ -1464,7 +1482,7 @@ __device__ __forceinline__ int run_ops(A* s, const PassDesc& P, uint64_t gbase, 
                 if constexpr (TWO) {
                     // Both states in registers, each member dispatched once
                     // for the two (and inner-product members, if any).
-                    n_ip += block_apply2<A>(s, tp, kPhi, G, op, ops + o + 1, nsub, coefs, wave_ip, n_ip, gbase);
+                    n_ip += block_apply2<A, !BATCH>(s, tp, kPhi, G, op, ops + o + 1, nsub, coefs, wave_ip, n_ip, gbase);
                 } else {
                     block_apply<A, PAIRS>(s, tp, G, op, ops + o + 1, nsub, coefs, gbase);
                 }
@@ -1602,7 +1620,8 @@ struct TileLds {
 
 // One tile t of pass P in the LDS array s (TileLds<A, TWO, TB>::kElems
 // elements).  PAIRS: single-state blocks take a thread's groups two at a time.
-template <typename A, bool TWO, int KMAX, int TB, bool PAIRS>
+// BATCH: the batched kernel (tighter on registers).
+template <typename A, bool TWO, int KMAX, int TB, bool PAIRS, bool BATCH>
 __device__ __forceinline__ void tile_body(A* s, A* __restrict__ psi, A* __restrict__ phi, const PassDesc& P,
                                           const DevOp* __restrict__ ops, const A* __restrict__ coefs,
                                           double2* __restrict__ ip_out, int64_t n_tiles, uint64_t t) {
@@ -1658,7 +1677,7 @@ __device__ __forceinline__ void tile_body(A* s, A* __restrict__ psi, A* __restri
     }
     __syncthreads();
 
-    const int n_ip = run_ops<A, TWO, KMAX, TB, false, PAIRS>(s, P, gbase, te, ops, coefs, wave_ip, nullptr);
+    const int n_ip = run_ops<A, TWO, KMAX, TB, false, PAIRS, BATCH>(s, P, gbase, te, ops, coefs, wave_ip, nullptr);
 
     if (TWO && static_cast<int>(threadIdx.x) < n_ip) {
         // Sum the per-wave values in fixed order: partial k of this tile.
@@ -1739,7 +1758,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(tile_w
     A* __restrict__ psi, A* __restrict__ phi, const PassDesc P, const DevOp* __restrict__ ops,
     const A* __restrict__ coefs, double2* __restrict__ ip_out, int64_t n_tiles) {
     __shared__ __align__(16) A s[TileLds<A, TWO, TB>::kElems];
-    tile_body<A, TWO, KMAX, TB, block_pairs(TB)>(s, psi, phi, P, ops, coefs, ip_out, n_tiles, blockIdx.x);
+    tile_body<A, TWO, KMAX, TB, block_pairs(TB), false>(s, psi, phi, P, ops, coefs, ip_out, n_tiles, blockIdx.x);
 }
 
 // Many instances' passes in one launch: workgroup b runs tile b - first[j]
@@ -1757,7 +1776,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(tile_w
         else hi = mid - 1;
     }
     const BatchItem& it = items[lo];
-    tile_body<A, TWO, KMAX, TB, false>(s, static_cast<A*>(it.psi), static_cast<A*>(it.phi), it.P,
+    tile_body<A, TWO, KMAX, TB, false, true>(s, static_cast<A*>(it.psi), static_cast<A*>(it.phi), it.P,
                                 static_cast<const DevOp*>(it.ops), static_cast<const A*>(it.coefs), it.ip_out,
                                 it.n_tiles, static_cast<uint64_t>(b - first[lo]));
 }
@@ -1859,7 +1878,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(stream
         // again and never used.
         load_tile(tn < n_tiles ? tn : n_tiles - 1);
         const uint64_t gbase = tile_base(static_cast<uint64_t>(t), outer);
-        run_ops<A, false, 2 | kPhaseKernel, TB, true, false>(s, P, gbase, te, ops, coefs, nullptr, frag);
+        run_ops<A, false, 2 | kPhaseKernel, TB, true, false, false>(s, P, gbase, te, ops, coefs, nullptr, frag);
         const uint64_t g0 = gbase | dj;
         uint64_t dr = 0;
 #pragma unroll
